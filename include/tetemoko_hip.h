@@ -1,0 +1,141 @@
+/*
+ * tetemoko_hip.h -- C ABI of the MI355X (gfx950) per-patch Godunov update.
+ *
+ * The library replaces the Fortran kernels behind tetemoko's Chombo plugin
+ * surface.  Every entry point takes plain pointers, sizes and index boxes;
+ * no Chombo, HIP or PyTorch types appear in the signatures.  The physics
+ * parameters, which the reference keeps in Fortran COMMON blocks
+ * (/IBC/, /PLASTIC/, /rs/, /slipweak/), are carried per context instead, so
+ * several scenarios can live in one process.
+ *
+ * Storage convention (Chombo FArrayBox): an array over an inclusive index box
+ * is stored in Fortran order, x fastest, then y, then component.
+ * State: 22 components (srcLinElast/LEINDEX.H:12-47).  Rate-and-state fault
+ * data: 11 components (ibcLinElast/RSINDEX.H:12-22).  Slip-weakening fault
+ * data: 9 components (ibcLinElast/SWIBC.H:129-142).
+ */
+#ifndef TETEMOKO_HIP_H
+#define TETEMOKO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMK_NCOMP 22
+#define TMK_NBDRY_RS 11
+#define TMK_NBDRY_SW 9
+#define TMK_MAXPATCH 8
+#define TMK_GHOST 4
+
+/* error codes; the reference aborts (MayDay::Error / Fortran stop) on the first four */
+#define TMK_OK 0
+#define TMK_E_PSI_NAN (-1)     /* RSIBCF.ChF:133-145 "dPsi is nan" */
+#define TMK_E_BAD_BC (-2)      /* RSIBC.cpp:269 "Invalid Boundary Type" */
+#define TMK_E_PRED_ORDER (-3)  /* LinElastPatchGodunov.cpp:313, :543, :645 */
+#define TMK_E_NO_BDRY (-4)     /* fault face reached without fault data (RSIBC::setBdryData not called) */
+#define TMK_E_BOX (-5)         /* an array box does not cover the cells the update reads or writes */
+#define TMK_E_ARG (-6)         /* invalid argument */
+#define TMK_E_HIP (-7)         /* HIP runtime failure */
+#define TMK_E_UNSUPPORTED (-8) /* option the reference aborts on (flattening, artificial viscosity, high-order limiter) */
+#define TMK_E_NOMEM (-9)       /* device memory exhausted */
+
+#define TMK_FAULT_RS 0
+#define TMK_FAULT_SW 1
+
+#define TMK_PRED_CTU 0
+#define TMK_PRED_PLM 1
+#define TMK_PRED_PPM 2
+
+/* arithmetic of the bulk update: FP64 everywhere, or FP32 state and bulk
+ * arithmetic with the fault solve, state evolution, slip and time in FP64 */
+#define TMK_PREC_FP64 0
+#define TMK_PREC_FP32_BULK 1
+
+typedef struct { int lo[2]; int hi[2]; } tmk_box;
+
+/* Replaces the COMMON blocks and the IBC constructor arguments
+ * (IBCCommon.fh:4-6, PlasticCommon.fh:4-5, RSCommon.fh:4-5, SWCommon.fh:4-5,
+ * RSIBC.cpp:17-49, SWIBC.cpp:15-50, amrGodunov.cpp:194-759). */
+typedef struct {
+    /* material: LINELASTSETF inputs (LinElastPhysicsF.ChF:1149-1158) */
+    double cs, cp, G;
+    double sxx0, syy0, szz0, sxy0, sxz0, syz0;
+    /* plasticity: PLASTICSETF (PlasticPhysicsF.ChF:6-25); p_b (cohesion) and p_h (hardening) are 0 in the reference */
+    int use_plasticity;
+    double p_mu, p_beta, p_eta, p_b, p_h;
+    /* boundary type per [dir*2 + side] (amrGodunov.cpp:600-611): 0 outflow, 1 free surface; lo-y is the fault */
+    int bc[4];
+    int fault_kind;
+    /* rate-and-state (RSSETF, RSIBCF.ChF:8-55) */
+    double r0, nuc_time, nsig, rs_a, rs_b, rs_V0, rs_f0, rs_L, rs_fw, rs_Vw, rs_fexp;
+    double x0, y0, rs_rupture_thr, ramp_x, ramp_w, ramp_a, ramp_Vw;
+    double rs_psi0;
+    int state_law; /* 0: RSSETBND psi update (RSIBCF.ChF:126-132); 1: aging law (extension) */
+    /* slip weakening (SWSETF, SWIBCF.ChF:6-27, and SWIBC members) */
+    double fs, fd, dw, sm_value, sw_rupture_thr;
+    int num_patches;
+    double xc[TMK_MAXPATCH], xw[TMK_MAXPATCH], zc[TMK_MAXPATCH], zw[TMK_MAXPATCH], tau_p[TMK_MAXPATCH];
+    double fric_center[2], fric_width[2], outside_f;
+    /* per-face extensions, host arrays indexed by (face i - face_lo), NULL = reference scalar;
+     * copied to the device by tmk_ctx_create */
+    int face_lo, nface;
+    const double *f_a, *f_b, *f_L, *f_fs, *f_fd, *f_dc, *f_syy, *f_tau;
+    /* numerics (amrGodunov.cpp:298-355) */
+    int normal_pred, fourth_order, prim_limiting, char_limiting;
+    double dx;
+    tmk_box domain;
+} tmk_params;
+
+typedef struct tmk_ctx tmk_ctx;
+typedef struct tmk_level tmk_level;
+
+/* Context = one scenario's parameters + stream + device scratch.
+ * Replaces LEPhysIBC::setFortranCommonLE/Plastic (LEPhysIBC.cpp:13-36) and the
+ * RSIBC/SWIBC constructors (RSIBC.cpp:17-49, SWIBC.cpp:15-50). */
+int tmk_ctx_create(tmk_ctx **ctx, const tmk_params *p, int precision);
+int tmk_ctx_destroy(tmk_ctx *ctx);
+const char *tmk_strerror(int code);
+/* 1 when the library was built for gfx950 and a device is visible */
+int tmk_device_ok(void);
+
+/* LinElastPatchGodunov::updateState (LinElastPatchGodunov.cpp:101-129), with
+ * LinElastPhysics::computeUpdate (LinElastPhysics.cpp:135-179) and the IBC
+ * calls it makes (RSIBC::primBC, RSIBC::updateBoundary).  Host FArrayBox data:
+ *   U   : state on U_box (valid grown by the ghost cells), 22 comps, updated in place
+ *   F0/F1: optional fluxes on surroundingNodes(valid, d), 22 comps (NULL to skip)
+ *   B   : fault data on B_box (j = 0 faces, x-ghosts), NULL when valid does not touch lo-y
+ *   max_wave_speed: MAXWAVESPEEDF (LinElastPhysicsF.ChF:10-21) */
+int tmk_patch_update_state(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_box, int ncomp,
+                           double *F0, double *F1, double *B, tmk_box B_box, int nbdry,
+                           double time, double dt, double *max_wave_speed);
+
+/* LinElastPatchGodunov::plasticUpdate (LinElastPatchGodunov.cpp:161-170) ->
+ * PLASTICCORRECTION (PlasticPhysicsF.ChF:30-69) on the valid box. */
+int tmk_patch_plastic_update(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_box, int ncomp, double dt);
+
+/* Device-resident level: LinElastLevelGodunov (LinElastLevelGodunov.cpp:45-398).
+ * boxes: disjoint valid boxes of one level.  Fault data is allocated for every
+ * box whose lo-y edge lies on the domain lo-y boundary. */
+int tmk_level_create(tmk_ctx *ctx, int nboxes, const tmk_box *boxes, tmk_level **out);
+int tmk_level_destroy(tmk_level *lev);
+/* copy one box's ghosted state (grow(box,4), 22 comps) and fault data
+ * (x in [lo-4, hi+4], j = 0, nbdry comps; NULL if the box has none) to/from the device */
+int tmk_level_upload(tmk_level *lev, int ibox, const double *U, const double *B);
+int tmk_level_download(tmk_level *lev, int ibox, double *U, double *B);
+/* LinElastLevelGodunov::step (LinElastLevelGodunov.cpp:174-398) followed by
+ * LinElastLevelGodunov::plasticUpdate (:643-667) when plasticity is on:
+ * same-level ghost exchange, one launch over every box of the level, fault data
+ * update.  dt_new = dx / max(cs,cp) (LinElastLevelGodunov.cpp:377). */
+int tmk_level_step(tmk_level *lev, double time, double dt, double *dt_new);
+/* nsteps level steps with time advancing by dt, captured once as a hipGraph */
+int tmk_level_run(tmk_level *lev, int nsteps, double time, double dt);
+/* synchronize and return the first error any launch reported */
+int tmk_level_sync(tmk_level *lev);
+
+/* Tile shape of the fused update (autotuner hook); 0 = library default. */
+int tmk_set_tile(tmk_ctx *ctx, int bx, int by);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on seeded inputs.
+
+FP64 tolerance (stated by the task): 1e-12 relative on state, slip and rupture
+time, measured per component as max|gpu - oracle| / max|oracle|.  Patches that
+do not touch the fault involve no transcendental function and are expected to
+be bit-identical (both sides evaluate the same expressions without FMA
+contraction).
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+import tmo
+from tetemoko_amd import capi, scenarios
+
+pytestmark = pytest.mark.gpu
+
+G = scenarios.GHOST
+TOL = 1e-12
+
+
+def _cfg(fault, pred, nx=48, ny=40, seed=7, prim=1, char=0, fourth=1, bc=(0, 0, 0, 0), antiplane=False):
+    if fault == "rs":
+        cfg = scenarios.config_c1(seed=seed, nx=nx, ny=ny, dx=25.0)
+        cfg["rs"]["state_law"] = 0 if seed % 2 else 1
+    else:
+        cfg = scenarios.config_c0(seed=seed, nx=nx, ny=ny, dx=100.0)
+    cfg["numerics"] = {"normal_predictor": pred, "fourth_order": fourth, "prim_limiting": prim,
+                       "char_limiting": char}
+    cfg["bc"] = bc
+    if antiplane:
+        bg = list(cfg["background"])
+        bg[4] = 3.0   # sxz0
+        bg[5] = -2.0  # syz0
+        cfg["background"] = tuple(bg)
+    return cfg
+
+
+def _state(cfg, seed, antiplane=False):
+    U, B = scenarios.level_arrays(cfg)
+    scenarios.perturb(U, seed=seed, amp_v=1.0, amp_s=5.0, antiplane=antiplane)
+    rng = np.random.default_rng(seed + 1)
+    if cfg["fault"] == "rs":
+        B[10] += rng.uniform(-0.02, 0.02, size=B.shape[1])
+        B[8] = rng.uniform(0, 1, size=B.shape[1])
+        B[9] = np.where(rng.uniform(size=B.shape[1]) < 0.5, 1e10, 0.01)
+    else:
+        B[7] = rng.uniform(0, 0.6, size=B.shape[1])
+        B[4] = rng.uniform(0, 0.01, size=B.shape[1])
+    return U, B
+
+
+def _oracle_patch(cfg, valid, U, ubox, B, bbox, time, dt, with_flux=False):
+    op = tmo.OracleParams(cfg)
+    L = tmo.lib()
+    Uf = tmo.FabArray(ubox[:2], ubox[2:], 22, data=U)
+    Bf = tmo.FabArray((bbox[0], 0), (bbox[2], 0), B.shape[0], data=B.reshape(B.shape[0], 1, -1)) if B is not None else None
+    F0 = F1 = None
+    if with_flux:
+        F0 = tmo.FabArray((valid[0], valid[1]), (valid[2] + 1, valid[3]), 22)
+        F1 = tmo.FabArray((valid[0], valid[1]), (valid[2], valid[3] + 1), 22)
+    e = L.tmo_update_state(Uf.fab, Bf.fab if Bf else None, F0.fab if F0 else None, F1.fab if F1 else None,
+                           op.ref, tmo.Box.make(*valid), time, dt)
+    return e, (F0.a if F0 else None), (F1.a if F1 else None)
+
+
+def _relerr(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    scale = np.maximum(np.max(np.abs(b), axis=1), 1e-300)
+    return np.max(np.abs(a - b), axis=1) / scale
+
+
+def _patch_case(cfg, valid, seed, precision=capi.PREC_FP64, antiplane=False, with_flux=False, tile=None):
+    nx, ny = cfg["nx"], cfg["ny"]
+    U, B = _state(cfg, seed, antiplane)
+    ubox = (valid[0] - G, valid[1] - G, valid[2] + G, valid[3] + G)
+    Ub = np.ascontiguousarray(U[:, ubox[1] + G:ubox[3] + G + 1, ubox[0] + G:ubox[2] + G + 1])
+    touches = valid[1] == 0
+    bbox = (valid[0] - G, 0, valid[2] + G, 0)
+    Bb = np.ascontiguousarray(B[:, bbox[0] + G:bbox[2] + G + 1]) if touches else None
+    dt = scenarios.dt_of(cfg)
+    time = 0.37
+    Ur, Br = Ub.copy(), (Bb.copy() if Bb is not None else None)
+    e, F0r, F1r = _oracle_patch(cfg, valid, Ur, ubox, Br, bbox, time, dt, with_flux)
+    assert e == 0
+    ctx = capi.Context(cfg, precision)
+    if tile:
+        ctx.set_tile(*tile)
+    Ug, Bg = Ub.copy(), (Bb.copy() if Bb is not None else None)
+    F0g = F1g = None
+    if with_flux:
+        F0g = np.zeros_like(F0r)
+        F1g = np.zeros_like(F1r)
+    mws = ctx.update_state(valid, Ug, ubox, Bg, bbox if touches else None, F0g, F1g, time=time, dt=dt)
+    ctx.close()
+    assert mws == max(cfg["cs"], cfg["cp"])
+    return (Ug, Ur), (Bg, Br), (F0g, F0r), (F1g, F1r), touches
+
+
+BOXES = {
+    "interior": (12, 10, 29, 25),
+    "fault": (8, 0, 27, 15),
+    "xlo_fault_corner": (0, 0, 19, 11),
+    "xhi": (30, 12, 47, 30),
+    "yhi": (10, 24, 33, 39),
+    "xhi_yhi_corner": (34, 28, 47, 39),
+    "whole": (0, 0, 47, 39),
+}
+
+
+@pytest.mark.parametrize("pred,prim,char,fourth", [
+    ("CTU", 1, 0, 1), ("PLM", 1, 0, 1), ("PLM", 1, 0, 0), ("PLM", 0, 1, 1), ("PLM", 0, 0, 0),
+    ("PPM", 1, 0, 1), ("PPM", 0, 1, 1), ("PPM", 0, 0, 1),
+])
+@pytest.mark.parametrize("fault", ["rs", "sw"])
+@pytest.mark.parametrize("where", list(BOXES))
+def test_patch_update_fp64(pred, prim, char, fourth, fault, where):
+    print("case", pred, prim, char, fourth, fault, where, flush=True)
+    cfg = _cfg(fault, pred, prim=prim, char=char, fourth=fourth, bc=(0, 1, 0, 1) if where.endswith("corner") else (0, 0, 0, 0))
+    (Ug, Ur), (Bg, Br), _, _, touches = _patch_case(cfg, BOXES[where], seed=11)
+    eu = _relerr(Ug, Ur)
+    assert np.all(eu <= TOL), "state rel err per comp: %s" % eu
+    if not touches:
+        assert np.array_equal(Ug, Ur), "non-fault patch not bit-identical: %s" % eu
+    if touches:
+        eb = _relerr(Bg, Br)
+        assert np.all(eb <= TOL), "fault rel err per comp: %s" % eb
+
+
+@pytest.mark.parametrize("where", ["interior", "fault", "whole"])
+def test_patch_antiplane(where):
+    cfg = _cfg("rs", "PPM", antiplane=True)
+    (Ug, Ur), (Bg, Br), _, _, touches = _patch_case(cfg, BOXES[where], seed=3, antiplane=True)
+    eu = _relerr(Ug, Ur)
+    assert np.all(eu <= TOL), eu
+    if touches:
+        assert np.all(_relerr(Bg, Br) <= TOL)
+
+
+@pytest.mark.parametrize("tile", [(32, 8), (32, 16), (64, 4), (64, 8), (32, 32), (64, 16)])
+def test_patch_tile_shapes(tile):
+    cfg = _cfg("rs", "PPM")
+    (Ug, Ur), (Bg, Br), _, _, _ = _patch_case(cfg, BOXES["whole"], seed=5, tile=tile)
+    assert np.all(_relerr(Ug, Ur) <= TOL)
+    assert np.all(_relerr(Bg, Br) <= TOL)
+
+
+def test_patch_fluxes():
+    cfg = _cfg("rs", "PPM")
+    _, _, (F0g, F0r), (F1g, F1r), _ = _patch_case(cfg, BOXES["fault"], seed=9, with_flux=True)
+    assert np.all(_relerr(F0g, F0r) <= TOL)
+    assert np.all(_relerr(F1g, F1r) <= TOL)
+
+
+def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=False):
+    U, B = _state(cfg, seed)
+    Ur, Br = U.copy(), B.copy()
+    dt = scenarios.dt_of(cfg)
+    assert tmo.level_steps(tmo.OracleParams(cfg), Ur, Br, 0.0, dt, nsteps) == 0
+    ctx = capi.Context(cfg, precision)
+    lev = capi.Level(ctx, boxes)
+    for k, b in enumerate(boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+    if use_run:
+        lev.run(nsteps, 0.0, dt)
+    else:
+        t = 0.0
+        for _ in range(nsteps):
+            lev.step(t, dt)
+            t += dt
+    lev.sync()
+    Ug, Bg = np.full_like(U, np.nan), np.full_like(B, np.nan)
+    for k, b in enumerate(boxes):
+        ub = scenarios.box_view(U, b)
+        bb = scenarios.box_bview(B, b) if b[1] == 0 else None
+        lev.download(k, ub, bb)
+        x0, y0, x1, y1 = b
+        Ug[:, y0 + G:y1 + G + 1, x0 + G:x1 + G + 1] = ub[:, G:-G, G:-G]
+        if bb is not None:
+            Bg[:, x0 + G:x1 + G + 1] = bb[:, G:-G]
+    lev.close()
+    ctx.close()
+    return (Ug[:, G:-G, G:-G], Ur[:, G:-G, G:-G]), (Bg[:, G:-G], Br[:, G:-G])
+
+
+@pytest.mark.parametrize("fault", ["rs", "sw"])
+@pytest.mark.parametrize("maxbox", [None, 16, 20])
+def test_level_steps(fault, maxbox):
+    cfg = _cfg(fault, "PPM", nx=64, ny=40)
+    boxes = [(0, 0, 63, 39)] if maxbox is None else scenarios.split_boxes(64, 40, maxbox)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, 6, seed=21)
+    assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg, Br) <= TOL), _relerr(Bg, Br)
+
+
+def test_level_graph_run_matches_steps():
+    cfg = _cfg("rs", "PPM", nx=64, ny=40)
+    a = _level_case(cfg, [(0, 0, 63, 39)], 7, seed=4, use_run=True)
+    (Ug, Ur), (Bg, Br) = a
+    assert np.all(_relerr(Ug, Ur) <= TOL)
+    assert np.all(_relerr(Bg, Br) <= TOL)
+
+
+def test_level_plasticity():
+    cfg = _cfg("rs", "PPM", nx=64, ny=40)
+    cfg["plasticity"] = 1
+    cfg["p_mu"] = 0.6
+    cfg["p_beta"] = 0.3
+    cfg["p_eta"] = 0.03 * scenarios.G_MPA
+    cfg["p_b"] = 1.0
+    cfg["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, scenarios.split_boxes(64, 40, 20), 5, seed=8)
+    assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg, Br) <= TOL)
+    assert np.max(Ur[9]) > 0.0, "plastic flow expected somewhere"
+
+
+def test_fp32_bulk_level():
+    cfg = _cfg("rs", "PPM", nx=64, ny=40)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 63, 39)], 6, seed=21, precision=capi.PREC_FP32_BULK)
+    assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= 1e-4), _relerr(Bg, Br)
+
+
+def test_psi_nan_error():
+    cfg = _cfg("rs", "PPM")
+    cfg["rs"]["state_law"] = 0
+    U, B = _state(cfg, 2)
+    B[10, 20] = np.nan
+    valid = (8, 0, 27, 15)
+    ubox = (valid[0] - G, valid[1] - G, valid[2] + G, valid[3] + G)
+    Ub = np.ascontiguousarray(U[:, ubox[1] + G:ubox[3] + G + 1, ubox[0] + G:ubox[2] + G + 1])
+    bbox = (valid[0] - G, 0, valid[2] + G, 0)
+    Bb = np.ascontiguousarray(B[:, bbox[0] + G:bbox[2] + G + 1])
+    ctx = capi.Context(cfg)
+    with pytest.raises(capi.TmkError) as ei:
+        ctx.update_state(valid, Ub, ubox, Bb, bbox, time=0.0, dt=scenarios.dt_of(cfg))
+    assert ei.value.code == capi.E_PSI_NAN
+    ctx.close()
+
+
+def test_argument_errors():
+    cfg = _cfg("rs", "PPM")
+    U, B = _state(cfg, 2)
+    ctx = capi.Context(cfg)
+    valid = (8, 0, 27, 15)
+    small = (valid[0] - 2, valid[1] - 2, valid[2] + 2, valid[3] + 2)
+    Us = np.zeros((22, small[3] - small[1] + 1, small[2] - small[0] + 1))
+    with pytest.raises(capi.TmkError) as ei:
+        ctx.update_state(valid, Us, small, None, None, time=0.0, dt=0.001)
+    assert ei.value.code == capi.E_BOX
+    ubox = (valid[0] - G, valid[1] - G, valid[2] + G, valid[3] + G)
+    Ub = np.zeros((22, ubox[3] - ubox[1] + 1, ubox[2] - ubox[0] + 1))
+    with pytest.raises(capi.TmkError) as ei:
+        ctx.update_state(valid, Ub, ubox, None, None, time=0.0, dt=0.001)
+    assert ei.value.code == capi.E_NO_BDRY
+    ctx.close()
+    bad = dict(cfg)
+    bad["bc"] = (0, 5, 0, 0)
+    with pytest.raises(capi.TmkError) as ei:
+        capi.Context(bad)
+    assert ei.value.code == capi.E_BAD_BC

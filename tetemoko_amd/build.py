@@ -1,0 +1,32 @@
+"""Build libtetemoko_hip.so for gfx950 in-tree (tetemoko_amd/_lib)."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I" + os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument"]
+
+
+def build(verbose=True):
+    src = os.path.join(HERE, "csrc", "tmk_kernels.hip")
+    out_dir = os.path.join(HERE, "_lib")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libtetemoko_hip.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-o", tmp, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libtetemoko_hip.so")
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,281 @@
+"""ctypes binding of the C ABI in include/tetemoko_hip.h (libtetemoko_hip.so).
+
+The shared library is built in-tree (tetemoko_amd/_lib) by build.py; loading
+fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libtetemoko_hip.so")
+
+NCOMP = 22
+NBDRY_RS = 11
+NBDRY_SW = 9
+MAXPATCH = 8
+GHOST = 4
+
+OK = 0
+E_PSI_NAN = -1
+E_BAD_BC = -2
+E_PRED_ORDER = -3
+E_NO_BDRY = -4
+E_BOX = -5
+E_ARG = -6
+E_HIP = -7
+E_UNSUPPORTED = -8
+E_NOMEM = -9
+
+PREC_FP64 = 0
+PREC_FP32_BULK = 1
+
+EXPORTED = (
+    "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok",
+    "tmk_patch_update_state", "tmk_patch_plastic_update",
+    "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
+    "tmk_level_step", "tmk_level_run", "tmk_level_sync", "tmk_set_tile",
+)
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_int * 2), ("hi", ctypes.c_int * 2)]
+
+    @staticmethod
+    def make(lo0, lo1, hi0, hi1):
+        b = Box()
+        b.lo[0], b.lo[1], b.hi[0], b.hi[1] = lo0, lo1, hi0, hi1
+        return b
+
+    def tup(self):
+        return (self.lo[0], self.lo[1], self.hi[0], self.hi[1])
+
+
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("cs", _D), ("cp", _D), ("G", _D),
+        ("sxx0", _D), ("syy0", _D), ("szz0", _D), ("sxy0", _D), ("sxz0", _D), ("syz0", _D),
+        ("use_plasticity", ctypes.c_int),
+        ("p_mu", _D), ("p_beta", _D), ("p_eta", _D), ("p_b", _D), ("p_h", _D),
+        ("bc", ctypes.c_int * 4),
+        ("fault_kind", ctypes.c_int),
+        ("r0", _D), ("nuc_time", _D), ("nsig", _D), ("rs_a", _D), ("rs_b", _D), ("rs_V0", _D), ("rs_f0", _D),
+        ("rs_L", _D), ("rs_fw", _D), ("rs_Vw", _D), ("rs_fexp", _D),
+        ("x0", _D), ("y0", _D), ("rs_rupture_thr", _D), ("ramp_x", _D), ("ramp_w", _D), ("ramp_a", _D),
+        ("ramp_Vw", _D),
+        ("rs_psi0", _D),
+        ("state_law", ctypes.c_int),
+        ("fs", _D), ("fd", _D), ("dw", _D), ("sm_value", _D), ("sw_rupture_thr", _D),
+        ("num_patches", ctypes.c_int),
+        ("xc", _D * MAXPATCH), ("xw", _D * MAXPATCH), ("zc", _D * MAXPATCH), ("zw", _D * MAXPATCH),
+        ("tau_p", _D * MAXPATCH),
+        ("fric_center", _D * 2), ("fric_width", _D * 2), ("outside_f", _D),
+        ("face_lo", ctypes.c_int), ("nface", ctypes.c_int),
+        ("f_a", _PD), ("f_b", _PD), ("f_L", _PD), ("f_fs", _PD), ("f_fd", _PD), ("f_dc", _PD),
+        ("f_syy", _PD), ("f_tau", _PD),
+        ("normal_pred", ctypes.c_int), ("fourth_order", ctypes.c_int), ("prim_limiting", ctypes.c_int),
+        ("char_limiting", ctypes.c_int),
+        ("dx", _D),
+        ("domain", Box),
+    ]
+
+
+class TmkError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("%s: tetemoko_hip error %d (%s)" % (where, code, strerror(code)))
+
+
+_lib = None
+
+
+def lib():
+    """Load libtetemoko_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtetemoko_hip.so is not built (%s); run python -c "
+                               "'import __graft_entry__; __graft_entry__.build()'" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        c_int, PBox = ctypes.c_int, ctypes.POINTER(Box)
+        vp = ctypes.c_void_p
+        L.tmk_ctx_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params), c_int]
+        L.tmk_ctx_destroy.argtypes = [vp]
+        L.tmk_strerror.argtypes = [c_int]
+        L.tmk_strerror.restype = ctypes.c_char_p
+        L.tmk_device_ok.argtypes = []
+        L.tmk_patch_update_state.argtypes = [vp, Box, _PD, Box, c_int, _PD, _PD, _PD, Box, c_int, _D, _D, _PD]
+        L.tmk_patch_plastic_update.argtypes = [vp, Box, _PD, Box, c_int, _D]
+        L.tmk_level_create.argtypes = [vp, c_int, PBox, ctypes.POINTER(vp)]
+        L.tmk_level_destroy.argtypes = [vp]
+        L.tmk_level_upload.argtypes = [vp, c_int, _PD, _PD]
+        L.tmk_level_download.argtypes = [vp, c_int, _PD, _PD]
+        L.tmk_level_step.argtypes = [vp, _D, _D, _PD]
+        L.tmk_level_run.argtypes = [vp, c_int, _D, _D]
+        L.tmk_level_sync.argtypes = [vp]
+        L.tmk_set_tile.argtypes = [vp, c_int, c_int]
+        _lib = L
+    return _lib
+
+
+def strerror(code):
+    try:
+        return lib().tmk_strerror(code).decode()
+    except RuntimeError:
+        return "library not loaded"
+
+
+def _check(code, where):
+    if code != 0:
+        raise TmkError(code, where)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PD)
+
+
+def make_params(cfg):
+    """tmk_params from a scenario config dict (see tetemoko_amd.scenarios); keeps face arrays alive."""
+    p = Params()
+    keep = []
+    p.cs, p.cp, p.G = cfg["cs"], cfg["cp"], cfg["G"]
+    bg = cfg.get("background", (0.0,) * 6)
+    p.sxx0, p.syy0, p.szz0, p.sxy0, p.sxz0, p.syz0 = bg
+    p.use_plasticity = int(cfg.get("plasticity", 0))
+    p.p_mu = cfg.get("p_mu", 0.5735)
+    p.p_beta = cfg.get("p_beta", p.p_mu / 2)
+    p.p_eta = cfg.get("p_eta", 0.2775)
+    p.p_b = cfg.get("p_b", 0.0)
+    p.p_h = cfg.get("p_h", 0.0)
+    for k, v in enumerate(cfg.get("bc", (0, 0, 0, 0))):
+        p.bc[k] = v
+    p.fault_kind = 0 if cfg.get("fault", "rs") == "rs" else 1
+    rs = cfg.get("rs", {})
+    for name, key, dflt in (("r0", "nuc_R", 0.0), ("nuc_time", "nuc_T", 0.0), ("nsig", "nuc_S", 0.0),
+                            ("rs_a", "a", 0.01), ("rs_b", "b", 0.014), ("rs_V0", "V0", 1e-6),
+                            ("rs_f0", "f0", 0.6), ("rs_L", "L", 0.02), ("rs_fw", "fw", 0.13),
+                            ("rs_Vw", "Vw", 0.17), ("rs_fexp", "fric_exp", -1.0), ("x0", "nuc_x", 0.0),
+                            ("y0", "nuc_y", 0.0), ("rs_rupture_thr", "rupture_thr", 0.001),
+                            ("ramp_x", "ramp_x", 0.0), ("ramp_w", "ramp_w", 1e40),
+                            ("ramp_a", "ramp_a", 0.0), ("ramp_Vw", "ramp_Vw", 0.0), ("rs_psi0", "psi", 0.6)):
+        setattr(p, name, float(rs.get(key, dflt)))
+    p.state_law = int(rs.get("state_law", 0))
+    sw = cfg.get("sw", {})
+    p.fs = sw.get("f_static", 0.677)
+    p.fd = sw.get("f_dynamic", 0.525)
+    p.dw = sw.get("d_weak", 0.4)
+    p.sm_value = sw.get("smooth", 0.0)
+    p.sw_rupture_thr = sw.get("rupture_thr", 0.001)
+    patches = sw.get("patches", [])
+    p.num_patches = len(patches)
+    for k, (xc, xw, tau) in enumerate(patches):
+        p.xc[k], p.xw[k], p.tau_p[k] = xc, xw, tau
+    p.fric_center[0], p.fric_center[1] = sw.get("fric_center", (0.0, 0.0))
+    p.fric_width[0], p.fric_width[1] = sw.get("fric_width", (1e40, 1e40))
+    p.outside_f = sw.get("outside_f", 1e4)
+    faces = cfg.get("faces", {})
+    p.face_lo = int(faces.get("lo", 0))
+    p.nface = 0
+    for name in ("a", "b", "L", "fs", "fd", "dc", "syy", "tau"):
+        arr = faces.get(name)
+        if arr is not None:
+            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            keep.append(arr)
+            p.nface = len(arr)
+            setattr(p, "f_" + name, arr.ctypes.data_as(_PD))
+    num = cfg.get("numerics", {})
+    p.normal_pred = {"CTU": 0, "PLM": 1, "PPM": 2}[num.get("normal_predictor", "PPM")]
+    p.fourth_order = int(num.get("fourth_order", 1))
+    p.prim_limiting = int(num.get("prim_limiting", 1))
+    p.char_limiting = int(num.get("char_limiting", 0))
+    p.dx = cfg["dx"]
+    p.domain = Box.make(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)
+    return p, keep
+
+
+class Context:
+    """One scenario: parameters (the reference's COMMON blocks), stream, device scratch."""
+
+    def __init__(self, cfg, precision=PREC_FP64):
+        self.cfg = cfg
+        self.params, self._keep = make_params(cfg)
+        h = ctypes.c_void_p()
+        _check(lib().tmk_ctx_create(ctypes.byref(h), ctypes.byref(self.params), precision), "tmk_ctx_create")
+        self.h = h
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            lib().tmk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tile(self, bx, by):
+        _check(lib().tmk_set_tile(self.h, bx, by), "tmk_set_tile")
+
+    def update_state(self, valid, U, ubox, B=None, bbox=None, F0=None, F1=None, time=0.0, dt=1.0):
+        """LinElastPatchGodunov::updateState on host FArrayBox arrays (in place). Returns max wave speed."""
+        mws = ctypes.c_double()
+        bb = Box.make(*bbox) if bbox is not None else Box.make(0, 0, -1, -1)
+        nb = B.shape[0] if B is not None else 0
+        code = lib().tmk_patch_update_state(self.h, Box.make(*valid), _ptr(U), Box.make(*ubox), NCOMP,
+                                            _ptr(F0), _ptr(F1), _ptr(B), bb, nb, time, dt, ctypes.byref(mws))
+        _check(code, "tmk_patch_update_state")
+        return mws.value
+
+    def plastic_update(self, valid, U, ubox, dt):
+        _check(lib().tmk_patch_plastic_update(self.h, Box.make(*valid), _ptr(U), Box.make(*ubox), NCOMP, dt),
+               "tmk_patch_plastic_update")
+
+
+class Level:
+    """Device-resident level of boxes (LinElastLevelGodunov)."""
+
+    def __init__(self, ctx, boxes):
+        self.ctx = ctx
+        self.boxes = [tuple(b) for b in boxes]
+        arr = (Box * len(boxes))(*[Box.make(*b) for b in boxes])
+        h = ctypes.c_void_p()
+        _check(lib().tmk_level_create(ctx.h, len(boxes), arr, ctypes.byref(h)), "tmk_level_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().tmk_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, ibox, U, B=None):
+        _check(lib().tmk_level_upload(self.h, ibox, _ptr(U), _ptr(B)), "tmk_level_upload")
+
+    def download(self, ibox, U, B=None):
+        _check(lib().tmk_level_download(self.h, ibox, _ptr(U), _ptr(B)), "tmk_level_download")
+
+    def step(self, time, dt):
+        dtn = ctypes.c_double()
+        _check(lib().tmk_level_step(self.h, time, dt, ctypes.byref(dtn)), "tmk_level_step")
+        return dtn.value
+
+    def run(self, nsteps, time, dt):
+        _check(lib().tmk_level_run(self.h, nsteps, time, dt), "tmk_level_run")
+
+    def sync(self):
+        _check(lib().tmk_level_sync(self.h), "tmk_level_sync")

@@ -1,0 +1,575 @@
+// tmk_physics.hpp -- pointwise physics of the per-patch Godunov update, device side.
+//
+// Every function restates one reference Fortran routine on the reduced
+// "slot" form the fused kernel carries.  For a sweep direction d the elastic
+// system only couples three velocity/traction pairs per side:
+//   slot N (normal)   : (v_n, s_nn)   dir 0: (vx, sxx)   dir 1: (vy, syy)
+//   slot S (in-plane) : (v_t, s_nt)   dir 0: (vy, sxy)   dir 1: (vx, sxy)
+//   slot A (antiplane): (v_z, s_nz)   dir 0: (vz, sxz)   dir 1: (vz, syz)
+// These are exactly the components read by RIEMANNF's wave characteristics,
+// GETFLUXF and GETADWDXF (LinElastPhysicsF.ChF:265-353, :601-765, :791-901);
+// the zero-speed invariants (syy - sxx lam/(2G+lam), ...) never reach a flux.
+// Side 2 (the mirrored half space, LEINDEX.H:23) flips the velocity sign in
+// direction 1 (rv = -1, LinElastPhysicsF.ChF:285-301).
+//
+// Expressions keep the reference's operation order so the FP64 path rounds
+// like the reference when compiled without FMA contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmk {
+
+// ---------------------------------------------------------------------------
+// parameters (device copy of tmk_params plus derived constants)
+// ---------------------------------------------------------------------------
+struct DevParams {
+    double cs, cp, G, rho, lam, K, Zp, Zs;
+    double den;    // 2G + lam   (LinElastPhysicsF.ChF:427 form)
+    double lam2G;  // lam + 2G   (LinElastPhysicsF.ChF:318 form)
+    double bg[22];
+    int plast;
+    double p_mu, p_beta, p_eta, p_b, p_h;
+    int bc[4];
+    int fault_kind;
+    double r0, nsig, rs_a, rs_b, rs_V0, rs_f0, rs_L, rs_fw, rs_Vw, rs_fexp;
+    double x0, rs_thr, ramp_x, ramp_w, ramp_a, ramp_Vw;
+    int state_law;
+    double fs, fd, dw, sm, sw_thr;
+    int npatch;
+    double xc[8], xw[8], taup[8];
+    double fric_c0, fric_w0, outside_f;
+    int face_lo, nface;
+    const double *f_a, *f_b, *f_L, *f_fs, *f_fd, *f_dc, *f_syy, *f_tau;
+    int pred, fourth, prim_lim, char_lim;
+    double dx;
+    int dlo[2], dhi[2];
+};
+
+template <typename T, int NS>
+struct Slots {
+    T v[2][NS];
+    T s[2][NS];
+};
+
+// elastic constants in the bulk arithmetic type
+template <typename T>
+struct Mat {
+    T rho, lam, G, Zp, Zs, den, lam2G;
+    __device__ __forceinline__ explicit Mat(const DevParams& p)
+        : rho((T)p.rho), lam((T)p.lam), G((T)p.G), Zp((T)p.Zp), Zs((T)p.Zs), den((T)p.den),
+          lam2G((T)p.lam2G) {}
+    __device__ __forceinline__ T Z(int k) const { return k == 0 ? Zp : Zs; }
+};
+
+// component index of (slot, velocity/stress) for direction d, side 1
+__host__ __device__ constexpr int slot_comp(int d, int k, int is_s)
+{
+    return d == 0 ? (k == 0 ? (is_s ? 3 : 0) : k == 1 ? (is_s ? 6 : 1) : (is_s ? 7 : 2))
+                  : (k == 0 ? (is_s ? 4 : 1) : k == 1 ? (is_s ? 6 : 0) : (is_s ? 8 : 2));
+}
+
+// ---------------------------------------------------------------------------
+// slopes and PPM (restated from the published method; see oracle for the same form)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T vl_limit(T c, T l, T r)
+{
+    T lim = fmin((T)2 * fabs(l), (T)2 * fabs(r));
+    lim = fmin(lim, fabs(c));
+    if (l * r < (T)0) lim = (T)0;
+    return lim * copysign((T)1, c);
+}
+
+// side classification of stencil cell: 0 centred, 1 domain lo, 2 domain hi
+__device__ __forceinline__ int cell_side(int c, int dlo, int dhi) { return c == dlo ? 1 : (c == dhi ? 2 : 0); }
+
+// second-order slope differences at stencil position q (0..4, centre 2)
+template <typename T>
+__device__ __forceinline__ void slope_diffs(const T* w, int q, int side, T& C, T& L, T& R)
+{
+    if (side == 0) {
+        const T dr = w[q + 1] - w[q];
+        const T dl = w[q] - w[q - 1];
+        R = dr; L = dl; C = (T)0.5 * (dr + dl);
+    } else if (side == 1) {
+        const T dr = w[q + 1] - w[q];
+        C = L = R = dr;
+    } else {
+        const T dl = w[q] - w[q - 1];
+        C = L = R = dl;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T vl_slope(const T* w, int q, int side, bool limit)
+{
+    T C, L, R;
+    slope_diffs(w, q, side, C, L, R);
+    return limit ? vl_limit(C, L, R) : C;
+}
+
+// PPM face increments (face value minus cell value) at the lo and hi faces of
+// the centre cell, Colella & Woodward (1984) eq. 1.6/1.9 on a uniform grid
+template <typename T>
+__device__ __forceinline__ void ppm_incr(const T* w, int ci, int dlo, int dhi, bool limit, T& dm, T& dp)
+{
+    const T sixth = (T)(1.0 / 6.0);
+    const int s0 = cell_side(ci, dlo, dhi);
+    const T d0 = vl_slope(w, 2, s0, limit);
+    T fl, fh;
+    if (ci == dlo) {
+        fl = w[2] - (T)0.5 * d0;
+    } else {
+        const T dmm = vl_slope(w, 1, cell_side(ci - 1, dlo, dhi), limit);
+        fl = (T)0.5 * (w[2] + w[1]) + sixth * (dmm - d0);
+    }
+    if (ci == dhi) {
+        fh = w[2] + (T)0.5 * d0;
+    } else {
+        const T dpp = vl_slope(w, 3, cell_side(ci + 1, dlo, dhi), limit);
+        fh = (T)0.5 * (w[3] + w[2]) + sixth * (d0 - dpp);
+    }
+    dm = ((T)0 - w[2]) + fl;
+    dp = ((T)0 - w[2]) + fh;
+}
+
+template <typename T>
+__device__ __forceinline__ void ppm_limit(T& m, T& p)
+{
+    if (m * p < (T)0) {
+        const T nm = copysign(fmin((T)2 * fabs(p), fabs(m)), m);
+        const T np = copysign(fmin((T)2 * fabs(m), fabs(p)), p);
+        m = nm; p = np;
+    } else {
+        m = (T)0; p = (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// normal predictor for one cell and one direction (LinElastPatchGodunov.cpp:471-702)
+// w[side][slot][v|s][5]: primitive stencil along d; outputs W- and W+ (primitive)
+// ---------------------------------------------------------------------------
+template <typename T, int NS>
+__device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m, int pred, T rv,
+                                            const T (&w)[2][NS][2][5], int ci, int dlo, int dhi,
+                                            T dtbydx, Slots<T, NS>& Wm, Slots<T, NS>& Wp)
+{
+    if (pred == 0) {  // CTUNormalPred: W+- = W
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                Wm.v[s][k] = Wp.v[s][k] = w[s][k][0][2];
+                Wm.s[s][k] = Wp.s[s][k] = w[s][k][1][2];
+            }
+        return;
+    }
+    const bool limit = P.char_lim || P.prim_lim;
+    const bool charl = P.char_lim != 0;
+    // characteristic increments: wave moving toward lo (m) / toward hi (p)
+    T cm_m[2][NS], cp_m[2][NS];  // chars of the minus-side increment
+    T cm_p[2][NS], cp_p[2][NS];  // chars of the plus-side increment
+    if (pred == 2) {
+        T dvm[2][NS], dsm[2][NS], dvp[2][NS], dsp[2][NS];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                ppm_incr(w[s][k][0], ci, dlo, dhi, limit, dvm[s][k], dvp[s][k]);
+                ppm_incr(w[s][k][1], ci, dlo, dhi, limit, dsm[s][k], dsp[s][k]);
+                if (limit && !charl) {
+                    ppm_limit(dvm[s][k], dvp[s][k]);
+                    ppm_limit(dsm[s][k], dsp[s][k]);
+                }
+            }
+        // CHARANALYSISF (LinElastPhysicsF.ChF:424-447)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const T Z = m.Z(k);
+                const T vm = s ? rv * dvm[s][k] : dvm[s][k];
+                const T vp = s ? rv * dvp[s][k] : dvp[s][k];
+                cm_m[s][k] = dsm[s][k] + vm * Z;
+                cp_m[s][k] = dsm[s][k] - vm * Z;
+                cm_p[s][k] = dsp[s][k] + vp * Z;
+                cp_p[s][k] = dsp[s][k] - vp * Z;
+                if (charl) {
+                    ppm_limit(cm_m[s][k], cm_p[s][k]);
+                    ppm_limit(cp_m[s][k], cp_p[s][k]);
+                }
+            }
+        // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const T sig = (k == 0 ? (T)P.cp : (T)P.cs) * dtbydx;
+                {
+                    const T mm = cm_m[s][k], pp = cm_p[s][k];
+                    cm_m[s][k] = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+                }
+                {
+                    const T mm = cp_m[s][k], pp = cp_p[s][k];
+                    cp_p[s][k] = pp + sig * ((mm - pp) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+                }
+            }
+    } else {  // PLM
+        T dv[2][NS], ds[2][NS], ovl[2][NS], ovr[2][NS], osl[2][NS], osr[2][NS];
+        const bool os = charl || P.fourth;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const T* ww = w[s][k][q];
+                    const int s0 = cell_side(ci, dlo, dhi);
+                    T d;
+                    if (P.fourth) {
+                        const T vl0 = vl_slope(ww, 2, s0, limit);
+                        if (s0 != 0) {
+                            d = vl0;
+                        } else {
+                            const T vlm = vl_slope(ww, 1, cell_side(ci - 1, dlo, dhi), limit);
+                            const T vlp = vl_slope(ww, 3, cell_side(ci + 1, dlo, dhi), limit);
+                            const T hi = ww[3] - (T)0.25 * vlp;
+                            const T lo = ww[1] + (T)0.25 * vlm;
+                            d = (T)(2.0 / 3.0) * (hi - lo);
+                        }
+                    } else {
+                        d = vl_slope(ww, 2, s0, limit);
+                    }
+                    T C, L, R;
+                    slope_diffs(ww, 2, s0, C, L, R);
+                    if (q == 0) { dv[s][k] = d; ovl[s][k] = os ? L : (T)0; ovr[s][k] = os ? R : (T)0; }
+                    else { ds[s][k] = d; osl[s][k] = os ? L : (T)0; osr[s][k] = os ? R : (T)0; }
+                }
+        T cm_d[2][NS], cp_d[2][NS];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const T Z = m.Z(k);
+                if (limit && !charl && P.fourth) {
+                    dv[s][k] = vl_limit(dv[s][k], ovl[s][k], ovr[s][k]);
+                    ds[s][k] = vl_limit(ds[s][k], osl[s][k], osr[s][k]);
+                }
+                const T vd = s ? rv * dv[s][k] : dv[s][k];
+                cm_d[s][k] = ds[s][k] + vd * Z;
+                cp_d[s][k] = ds[s][k] - vd * Z;
+                if (charl) {
+                    const T vl = s ? rv * ovl[s][k] : ovl[s][k];
+                    const T vr = s ? rv * ovr[s][k] : ovr[s][k];
+                    const T cml = osl[s][k] + vl * Z, cpl = osl[s][k] - vl * Z;
+                    const T cmr = osr[s][k] + vr * Z, cpr = osr[s][k] - vr * Z;
+                    cm_d[s][k] = vl_limit(cm_d[s][k], cml, cmr);
+                    cp_d[s][k] = vl_limit(cp_d[s][k], cpl, cpr);
+                }
+                // PLM characteristic predictor, lmin = -cp, lmax = +cp
+                const T lk = k == 0 ? (T)P.cp : (T)P.cs;
+                const T lmin = -(T)P.cp, lmax = (T)P.cp;
+                // wave k-minus (lambda = -lk < 0)
+                cm_m[s][k] = -((T)0.5 * ((T)1 + dtbydx * (-lk)) * cm_d[s][k]);
+                cm_p[s][k] = (T)0.5 * ((T)1 - dtbydx * lmax) * cm_d[s][k];
+                // wave k-plus (lambda = +lk > 0)
+                cp_m[s][k] = -((T)0.5 * ((T)1 + dtbydx * lmin) * cp_d[s][k]);
+                cp_p[s][k] = (T)0.5 * ((T)1 - dtbydx * lk) * cp_d[s][k];
+            }
+    }
+    // CHARSYNTHESISF (LinElastPhysicsF.ChF:526-550), then W+- = increment + W
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T Z = m.Z(k);
+            T vmi = (cm_m[s][k] - cp_m[s][k]) / (T)2 / Z;
+            T vpi = (cm_p[s][k] - cp_p[s][k]) / (T)2 / Z;
+            if (s) { vmi = rv * (cm_m[s][k] - cp_m[s][k]) / (T)2 / Z; vpi = rv * (cm_p[s][k] - cp_p[s][k]) / (T)2 / Z; }
+            const T smi = (cm_m[s][k] + cp_m[s][k]) / (T)2;
+            const T spi = (cm_p[s][k] + cp_p[s][k]) / (T)2;
+            Wm.v[s][k] = vmi + w[s][k][0][2];
+            Wp.v[s][k] = vpi + w[s][k][0][2];
+            Wm.s[s][k] = smi + w[s][k][1][2];
+            Wp.s[s][k] = spi + w[s][k][1][2];
+        }
+}
+
+// outgoing-to-the-right characteristic of a left state (RIEMANNF wxxp, LinElastPhysicsF.ChF:677-679,733-735)
+template <typename T, int NS>
+__device__ __forceinline__ void right_chars(const Mat<T>& m, T rv, const Slots<T, NS>& L, T (&c)[2][NS])
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T rZ = s ? rv * m.Z(k) : m.Z(k);
+            c[s][k] = L.s[s][k] - rZ * L.v[s][k];
+        }
+}
+// left-moving characteristic of a right state (RIEMANNF wxxm, :681-683,738-740)
+template <typename T, int NS>
+__device__ __forceinline__ void left_chars(const Mat<T>& m, T rv, const Slots<T, NS>& R, T (&c)[2][NS])
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T rZ = s ? rv * m.Z(k) : m.Z(k);
+            c[s][k] = R.s[s][k] + rZ * R.v[s][k];
+        }
+}
+
+// RIEMANNF face state from the two characteristics (:693-702, :750-759)
+template <typename T, int NS>
+__device__ __forceinline__ void riemann_state(const Mat<T>& m, T rv, const T (&wp)[2][NS], const T (&wm)[2][NS],
+                                              Slots<T, NS>& F)
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T Z = m.Z(k);
+            F.v[s][k] = s ? rv * (wm[s][k] - wp[s][k]) / (T)2 / Z : (wm[s][k] - wp[s][k]) / (T)2 / Z;
+            F.s[s][k] = (wm[s][k] + wp[s][k]) / (T)2;
+        }
+}
+
+// LINELASTOUTBCF / LINELASTFREEBCF (LinElastPhysicsF.ChF:906-1144) on the face state.
+// lohisign -1: inside chars are the left-moving wm; +1: right-moving wp.
+template <typename T, int NS>
+__device__ __forceinline__ void bc_state(const DevParams& P, const Mat<T>& m, int d, T rv, int type, int lohisign,
+                                         const T (&cin)[2][NS], Slots<T, NS>& F)
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T Z = m.Z(k);
+            T wm, wp;
+            if (type == 0) {
+                const int cv = slot_comp(d, k, 0) + 11 * s, cs_ = slot_comp(d, k, 1) + 11 * s;
+                const T bv = (T)P.bg[cv], bs = (T)P.bg[cs_];
+                const T rZ = s ? rv * Z : Z;
+                if (lohisign == -1) { wm = cin[s][k]; wp = bs - rZ * bv; }
+                else { wp = cin[s][k]; wm = bs + rZ * bv; }
+            } else {
+                if (lohisign == -1) { wm = cin[s][k]; wp = -wm; }
+                else { wp = cin[s][k]; wm = -wp; }
+            }
+            F.v[s][k] = s ? rv * (wm - wp) / ((T)2 * Z) : (wm - wp) / ((T)2 * Z);
+            F.s[s][k] = (wm + wp) / (T)2;
+        }
+}
+
+// GETADWDXF applied as W(d1) += A(d2) (WHalf(d2)[+1] - WHalf(d2)[0]) (LinElastPhysicsF.ChF:791-901,
+// LinElastPatchGodunov.cpp:408-416).  sc = -(1/2) dt/dx, times rv(d2) on side 2.
+template <typename T, int NS>
+__device__ __forceinline__ void transverse_add(const Mat<T>& m, T sc1, T sc2, const Slots<T, NS>& Hlo,
+                                               const Slots<T, NS>& Hhi, Slots<T, NS>& W)
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const T sc = s ? sc2 : sc1;
+        const T dNv = Hhi.v[s][0] - Hlo.v[s][0], dNs = Hhi.s[s][0] - Hlo.s[s][0];
+        const T dSv = Hhi.v[s][1] - Hlo.v[s][1], dSs = Hhi.s[s][1] - Hlo.s[s][1];
+        W.v[s][0] = W.v[s][0] + (-(sc * dSs / m.rho));
+        W.s[s][0] = W.s[s][0] + (-(sc * dNv * m.lam));
+        W.v[s][1] = W.v[s][1] + (-(sc * dNs / m.rho));
+        W.s[s][1] = W.s[s][1] + (-(sc * dSv * m.G));
+        if (NS == 3) {
+            const T dAs = Hhi.s[s][NS - 1] - Hlo.s[s][NS - 1];
+            W.v[s][NS - 1] = W.v[s][NS - 1] + (-(sc * dAs / m.rho));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fault physics (FP64 always)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double face_val(const double* arr, const DevParams& P, int i, double dflt)
+{
+    if (!arr) return dflt;
+    int k = i - P.face_lo;
+    k = k < 0 ? 0 : (k >= P.nface ? P.nface - 1 : k);
+    return arr[k];
+}
+
+__device__ __forceinline__ double gaussianfunc(double w, double r) { return exp(-(0.5 * ((r / w) * (r / w)))); }
+__device__ __forceinline__ double smoothfunc(double w, double r)
+{
+    if (r >= w) return 0.0;
+    if (r <= 0.0) return 1.0;
+    return exp(r * r / (r * r - w * w));
+}
+
+// RSNEWTON (RSIBCF.ChF:336-401): bracketed Newton on the slip rate.  Each lane
+// keeps its own iterate; the wave runs until its last lane has converged.
+__device__ __forceinline__ double rs_newton(const DevParams& P, double phi, double psi, double sn, double c_rs_a,
+                                            double& Vio)
+{
+    const double atolG = 2.220446049250313080847e-16 * 1e4;
+    double V = Vio;
+    double Vmin = 0.0, Vmax = phi / P.Zs;
+    if (V < Vmin || V > Vmax) V = (Vmin + Vmax) / 2.0;
+    const double alpha = sn * c_rs_a;
+    const double beta = exp(psi / c_rs_a) / (2.0 * P.rs_V0);
+    const double eta = 0.5 * P.Zs;
+    double fval = alpha * asinh(beta * V) + eta * V - phi;
+    for (int it = 1; it <= 100; ++it) {
+        if (fval < 0.0) Vmin = V; else Vmax = V;
+        const double dfval = alpha * beta / sqrt(beta * beta * (V * V) + 1.0) + eta;
+        const double dV = -(fval / dfval);
+        V = V + dV;
+        if (V < Vmin || V > Vmax) V = (Vmin + Vmax) / 2.0;
+        fval = alpha * asinh(beta * V) + eta * V - phi;
+        if (fabs(fval) <= atolG) break;
+        if (fabs(dV) <= 0.0) break;
+    }
+    Vio = V;
+    return alpha * asinh(beta * V);
+}
+
+// inside state of a fault face (W- of the cell above, physical components)
+struct FaultIn {
+    double vx[2], vy[2], vz[2], syy[2], sxy[2], syz[2];
+};
+struct FaultOut {
+    double vx[2], vy[2], vz[2], syy[2], sxy[2], syz[2];
+};
+
+__device__ __forceinline__ double ramp_of(const DevParams& P, double x)
+{
+    return fmin(1.0, fmax(0.0, (x - P.ramp_x) / P.ramp_w));
+}
+
+// RSFAULTBCF (RSIBCF.ChF:158-323)
+__device__ __forceinline__ void rs_fault(const DevParams& P, int i, double psi, const FaultIn& in, FaultOut& o)
+{
+    const double Zp = P.Zp, Zs = P.Zs;
+    const double x = (i + 0.5) * P.dx - P.x0;
+    const double wpxy1 = -in.sxy[0] - Zs * in.vx[0];
+    const double wpyy1 = -in.syy[0] - Zp * in.vy[0];
+    const double wpxy2 = in.sxy[1] - Zs * in.vx[1];
+    const double wpyy2 = in.syy[1] - Zp * in.vy[1];
+    const double wpyz1 = 0.0, wpyz2 = 0.0;
+    double tau_xy_0 = P.nsig * gaussianfunc(P.r0, x);
+    if (P.f_tau) tau_xy_0 = tau_xy_0 + face_val(P.f_tau, P, i, 0.0);
+    const double tau_yz_0 = 0.0;
+    const double phi_xy = 0.5 * (wpxy2 - wpxy1) + tau_xy_0;
+    const double phi_yz = 0.5 * (wpyz2 - wpyz1) + tau_yz_0;
+    const double phi = sqrt(phi_xy * phi_xy + phi_yz * phi_yz);
+    const double syy = -((wpyy1 - wpyy2) / 2.0);
+    const double vy = -((wpyy1 + wpyy2) / 2.0 / Zp);
+    const double dvx = in.vx[0] - in.vx[1], dvz = in.vz[0] - in.vz[1];
+    double V = sqrt(dvx * dvx + dvz * dvz);
+    const double c_rs_a = face_val(P.f_a, P, i, P.rs_a) + P.ramp_a * ramp_of(P, x);
+    double sn = -syy;
+    if (P.f_syy) sn = -(syy + face_val(P.f_syy, P, i, 0.0));
+    const double tau = rs_newton(P, phi, psi, sn, c_rs_a, V);
+    const double sxy = (phi_xy / phi) * tau - tau_xy_0;
+    const double syz = (phi_yz / phi) * tau - tau_yz_0;
+    const double Vx = (phi_xy / phi) * V;
+    const double Vz = (phi_yz / phi) * V;
+    const double vx1 = -((sxy + wpxy1) / Zs);
+    const double vz1 = -((syz + wpyz1) / Zs);
+    o.vx[0] = vx1; o.vy[0] = vy; o.vz[0] = vz1; o.syy[0] = syy; o.sxy[0] = sxy; o.syz[0] = syz;
+    o.vx[1] = vx1 - Vx; o.vy[1] = vy; o.vz[1] = vz1 - Vz; o.syy[1] = syy; o.sxy[1] = sxy; o.syz[1] = syz;
+}
+
+// RSSETBND psi update (RSIBCF.ChF:125-146) and the aging-law extension
+__device__ __forceinline__ double rs_psi_update(const DevParams& P, int i, double psi, double V, double dt)
+{
+    const double x = (i + 0.5) * P.dx - P.x0;
+    const double c_ramp = ramp_of(P, x);
+    const double c_rs_a = face_val(P.f_a, P, i, P.rs_a) + P.ramp_a * c_ramp;
+    const double L = face_val(P.f_L, P, i, P.rs_L);
+    const double b = face_val(P.f_b, P, i, P.rs_b);
+    if (P.state_law == 0) {
+        const double f = c_rs_a * asinh(V * exp(psi / c_rs_a) / (2.0 * P.rs_V0));
+        double fss = P.rs_f0 - (b - c_rs_a) * log(V / P.rs_V0);
+        if (P.rs_fexp > 0.0) {
+            const double c_rs_Vw = P.rs_Vw + P.ramp_Vw * c_ramp;
+            const double r = V / c_rs_Vw;
+            const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+            fss = P.rs_fw + (fss - P.rs_fw) / pow(1.0 + r8, 0.125);
+        }
+        return psi - dt * (V / L) * (f - fss);
+    }
+    return psi + dt * (b * P.rs_V0 / L) * (exp((P.rs_f0 - psi) / b) - V / P.rs_V0);
+}
+
+// SWFAULTBCF (SWIBCF.ChF:79-267), side 2 at IX2_* = 11+n; slip = fault slip-total
+__device__ __forceinline__ void sw_fault(const DevParams& P, int i, double slip, const FaultIn& in, FaultOut& o)
+{
+    const double Zp = P.Zp, Zs = P.Zs;
+    const double x = (i + 0.5) * P.dx - P.fric_c0;
+    const double wpxy1 = -in.sxy[0] - Zs * in.vx[0];
+    const double wpyy1 = -in.syy[0] - Zp * in.vy[0];
+    const double wpyz1 = -in.syz[0] - Zs * in.vz[0];
+    const double wpxy2 = in.sxy[1] - Zs * in.vx[1];
+    const double wpyy2 = in.syy[1] - Zp * in.vy[1];
+    const double wpyz2 = in.syz[1] - Zs * in.vz[1];
+    double tau_0 = 0.0;
+    for (int ip = 0; ip < P.npatch; ++ip) {
+        const double valx = smoothfunc(P.sm, fabs(x - P.xc[ip]) - P.xw[ip]);
+        tau_0 = tau_0 + P.taup[ip] * valx;
+    }
+    if (P.f_tau) tau_0 = tau_0 + face_val(P.f_tau, P, i, 0.0);
+    const double ta = 0.5 * (wpxy2 - wpxy1) + tau_0;
+    const double tb = 0.5 * (wpyz2 - wpyz1);
+    const double tau_lock = sqrt(ta * ta + tb * tb);
+    const double syy = -((wpyy1 - wpyy2) / 2.0);
+    const double vy = -((wpyy1 + wpyy2) / 2.0 / Zp);
+    const double valx = (fabs(x) >= P.fric_w0) ? 0.0 : 1.0;
+    const double fs = face_val(P.f_fs, P, i, P.fs);
+    const double fd = face_val(P.f_fd, P, i, P.fd);
+    const double dw = face_val(P.f_dc, P, i, P.dw);
+    double fric = fs * valx + (1.0 - valx) * P.outside_f;
+    fric = fric - (fric - fd) * fmin(1.0, slip / dw);
+    double sn_syy = syy;
+    if (P.f_syy) sn_syy = syy + face_val(P.f_syy, P, i, 0.0);
+    const double tau = fmin(tau_lock, -sn_syy * fric) - tau_0;
+    const double phi_xy = 0.5 * (wpxy2 - wpxy1);
+    const double phi_yz = 0.5 * (wpyz2 - wpyz1);
+    const double phi = sqrt(phi_xy * phi_xy + phi_yz * phi_yz);
+    const double sxy = (phi_xy / phi) * tau;
+    const double syz = (phi_yz / phi) * tau;
+    o.vx[0] = -((sxy + wpxy1) / Zs);
+    o.vz[0] = -((syz + wpyz1) / Zs);
+    o.vy[0] = vy; o.syy[0] = syy; o.sxy[0] = sxy; o.syz[0] = syz;
+    o.vx[1] = (sxy - wpxy2) / Zs;
+    o.vz[1] = (syz - wpyz2) / Zs;
+    o.vy[1] = vy; o.syy[1] = syy; o.sxy[1] = sxy; o.syz[1] = syz;
+}
+
+// PLASTICFLOW + INVARIANTS + YIELD (PlasticPhysicsF.ChF:74-181)
+template <typename T>
+__device__ __forceinline__ void plastic_flow(const DevParams& P, T dt, T& sxx, T& syy, T& szz, T& sxy, T& sxz,
+                                             T& syz, T& plam, T& pgam)
+{
+    T sigma = (sxx + syy + szz) / (T)3;
+    const T a = sxx - syy, b = syy - szz, c = szz - sxx;
+    const T J2 = (a * a + b * b + c * c) / (T)6 + sxy * sxy + sxz * sxz + syz * syz;
+    T tau = sqrt(J2);
+    const T bb = (T)P.p_b + pgam * (T)P.p_h;
+    const T Y = tau - fmax(-(T)P.p_mu * sigma + bb, (T)0);
+    if (Y <= (T)0) { plam = (T)0; return; }
+    const T G = (T)P.G, K = (T)P.K;
+    plam = Y / (dt * ((T)P.p_h + G + (T)P.p_mu * (T)P.p_beta * K) + (T)P.p_eta);
+    pgam = pgam + dt * plam;
+    const T sig_e = sigma;
+    tau = tau - dt * plam * G;
+    sigma = sigma - dt * plam * K * (T)P.p_beta;
+    const T factor = tau / (tau + dt * plam * G);
+    sxx = sigma + factor * (sxx - sig_e);
+    syy = sigma + factor * (syy - sig_e);
+    szz = sigma + factor * (szz - sig_e);
+    sxy = factor * sxy;
+    sxz = factor * sxz;
+    syz = factor * syz;
+}
+
+}  // namespace tmk

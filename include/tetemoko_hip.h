@@ -129,6 +129,13 @@ int tmk_level_download(tmk_level *lev, int ibox, double *U, double *B);
 int tmk_level_step(tmk_level *lev, double time, double dt, double *dt_new);
 /* nsteps level steps with time advancing by dt, captured once as a hipGraph */
 int tmk_level_run(tmk_level *lev, int nsteps, double time, double dt);
+/* tmk_level_run bracketed by HIP events recorded on the level's own stream;
+ * waits for completion and returns the elapsed milliseconds in *ms */
+int tmk_level_run_timed(tmk_level *lev, int nsteps, double time, double dt, double *ms);
+/* nsteps eager level steps (host-supplied times, no graph) bracketed by HIP
+ * events on the level's stream; *ms_per_step = mean elapsed time per step.  For a
+ * single-box level each step is exactly one launch of the fused update kernel. */
+int tmk_level_profile(tmk_level *lev, int nsteps, double time, double dt, double *ms_per_step);
 /* synchronize and return the first error any launch reported */
 int tmk_level_sync(tmk_level *lev);
 

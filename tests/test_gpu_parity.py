@@ -211,6 +211,7 @@ def test_level_plasticity():
     cfg["p_beta"] = 0.3
     cfg["p_eta"] = 0.03 * scenarios.G_MPA
     cfg["p_b"] = 1.0
+    cfg["rs"]["state_law"] = 1
     cfg["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
     (Ug, Ur), (Bg, Br) = _level_case(cfg, scenarios.split_boxes(64, 40, 20), 5, seed=8)
     assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)

@@ -35,7 +35,7 @@ EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok",
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
-    "tmk_level_step", "tmk_level_run", "tmk_level_sync", "tmk_set_tile",
+    "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
 )
 
 
@@ -117,6 +117,8 @@ def lib():
         L.tmk_level_download.argtypes = [vp, c_int, _PD, _PD]
         L.tmk_level_step.argtypes = [vp, _D, _D, _PD]
         L.tmk_level_run.argtypes = [vp, c_int, _D, _D]
+        L.tmk_level_run_timed.argtypes = [vp, c_int, _D, _D, _PD]
+        L.tmk_level_profile.argtypes = [vp, c_int, _D, _D, _PD]
         L.tmk_level_sync.argtypes = [vp]
         L.tmk_set_tile.argtypes = [vp, c_int, c_int]
         _lib = L
@@ -276,6 +278,18 @@ class Level:
 
     def run(self, nsteps, time, dt):
         _check(lib().tmk_level_run(self.h, nsteps, time, dt), "tmk_level_run")
+
+    def run_timed(self, nsteps, time, dt):
+        """Run nsteps; returns elapsed ms from HIP events on the level's stream."""
+        ms = ctypes.c_double()
+        _check(lib().tmk_level_run_timed(self.h, nsteps, time, dt, ctypes.byref(ms)), "tmk_level_run_timed")
+        return ms.value
+
+    def profile(self, nsteps, time, dt):
+        """Mean ms per eager step (one fused launch per step for a single-box level)."""
+        ms = ctypes.c_double()
+        _check(lib().tmk_level_profile(self.h, nsteps, time, dt, ctypes.byref(ms)), "tmk_level_profile")
+        return ms.value
 
     def sync(self):
         _check(lib().tmk_level_sync(self.h), "tmk_level_sync")

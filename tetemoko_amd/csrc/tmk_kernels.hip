@@ -1411,6 +1411,52 @@ extern "C" int tmk_level_run(tmk_level* L, int nsteps, double time, double dt)
     return TMK_OK;
 }
 
+extern "C" int tmk_level_run_timed(tmk_level* L, int nsteps, double time, double dt, double* ms)
+{
+    if (!L || !ms) return TMK_E_ARG;
+    hipStream_t st = L->ctx->stream;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, st));
+    int e = tmk_level_run(L, nsteps, time, dt);
+    if (e) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return e; }
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double dt, double* ms)
+{
+    if (!L || !ms || nsteps <= 0) return TMK_E_ARG;
+    int e = level_prepare(L);
+    if (e) return e;
+    hipStream_t st = L->ctx->stream;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, st));
+    double t = time;
+    for (int k = 0; k < nsteps && !e; ++k) {
+        e = level_step_any(L, t, dt, false);
+        t = t + dt;
+    }
+    if (e) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); return e; }
+    HIPCHK(hipEventRecord(b, st));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    *ms = (double)f / nsteps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return TMK_OK;
+}
+
 extern "C" int tmk_level_sync(tmk_level* L)
 {
     if (!L) return TMK_E_ARG;

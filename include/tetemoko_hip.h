@@ -17,6 +17,8 @@
 #ifndef TETEMOKO_HIP_H
 #define TETEMOKO_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -88,6 +90,7 @@ typedef struct {
 
 typedef struct tmk_ctx tmk_ctx;
 typedef struct tmk_level tmk_level;
+typedef struct tmk_batch tmk_batch;
 
 /* Context = one scenario's parameters + stream + device scratch.
  * Replaces LEPhysIBC::setFortranCommonLE/Plastic (LEPhysIBC.cpp:13-36) and the
@@ -97,6 +100,8 @@ int tmk_ctx_destroy(tmk_ctx *ctx);
 const char *tmk_strerror(int code);
 /* 1 when the library was built for gfx950 and a device is visible */
 int tmk_device_ok(void);
+/* free and total device memory (admission control of the serving loop) */
+int tmk_mem_info(size_t *free_bytes, size_t *total_bytes);
 
 /* LinElastPatchGodunov::updateState (LinElastPatchGodunov.cpp:101-129), with
  * LinElastPhysics::computeUpdate (LinElastPhysics.cpp:135-179) and the IBC
@@ -117,6 +122,8 @@ int tmk_patch_plastic_update(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_b
  * boxes: disjoint valid boxes of one level.  Fault data is allocated for every
  * box whose lo-y edge lies on the domain lo-y boundary. */
 int tmk_level_create(tmk_ctx *ctx, int nboxes, const tmk_box *boxes, tmk_level **out);
+/* device bytes tmk_level_create would allocate for these boxes */
+int tmk_level_footprint(const tmk_ctx *ctx, int nboxes, const tmk_box *boxes, size_t *bytes);
 int tmk_level_destroy(tmk_level *lev);
 /* copy one box's ghosted state (grow(box,4), 22 comps) and fault data
  * (x in [lo-4, hi+4], j = 0, nbdry comps; NULL if the box has none) to/from the device */
@@ -138,6 +145,23 @@ int tmk_level_run_timed(tmk_level *lev, int nsteps, double time, double dt, doub
 int tmk_level_profile(tmk_level *lev, int nsteps, double time, double dt, double *ms_per_step);
 /* synchronize and return the first error any launch reported */
 int tmk_level_sync(tmk_level *lev);
+
+/* Batches: the boxes of every member level (one scenario each, own parameters,
+ * clock and step) are updated in one shared fused launch per step.  Members
+ * join and leave between steps (continuous batching); a member's results are
+ * bit-identical to stepping it alone.  Members must share precision, tile
+ * shape, fault law and anti-plane mode.  No reference counterpart: the
+ * reference runs one scenario per process (amrGodunov.cpp:81-166). */
+int tmk_batch_create(tmk_batch **out);
+int tmk_batch_destroy(tmk_batch *b);
+/* join with the member's clock (time of its next step) and step dt */
+int tmk_batch_add(tmk_batch *b, tmk_level *lev, double time, double dt);
+/* leave; the level can then be downloaded or stepped alone */
+int tmk_batch_remove(tmk_batch *b, tmk_level *lev);
+/* advance every member by nsteps of its own dt */
+int tmk_batch_run(tmk_batch *b, int nsteps);
+/* synchronize; first error reported by any member (0 = none) */
+int tmk_batch_sync(tmk_batch *b);
 
 /* Tile shape of the fused update (autotuner hook); 0 = library default. */
 int tmk_set_tile(tmk_ctx *ctx, int bx, int by);

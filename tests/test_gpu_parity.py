@@ -264,3 +264,76 @@ def test_argument_errors():
     with pytest.raises(capi.TmkError) as ei:
         capi.Context(bad)
     assert ei.value.code == capi.E_BAD_BC
+
+
+def _fresh_level(cfg, seed, precision=capi.PREC_FP32_BULK, boxes=None):
+    U, B = scenarios.level_arrays(cfg)
+    scenarios.perturb(U, seed=seed, amp_v=0.2, amp_s=1.0)
+    ctx = capi.Context(cfg, precision)
+    boxes = boxes or [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)]
+    lev = capi.Level(ctx, boxes)
+    for k, b in enumerate(boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+    return ctx, lev, U, B, boxes
+
+
+def _download(lev, U, B, boxes):
+    out_u, out_b = np.full_like(U, np.nan), np.full_like(B, np.nan)
+    for k, b in enumerate(boxes):
+        ub = scenarios.box_view(U, b)
+        bb = scenarios.box_bview(B, b) if b[1] == 0 else None
+        lev.download(k, ub, bb)
+        x0, y0, x1, y1 = b
+        out_u[:, y0 + G:y1 + G + 1, x0 + G:x1 + G + 1] = ub[:, G:-G, G:-G]
+        if bb is not None:
+            out_b[:, x0 + G:x1 + G + 1] = bb[:, G:-G]
+    return out_u[:, G:-G, G:-G], out_b[:, G:-G]
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+def test_batch_member_equals_alone(precision):
+    """Continuous batching: a scenario's result is bit-identical alone and in the mix,
+    with members joining and leaving at different steps."""
+    cfgs = [scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0),
+            scenarios.config_c1(seed=2, nx=64, ny=64, dx=40.0),
+            scenarios.config_c0(seed=3, nx=80, ny=40)]
+    cfgs[2]["fault"] = "sw"
+    # alone
+    alone = []
+    for k, cfg in enumerate(cfgs[:2]):
+        ctx, lev, U, B, boxes = _fresh_level(cfg, 10 + k, precision)
+        lev.run(9, 0.0, scenarios.dt_of(cfg))
+        lev.sync()
+        alone.append(_download(lev, U, B, boxes))
+        lev.close()
+        ctx.close()
+    # mixed: scenario 1 joins after 3 steps of scenario 0; scenario 0 leaves after its 9th step
+    ctx0, lev0, U0, B0, bx0 = _fresh_level(cfgs[0], 10, precision)
+    ctx1, lev1, U1, B1, bx1 = _fresh_level(cfgs[1], 11, precision, boxes=scenarios.split_boxes(64, 64, 24))
+    batch = capi.Batch()
+    batch.add(lev0, 0.0, scenarios.dt_of(cfgs[0]))
+    batch.run(3)
+    batch.add(lev1, 0.0, scenarios.dt_of(cfgs[1]))
+    batch.run(6)
+    batch.sync()
+    batch.remove(lev0)
+    batch.run(3)
+    batch.remove(lev1)
+    lev1.run(0, 0.0, scenarios.dt_of(cfgs[1]))
+    got0 = _download(lev0, U0, B0, bx0)
+    got1 = _download(lev1, U1, B1, bx1)
+    assert np.array_equal(got0[0], alone[0][0]) and np.array_equal(got0[1], alone[0][1])
+    assert np.array_equal(got1[0], alone[1][0]) and np.array_equal(got1[1], alone[1][1])
+    batch.close()
+    for o in (lev0, lev1, ctx0, ctx1):
+        o.close()
+
+
+def test_mem_info_and_footprint():
+    free, total = capi.mem_info()
+    assert 0 < free <= total
+    cfg = scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0)
+    ctx = capi.Context(cfg)
+    n = capi.level_footprint(ctx, [(0, 0, 95, 47)])
+    assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + 2 * (96 + 8) * 11 * 8
+    ctx.close()

@@ -12,7 +12,7 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared", "-ffp
 
 
 def build(verbose=True):
-    src = os.path.join(HERE, "csrc", "tmk_kernels.hip")
+    src = os.path.join(HERE, "csrc", "tmk_api.hip")
     out_dir = os.path.join(HERE, "_lib")
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "libtetemoko_hip.so")

@@ -32,7 +32,9 @@ PREC_FP64 = 0
 PREC_FP32_BULK = 1
 
 EXPORTED = (
-    "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok",
+    "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
+    "tmk_level_footprint", "tmk_batch_create", "tmk_batch_destroy", "tmk_batch_add", "tmk_batch_remove",
+    "tmk_batch_run", "tmk_batch_sync",
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
@@ -121,6 +123,15 @@ def lib():
         L.tmk_level_profile.argtypes = [vp, c_int, _D, _D, _PD]
         L.tmk_level_sync.argtypes = [vp]
         L.tmk_set_tile.argtypes = [vp, c_int, c_int]
+        sz = ctypes.c_size_t
+        L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
+        L.tmk_batch_create.argtypes = [ctypes.POINTER(vp)]
+        L.tmk_batch_destroy.argtypes = [vp]
+        L.tmk_batch_add.argtypes = [vp, vp, _D, _D]
+        L.tmk_batch_remove.argtypes = [vp, vp]
+        L.tmk_batch_run.argtypes = [vp, c_int]
+        L.tmk_batch_sync.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -293,3 +304,49 @@ class Level:
 
     def sync(self):
         _check(lib().tmk_level_sync(self.h), "tmk_level_sync")
+
+
+def mem_info():
+    """(free, total) device bytes."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().tmk_mem_info(ctypes.byref(f), ctypes.byref(t)), "tmk_mem_info")
+    return f.value, t.value
+
+
+def level_footprint(ctx, boxes):
+    arr = (Box * len(boxes))(*[Box.make(*b) for b in boxes])
+    n = ctypes.c_size_t()
+    _check(lib().tmk_level_footprint(ctx.h, len(boxes), arr, ctypes.byref(n)), "tmk_level_footprint")
+    return n.value
+
+
+class Batch:
+    """Continuous batch: member levels (scenarios) advance together in shared launches."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().tmk_batch_create(ctypes.byref(h)), "tmk_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().tmk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, level, time, dt):
+        _check(lib().tmk_batch_add(self.h, level.h, time, dt), "tmk_batch_add")
+
+    def remove(self, level):
+        _check(lib().tmk_batch_remove(self.h, level.h), "tmk_batch_remove")
+
+    def run(self, nsteps):
+        _check(lib().tmk_batch_run(self.h, nsteps), "tmk_batch_run")
+
+    def sync(self):
+        _check(lib().tmk_batch_sync(self.h), "tmk_batch_sync")

@@ -1,0 +1,1127 @@
+// tmk_api.hip -- host side of libtetemoko_hip: contexts, device-resident levels,
+// multi-scenario batches and the C ABI of include/tetemoko_hip.h.
+//
+// A "group" is the unit of one fused launch: a device array of scenario blocks
+// (parameters + clock), the box descriptors of every member level for both
+// buffer parities, the tile work list and the ghost-exchange plan.  A level
+// stepping on its own is a one-member group; a batch concatenates the levels of
+// many scenarios into one group so that their boxes share each launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tetemoko_hip.h"
+#include "tmk_kernels.hpp"
+
+using namespace tmk;
+
+#define HIPCHK(x)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) {                                                             \
+            fprintf(stderr, "tetemoko_hip: %s failed: %s\n", #x, hipGetErrorString(e_));   \
+            return e_ == hipErrorOutOfMemory ? TMK_E_NOMEM : TMK_E_HIP;                     \
+        }                                                                                   \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= n) return TMK_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return TMK_E_NOMEM;
+        n = bytes;
+        return TMK_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+inline bool box_contains(const tmk_box& o, const tmk_box& b)
+{
+    return b.lo[0] >= o.lo[0] && b.lo[1] >= o.lo[1] && b.hi[0] <= o.hi[0] && b.hi[1] <= o.hi[1];
+}
+inline tmk_box box_and(tmk_box a, const tmk_box& b)
+{
+    for (int d = 0; d < 2; ++d) {
+        a.lo[d] = std::max(a.lo[d], b.lo[d]);
+        a.hi[d] = std::min(a.hi[d], b.hi[d]);
+    }
+    return a;
+}
+inline tmk_box box_grow(tmk_box a, int n)
+{
+    a.lo[0] -= n; a.lo[1] -= n; a.hi[0] += n; a.hi[1] += n;
+    return a;
+}
+inline bool box_empty(const tmk_box& a) { return a.hi[0] < a.lo[0] || a.hi[1] < a.lo[1]; }
+
+void make_tiles(const tmk_box& v, int patch, int tx, int ty, std::vector<Tile>& out)
+{
+    for (int y0 = v.lo[1]; y0 <= v.hi[1]; y0 += ty)
+        for (int x0 = v.lo[0]; x0 <= v.hi[0]; x0 += tx) {
+            Tile t;
+            t.patch = patch;
+            t.x0 = x0;
+            t.y0 = y0;
+            t.x1 = std::min(x0 + tx - 1, v.hi[0]);
+            t.y1 = std::min(y0 + ty - 1, v.hi[1]);
+            out.push_back(t);
+        }
+}
+
+// Workgroup g runs on XCD g % 8.  Give each XCD a contiguous band of the tile
+// list so that halos shared by neighbouring tiles stay in that XCD's L2.
+void xcd_order(std::vector<Tile>& t)
+{
+    const size_t nx = 8, n = t.size();
+    if (n < 64) return;
+    const size_t per = (n + nx - 1) / nx;
+    std::vector<Tile> o;
+    o.reserve(n);
+    for (size_t g = 0; g < per * nx; ++g) {
+        const size_t src = (g % nx) * per + g / nx;
+        if (src < n) o.push_back(t[src]);
+    }
+    t.swap(o);
+}
+
+// state planes that sweeps read with a stencil (double-buffered); the others
+// (szz, lam, gam and, in mode II, the anti-plane planes) are updated in place
+bool is_stencil(int c, int ns)
+{
+    const int k = c % 11;
+    if (k == 0 || k == 1 || k == 3 || k == 4 || k == 6) return true;
+    if (ns == 3 && (k == 2 || k == 7 || k == 8)) return true;
+    return false;
+}
+
+int stencil_mask(int ns)
+{
+    int m = 0;
+    for (int c = 0; c < TMK_NCOMP; ++c)
+        if (is_stencil(c, ns)) m |= 1 << c;
+    return m;
+}
+
+}  // namespace
+
+struct Group {
+    size_t tsize = 8;
+    int ns = 2, bx = 32, by = 16;
+    Scen* scen_dev = nullptr;
+    int nscen = 0;
+    void* patches_dev[2] = {nullptr, nullptr};
+    Tile* tiles_dev = nullptr;
+    int ntiles = 0;
+    CopyRect* rects_dev = nullptr;
+    int nrects = 0;
+    CopyRect* brects_dev = nullptr;
+    int nbrects = 0;
+    int nb = 11;
+    int plastic = 1;
+    int g = 0;  // parity of the next launch
+    hipGraphExec_t gexec = nullptr;
+    void release()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        for (auto& p : patches_dev) { if (p) (void)hipFree(p); p = nullptr; }
+        if (scen_dev) (void)hipFree(scen_dev);
+        if (tiles_dev) (void)hipFree(tiles_dev);
+        if (rects_dev) (void)hipFree(rects_dev);
+        if (brects_dev) (void)hipFree(brects_dev);
+        scen_dev = nullptr; tiles_dev = nullptr; rects_dev = nullptr; brects_dev = nullptr;
+        nscen = ntiles = nrects = nbrects = 0;
+    }
+};
+
+struct tmk_ctx {
+    tmk_params hp;
+    DevParams P;
+    int precision = TMK_PREC_FP64;
+    hipStream_t stream = nullptr;
+    std::vector<double*> face_dev;
+    DevBuf scratch[8];
+    int bx = 32, by = 16;
+    bool antiplane = true;
+};
+
+struct tmk_batch;
+
+struct tmk_level {
+    tmk_ctx* ctx = nullptr;
+    std::vector<tmk_box> boxes;
+    std::vector<int> has_b;
+    int ns = 2;
+    size_t tsize = 8;
+    std::vector<void*> ubuf[2];
+    std::vector<void*> upt;
+    std::vector<double*> bbuf[2];
+    int cur = 0;                  // buffer parity holding the current state
+    double time = 0.0, dt = 0.0;  // clock of this scenario
+    std::vector<char> hpatch[2];  // Patch<T> descriptors for parity 0 / 1 (scen index 0)
+    std::vector<Tile> tiles;
+    std::vector<CopyRect> rects, brects;
+    Group own;
+    bool own_built = false;
+    bool dirty = true;            // uploaded data not yet exchanged
+    tmk_batch* batch = nullptr;
+    size_t bytes = 0;             // device bytes held
+};
+
+struct tmk_batch {
+    hipStream_t stream = nullptr;
+    std::vector<tmk_level*> members;
+    std::vector<int> phase;       // member parity at group parity 0
+    Group grp;
+    bool built = false;
+};
+
+static const char* k_errs[] = {"ok",
+                               "NaN state variable psi (dPsi is nan)",
+                               "invalid boundary type",
+                               "invalid normal predictor or limiter combination",
+                               "fault face without fault data",
+                               "array box does not cover the update stencil",
+                               "invalid argument",
+                               "HIP runtime failure",
+                               "unsupported option",
+                               "device memory exhausted"};
+
+extern "C" const char* tmk_strerror(int code)
+{
+    const int k = -code;
+    if (k < 0 || k >= (int)(sizeof(k_errs) / sizeof(k_errs[0]))) return "unknown error";
+    return k_errs[k];
+}
+
+extern "C" int tmk_device_ok(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return strstr(prop.gcnArchName, "gfx950") != nullptr ? 1 : 0;
+}
+
+extern "C" int tmk_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return TMK_E_ARG;
+    HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    return TMK_OK;
+}
+
+static void fill_dev_params(const tmk_params& h, DevParams& P)
+{
+    memset(&P, 0, sizeof P);
+    // LINELASTSETF (LinElastPhysicsF.ChF:1165-1207)
+    P.cs = h.cs; P.cp = h.cp; P.G = h.G;
+    P.rho = h.G / (h.cs * h.cs);
+    P.lam = (h.cp * h.cp) * P.rho - 2.0 * h.G;
+    P.Zs = P.rho * h.cs;
+    P.Zp = P.rho * h.cp;
+    P.K = P.lam + 2.0 * h.G / 3.0;
+    P.den = 2.0 * h.G + P.lam;
+    P.lam2G = P.lam + 2.0 * h.G;
+    const double s9[9] = {0.0, 0.0, 0.0, h.sxx0, h.syy0, h.szz0, h.sxy0, h.sxz0, h.syz0};
+    for (int s = 0; s < 2; ++s) {
+        for (int c = 0; c < 9; ++c) P.bg[11 * s + c] = s9[c];
+        P.bg[11 * s + 9] = 0.0;
+        P.bg[11 * s + 10] = 0.0;
+    }
+    P.plast = h.use_plasticity;
+    P.p_mu = h.p_mu; P.p_beta = h.p_beta; P.p_eta = h.p_eta; P.p_b = h.p_b; P.p_h = h.p_h;
+    for (int k = 0; k < 4; ++k) P.bc[k] = h.bc[k];
+    P.fault_kind = h.fault_kind;
+    P.r0 = h.r0; P.nsig = h.nsig; P.rs_a = h.rs_a; P.rs_b = h.rs_b; P.rs_V0 = h.rs_V0; P.rs_f0 = h.rs_f0;
+    P.rs_L = h.rs_L; P.rs_fw = h.rs_fw; P.rs_Vw = h.rs_Vw; P.rs_fexp = h.rs_fexp;
+    P.x0 = h.x0; P.rs_thr = h.rs_rupture_thr; P.ramp_x = h.ramp_x; P.ramp_w = h.ramp_w; P.ramp_a = h.ramp_a;
+    P.ramp_Vw = h.ramp_Vw;
+    P.state_law = h.state_law;
+    P.fs = h.fs; P.fd = h.fd; P.dw = h.dw; P.sm = h.sm_value; P.sw_thr = h.sw_rupture_thr;
+    P.npatch = h.num_patches;
+    for (int k = 0; k < TMK_MAXPATCH; ++k) { P.xc[k] = h.xc[k]; P.xw[k] = h.xw[k]; P.taup[k] = h.tau_p[k]; }
+    P.fric_c0 = h.fric_center[0]; P.fric_w0 = h.fric_width[0]; P.outside_f = h.outside_f;
+    P.face_lo = h.face_lo; P.nface = h.nface;
+    P.pred = h.normal_pred; P.fourth = h.fourth_order; P.prim_lim = h.prim_limiting; P.char_lim = h.char_limiting;
+    P.dx = h.dx;
+    P.dlo[0] = h.domain.lo[0]; P.dlo[1] = h.domain.lo[1];
+    P.dhi[0] = h.domain.hi[0]; P.dhi[1] = h.domain.hi[1];
+}
+
+extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
+{
+    if (!out || !p) return TMK_E_ARG;
+    *out = nullptr;
+    if (precision != TMK_PREC_FP64 && precision != TMK_PREC_FP32_BULK) return TMK_E_ARG;
+    if (p->normal_pred < 0 || p->normal_pred > 2) return TMK_E_PRED_ORDER;
+    if (p->normal_pred != 0 && p->char_limiting && p->prim_limiting) return TMK_E_PRED_ORDER;
+    for (int k = 0; k < 4; ++k)
+        if (k != 2 && p->bc[k] != 0 && p->bc[k] != 1) return TMK_E_BAD_BC;
+    if (p->fault_kind != TMK_FAULT_RS && p->fault_kind != TMK_FAULT_SW) return TMK_E_ARG;
+    if (p->num_patches < 0 || p->num_patches > TMK_MAXPATCH) return TMK_E_ARG;
+    if (!(p->dx > 0.0) || p->domain.hi[0] - p->domain.lo[0] < 3 || p->domain.hi[1] - p->domain.lo[1] < 3)
+        return TMK_E_ARG;
+    tmk_ctx* c = new (std::nothrow) tmk_ctx;
+    if (!c) return TMK_E_NOMEM;
+    c->hp = *p;
+    c->precision = precision;
+    fill_dev_params(*p, c->P);
+    c->antiplane = !(p->sxz0 == 0.0 && p->syz0 == 0.0);
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return TMK_E_HIP;
+    }
+    const double* fa[8] = {p->f_a, p->f_b, p->f_L, p->f_fs, p->f_fd, p->f_dc, p->f_syy, p->f_tau};
+    const double** fd[8] = {&c->P.f_a, &c->P.f_b, &c->P.f_L, &c->P.f_fs, &c->P.f_fd, &c->P.f_dc, &c->P.f_syy,
+                            &c->P.f_tau};
+    for (int k = 0; k < 8; ++k) {
+        *fd[k] = nullptr;
+        if (!fa[k]) continue;
+        if (p->nface <= 0) { tmk_ctx_destroy(c); return TMK_E_ARG; }
+        double* d = nullptr;
+        if (hipMalloc(&d, sizeof(double) * p->nface) != hipSuccess) { tmk_ctx_destroy(c); return TMK_E_NOMEM; }
+        c->face_dev.push_back(d);
+        if (hipMemcpy(d, fa[k], sizeof(double) * p->nface, hipMemcpyHostToDevice) != hipSuccess) {
+            tmk_ctx_destroy(c);
+            return TMK_E_HIP;
+        }
+        *fd[k] = d;
+    }
+    c->hp.f_a = c->hp.f_b = c->hp.f_L = c->hp.f_fs = c->hp.f_fd = c->hp.f_dc = c->hp.f_syy = c->hp.f_tau = nullptr;
+    *out = c;
+    return TMK_OK;
+}
+
+extern "C" int tmk_ctx_destroy(tmk_ctx* c)
+{
+    if (!c) return TMK_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (double* d : c->face_dev) (void)hipFree(d);
+    for (auto& b : c->scratch) b.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TMK_OK;
+}
+
+static bool tile_ok(int bx, int by)
+{
+    return (bx == 32 && (by == 8 || by == 16 || by == 32)) || (bx == 64 && (by == 4 || by == 8 || by == 16));
+}
+
+extern "C" int tmk_set_tile(tmk_ctx* c, int bx, int by)
+{
+    if (!c) return TMK_E_ARG;
+    if (bx == 0) bx = 32;
+    if (by == 0) by = 16;
+    if (!tile_ok(bx, by)) return TMK_E_ARG;
+    c->bx = bx;
+    c->by = by;
+    return TMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+template <typename T, int NS, int BX, int BY>
+static int launch_tiles_t(hipStream_t st, const KArgs<T>& a)
+{
+    constexpr int NT = BX * BY;
+    constexpr int NC = 4 * NS;
+    const size_t s1 = (size_t)NC * BY * (BX + 4), s2 = (size_t)NC * (BY + 4) * BX, s3 = (size_t)8 * NS * NT;
+    const size_t shm = std::max(s1, std::max(s2, s3)) * sizeof(T);
+    if (a.ntiles == 0) return TMK_OK;
+    if (shm > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_godunov<T, NS, BX, BY>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+    hipLaunchKernelGGL((k_godunov<T, NS, BX, BY>), dim3(a.ntiles), dim3(NT), shm, st, a);
+    return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
+}
+
+template <typename T, int NS>
+static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a)
+{
+    if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16>(st, a);
+    if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8>(st, a);
+    if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32>(st, a);
+    if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8>(st, a);
+    if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4>(st, a);
+    if (bx == 64 && by == 16) return launch_tiles_t<T, NS, 64, 16>(st, a);
+    return TMK_E_ARG;
+}
+
+template <typename T>
+static int launch_tiles(hipStream_t st, int ns, int bx, int by, const KArgs<T>& a)
+{
+    return ns == 3 ? launch_tiles_ns<T, 3>(st, bx, by, a) : launch_tiles_ns<T, 2>(st, bx, by, a);
+}
+
+// one level step of every member of a group: fused update, ghost exchange of the
+// new state, clock advance
+template <typename T>
+static int group_enqueue_step_t(Group& G, hipStream_t st)
+{
+    KArgs<T> a;
+    a.scen = G.scen_dev;
+    a.patches = (const Patch<T>*)G.patches_dev[G.g];
+    a.tiles = G.tiles_dev;
+    a.ntiles = G.ntiles;
+    a.plastic = G.plastic;
+    int e = launch_tiles<T>(st, G.ns, G.bx, G.by, a);
+    if (e) return e;
+    const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g];  // its out planes hold the new state
+    if (G.nrects > 0) {
+        hipLaunchKernelGGL(k_exchange<T>, dim3(4, G.nrects), dim3(256), 0, st, pn, G.rects_dev, G.nrects,
+                           stencil_mask(G.ns));
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
+    if (G.nbrects > 0) {
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects, G.nb);
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
+    hipLaunchKernelGGL(k_advance_time, dim3((G.nscen + 63) / 64), dim3(64), 0, st, G.scen_dev, G.nscen);
+    if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    G.g ^= 1;
+    return TMK_OK;
+}
+
+static int group_enqueue_step(Group& G, hipStream_t st)
+{
+    return G.tsize == sizeof(double) ? group_enqueue_step_t<double>(G, st) : group_enqueue_step_t<float>(G, st);
+}
+
+// exchange the current state (before the first step after an upload)
+template <typename T>
+static int group_exchange_now_t(Group& G, hipStream_t st)
+{
+    // patches_dev[g ^ 1].out are the planes holding the current state
+    const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g ^ 1];
+    if (G.nrects > 0) {
+        hipLaunchKernelGGL(k_exchange<T>, dim3(4, G.nrects), dim3(256), 0, st, pn, G.rects_dev, G.nrects,
+                           stencil_mask(G.ns));
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
+    if (G.nbrects > 0) {
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects, G.nb);
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
+    return TMK_OK;
+}
+
+// n steps; a two-step graph (parities 0 -> 1 -> 0) is captured once per group
+// layout and dt and replayed
+static int group_run(Group& G, hipStream_t st, int nsteps)
+{
+    int e;
+    if (nsteps <= 0) return TMK_OK;
+    if (G.g != 0) {
+        if ((e = group_enqueue_step(G, st))) return e;
+        --nsteps;
+    }
+    if (nsteps >= 2 && !G.gexec) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        e = group_enqueue_step(G, st);
+        if (!e) e = group_enqueue_step(G, st);
+        hipError_t ce = hipStreamEndCapture(st, &gr);
+        if (e) return e;
+        if (ce != hipSuccess) return TMK_E_HIP;
+        hipError_t ie = hipGraphInstantiate(&G.gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) return TMK_E_HIP;
+    }
+    for (int k = 0; k + 1 < nsteps; k += 2) HIPCHK(hipGraphLaunch(G.gexec, st));
+    if (nsteps % 2)
+        if ((e = group_enqueue_step(G, st))) return e;
+    return TMK_OK;
+}
+
+// Build a group from member levels.  Member m's buffer parity at group parity g
+// is phase[m] ^ g.
+template <typename T>
+static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std::vector<int>& phase, int plastic)
+{
+    G.release();
+    if (mem.empty()) return TMK_OK;
+    tmk_level* L0 = mem[0];
+    G.tsize = sizeof(T);
+    G.ns = L0->ns;
+    G.bx = L0->ctx->bx;
+    G.by = L0->ctx->by;
+    G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    G.plastic = plastic;
+    G.g = 0;
+    std::vector<Scen> sc(mem.size());
+    std::vector<Patch<T>> pts[2];
+    std::vector<Tile> tiles;
+    std::vector<CopyRect> rects, brects;
+    int poff = 0;
+    for (size_t m = 0; m < mem.size(); ++m) {
+        tmk_level* L = mem[m];
+        memset(&sc[m], 0, sizeof(Scen));
+        sc[m].P = L->ctx->P;
+        sc[m].time = L->time;
+        sc[m].dt = L->dt;
+        const int nbox = (int)L->boxes.size();
+        for (int g = 0; g < 2; ++g) {
+            const Patch<T>* src = (const Patch<T>*)L->hpatch[phase[m] ^ g].data();
+            for (int b = 0; b < nbox; ++b) {
+                Patch<T> p = src[b];
+                p.scen = (int)m;
+                pts[g].push_back(p);
+            }
+        }
+        for (Tile t : L->tiles) { t.patch += poff; tiles.push_back(t); }
+        for (CopyRect r : L->rects) { r.src += poff; r.dst += poff; rects.push_back(r); }
+        for (CopyRect r : L->brects) { r.src += poff; r.dst += poff; brects.push_back(r); }
+        poff += nbox;
+    }
+    xcd_order(tiles);
+    G.nscen = (int)sc.size();
+    G.ntiles = (int)tiles.size();
+    G.nrects = (int)rects.size();
+    G.nbrects = (int)brects.size();
+    HIPCHK(hipMalloc(&G.scen_dev, sizeof(Scen) * sc.size()));
+    HIPCHK(hipMemcpy(G.scen_dev, sc.data(), sizeof(Scen) * sc.size(), hipMemcpyHostToDevice));
+    for (int g = 0; g < 2; ++g) {
+        HIPCHK(hipMalloc(&G.patches_dev[g], sizeof(Patch<T>) * pts[g].size()));
+        HIPCHK(hipMemcpy(G.patches_dev[g], pts[g].data(), sizeof(Patch<T>) * pts[g].size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&G.tiles_dev, sizeof(Tile) * std::max<size_t>(tiles.size(), 1)));
+    if (!tiles.empty()) HIPCHK(hipMemcpy(G.tiles_dev, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    if (!rects.empty()) {
+        HIPCHK(hipMalloc(&G.rects_dev, sizeof(CopyRect) * rects.size()));
+        HIPCHK(hipMemcpy(G.rects_dev, rects.data(), sizeof(CopyRect) * rects.size(), hipMemcpyHostToDevice));
+    }
+    if (!brects.empty()) {
+        HIPCHK(hipMalloc(&G.brects_dev, sizeof(CopyRect) * brects.size()));
+        HIPCHK(hipMemcpy(G.brects_dev, brects.data(), sizeof(CopyRect) * brects.size(), hipMemcpyHostToDevice));
+    }
+    return TMK_OK;
+}
+
+static int group_build(Group& G, const std::vector<tmk_level*>& mem, const std::vector<int>& phase, int plastic)
+{
+    if (mem.empty()) { G.release(); return TMK_OK; }
+    return mem[0]->tsize == sizeof(double) ? group_build_t<double>(G, mem, phase, plastic)
+                                           : group_build_t<float>(G, mem, phase, plastic);
+}
+
+static int group_set_clock(Group& G, hipStream_t st, int m, double time, double dt)
+{
+    double td[2] = {time, dt};
+    HIPCHK(hipMemcpyAsync((char*)(G.scen_dev + m) + offsetof(Scen, time), td, sizeof td, hipMemcpyHostToDevice, st));
+    return TMK_OK;
+}
+
+static int group_error(Group& G, hipStream_t st, int m, int* code)
+{
+    int e = 0;
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(&e, (char*)(G.scen_dev + m) + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost));
+    *code = e;
+    return TMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-patch drop-in (host FArrayBox data)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, double* F0, double* F1, double* B,
+                          tmk_box bb, int nb, double time, double dt, bool plastic_only)
+{
+    const size_t unx = (size_t)(ub.hi[0] - ub.lo[0] + 1), uny = (size_t)(ub.hi[1] - ub.lo[1] + 1);
+    const size_t plane = unx * uny;
+    const size_t nU = plane * TMK_NCOMP;
+    const size_t bnx = B ? (size_t)(bb.hi[0] - bb.lo[0] + 1) : 0;
+    const size_t nB = bnx * (size_t)nb;
+    const size_t nF0 = F0 ? (size_t)(valid.hi[0] - valid.lo[0] + 2) * (valid.hi[1] - valid.lo[1] + 1) * TMK_NCOMP : 0;
+    const size_t nF1 = F1 ? (size_t)(valid.hi[0] - valid.lo[0] + 1) * (valid.hi[1] - valid.lo[1] + 2) * TMK_NCOMP : 0;
+    int e;
+    if ((e = c->scratch[1].ensure(nU * sizeof(T)))) return e;
+    if ((e = c->scratch[2].ensure(nU * sizeof(T)))) return e;
+    if ((e = c->scratch[3].ensure(std::max<size_t>(2 * nB, 1) * sizeof(double)))) return e;
+    if ((e = c->scratch[4].ensure(std::max<size_t>(nF0 + nF1, 1) * sizeof(double)))) return e;
+    hipStream_t st = c->stream;
+    T* Uin = (T*)c->scratch[1].p;
+    T* Uout = (T*)c->scratch[2].p;
+    double* Bd = (double*)c->scratch[3].p;
+    double* Fd = (double*)c->scratch[4].p;
+    std::vector<float> tmp;
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(hipMemcpyAsync(Uin, U, nU * sizeof(double), hipMemcpyHostToDevice, st));
+    } else {
+        tmp.resize(nU);
+        for (size_t k = 0; k < nU; ++k) tmp[k] = (float)U[k];
+        HIPCHK(hipMemcpyAsync(Uin, tmp.data(), nU * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(Uout, Uin, nU * sizeof(T), hipMemcpyDeviceToDevice, st));
+    if (plastic_only) {
+        const int vnx = valid.hi[0] - valid.lo[0] + 1, vny = valid.hi[1] - valid.lo[1] + 1;
+        const int n = vnx * vny;
+        hipLaunchKernelGGL(k_plastic<T>, dim3((n + 255) / 256), dim3(256), 0, st, c->P, Uout, ub.lo[0], ub.lo[1],
+                           (int)unx, plane, valid.lo[0], valid.lo[1], vnx, vny, dt);
+        HIPCHK(hipGetLastError());
+    } else {
+        if (B) {
+            HIPCHK(hipMemcpyAsync(Bd, B, nB * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(Bd + nB, Bd, nB * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        Patch<T> pt;
+        memset(&pt, 0, sizeof pt);
+        for (int k = 0; k < TMK_NCOMP; ++k) { pt.in[k] = Uin + k * plane; pt.out[k] = Uout + k * plane; }
+        pt.ulo0 = ub.lo[0]; pt.ulo1 = ub.lo[1]; pt.unx = (int)unx;
+        pt.bin = B ? Bd : nullptr;
+        pt.bout = B ? Bd + nB : nullptr;
+        pt.blo0 = bb.lo[0]; pt.bnx = (int)bnx;
+        pt.F[0] = F0 ? Fd : nullptr;
+        pt.F[1] = F1 ? Fd + nF0 : nullptr;
+        pt.vlo0 = valid.lo[0]; pt.vlo1 = valid.lo[1]; pt.vhi0 = valid.hi[0]; pt.vhi1 = valid.hi[1];
+        pt.scen = 0;
+        std::vector<Tile> tiles;
+        make_tiles(valid, 0, c->bx - 2, c->by - 2, tiles);
+        const size_t off_t = ((sizeof(Patch<T>) + 255) / 256) * 256;
+        const size_t off_s = off_t + ((sizeof(Tile) * tiles.size() + 255) / 256) * 256;
+        if ((e = c->scratch[5].ensure(off_s + sizeof(Scen)))) return e;
+        char* meta = (char*)c->scratch[5].p;
+        Scen sc;
+        memset(&sc, 0, sizeof sc);
+        sc.P = c->P;
+        sc.time = time;
+        sc.dt = dt;
+        HIPCHK(hipMemcpyAsync(meta, &pt, sizeof pt, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(meta + off_t, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(meta + off_s, &sc, sizeof sc, hipMemcpyHostToDevice, st));
+        KArgs<T> a;
+        a.scen = (Scen*)(meta + off_s);
+        a.patches = (const Patch<T>*)meta;
+        a.tiles = (const Tile*)(meta + off_t);
+        a.ntiles = (int)tiles.size();
+        a.plastic = 0;  // updateState does not include the plastic return
+        // the anti-plane planes are skipped only when background and data are zero there
+        int ns = 3;
+        if (!c->antiplane) {
+            bool zero = true;
+            for (int s = 0; s < 2 && zero; ++s)
+                for (int cc : {2, 7, 8}) {
+                    const double* q = U + (size_t)(11 * s + cc) * plane;
+                    for (size_t k = 0; k < plane; ++k)
+                        if (q[k] != 0.0) { zero = false; break; }
+                    if (!zero) break;
+                }
+            ns = zero ? 2 : 3;
+        }
+        if ((e = launch_tiles<T>(st, ns, c->bx, c->by, a))) return e;
+        int herr = 0;
+        HIPCHK(hipMemcpyAsync(&herr, meta + off_s + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (herr) return herr;
+        if (B) HIPCHK(hipMemcpy(B, Bd + nB, nB * sizeof(double), hipMemcpyDeviceToHost));
+        if (F0) HIPCHK(hipMemcpy(F0, Fd, nF0 * sizeof(double), hipMemcpyDeviceToHost));
+        if (F1) HIPCHK(hipMemcpy(F1, Fd + nF0, nF1 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (sizeof(T) == sizeof(double)) {
+        HIPCHK(hipMemcpyAsync(U, Uout, nU * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } else {
+        tmp.resize(nU);
+        HIPCHK(hipMemcpyAsync(tmp.data(), Uout, nU * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t k = 0; k < nU; ++k) U[k] = (double)tmp[k];
+    }
+    return TMK_OK;
+}
+
+extern "C" int tmk_patch_update_state(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp, double* F0,
+                                      double* F1, double* B, tmk_box bb, int nbdry, double time, double dt,
+                                      double* max_wave_speed)
+{
+    if (!c || !U || ncomp != TMK_NCOMP) return TMK_E_ARG;
+    if (box_empty(valid)) return TMK_E_ARG;
+    const tmk_box dom = c->hp.domain;
+    if (!box_contains(dom, valid)) return TMK_E_BOX;
+    // the update reads W on grow(valid,3) & domain and writes the valid cells
+    if (!box_contains(ub, box_and(box_grow(valid, 3), dom))) return TMK_E_BOX;
+    const bool fault = valid.lo[1] == dom.lo[1];
+    if (fault) {
+        if (!B) return TMK_E_NO_BDRY;
+        const int need = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+        if (nbdry != need) return TMK_E_ARG;
+        const int lo = std::max(valid.lo[0] - 1, dom.lo[0]);
+        const int hi = std::min(valid.hi[0] + 1, dom.hi[0]);
+        if (bb.lo[0] > lo || bb.hi[0] < hi) return TMK_E_BOX;
+    } else {
+        B = nullptr;
+    }
+    if (max_wave_speed) *max_wave_speed = std::max(c->hp.cs, c->hp.cp);
+    if (c->precision == TMK_PREC_FP64)
+        return patch_update_t<double>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
+    return patch_update_t<float>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
+}
+
+extern "C" int tmk_patch_plastic_update(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp, double dt)
+{
+    if (!c || !U || ncomp != TMK_NCOMP) return TMK_E_ARG;
+    if (box_empty(valid) || !box_contains(ub, valid)) return TMK_E_BOX;
+    if (c->precision == TMK_PREC_FP64)
+        return patch_update_t<double>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
+    return patch_update_t<float>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident level
+// ---------------------------------------------------------------------------
+template <typename T>
+static void level_fill_patches(tmk_level* L)
+{
+    const int nbox = (int)L->boxes.size();
+    for (int cur = 0; cur < 2; ++cur) {
+        L->hpatch[cur].assign(sizeof(Patch<T>) * nbox, 0);
+        Patch<T>* pts = (Patch<T>*)L->hpatch[cur].data();
+        for (int b = 0; b < nbox; ++b) {
+            const tmk_box v = L->boxes[b];
+            const tmk_box ub = box_grow(v, TMK_GHOST);
+            const size_t unx = ub.hi[0] - ub.lo[0] + 1, uny = ub.hi[1] - ub.lo[1] + 1, plane = unx * uny;
+            Patch<T>& p = pts[b];
+            T* A = (T*)L->ubuf[cur][b];
+            T* Bn = (T*)L->ubuf[1 - cur][b];
+            T* W = (T*)L->upt[b];
+            for (int c = 0; c < TMK_NCOMP; ++c) {
+                if (is_stencil(c, L->ns)) { p.in[c] = A + c * plane; p.out[c] = Bn + c * plane; }
+                else { p.in[c] = W + c * plane; p.out[c] = W + c * plane; }
+            }
+            p.ulo0 = ub.lo[0]; p.ulo1 = ub.lo[1]; p.unx = (int)unx;
+            if (L->has_b[b]) {
+                p.bin = L->bbuf[cur][b];
+                p.bout = L->bbuf[1 - cur][b];
+                p.blo0 = v.lo[0] - TMK_GHOST;
+                p.bnx = v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST;
+            }
+            p.vlo0 = v.lo[0]; p.vlo1 = v.lo[1]; p.vhi0 = v.hi[0]; p.vhi1 = v.hi[1];
+            p.scen = 0;
+        }
+    }
+}
+
+static size_t level_bytes(const tmk_ctx* c, int nboxes, const tmk_box* boxes)
+{
+    const size_t ts = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const tmk_box dom = c->hp.domain;
+    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    size_t tot = 0;
+    for (int b = 0; b < nboxes; ++b) {
+        const tmk_box ub = box_grow(boxes[b], TMK_GHOST);
+        const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+        tot += 3 * plane * TMK_NCOMP * ts;
+        if (boxes[b].lo[1] == dom.lo[1])
+            tot += 2 * (size_t)(boxes[b].hi[0] - boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb * sizeof(double);
+    }
+    return tot;
+}
+
+extern "C" int tmk_level_footprint(const tmk_ctx* c, int nboxes, const tmk_box* boxes, size_t* bytes)
+{
+    if (!c || nboxes <= 0 || !boxes || !bytes) return TMK_E_ARG;
+    *bytes = level_bytes(c, nboxes, boxes);
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tmk_level** out)
+{
+    if (!c || !out || nboxes <= 0 || !boxes) return TMK_E_ARG;
+    *out = nullptr;
+    const tmk_box dom = c->hp.domain;
+    for (int b = 0; b < nboxes; ++b)
+        if (box_empty(boxes[b]) || !box_contains(dom, boxes[b])) return TMK_E_BOX;
+    tmk_level* L = new (std::nothrow) tmk_level;
+    if (!L) return TMK_E_NOMEM;
+    L->ctx = c;
+    L->boxes.assign(boxes, boxes + nboxes);
+    L->ns = c->antiplane ? 3 : 2;
+    L->tsize = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    L->bytes = level_bytes(c, nboxes, boxes);
+    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    for (int b = 0; b < nboxes; ++b) {
+        const tmk_box v = boxes[b];
+        const tmk_box ub = box_grow(v, TMK_GHOST);
+        const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+        const size_t bytes = plane * TMK_NCOMP * L->tsize;
+        void *a = nullptr, *bb = nullptr, *w = nullptr;
+        const bool ok = hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&bb, bytes) == hipSuccess &&
+                        hipMalloc(&w, bytes) == hipSuccess;
+        L->ubuf[0].push_back(a);
+        L->ubuf[1].push_back(bb);
+        L->upt.push_back(w);
+        const int hb = v.lo[1] == dom.lo[1];
+        L->has_b.push_back(hb);
+        double *b0 = nullptr, *b1 = nullptr;
+        bool okb = true;
+        if (hb) {
+            const size_t nB = (size_t)(v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST) * nb;
+            okb = hipMalloc(&b0, nB * sizeof(double)) == hipSuccess && hipMalloc(&b1, nB * sizeof(double)) == hipSuccess;
+        }
+        L->bbuf[0].push_back(b0);
+        L->bbuf[1].push_back(b1);
+        if (!ok || !okb) {
+            tmk_level_destroy(L);
+            return TMK_E_NOMEM;
+        }
+    }
+    if (L->tsize == sizeof(double)) level_fill_patches<double>(L);
+    else level_fill_patches<float>(L);
+    for (int b = 0; b < nboxes; ++b) make_tiles(boxes[b], b, c->bx - 2, c->by - 2, L->tiles);
+    // exchange plan: ghost cells (inside the domain) of each box covered by another box's valid cells
+    for (int d = 0; d < nboxes; ++d) {
+        const tmk_box g = box_and(box_grow(boxes[d], TMK_GHOST), dom);
+        for (int s = 0; s < nboxes; ++s) {
+            if (s == d) continue;
+            const tmk_box r = box_and(g, boxes[s]);
+            if (box_empty(r)) continue;
+            L->rects.push_back(CopyRect{s, d, r.lo[0], r.lo[1], r.hi[0], r.hi[1]});
+            if (L->has_b[d] && L->has_b[s]) {
+                const int lo = std::max(boxes[d].lo[0] - TMK_GHOST, boxes[s].lo[0]);
+                const int hi = std::min(boxes[d].hi[0] + TMK_GHOST, boxes[s].hi[0]);
+                if (lo <= hi) L->brects.push_back(CopyRect{s, d, lo, 0, hi, 0});
+            }
+        }
+    }
+    *out = L;
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_destroy(tmk_level* L)
+{
+    if (!L) return TMK_OK;
+    if (L->batch) return TMK_E_ARG;
+    if (L->ctx && L->ctx->stream) (void)hipStreamSynchronize(L->ctx->stream);
+    L->own.release();
+    for (int k = 0; k < 2; ++k) {
+        for (void* p : L->ubuf[k]) if (p) (void)hipFree(p);
+        for (double* p : L->bbuf[k]) if (p) (void)hipFree(p);
+    }
+    for (void* p : L->upt) if (p) (void)hipFree(p);
+    delete L;
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const double* B)
+{
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch) return TMK_E_ARG;
+    tmk_ctx* c = L->ctx;
+    const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
+    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    const size_t nU = plane * TMK_NCOMP;
+    if (L->ns == 2) {  // mode II level: the anti-plane planes must be zero
+        for (int s = 0; s < 2; ++s)
+            for (int cc : {2, 7, 8})
+                for (size_t k = 0; k < plane; ++k)
+                    if (U[k + (size_t)(11 * s + cc) * plane] != 0.0) return TMK_E_ARG;
+    }
+    std::vector<float> tmp;
+    const void* src = U;
+    if (L->tsize == sizeof(float)) {
+        tmp.resize(nU);
+        for (size_t k = 0; k < nU; ++k) tmp[k] = (float)U[k];
+        src = tmp.data();
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t bytes = nU * L->tsize;
+    for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->ubuf[k][b], src, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->upt[b], src, bytes, hipMemcpyHostToDevice));
+    if (L->has_b[b]) {
+        if (!B) return TMK_E_NO_BDRY;
+        const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
+        for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->bbuf[k][b], B, nB * sizeof(double), hipMemcpyHostToDevice));
+    }
+    L->dirty = true;
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
+{
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch) return TMK_E_ARG;
+    tmk_ctx* c = L->ctx;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
+    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    const size_t nU = plane * TMK_NCOMP;
+    std::vector<char> a(nU * L->tsize), w(nU * L->tsize);
+    HIPCHK(hipMemcpy(a.data(), L->ubuf[L->cur][b], nU * L->tsize, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w.data(), L->upt[b], nU * L->tsize, hipMemcpyDeviceToHost));
+    for (int cc = 0; cc < TMK_NCOMP; ++cc) {
+        const char* src = is_stencil(cc, L->ns) ? a.data() : w.data();
+        for (size_t k = 0; k < plane; ++k) {
+            const size_t q = k + (size_t)cc * plane;
+            U[q] = L->tsize == sizeof(double) ? ((const double*)src)[q] : (double)((const float*)src)[q];
+        }
+    }
+    if (L->has_b[b] && B) {
+        const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
+        HIPCHK(hipMemcpy(B, L->bbuf[L->cur][b], nB * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return TMK_OK;
+}
+
+// make the level's own one-member group current (built lazily, rebuilt when the
+// level left a batch at another parity)
+static int level_own_group(tmk_level* L)
+{
+    if (!L->own_built) {
+        std::vector<tmk_level*> m{L};
+        std::vector<int> ph{L->cur};
+        int e = group_build(L->own, m, ph, 1);
+        if (e) return e;
+        L->own_built = true;
+    }
+    return TMK_OK;
+}
+
+static int level_prepare(tmk_level* L, double time, double dt)
+{
+    int e = level_own_group(L);
+    if (e) return e;
+    hipStream_t st = L->ctx->stream;
+    // group parity tracks the level parity (phase 0 at build time == cur at build)
+    if (L->dirty) {
+        L->dirty = false;
+        e = L->tsize == sizeof(double) ? group_exchange_now_t<double>(L->own, st)
+                                       : group_exchange_now_t<float>(L->own, st);
+        if (e) return e;
+    }
+    // the clock lives in device memory, so captured graphs stay valid across calls
+    L->time = time;
+    L->dt = dt;
+    return group_set_clock(L->own, st, 0, time, dt);
+}
+
+extern "C" int tmk_level_step(tmk_level* L, double time, double dt, double* dt_new)
+{
+    if (!L || L->batch) return TMK_E_ARG;
+    int e = level_prepare(L, time, dt);
+    if (e) return e;
+    if ((e = group_enqueue_step(L->own, L->ctx->stream))) return e;
+    L->cur ^= 1;
+    L->time = time + dt;
+    if (dt_new) *dt_new = L->ctx->hp.dx / std::max(L->ctx->hp.cs, L->ctx->hp.cp);
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_run(tmk_level* L, int nsteps, double time, double dt)
+{
+    if (!L || nsteps < 0 || L->batch) return TMK_E_ARG;
+    if (nsteps == 0) return TMK_OK;
+    int e = level_prepare(L, time, dt);
+    if (e) return e;
+    if ((e = group_run(L->own, L->ctx->stream, nsteps))) return e;
+    L->cur ^= (nsteps & 1);
+    double t = time;
+    for (int k = 0; k < nsteps; ++k) t = t + dt;
+    L->time = t;
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_run_timed(tmk_level* L, int nsteps, double time, double dt, double* ms)
+{
+    if (!L || !ms) return TMK_E_ARG;
+    hipStream_t st = L->ctx->stream;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int e = level_prepare(L, time, dt);
+    if (!e && hipEventRecord(a, st) != hipSuccess) e = TMK_E_HIP;
+    if (!e) e = tmk_level_run(L, nsteps, time, dt);
+    if (!e && hipEventRecord(b, st) != hipSuccess) e = TMK_E_HIP;
+    if (!e && hipEventSynchronize(b) != hipSuccess) e = TMK_E_HIP;
+    float f = 0.f;
+    if (!e && hipEventElapsedTime(&f, a, b) != hipSuccess) e = TMK_E_HIP;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!e) *ms = f;
+    return e;
+}
+
+extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double dt, double* ms)
+{
+    if (!L || !ms || nsteps <= 0 || L->batch) return TMK_E_ARG;
+    int e = level_prepare(L, time, dt);
+    if (e) return e;
+    hipStream_t st = L->ctx->stream;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    if (hipEventRecord(a, st) != hipSuccess) e = TMK_E_HIP;
+    for (int k = 0; k < nsteps && !e; ++k) {
+        e = group_enqueue_step(L->own, st);
+        L->cur ^= 1;
+        L->time = L->time + dt;
+    }
+    if (!e && hipEventRecord(b, st) != hipSuccess) e = TMK_E_HIP;
+    if (!e && hipEventSynchronize(b) != hipSuccess) e = TMK_E_HIP;
+    float f = 0.f;
+    if (!e && hipEventElapsedTime(&f, a, b) != hipSuccess) e = TMK_E_HIP;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!e) *ms = (double)f / nsteps;
+    return e;
+}
+
+extern "C" int tmk_level_sync(tmk_level* L)
+{
+    if (!L || L->batch) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    if (!L->own_built) return TMK_OK;
+    int code = 0;
+    int e = group_error(L->own, L->ctx->stream, 0, &code);
+    if (e) return e;
+    if (code) {
+        const int z = 0;
+        HIPCHK(hipMemcpy((char*)(L->own.scen_dev) + offsetof(Scen, err), &z, sizeof z, hipMemcpyHostToDevice));
+    }
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// batches: boxes of many scenarios in shared launches (continuous batching)
+// ---------------------------------------------------------------------------
+extern "C" int tmk_batch_create(tmk_batch** out)
+{
+    if (!out) return TMK_E_ARG;
+    tmk_batch* B = new (std::nothrow) tmk_batch;
+    if (!B) return TMK_E_NOMEM;
+    if (hipStreamCreate(&B->stream) != hipSuccess) {
+        delete B;
+        return TMK_E_HIP;
+    }
+    *out = B;
+    return TMK_OK;
+}
+
+extern "C" int tmk_batch_destroy(tmk_batch* B)
+{
+    if (!B) return TMK_OK;
+    (void)hipStreamSynchronize(B->stream);
+    for (tmk_level* L : B->members) L->batch = nullptr;
+    B->grp.release();
+    (void)hipStreamDestroy(B->stream);
+    delete B;
+    return TMK_OK;
+}
+
+// pull each member's clock and parity out of the group before the layout changes
+static int batch_detach_all(tmk_batch* B)
+{
+    HIPCHK(hipStreamSynchronize(B->stream));
+    if (!B->built) return TMK_OK;
+    std::vector<Scen> sc(B->members.size());
+    if (!sc.empty())
+        HIPCHK(hipMemcpy(sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+    for (size_t m = 0; m < B->members.size(); ++m) {
+        tmk_level* L = B->members[m];
+        L->cur = B->phase[m] ^ B->grp.g;
+        L->time = sc[m].time;
+        if (sc[m].err) L->own_built = false;
+        B->phase[m] = L->cur;
+    }
+    B->grp.release();
+    B->built = false;
+    return TMK_OK;
+}
+
+static int batch_rebuild(tmk_batch* B)
+{
+    for (size_t m = 0; m < B->members.size(); ++m) B->phase[m] = B->members[m]->cur;
+    int e = group_build(B->grp, B->members, B->phase, 1);
+    if (e) return e;
+    B->built = !B->members.empty();
+    // exchange members that were uploaded since their last step
+    for (tmk_level* L : B->members) {
+        if (!L->dirty) continue;
+        L->dirty = false;
+        // a one-member exchange on the batch stream
+        Group tmp;
+        std::vector<tmk_level*> m{L};
+        std::vector<int> ph{L->cur};
+        if ((e = group_build(tmp, m, ph, 1))) return e;
+        e = L->tsize == sizeof(double) ? group_exchange_now_t<double>(tmp, B->stream)
+                                       : group_exchange_now_t<float>(tmp, B->stream);
+        (void)hipStreamSynchronize(B->stream);
+        tmp.release();
+        if (e) return e;
+    }
+    return TMK_OK;
+}
+
+extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
+{
+    if (!B || !L || L->batch) return TMK_E_ARG;
+    if (!B->members.empty()) {
+        tmk_level* L0 = B->members[0];
+        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by ||
+            L0->ctx->hp.fault_kind != L->ctx->hp.fault_kind)
+            return TMK_E_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    int e = batch_detach_all(B);
+    if (e) return e;
+    L->time = time;
+    L->dt = dt;
+    L->batch = B;
+    B->members.push_back(L);
+    B->phase.push_back(L->cur);
+    return batch_rebuild(B);
+}
+
+extern "C" int tmk_batch_remove(tmk_batch* B, tmk_level* L)
+{
+    if (!B || !L || L->batch != B) return TMK_E_ARG;
+    int e = batch_detach_all(B);
+    if (e) return e;
+    const auto it = std::find(B->members.begin(), B->members.end(), L);
+    const size_t m = it - B->members.begin();
+    B->members.erase(it);
+    B->phase.erase(B->phase.begin() + m);
+    L->batch = nullptr;
+    L->own_built = false;  // rebuild its own group at its current parity
+    L->own.release();
+    return batch_rebuild(B);
+}
+
+extern "C" int tmk_batch_run(tmk_batch* B, int nsteps)
+{
+    if (!B || nsteps < 0) return TMK_E_ARG;
+    if (!B->built || nsteps == 0) return TMK_OK;
+    int e = group_run(B->grp, B->stream, nsteps);
+    if (e) return e;
+    for (tmk_level* L : B->members)
+        for (int k = 0; k < nsteps; ++k) L->time = L->time + L->dt;
+    return TMK_OK;
+}
+
+extern "C" int tmk_batch_sync(tmk_batch* B)
+{
+    if (!B) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(B->stream));
+    if (!B->built) return TMK_OK;
+    std::vector<Scen> sc(B->members.size());
+    HIPCHK(hipMemcpy(sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+    for (const Scen& s : sc)
+        if (s.err) return s.err;
+    return TMK_OK;
+}

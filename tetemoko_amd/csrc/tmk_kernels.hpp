@@ -1,0 +1,672 @@
+// tmk_kernels.hpp -- fused per-tile Godunov update for gfx950 (CDNA4, wave64).
+//
+// One workgroup updates one tile of TX x TY valid cells of one box
+// (LinElastPatchGodunov::updateState, LinElastPatchGodunov.cpp:101-129, run on
+// a sub-rectangle; every value the reference computes is a function of its
+// location only, so tiling a box reproduces the box result exactly).
+// Thread (lx, ly) owns cell (x0-1+lx, y0-1+ly) of the tile grown by one ring
+// (the reference's slopeBox, LinElastPatchGodunov.cpp:244-246).
+//
+// Stages (each ends with a workgroup barrier):
+//   1  x-sweep stencil staged in LDS, normal predictor in x (per thread)
+//   2  y-sweep stencil staged in LDS, normal predictor in y
+//   3  predictor Riemann states WHalf1 on the lo x- and y-face of each thread,
+//      domain boundary states and the fault solve (RIEMANNF / RSIBC::primBC)
+//   4  transverse correction W(d1) += A(d2) dWHalf1(d2)  (GETADWDXF)
+//   5  final Riemann states on valid faces, fault solve and fault-data update
+//   6  flux difference, U += dU, optional Drucker-Prager return
+//
+// Only the components a sweep can move are carried (tmk_physics.hpp): the
+// state planes vz/sxz/syz are skipped when the anti-plane fields are zero
+// (NS == 2), and lam/gam are only touched by the plastic return.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+
+#include "tetemoko_hip.h"
+#include "tmk_physics.hpp"
+
+namespace tmk {
+
+// one box of the level as seen by the kernel
+template <typename T>
+struct Patch {
+    const T* in[22];   // state planes read this step
+    T* out[22];        // state planes written this step (== in for pointwise planes)
+    int ulo0, ulo1, unx; // U array box (ghosted) lo and x extent
+    const double* bin; // fault data read (NULL: box has none)
+    double* bout;      // fault data written
+    int blo0, bnx;     // fault-data x lo and extent
+    double* F[2];      // optional face fluxes (22 comps, on surroundingNodes(box,d)) or NULL
+    int vlo0, vlo1, vhi0, vhi1; // valid box
+    int scen;          // index of the scenario (parameters, time, step) this box belongs to
+};
+
+// one scenario's parameters, clock and error word; device resident so that
+// boxes of several scenarios can share one launch
+struct Scen {
+    DevParams P;
+    double time;  // time at the start of the current step
+    double dt;
+    int err;      // first error reported by any launch (0 = none)
+    int pad;
+};
+
+struct Tile {
+    int patch;
+    int x0, y0, x1, y1;  // valid cells of this tile (inclusive)
+};
+
+template <typename T>
+struct KArgs {
+    Scen* scen;
+    const Patch<T>* patches;
+    const Tile* tiles;
+    int ntiles;
+    int plastic;   // 0: skip the Drucker-Prager return even if the scenario enables it
+};
+
+__device__ __forceinline__ void report(int* err, int code)
+{
+    // vector atomic from the lanes that hit the error
+    atomicCAS(err, 0, code);
+}
+
+template <typename T>
+__device__ __forceinline__ size_t uidx(const Patch<T>& p, int i, int j, size_t plane)
+{
+    return (size_t)(i - p.ulo0) + (size_t)(j - p.ulo1) * (size_t)p.unx;
+}
+
+// stage-1/2 staging: component list per direction, [side][slot][v|s]
+template <int NS>
+__device__ __forceinline__ int stage_comp(int d, int c)
+{
+    const int s = c / (2 * NS);
+    const int r = c - s * 2 * NS;
+    return slot_comp(d, r >> 1, r & 1) + 11 * s;
+}
+
+template <typename T, int NS, int BX, int BY>
+__global__ void __launch_bounds__(BX * BY)
+k_godunov(const KArgs<T> a)
+{
+    constexpr int NT = BX * BY;
+    constexpr int NC = 4 * NS;  // staged components per direction
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+
+    const Tile tl = a.tiles[blockIdx.x];
+    const Patch<T>& pt = a.patches[tl.patch];
+    Scen& S = a.scen[pt.scen];
+    const DevParams& P = S.P;
+    const Mat<T> m(P);
+    int* const errp = &S.err;
+    const int tid = threadIdx.x;
+    const int lx = tid % BX, ly = tid / BX;
+    const int i = tl.x0 - 1 + lx, j = tl.y0 - 1 + ly;
+    const int tnx = tl.x1 - tl.x0 + 1, tny = tl.y1 - tl.y0 + 1;
+    const int dlo0 = P.dlo[0], dlo1 = P.dlo[1], dhi0 = P.dhi[0], dhi1 = P.dhi[1];
+    const bool colD = i >= dlo0 && i <= dhi0;
+    const bool rowD = j >= dlo1 && j <= dhi1;
+    const bool r1 = colD && rowD && lx <= tnx + 1 && ly <= tny + 1;
+    const bool valid = lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny;
+    const double time = S.time;
+    const double dt = S.dt;
+    const T dtbydx = (T)(dt / P.dx);
+    const int pred = P.pred;
+    const T rv1 = (T)-1;
+
+    Slots<T, NS> WmX, WpX, WmY, WpY;
+
+    // ---------------- stage 1: x predictor ----------------
+    {
+        constexpr int SW = BX + 4;
+        for (int e = tid; e < NC * BY * SW; e += NT) {
+            const int c = e / (BY * SW);
+            const int r = e - c * (BY * SW);
+            const int yy = r / SW, xx = r - yy * SW;
+            const int gi = tl.x0 - 3 + xx, gj = tl.y0 - 1 + yy;
+            T v = (T)0;
+            // only the tile's stencil region (x: tile +-3, y: tile +-1), inside the domain
+            if (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1)
+                v = pt.in[stage_comp<NS>(0, c)][uidx(pt, gi, gj, 0)];
+            sm[e] = v;
+        }
+        __syncthreads();
+        if (r1) {
+            T w[2][NS][2][5];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const int c = s * 2 * NS + k * 2 + q;
+#pragma unroll
+                        for (int o = 0; o < 5; ++o) w[s][k][q][o] = sm[(c * BY + ly) * SW + lx + o];
+                    }
+            normal_pred<T, NS>(P, m, pred, (T)1, w, i, dlo0, dhi0, dtbydx, WmX, WpX);
+        }
+        __syncthreads();
+    }
+    // ---------------- stage 2: y predictor ----------------
+    {
+        constexpr int SH = BY + 4;
+        for (int e = tid; e < NC * SH * BX; e += NT) {
+            const int c = e / (SH * BX);
+            const int r = e - c * (SH * BX);
+            const int yy = r / BX, xx = r - yy * BX;
+            const int gi = tl.x0 - 1 + xx, gj = tl.y0 - 3 + yy;
+            T v = (T)0;
+            // only the tile's stencil region (x: tile +-1, y: tile +-3), inside the domain
+            if (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3)
+                v = pt.in[stage_comp<NS>(1, c)][uidx(pt, gi, gj, 0)];
+            sm[e] = v;
+        }
+        __syncthreads();
+        if (r1) {
+            T w[2][NS][2][5];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const int c = s * 2 * NS + k * 2 + q;
+#pragma unroll
+                        for (int o = 0; o < 5; ++o) w[s][k][q][o] = sm[(c * SH + ly + o) * BX + lx];
+                    }
+            normal_pred<T, NS>(P, m, pred, rv1, w, j, dlo1, dhi1, dtbydx, WmY, WpY);
+        }
+        __syncthreads();
+    }
+
+    // exchange helpers: region r of the LDS holds NV values per thread
+    auto put = [&](int slot, T v) { sm[slot * NT + tid] = v; };
+    auto get = [&](int slot, int t) -> T { return sm[slot * NT + t]; };
+
+    // ---------------- stage 3: predictor face states ----------------
+    Slots<T, NS> H0, H1;
+    double tmp_state = 0.0;  // RS: psi of the half step; SW: slip-total of the half step
+    {
+        T cpx[2][NS], cpy[2][NS];
+        if (r1) {
+            right_chars<T, NS>(m, (T)1, WpX, cpx);
+            right_chars<T, NS>(m, rv1, WpY, cpy);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    put(s * NS + k, cpx[s][k]);
+                    put(2 * NS + s * NS + k, cpy[s][k]);
+                }
+        }
+        __syncthreads();
+        // x faces: lo face of cell i, rows of the ring-grown tile
+        if (lx >= 1 && lx <= tnx + 1 && rowD && ly <= tny + 1 && i >= dlo0 && i <= dhi0 + 1) {
+            T wpL[2][NS], wmR[2][NS];
+            if (i == dlo0) {
+                left_chars<T, NS>(m, (T)1, WmX, wmR);
+                bc_state<T, NS>(P, m, 0, (T)1, P.bc[0], -1, wmR, H0);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) wpL[s][k] = get(s * NS + k, tid - 1);
+                if (i == dhi0 + 1) {
+                    bc_state<T, NS>(P, m, 0, (T)1, P.bc[1], +1, wpL, H0);
+                } else {
+                    left_chars<T, NS>(m, (T)1, WmX, wmR);
+                    riemann_state<T, NS>(m, (T)1, wpL, wmR, H0);
+                }
+            }
+        }
+        // y faces: lo face of cell j
+        if (ly >= 1 && ly <= tny + 1 && colD && lx <= tnx + 1 && j >= dlo1 && j <= dhi1 + 1) {
+            T wpL[2][NS], wmR[2][NS];
+            if (j == dlo1) {
+                // fault face (RSIBC::primBC lo-y branch, RSIBC.cpp:220-246)
+                if (!pt.bin) {
+                    report(errp, TMK_E_NO_BDRY);
+                } else {
+                    FaultIn fin;
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        fin.vy[s] = (double)WmY.v[s][0]; fin.syy[s] = (double)WmY.s[s][0];
+                        fin.vx[s] = (double)WmY.v[s][1]; fin.sxy[s] = (double)WmY.s[s][1];
+                        fin.vz[s] = NS == 3 ? (double)WmY.v[s][NS - 1] : 0.0;
+                        fin.syz[s] = NS == 3 ? (double)WmY.s[s][NS - 1] : 0.0;
+                    }
+                    FaultOut fo;
+                    const size_t bi = (size_t)(i - pt.blo0);
+                    const size_t bp = (size_t)pt.bnx;
+                    if (P.fault_kind == 0) {
+                        rs_fault(P, i, pt.bin[bi + 10 * bp], fin, fo);
+                        // RSSETBND into the half-step copy (RSIBC.cpp:325-333): only psi is consumed
+                        const double Vx = fo.vx[0] - fo.vx[1];
+                        const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
+                        const double V = sqrt(Vx * Vx + Vz * Vz);
+                        tmp_state = rs_psi_update(P, i, pt.bin[bi + 10 * bp], V, 0.5 * dt);
+                        if (isnan(tmp_state)) report(errp, TMK_E_PSI_NAN);
+                    } else {
+                        sw_fault(P, i, pt.bin[bi + 7 * bp], fin, fo);
+                        const double Vx = fo.vx[0] - fo.vx[1];
+                        const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
+                        const double V = sqrt(Vx * Vx + Vz * Vz);
+                        tmp_state = pt.bin[bi + 7 * bp] + (0.5 * dt) * V;
+                    }
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        H1.v[s][0] = (T)fo.vy[s]; H1.s[s][0] = (T)fo.syy[s];
+                        H1.v[s][1] = (T)fo.vx[s]; H1.s[s][1] = (T)fo.sxy[s];
+                        if (NS == 3) { H1.v[s][NS - 1] = (T)fo.vz[s]; H1.s[s][NS - 1] = (T)fo.syz[s]; }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) wpL[s][k] = get(2 * NS + s * NS + k, tid - BX);
+                if (j == dhi1 + 1) {
+                    bc_state<T, NS>(P, m, 1, rv1, P.bc[3], +1, wpL, H1);
+                } else {
+                    left_chars<T, NS>(m, rv1, WmY, wmR);
+                    riemann_state<T, NS>(m, rv1, wpL, wmR, H1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---------------- stage 4: transverse corrections ----------------
+    {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                put(s * NS + k, H0.v[s][k]);
+                put(2 * NS + s * NS + k, H0.s[s][k]);
+                put(4 * NS + s * NS + k, H1.v[s][k]);
+                put(6 * NS + s * NS + k, H1.s[s][k]);
+            }
+        __syncthreads();
+        const T scale = (T)(-(1.0 / 2.0) * dt / P.dx);
+        // x predictor (cells of ccBox[1]: ring columns, valid rows) corrected with WHalf1[1]
+        if (colD && lx <= tnx + 1 && ly >= 1 && ly <= tny) {
+            Slots<T, NS> Hhi;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    Hhi.v[s][k] = get(4 * NS + s * NS + k, tid + BX);
+                    Hhi.s[s][k] = get(6 * NS + s * NS + k, tid + BX);
+                }
+            transverse_add<T, NS>(m, scale, rv1 * scale, H1, Hhi, WmX);
+            transverse_add<T, NS>(m, scale, rv1 * scale, H1, Hhi, WpX);
+        }
+        // y predictor (cells of ccBox[0]: valid columns, ring rows) corrected with WHalf1[0]
+        if (rowD && ly <= tny + 1 && lx >= 1 && lx <= tnx) {
+            Slots<T, NS> Hhi;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    Hhi.v[s][k] = get(s * NS + k, tid + 1);
+                    Hhi.s[s][k] = get(2 * NS + s * NS + k, tid + 1);
+                }
+            transverse_add<T, NS>(m, scale, scale, H0, Hhi, WmY);
+            transverse_add<T, NS>(m, scale, scale, H0, Hhi, WpY);
+        }
+        __syncthreads();
+    }
+    // ---------------- stage 5: final face states ----------------
+    Slots<T, NS> F0, F1;
+    {
+        T cpx[2][NS], cpy[2][NS];
+        right_chars<T, NS>(m, (T)1, WpX, cpx);
+        right_chars<T, NS>(m, rv1, WpY, cpy);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                put(s * NS + k, cpx[s][k]);
+                put(2 * NS + s * NS + k, cpy[s][k]);
+            }
+        __syncthreads();
+        if (lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny && i >= dlo0 && i <= dhi0 + 1) {
+            T wpL[2][NS], wmR[2][NS];
+            if (i == dlo0) {
+                left_chars<T, NS>(m, (T)1, WmX, wmR);
+                bc_state<T, NS>(P, m, 0, (T)1, P.bc[0], -1, wmR, F0);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) wpL[s][k] = get(s * NS + k, tid - 1);
+                if (i == dhi0 + 1) {
+                    bc_state<T, NS>(P, m, 0, (T)1, P.bc[1], +1, wpL, F0);
+                } else {
+                    left_chars<T, NS>(m, (T)1, WmX, wmR);
+                    riemann_state<T, NS>(m, (T)1, wpL, wmR, F0);
+                }
+            }
+        }
+        if (ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx && j >= dlo1 && j <= dhi1 + 1) {
+            T wpL[2][NS], wmR[2][NS];
+            if (j == dlo1) {
+                if (pt.bin) {
+                    FaultIn fin;
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        fin.vy[s] = (double)WmY.v[s][0]; fin.syy[s] = (double)WmY.s[s][0];
+                        fin.vx[s] = (double)WmY.v[s][1]; fin.sxy[s] = (double)WmY.s[s][1];
+                        fin.vz[s] = NS == 3 ? (double)WmY.v[s][NS - 1] : 0.0;
+                        fin.syz[s] = NS == 3 ? (double)WmY.s[s][NS - 1] : 0.0;
+                    }
+                    FaultOut fo;
+                    const size_t bi = (size_t)(i - pt.blo0);
+                    const size_t bp = (size_t)pt.bnx;
+                    const double* B = pt.bin;
+                    double* Bo = pt.bout;
+                    const double tnew = time + dt;
+                    if (P.fault_kind == 0) {
+                        rs_fault(P, i, tmp_state, fin, fo);
+                        // RSSETBND final (RSIBCF.ChF:61-151) on m_bdryData
+                        const double x = (i + 0.5) * P.dx - P.x0;
+                        double tau_xy_0 = P.nsig * gaussianfunc(P.r0, x);
+                        if (P.f_tau) tau_xy_0 = tau_xy_0 + face_val(P.f_tau, P, i, 0.0);
+                        const double Vx = fo.vx[0] - fo.vx[1];
+                        const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
+                        const double V = sqrt(Vx * Vx + Vz * Vz);
+                        const double dX = B[bi + 6 * bp], dZ = B[bi + 7 * bp], dT = B[bi + 8 * bp];
+                        const double RT = B[bi + 9 * bp], psi = B[bi + 10 * bp];
+                        Bo[bi + 0 * bp] = Vx;
+                        Bo[bi + 1 * bp] = Vz;
+                        Bo[bi + 2 * bp] = V;
+                        Bo[bi + 3 * bp] = fo.sxy[0] + tau_xy_0;
+                        Bo[bi + 4 * bp] = (NS == 3 ? fo.syz[0] : 0.0) + 0.0;
+                        Bo[bi + 6 * bp] = dX + dt * Vx;
+                        Bo[bi + 7 * bp] = dZ + dt * Vz;
+                        Bo[bi + 8 * bp] = dT + dt * V;
+                        if (RT > tnew && V > P.rs_thr) Bo[bi + 9 * bp] = tnew;
+                        else if (RT > tnew) Bo[bi + 9 * bp] = 1e10;
+                        else Bo[bi + 9 * bp] = RT;
+                        Bo[bi + 5 * bp] = fo.syy[0];
+                        const double np = rs_psi_update(P, i, psi, V, dt);
+                        if (isnan(np)) report(errp, TMK_E_PSI_NAN);
+                        Bo[bi + 10 * bp] = np;
+                    } else {
+                        sw_fault(P, i, tmp_state, fin, fo);
+                        // SWSETBND final (SWIBCF.ChF:32-72) on m_bdryData (Bout aliases Bin)
+                        const double Vx = fo.vx[0] - fo.vx[1];
+                        const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
+                        const double b5 = B[bi + 5 * bp], b6 = B[bi + 6 * bp], b7 = B[bi + 7 * bp];
+                        const double b8 = B[bi + 8 * bp];
+                        const double V = sqrt(Vx * Vx + Vz * Vz);
+                        Bo[bi + 0 * bp] = Vx;
+                        Bo[bi + 1 * bp] = Vz;
+                        Bo[bi + 2 * bp] = fo.sxy[0];
+                        Bo[bi + 3 * bp] = NS == 3 ? fo.syz[0] : 0.0;
+                        Bo[bi + 4 * bp] = V;
+                        Bo[bi + 5 * bp] = b5 + dt * Vx;
+                        Bo[bi + 6 * bp] = b6 + dt * Vz;
+                        Bo[bi + 7 * bp] = b7 + dt * V;
+                        Bo[bi + 8 * bp] = (b8 > tnew && V > P.sw_thr) ? tnew : b8;
+                    }
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        F1.v[s][0] = (T)fo.vy[s]; F1.s[s][0] = (T)fo.syy[s];
+                        F1.v[s][1] = (T)fo.vx[s]; F1.s[s][1] = (T)fo.sxy[s];
+                        if (NS == 3) { F1.v[s][NS - 1] = (T)fo.vz[s]; F1.s[s][NS - 1] = (T)fo.syz[s]; }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) wpL[s][k] = get(2 * NS + s * NS + k, tid - BX);
+                if (j == dhi1 + 1) {
+                    bc_state<T, NS>(P, m, 1, rv1, P.bc[3], +1, wpL, F1);
+                } else {
+                    left_chars<T, NS>(m, rv1, WmY, wmR);
+                    riemann_state<T, NS>(m, rv1, wpL, wmR, F1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---------------- stage 6: flux difference and update ----------------
+    {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                put(s * NS + k, F0.v[s][k]);
+                put(2 * NS + s * NS + k, F0.s[s][k]);
+                put(4 * NS + s * NS + k, F1.v[s][k]);
+                put(6 * NS + s * NS + k, F1.s[s][k]);
+            }
+        __syncthreads();
+        // GETFLUXF (LinElastPhysicsF.ChF:303-350) from a face state, in slot form:
+        // f[s][0]=F(vn) f[s][1]=F(snn) f[s][2]=F(vt) f[s][3]=F(snt) f[s][4]=F(s_transverse normal)
+        // f[s][5]=F(va) f[s][6]=F(sna)
+        auto flux = [&](const Slots<T, NS>& S, T rv, T (&f)[2][7]) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const T r = s ? rv : (T)1;
+                if (s == 0) {
+                    f[s][0] = -(S.s[s][0] / m.rho);
+                    f[s][2] = -(S.s[s][1] / m.rho);
+                    f[s][1] = -(S.v[s][0] * m.lam2G);
+                    f[s][4] = -(S.v[s][0] * m.lam);
+                    f[s][3] = -(S.v[s][1] * m.G);
+                    if (NS == 3) { f[s][5] = -(S.s[s][NS - 1] / m.rho); f[s][6] = -(S.v[s][NS - 1] * m.G); }
+                } else {
+                    f[s][0] = -(r * S.s[s][0] / m.rho);
+                    f[s][2] = -(r * S.s[s][1] / m.rho);
+                    f[s][1] = -(r * S.v[s][0] * m.lam2G);
+                    f[s][4] = -(r * S.v[s][0] * m.lam);
+                    f[s][3] = -(r * S.v[s][1] * m.G);
+                    if (NS == 3) { f[s][5] = -(r * S.s[s][NS - 1] / m.rho); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
+                }
+            }
+        };
+        // optional flux output for flux registers (LinElastPatchGodunov.cpp:117-118, LinElastPhysics.cpp:153)
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) {
+            T f[2][7];
+            flux(F0, (T)1, f);
+            double* Fo = pt.F[0];
+            const int fnx = pt.vhi0 - pt.vlo0 + 2, fny = pt.vhi1 - pt.vlo1 + 1;
+            const size_t fp = (size_t)fnx * fny;
+            const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int o = 11 * s;
+                Fo[fi + (o + 0) * fp] = (double)f[s][0];
+                Fo[fi + (o + 1) * fp] = (double)f[s][2];
+                Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : -(s ? 1.0 : 1.0) * 0.0 * 0.0;
+                Fo[fi + (o + 3) * fp] = (double)f[s][1];
+                Fo[fi + (o + 4) * fp] = (double)f[s][4];
+                Fo[fi + (o + 5) * fp] = (double)f[s][4];
+                Fo[fi + (o + 6) * fp] = (double)f[s][3];
+                Fo[fi + (o + 7) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
+                Fo[fi + (o + 8) * fp] = 0.0;
+                Fo[fi + (o + 9) * fp] = 0.0;
+                Fo[fi + (o + 10) * fp] = 0.0;
+            }
+        }
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) {
+            T f[2][7];
+            flux(F1, rv1, f);
+            double* Fo = pt.F[1];
+            const int fnx = pt.vhi0 - pt.vlo0 + 1, fny = pt.vhi1 - pt.vlo1 + 2;
+            const size_t fp = (size_t)fnx * fny;
+            const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int o = 11 * s;
+                Fo[fi + (o + 1) * fp] = (double)f[s][0];
+                Fo[fi + (o + 0) * fp] = (double)f[s][2];
+                Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : 0.0;
+                Fo[fi + (o + 4) * fp] = (double)f[s][1];
+                Fo[fi + (o + 5) * fp] = (double)f[s][4];
+                Fo[fi + (o + 3) * fp] = (double)f[s][4];
+                Fo[fi + (o + 6) * fp] = (double)f[s][3];
+                Fo[fi + (o + 8) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
+                Fo[fi + (o + 7) * fp] = 0.0;
+                Fo[fi + (o + 9) * fp] = 0.0;
+                Fo[fi + (o + 10) * fp] = 0.0;
+            }
+        }
+        if (valid) {
+            Slots<T, NS> F0h, F1h;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    F0h.v[s][k] = get(s * NS + k, tid + 1);
+                    F0h.s[s][k] = get(2 * NS + s * NS + k, tid + 1);
+                    F1h.v[s][k] = get(4 * NS + s * NS + k, tid + BX);
+                    F1h.s[s][k] = get(6 * NS + s * NS + k, tid + BX);
+                }
+            T fxl[2][7], fxh[2][7], fyl[2][7], fyh[2][7];
+            flux(F0, (T)1, fxl);
+            flux(F0h, (T)1, fxh);
+            flux(F1, rv1, fyl);
+            flux(F1h, rv1, fyh);
+            const T sc = (T)(-dt / P.dx);
+            const size_t u = uidx(pt, i, j, 0);
+            // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
+            auto upd = [&](int c, T dx_, T dy_) {
+                const T dU = ((T)0 + dx_) + dy_;
+                pt.out[c][u] = pt.in[c][u] + dU * sc;
+            };
+            T st[2][6];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int o = 11 * s;
+                // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
+                // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
+                const T dvx_x = fxh[s][0] - fxl[s][0], dvx_y = fyh[s][2] - fyl[s][2];
+                const T dvy_x = fxh[s][2] - fxl[s][2], dvy_y = fyh[s][0] - fyl[s][0];
+                const T dsxx_x = fxh[s][1] - fxl[s][1], dsxx_y = fyh[s][4] - fyl[s][4];
+                const T dsyy_x = fxh[s][4] - fxl[s][4], dsyy_y = fyh[s][1] - fyl[s][1];
+                const T dszz_x = fxh[s][4] - fxl[s][4], dszz_y = fyh[s][4] - fyl[s][4];
+                const T dsxy_x = fxh[s][3] - fxl[s][3], dsxy_y = fyh[s][3] - fyl[s][3];
+                upd(o + 0, dvx_x, dvx_y);
+                upd(o + 1, dvy_x, dvy_y);
+                const T sxx = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
+                const T syy = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
+                const T szz = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
+                const T sxy = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
+                st[s][0] = sxx; st[s][1] = syy; st[s][2] = szz; st[s][3] = sxy;
+                if (NS == 3) {
+                    const T dvz_x = fxh[s][5] - fxl[s][5], dvz_y = fyh[s][5] - fyl[s][5];
+                    const T dsxz_x = fxh[s][6] - fxl[s][6], dsxz_y = (T)0 - (T)0;
+                    const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[s][6] - fyl[s][6];
+                    upd(o + 2, dvz_x, dvz_y);
+                    st[s][4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
+                    st[s][5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
+                } else {
+                    st[s][4] = (T)0;
+                    st[s][5] = (T)0;
+                }
+            }
+            if (P.plast && a.plastic) {
+                // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int o = 11 * s;
+                    T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
+                    T sxz = NS == 3 ? st[s][4] : pt.in[o + 7][u];
+                    T syz = NS == 3 ? st[s][5] : pt.in[o + 8][u];
+                    plastic_flow<T>(P, (T)dt, st[s][0], st[s][1], st[s][2], st[s][3], sxz, syz, lam, gam);
+                    st[s][4] = sxz; st[s][5] = syz;
+                    pt.out[o + 9][u] = lam;
+                    pt.out[o + 10][u] = gam;
+                    if (NS == 2) { pt.out[o + 7][u] = sxz; pt.out[o + 8][u] = syz; }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int o = 11 * s;
+                pt.out[o + 3][u] = st[s][0];
+                pt.out[o + 4][u] = st[s][1];
+                pt.out[o + 5][u] = st[s][2];
+                pt.out[o + 6][u] = st[s][3];
+                if (NS == 3) { pt.out[o + 7][u] = st[s][4]; pt.out[o + 8][u] = st[s][5]; }
+            }
+        }
+    }
+}
+
+// LinElastPatchGodunov::plasticUpdate on its own (PLASTICCORRECTION over the valid box)
+template <typename T>
+__global__ void k_plastic(const DevParams P, T* U, int ulo0, int ulo1, int unx, size_t plane, int vlo0, int vlo1,
+                          int vnx, int vny, double dt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= vnx * vny) return;
+    const int i = vlo0 + t % vnx, j = vlo1 + t / vnx;
+    const size_t u = (size_t)(i - ulo0) + (size_t)(j - ulo1) * unx;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int o = 11 * s;
+        T sxx = U[u + (o + 3) * plane], syy = U[u + (o + 4) * plane], szz = U[u + (o + 5) * plane];
+        T sxy = U[u + (o + 6) * plane], sxz = U[u + (o + 7) * plane], syz = U[u + (o + 8) * plane];
+        T lam = U[u + (o + 9) * plane], gam = U[u + (o + 10) * plane];
+        plastic_flow<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
+        U[u + (o + 3) * plane] = sxx; U[u + (o + 4) * plane] = syy; U[u + (o + 5) * plane] = szz;
+        U[u + (o + 6) * plane] = sxy; U[u + (o + 7) * plane] = sxz; U[u + (o + 8) * plane] = syz;
+        U[u + (o + 9) * plane] = lam; U[u + (o + 10) * plane] = gam;
+    }
+}
+
+// same-level ghost exchange (LevelData::exchange, LinElastLevelGodunov.cpp:222-223):
+// copy rectangles of valid cells of a source box into the ghost cells of a destination box
+struct CopyRect {
+    int src, dst;      // patch indices
+    int lo0, lo1, hi0, hi1;
+};
+
+template <typename T>
+__global__ void k_exchange(const Patch<T>* patches, const CopyRect* rects, int nrects, int ncomp_mask)
+{
+    const CopyRect r = rects[blockIdx.y];
+    const Patch<T>& S = patches[r.src];
+    const Patch<T>& D = patches[r.dst];
+    const int nx = r.hi0 - r.lo0 + 1, ny = r.hi1 - r.lo1 + 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nx * ny; t += gridDim.x * blockDim.x) {
+        const int i = r.lo0 + t % nx, j = r.lo1 + t / nx;
+        const size_t si = (size_t)(i - S.ulo0) + (size_t)(j - S.ulo1) * S.unx;
+        const size_t di = (size_t)(i - D.ulo0) + (size_t)(j - D.ulo1) * D.unx;
+        for (int c = 0; c < 22; ++c)
+            if ((ncomp_mask >> c) & 1) D.out[c][di] = S.out[c][si];
+    }
+}
+
+// fault-data ghost exchange along x (m_B.exchange, LinElastLevelGodunov.cpp:223)
+template <typename T>
+__global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int nrects, int nb)
+{
+    const CopyRect r = rects[blockIdx.y];
+    const Patch<T>& S = patches[r.src];
+    const Patch<T>& D = patches[r.dst];
+    const int nx = r.hi0 - r.lo0 + 1;
+    for (int t = threadIdx.x; t < nx * nb; t += blockDim.x) {
+        const int i = r.lo0 + t % nx, c = t / nx;
+        D.bout[(size_t)(i - D.blo0) + (size_t)c * D.bnx] = S.bout[(size_t)(i - S.blo0) + (size_t)c * S.bnx];
+    }
+}
+
+// every scenario's clock advances by its own step (AMRLevelLinElast.cpp:478, m_time += m_dt)
+__global__ void k_advance_time(Scen* s, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) s[k].time = s[k].time + s[k].dt;
+}
+
+}  // namespace tmk
+

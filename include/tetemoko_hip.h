@@ -129,6 +129,11 @@ int tmk_level_destroy(tmk_level *lev);
  * (x in [lo-4, hi+4], j = 0, nbdry comps; NULL if the box has none) to/from the device */
 int tmk_level_upload(tmk_level *lev, int ibox, const double *U, const double *B);
 int tmk_level_download(tmk_level *lev, int ibox, double *U, double *B);
+/* fill one box's ghosted state with a uniform 22-component state (LEINITF,
+ * LinElastPhysicsF.ChF:1215-1263) on the device and upload its fault data */
+int tmk_level_fill(tmk_level *lev, int ibox, const double *state22, const double *B);
+/* copy one box's fault data (x in [lo-4, hi+4], nbdry comps) to the host */
+int tmk_level_download_fault(tmk_level *lev, int ibox, double *B);
 /* LinElastLevelGodunov::step (LinElastLevelGodunov.cpp:174-398) followed by
  * LinElastLevelGodunov::plasticUpdate (:643-667) when plasticity is on:
  * same-level ghost exchange, one launch over every box of the level, fault data

@@ -34,7 +34,7 @@ PREC_FP32_BULK = 1
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
     "tmk_level_footprint", "tmk_batch_create", "tmk_batch_destroy", "tmk_batch_add", "tmk_batch_remove",
-    "tmk_batch_run", "tmk_batch_sync",
+    "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill", "tmk_level_download_fault",
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
@@ -132,6 +132,8 @@ def lib():
         L.tmk_batch_remove.argtypes = [vp, vp]
         L.tmk_batch_run.argtypes = [vp, c_int]
         L.tmk_batch_sync.argtypes = [vp]
+        L.tmk_level_fill.argtypes = [vp, c_int, _PD, _PD]
+        L.tmk_level_download_fault.argtypes = [vp, c_int, _PD]
         _lib = L
     return _lib
 
@@ -281,6 +283,14 @@ class Level:
 
     def download(self, ibox, U, B=None):
         _check(lib().tmk_level_download(self.h, ibox, _ptr(U), _ptr(B)), "tmk_level_download")
+
+    def fill(self, ibox, state22, B=None):
+        """Uniform state on the device (no host state array) plus the box's fault data."""
+        st = np.ascontiguousarray(state22, dtype=np.float64)
+        _check(lib().tmk_level_fill(self.h, ibox, _ptr(st), _ptr(B)), "tmk_level_fill")
+
+    def download_fault(self, ibox, B):
+        _check(lib().tmk_level_download_fault(self.h, ibox, _ptr(B)), "tmk_level_download_fault")
 
     def step(self, time, dt):
         dtn = ctypes.c_double()

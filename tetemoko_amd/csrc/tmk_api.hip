@@ -391,7 +391,7 @@ static int group_enqueue_step_t(Group& G, hipStream_t st)
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     if (G.nbrects > 0) {
-        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects, G.nb);
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects);
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     hipLaunchKernelGGL(k_advance_time, dim3((G.nscen + 63) / 64), dim3(64), 0, st, G.scen_dev, G.nscen);
@@ -417,7 +417,7 @@ static int group_exchange_now_t(Group& G, hipStream_t st)
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     if (G.nbrects > 0) {
-        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects, G.nb);
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects);
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     return TMK_OK;
@@ -588,7 +588,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         pt.ulo0 = ub.lo[0]; pt.ulo1 = ub.lo[1]; pt.unx = (int)unx;
         pt.bin = B ? Bd : nullptr;
         pt.bout = B ? Bd + nB : nullptr;
-        pt.blo0 = bb.lo[0]; pt.bnx = (int)bnx;
+        pt.blo0 = bb.lo[0]; pt.bnx = (int)bnx; pt.bnb = nb;
         pt.F[0] = F0 ? Fd : nullptr;
         pt.F[1] = F1 ? Fd + nF0 : nullptr;
         pt.vlo0 = valid.lo[0]; pt.vlo1 = valid.lo[1]; pt.vhi0 = valid.hi[0]; pt.vhi1 = valid.hi[1];
@@ -711,6 +711,7 @@ static void level_fill_patches(tmk_level* L)
                 p.bout = L->bbuf[1 - cur][b];
                 p.blo0 = v.lo[0] - TMK_GHOST;
                 p.bnx = v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST;
+                p.bnb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
             }
             p.vlo0 = v.lo[0]; p.vlo1 = v.lo[1]; p.vhi0 = v.hi[0]; p.vhi1 = v.hi[1];
             p.scen = 0;
@@ -850,6 +851,53 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
         for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->bbuf[k][b], B, nB * sizeof(double), hipMemcpyHostToDevice));
     }
     L->dirty = true;
+    return TMK_OK;
+}
+
+template <typename T>
+static int level_fill_t(tmk_level* L, int b, const double* state)
+{
+    const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
+    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    StateVec v;
+    for (int c = 0; c < TMK_NCOMP; ++c) v.c[c] = state[c];
+    const int nblk = (int)std::min<size_t>((plane + 255) / 256, 4096);
+    void* bufs[3] = {L->ubuf[0][b], L->ubuf[1][b], L->upt[b]};
+    for (void* q : bufs) {
+        hipLaunchKernelGGL(k_fill<T>, dim3(nblk), dim3(256), 0, L->ctx->stream, (T*)q, plane, v);
+        HIPCHK(hipGetLastError());
+    }
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_fill(tmk_level* L, int b, const double* state, const double* B)
+{
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !state || L->batch) return TMK_E_ARG;
+    if (L->ns == 2 && (state[2] != 0.0 || state[7] != 0.0 || state[8] != 0.0 || state[13] != 0.0 ||
+                       state[18] != 0.0 || state[19] != 0.0))
+        return TMK_E_ARG;
+    int e = L->tsize == sizeof(double) ? level_fill_t<double>(L, b, state) : level_fill_t<float>(L, b, state);
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    if (L->has_b[b]) {
+        if (!B) return TMK_E_NO_BDRY;
+        const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
+        for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->bbuf[k][b], B, nB * sizeof(double), hipMemcpyHostToDevice));
+    }
+    L->dirty = true;
+    return TMK_OK;
+}
+
+/* fault data only (no state) of one box, e.g. for result records of a running scenario */
+extern "C" int tmk_level_download_fault(tmk_level* L, int b, double* B)
+{
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !B || L->batch) return TMK_E_ARG;
+    if (!L->has_b[b]) return TMK_E_NO_BDRY;
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
+    HIPCHK(hipMemcpy(B, L->bbuf[L->cur][b], nB * sizeof(double), hipMemcpyDeviceToHost));
     return TMK_OK;
 }
 
@@ -1073,8 +1121,7 @@ extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
     if (!B || !L || L->batch) return TMK_E_ARG;
     if (!B->members.empty()) {
         tmk_level* L0 = B->members[0];
-        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by ||
-            L0->ctx->hp.fault_kind != L->ctx->hp.fault_kind)
+        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by)
             return TMK_E_ARG;
     }
     HIPCHK(hipStreamSynchronize(L->ctx->stream));

@@ -39,9 +39,14 @@ struct Patch {
     const double* bin; // fault data read (NULL: box has none)
     double* bout;      // fault data written
     int blo0, bnx;     // fault-data x lo and extent
+    int bnb;           // fault-data components (11 rate-and-state, 9 slip weakening)
     double* F[2];      // optional face fluxes (22 comps, on surroundingNodes(box,d)) or NULL
     int vlo0, vlo1, vhi0, vhi1; // valid box
     int scen;          // index of the scenario (parameters, time, step) this box belongs to
+};
+
+struct StateVec {
+    double c[22];
 };
 
 // one scenario's parameters, clock and error word; device resident so that
@@ -649,12 +654,13 @@ __global__ void k_exchange(const Patch<T>* patches, const CopyRect* rects, int n
 
 // fault-data ghost exchange along x (m_B.exchange, LinElastLevelGodunov.cpp:223)
 template <typename T>
-__global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int nrects, int nb)
+__global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int nrects)
 {
     const CopyRect r = rects[blockIdx.y];
     const Patch<T>& S = patches[r.src];
     const Patch<T>& D = patches[r.dst];
     const int nx = r.hi0 - r.lo0 + 1;
+    const int nb = D.bnb;
     for (int t = threadIdx.x; t < nx * nb; t += blockDim.x) {
         const int i = r.lo0 + t % nx, c = t / nx;
         D.bout[(size_t)(i - D.blo0) + (size_t)c * D.bnx] = S.bout[(size_t)(i - S.blo0) + (size_t)c * S.bnx];
@@ -662,6 +668,14 @@ __global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int 
 }
 
 // every scenario's clock advances by its own step (AMRLevelLinElast.cpp:478, m_time += m_dt)
+// fill every plane of a box with a uniform state (LEINITF, LinElastPhysicsF.ChF:1215-1263)
+template <typename T>
+__global__ void k_fill(T* U, size_t plane, StateVec v)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (size_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 22; ++c) U[q + c * plane] = (T)v.c[c];
+}
+
 __global__ void k_advance_time(Scen* s, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;

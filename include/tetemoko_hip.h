@@ -151,6 +151,22 @@ int tmk_level_profile(tmk_level *lev, int nsteps, double time, double dt, double
 /* synchronize and return the first error any launch reported */
 int tmk_level_sync(tmk_level *lev);
 
+/* Refinement tagging of a level's current state: AMRLevelLinElast::tagCells
+ * (AMRLevelLinElast.cpp:577-717) with the IBC nucleation override
+ * (RSIBC.cpp:341-377, SWIBC.cpp:336-396), tag buffer growth and domain clip.
+ * stress_thresh / sliprate_thresh > 0 add the stress-gradient (sxx, syy, sxy)
+ * and fault slip-rate criteria (extensions); <= 0 disables them.
+ * tags: (domain nx) * (domain ny) bytes, x fastest, 1 = tagged. */
+typedef struct {
+    double refine_thresh, plastic_thresh;
+    int tag_buffer;
+    double x_coarsen[2], y_coarsen[2], slope_coarsen, width_coarsen;
+    double x_coarsen2[2], y_coarsen2[2];
+    int level, coarsen2_max_level;
+    double stress_thresh, sliprate_thresh;
+} tmk_tagparams;
+int tmk_level_tag(tmk_level *lev, const tmk_tagparams *tp, unsigned char *tags);
+
 /* Batches: the boxes of every member level (one scenario each, own parameters,
  * clock and step) are updated in one shared fused launch per step.  Members
  * join and leave between steps (continuous batching); a member's results are

@@ -1287,3 +1287,86 @@ int tmo_level_steps(double *U, double *B, const tmo_params *p, int nx, int ny,
     const int mb = nx > ny ? nx : ny;
     return tmo_level_steps_boxed(U, B, p, nx, ny, time, dt, nsteps, mb, nthreads);
 }
+
+/* ---------------- refinement tagging ---------------- */
+
+int tmo_tag_level(const double *U, const double *B, const tmo_params *p, const tmo_tagparams *tp,
+                  int nx, int ny, unsigned char *tags)
+{
+    const int g = 4;
+    const int LX = nx + 2 * g, LY = ny + 2 * g;
+    tmo_fab Uf;
+    Uf.d = (double *)U;
+    Uf.b = bx(-g, -g, nx - 1 + g, ny - 1 + g);
+    Uf.nc = TMO_NCOMP;
+    (void)LY;
+    const tmo_box dom = bx(0, 0, nx - 1, ny - 1);
+    tmo_fab grad;
+    fab_new(&grad, dom, 1, 0.0);
+    tmo_fullgrad(&grad, &Uf, dom, dom);
+    /* IBC override of the gradient magnitude on the nucleation cells (RSIBC.cpp:351-368,
+     * SWIBC.cpp:341-372) */
+    if (p->fault_kind == 0) {
+        const int i0 = (int)floor(p->x0 / p->dx), i1 = (int)ceil(p->x0 / p->dx);
+        for (int i = i0; i <= i1; ++i)
+            if (i >= 0 && i < nx) *AT(&grad, i, 0, 0) = 1.0;
+    } else {
+        for (int k = 0; k < p->num_patches; ++k) {
+            const int i0 = (int)floor((p->fric_center[0] + p->xc[k] - p->xw[k]) / p->dx);
+            const int i1 = (int)ceil((p->fric_center[0] + p->xc[k] + p->xw[k]) / p->dx);
+            for (int i = i0; i <= i1; ++i)
+                if (i >= 0 && i < nx) *AT(&grad, i, 0, 0) = 1.0;
+        }
+    }
+    unsigned char *raw = (unsigned char *)calloc((size_t)nx * ny, 1);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double r = 0.0;
+            const double x = i * p->dx;
+            r = fmax(r, fmax(tp->x_coarsen[0] - x, x - tp->x_coarsen[1]));
+            const double y = j * p->dx;
+            r = fmax(r, fmax(tp->y_coarsen[0] - y, y - tp->y_coarsen[1]));
+            const double thr = tp->refine_thresh * (1 + r * tp->slope_coarsen / tp->width_coarsen);
+            if (!(tp->level < tp->coarsen2_max_level ||
+                  (x >= tp->x_coarsen2[0] && x <= tp->x_coarsen2[1] && y >= tp->y_coarsen2[0] &&
+                   y <= tp->y_coarsen2[1])))
+                continue;
+            const double *u = U + (size_t)(i + g) + (size_t)(j + g) * LX;
+            const size_t pl = (size_t)LX * LY;
+            int t = *AT(&grad, i, j, 0) >= thr || u[10 * pl] >= tp->plastic_thresh || u[21 * pl] >= tp->plastic_thresh;
+            if (!t && tp->stress_thresh > 0.0) {
+                double gs = 0.0;
+                for (int d = 0; d < 2; ++d) {
+                    const int io = d == 0, jo = d == 1;
+                    const int ci = d == 0 ? i : j;
+                    for (int c = 3; c <= 6; ++c) {
+                        if (c == 5) continue;   /* sxx, syy, sxy */
+                        double v;
+                        if (ci == dom.lo[d]) v = fabs(*AT(&Uf, i + io, j + jo, c) - *AT(&Uf, i, j, c));
+                        else if (ci == dom.hi[d]) v = fabs(*AT(&Uf, i, j, c) - *AT(&Uf, i - io, j - jo, c));
+                        else v = 0.5 * fabs(*AT(&Uf, i + io, j + jo, c) - *AT(&Uf, i - io, j - jo, c));
+                        gs = fmax(gs, v);
+                    }
+                }
+                t = gs >= tp->stress_thresh;
+            }
+            if (!t && tp->sliprate_thresh > 0.0 && j == 0 && B) {
+                const int vc = p->fault_kind == 0 ? 2 : 4;
+                t = B[(size_t)(i + g) + (size_t)vc * LX] >= tp->sliprate_thresh;
+            }
+            raw[(size_t)i + (size_t)j * nx] = (unsigned char)t;
+        }
+    /* IntVectSet::grow(tag_buffer), then clip to the domain */
+    const int b = tp->tag_buffer;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            unsigned char t = 0;
+            for (int jj = j - b; jj <= j + b && !t; ++jj)
+                for (int ii = i - b; ii <= i + b && !t; ++ii)
+                    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) t = raw[(size_t)ii + (size_t)jj * nx];
+            tags[(size_t)i + (size_t)j * nx] = t;
+        }
+    free(raw);
+    fab_free(&grad);
+    return 0;
+}

@@ -121,6 +121,22 @@ int tmo_update_state(tmo_fab *U, tmo_fab *B, tmo_fab *F0, tmo_fab *F1,
 /* AMRLevelLinElast tagging gradient: GETFULLGRADF/MAXF in both directions */
 void tmo_fullgrad(tmo_fab *grad, const tmo_fab *U, tmo_box box, tmo_box domain);
 
+/* Refinement tagging (AMRLevelLinElast::tagCells, AMRLevelLinElast.cpp:577-717, and the IBC
+ * override RSIBC::tagCellsInit / SWIBC::tagCellsInit) on a uniform level: U ghosted by 4,
+ * B the level's fault data; tags is nx*ny bytes, x fastest.  stress_thresh and
+ * sliprate_thresh (> 0) add the stress-gradient and fault slip-rate criteria (extensions). */
+typedef struct {
+    /* stress_thresh: undivided difference of sxx, syy, sxy (side 1); sliprate_thresh: fault V */
+    double refine_thresh, plastic_thresh;
+    int tag_buffer;
+    double x_coarsen[2], y_coarsen[2], slope_coarsen, width_coarsen;
+    double x_coarsen2[2], y_coarsen2[2];
+    int level, coarsen2_max_level;
+    double stress_thresh, sliprate_thresh;
+} tmo_tagparams;
+int tmo_tag_level(const double *U, const double *B, const tmo_params *p, const tmo_tagparams *tp,
+                  int nx, int ny, unsigned char *tags);
+
 /* Uniform-grid level driver used by the tests and the CPU baseline:
  * nx x ny valid cells at (0,0), a single box, U ghosted by 4, B with 4 x-ghosts. */
 int tmo_level_steps(double *U, double *B, const tmo_params *p, int nx, int ny,

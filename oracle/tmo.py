@@ -67,6 +67,13 @@ class Params(ctypes.Structure):
     ]
 
 
+class TagParams(ctypes.Structure):
+    _fields_ = [("refine_thresh", _D), ("plastic_thresh", _D), ("tag_buffer", ctypes.c_int),
+                ("x_coarsen", _D * 2), ("y_coarsen", _D * 2), ("slope_coarsen", _D), ("width_coarsen", _D),
+                ("x_coarsen2", _D * 2), ("y_coarsen2", _D * 2), ("level", ctypes.c_int),
+                ("coarsen2_max_level", ctypes.c_int), ("stress_thresh", _D), ("sliprate_thresh", _D)]
+
+
 _lib = None
 
 
@@ -106,6 +113,9 @@ def lib():
         L.tmo_level_steps_boxed.argtypes = [_PD, _PD, PP, ctypes.c_int, ctypes.c_int, _D, _D,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.tmo_level_steps_boxed.restype = ctypes.c_int
+        L.tmo_tag_level.argtypes = [_PD, _PD, PP, ctypes.POINTER(TagParams), ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_ubyte)]
+        L.tmo_tag_level.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -226,3 +236,18 @@ def level_steps(op, U, B, time, dt, nsteps, nthreads=1, maxbox=None):
                                  time, dt, nsteps, nthreads)
     return L.tmo_level_steps_boxed(U.ctypes.data_as(_PD), B.ctypes.data_as(_PD), op.ref, nx, ny,
                                    time, dt, nsteps, maxbox, nthreads)
+
+
+def tag_level(op, U, B, tp_fields):
+    """Tags (ny, nx) uint8 of a uniform level; tp_fields: dict of TagParams fields."""
+    tp = TagParams()
+    for k, v in tp_fields.items():
+        if isinstance(v, (tuple, list)):
+            getattr(tp, k)[:] = v
+        else:
+            setattr(tp, k, v)
+    nx, ny = op.cfg["nx"], op.cfg["ny"]
+    out = np.zeros((ny, nx), dtype=np.uint8)
+    assert lib().tmo_tag_level(U.ctypes.data_as(_PD), B.ctypes.data_as(_PD), op.ref, ctypes.byref(tp), nx, ny,
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))) == 0
+    return out

@@ -337,3 +337,39 @@ def test_mem_info_and_footprint():
     n = capi.level_footprint(ctx, [(0, 0, 95, 47)])
     assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + 2 * (96 + 8) * 11 * 8
     ctx.close()
+
+
+@pytest.mark.parametrize("fault", ["rs", "sw"])
+@pytest.mark.parametrize("maxbox", [None, 16])
+def test_tags_bit_exact(fault, maxbox):
+    cfg = _cfg(fault, "PPM", nx=64, ny=40)
+    if fault == "sw":
+        cfg["sw"]["patches"] = [(2000.0, 800.0, 5.0)]
+    cfg["rs"] = cfg.get("rs", {})
+    boxes = [(0, 0, 63, 39)] if maxbox is None else scenarios.split_boxes(64, 40, maxbox)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, 3, seed=12)
+    tps = [dict(refine_thresh=0.4, plastic_thresh=1e-5, tag_buffer=2),
+           dict(refine_thresh=0.3, plastic_thresh=1e-5, tag_buffer=1, x_coarsen=(300.0, 900.0),
+                y_coarsen=(-1.0, 400.0), slope_coarsen=2.0, width_coarsen=0.005 * 1000, level=1,
+                coarsen2_max_level=1, x_coarsen2=(0.0, 1200.0), y_coarsen2=(0.0, 600.0)),
+           dict(refine_thresh=5.0, plastic_thresh=1.0, tag_buffer=0, stress_thresh=4.0, sliprate_thresh=0.5)]
+    # rebuild the level from the oracle's evolved state and tag on the device
+    U = np.full((22, 40 + 2 * G, 64 + 2 * G), 0.0)
+    U[:, G:-G, G:-G] = Ur
+    B = np.zeros((Br.shape[0], 64 + 2 * G))
+    B[:, G:-G] = Br
+    ctx = capi.Context(cfg)
+    lev = capi.Level(ctx, boxes)
+    for k, b in enumerate(boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+    for tpf in tps:
+        full = dict(refine_thresh=0.01, plastic_thresh=1e-5, tag_buffer=2, x_coarsen=(-1e40, 1e40),
+                    y_coarsen=(-1e40, 1e40), slope_coarsen=0.0, width_coarsen=1e40, x_coarsen2=(-1e40, 1e40),
+                    y_coarsen2=(-1e40, 1e40), level=0, coarsen2_max_level=3, stress_thresh=0.0, sliprate_thresh=0.0)
+        full.update(tpf)
+        want = tmo.tag_level(tmo.OracleParams(cfg), U, B, full)
+        got = lev.tag(capi.TagParams.make(**full), 64, 40)
+        assert np.array_equal(got, want), (np.sum(got), np.sum(want))
+        assert 0 < np.sum(want) < 64 * 40
+    lev.close()
+    ctx.close()

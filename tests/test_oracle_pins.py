@@ -569,3 +569,33 @@ def test_plane_p_wave_converges_second_order():
         errs.append(np.max(np.abs(U[0, g + 1, g + n // 4: g + 3 * n // 4] - ex)))
     order = np.log2(errs[0] / errs[1])
     assert order > 1.7, errs
+
+
+def test_tagging_rules():
+    """Tag semantics of AMRLevelLinElast::tagCells on hand-made states."""
+    cfg = scenarios.config_c1(seed=1, nx=40, ny=30, dx=100.0)
+    cfg["rs"]["nuc_x"] = 1050.0
+    op = tmo.OracleParams(cfg)
+    U, B = scenarios.level_arrays(cfg)
+    g = scenarios.GHOST
+    base = dict(refine_thresh=0.5, plastic_thresh=1e-5, tag_buffer=0, x_coarsen=(-1e40, 1e40),
+                y_coarsen=(-1e40, 1e40), slope_coarsen=0.0, width_coarsen=1e40, x_coarsen2=(-1e40, 1e40),
+                y_coarsen2=(-1e40, 1e40), level=0, coarsen2_max_level=3, stress_thresh=0.0, sliprate_thresh=0.0)
+    # quiescent: only the nucleation cells floor(x0/dx)..ceil(x0/dx) on row 0 (gradient forced to 1)
+    t = tmo.tag_level(op, U, B, base)
+    assert set(zip(*np.nonzero(t))) == {(0, 10), (0, 11)}
+    # a velocity spike of 2 at (20, 15): centred differences 1 at its four neighbours
+    U[0, 15 + g, 20 + g] = 2.0
+    t = tmo.tag_level(op, U, B, base)
+    assert t[15, 19] and t[15, 21] and t[14, 20] and t[16, 20] and not t[15, 20]
+    # buffer growth: every tagged cell grows to a 3x3 square (the plus of four cells -> its union)
+    t = tmo.tag_level(op, U, B, dict(base, tag_buffer=1))
+    assert t[14:17, 18:23].all() and t[13:18, 19:22].all()
+    assert not t[13, 18] and not t[12, 20] and not t[17, 22]
+    # plastic strain tags regardless of the gradient
+    U[10, 5 + g, 5 + g] = 1e-4
+    t = tmo.tag_level(op, U, B, base)
+    assert t[5, 5]
+    # coarsen region raises the threshold away from it (cell corners x = i dx)
+    t = tmo.tag_level(op, U, B, dict(base, x_coarsen=(0.0, 500.0), slope_coarsen=10.0, width_coarsen=100.0))
+    assert not t[15, 19]

@@ -34,7 +34,7 @@ PREC_FP32_BULK = 1
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
     "tmk_level_footprint", "tmk_batch_create", "tmk_batch_destroy", "tmk_batch_add", "tmk_batch_remove",
-    "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill", "tmk_level_download_fault",
+    "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill", "tmk_level_download_fault", "tmk_level_tag",
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
@@ -87,6 +87,29 @@ class Params(ctypes.Structure):
     ]
 
 
+class TagParams(ctypes.Structure):
+    _fields_ = [("refine_thresh", _D), ("plastic_thresh", _D), ("tag_buffer", ctypes.c_int),
+                ("x_coarsen", _D * 2), ("y_coarsen", _D * 2), ("slope_coarsen", _D), ("width_coarsen", _D),
+                ("x_coarsen2", _D * 2), ("y_coarsen2", _D * 2), ("level", ctypes.c_int),
+                ("coarsen2_max_level", ctypes.c_int), ("stress_thresh", _D), ("sliprate_thresh", _D)]
+
+    @staticmethod
+    def make(refine_thresh=0.01, plastic_thresh=1e-5, tag_buffer=2, x_coarsen=(-1e40, 1e40),
+             y_coarsen=(-1e40, 1e40), slope_coarsen=0.0, width_coarsen=1e40, x_coarsen2=(-1e40, 1e40),
+             y_coarsen2=(-1e40, 1e40), level=0, coarsen2_max_level=3, stress_thresh=0.0, sliprate_thresh=0.0):
+        """Defaults of amrGodunov.cpp:263-450."""
+        t = TagParams()
+        t.refine_thresh, t.plastic_thresh, t.tag_buffer = refine_thresh, plastic_thresh, tag_buffer
+        t.x_coarsen[:] = x_coarsen
+        t.y_coarsen[:] = y_coarsen
+        t.slope_coarsen, t.width_coarsen = slope_coarsen, width_coarsen
+        t.x_coarsen2[:] = x_coarsen2
+        t.y_coarsen2[:] = y_coarsen2
+        t.level, t.coarsen2_max_level = level, coarsen2_max_level
+        t.stress_thresh, t.sliprate_thresh = stress_thresh, sliprate_thresh
+        return t
+
+
 class TmkError(RuntimeError):
     def __init__(self, code, where=""):
         self.code = code
@@ -134,6 +157,7 @@ def lib():
         L.tmk_batch_sync.argtypes = [vp]
         L.tmk_level_fill.argtypes = [vp, c_int, _PD, _PD]
         L.tmk_level_download_fault.argtypes = [vp, c_int, _PD]
+        L.tmk_level_tag.argtypes = [vp, ctypes.POINTER(TagParams), ctypes.POINTER(ctypes.c_ubyte)]
         _lib = L
     return _lib
 
@@ -288,6 +312,13 @@ class Level:
         """Uniform state on the device (no host state array) plus the box's fault data."""
         st = np.ascontiguousarray(state22, dtype=np.float64)
         _check(lib().tmk_level_fill(self.h, ibox, _ptr(st), _ptr(B)), "tmk_level_fill")
+
+    def tag(self, tp, nx, ny):
+        """Refinement tags of the current state, (ny, nx) uint8."""
+        out = np.zeros((ny, nx), dtype=np.uint8)
+        _check(lib().tmk_level_tag(self.h, ctypes.byref(tp), out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))),
+               "tmk_level_tag")
+        return out
 
     def download_fault(self, ibox, B):
         _check(lib().tmk_level_download_fault(self.h, ibox, _ptr(B)), "tmk_level_download_fault")

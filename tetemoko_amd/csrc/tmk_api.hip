@@ -1172,3 +1172,46 @@ extern "C" int tmk_batch_sync(tmk_batch* B)
         if (s.err) return s.err;
     return TMK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// refinement tagging
+// ---------------------------------------------------------------------------
+template <typename T>
+static int level_tag_t(tmk_level* L, const tmk_tagparams* tp, unsigned char* tags)
+{
+    tmk_ctx* c = L->ctx;
+    hipStream_t st = c->stream;
+    const tmk_box dom = c->hp.domain;
+    const int nx = dom.hi[0] - dom.lo[0] + 1, ny = dom.hi[1] - dom.lo[1] + 1;
+    const size_t n = (size_t)nx * ny;
+    int e;
+    if ((e = c->scratch[6].ensure(2 * n))) return e;
+    unsigned char* raw = (unsigned char*)c->scratch[6].p;
+    HIPCHK(hipMemsetAsync(raw, 0, 2 * n, st));
+    TagArgs ta;
+    ta.refine_thresh = tp->refine_thresh; ta.plastic_thresh = tp->plastic_thresh;
+    ta.xc0 = tp->x_coarsen[0]; ta.xc1 = tp->x_coarsen[1]; ta.yc0 = tp->y_coarsen[0]; ta.yc1 = tp->y_coarsen[1];
+    ta.slope = tp->slope_coarsen; ta.width = tp->width_coarsen;
+    ta.x2c0 = tp->x_coarsen2[0]; ta.x2c1 = tp->x_coarsen2[1]; ta.y2c0 = tp->y_coarsen2[0]; ta.y2c1 = tp->y_coarsen2[1];
+    ta.level = tp->level; ta.c2max = tp->coarsen2_max_level;
+    ta.stress_thresh = tp->stress_thresh; ta.sliprate_thresh = tp->sliprate_thresh;
+    ta.nx = nx; ta.ny = ny;
+    // patches_dev[g].in hold the current state
+    const Patch<T>* pd = (const Patch<T>*)L->own.patches_dev[L->own.g];
+    hipLaunchKernelGGL(k_tag<T>, dim3(64, (unsigned)L->boxes.size()), dim3(256), 0, st, pd, L->own.scen_dev, ta, raw);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_dilate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, raw + n, nx, ny,
+                       tp->tag_buffer);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(tags, raw + n, n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_tag(tmk_level* L, const tmk_tagparams* tp, unsigned char* tags)
+{
+    if (!L || !tp || !tags || L->batch || tp->tag_buffer < 0) return TMK_E_ARG;
+    int e = level_prepare(L, L->time, L->dt);
+    if (e) return e;
+    return L->tsize == sizeof(double) ? level_tag_t<double>(L, tp, tags) : level_tag_t<float>(L, tp, tags);
+}

@@ -676,6 +676,96 @@ __global__ void k_fill(T* U, size_t plane, StateVec v)
         for (int c = 0; c < 22; ++c) U[q + c * plane] = (T)v.c[c];
 }
 
+// refinement tags (AMRLevelLinElast::tagCells, AMRLevelLinElast.cpp:577-717; RSIBC::tagCellsInit
+// RSIBC.cpp:341-377; SWIBC::tagCellsInit SWIBC.cpp:336-396) on the current state of a level
+struct TagArgs {
+    double refine_thresh, plastic_thresh;
+    double xc0, xc1, yc0, yc1, slope, width;
+    double x2c0, x2c1, y2c0, y2c1;
+    int level, c2max;
+    double stress_thresh, sliprate_thresh;
+    int nx, ny;  // domain extent (lo = P.dlo)
+};
+
+template <typename T>
+__device__ __forceinline__ double undiv(const Patch<T>& p, int c, int i, int j, int d, int dlo, int dhi)
+{
+    const int io = d == 0, jo = d == 1;
+    const int ci = d == 0 ? i : j;
+    const T* q = p.in[c];
+    const size_t u = (size_t)(i - p.ulo0) + (size_t)(j - p.ulo1) * p.unx;
+    const size_t s = d == 0 ? 1 : (size_t)p.unx;
+    (void)io; (void)jo;
+    if (ci == dlo) return fabs((double)q[u + s] - (double)q[u]);
+    if (ci == dhi) return fabs((double)q[u] - (double)q[u - s]);
+    return 0.5 * fabs((double)q[u + s] - (double)q[u - s]);
+}
+
+template <typename T>
+__global__ void k_tag(const Patch<T>* patches, const Scen* scen, TagArgs ta, unsigned char* raw)
+{
+    const Patch<T>& p = patches[blockIdx.y];
+    const DevParams& P = scen[p.scen].P;
+    const int vnx = p.vhi0 - p.vlo0 + 1, vny = p.vhi1 - p.vlo1 + 1;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < vnx * vny; t += gridDim.x * blockDim.x) {
+        const int i = p.vlo0 + t % vnx, j = p.vlo1 + t / vnx;
+        // GETFULLGRADF over the side-1 velocities, then MAXF over directions
+        double gm = 0.0;
+        for (int d = 0; d < 2; ++d) {
+            double g = 0.0;
+            for (int c = 0; c < 3; ++c) g = fmax(g, undiv(p, c, i, j, d, P.dlo[d], P.dhi[d]));
+            gm = fmax(gm, fabs(g));
+        }
+        // IBC override on the nucleation cells of the fault row
+        if (j == P.dlo[1]) {
+            if (P.fault_kind == 0) {
+                const int i0 = (int)floor(P.x0 / P.dx), i1 = (int)ceil(P.x0 / P.dx);
+                if (i >= i0 && i <= i1) gm = 1.0;
+            } else {
+                for (int k = 0; k < P.npatch; ++k) {
+                    const int i0 = (int)floor((P.fric_c0 + P.xc[k] - P.xw[k]) / P.dx);
+                    const int i1 = (int)ceil((P.fric_c0 + P.xc[k] + P.xw[k]) / P.dx);
+                    if (i >= i0 && i <= i1) gm = 1.0;
+                }
+            }
+        }
+        double r = 0.0;
+        const double x = i * P.dx;
+        r = fmax(r, fmax(ta.xc0 - x, x - ta.xc1));
+        const double y = j * P.dx;
+        r = fmax(r, fmax(ta.yc0 - y, y - ta.yc1));
+        const double thr = ta.refine_thresh * (1 + r * ta.slope / ta.width);
+        unsigned char tag = 0;
+        if (ta.level < ta.c2max || (x >= ta.x2c0 && x <= ta.x2c1 && y >= ta.y2c0 && y <= ta.y2c1)) {
+            const size_t u = (size_t)(i - p.ulo0) + (size_t)(j - p.ulo1) * p.unx;
+            tag = gm >= thr || (double)p.in[10][u] >= ta.plastic_thresh || (double)p.in[21][u] >= ta.plastic_thresh;
+            if (!tag && ta.stress_thresh > 0.0) {
+                double gs = 0.0;
+                for (int d = 0; d < 2; ++d)
+                    for (int c : {3, 4, 6}) gs = fmax(gs, undiv(p, c, i, j, d, P.dlo[d], P.dhi[d]));
+                tag = gs >= ta.stress_thresh;
+            }
+            if (!tag && ta.sliprate_thresh > 0.0 && j == P.dlo[1] && p.bin) {
+                const int vc = P.fault_kind == 0 ? 2 : 4;
+                tag = p.bin[(size_t)(i - p.blo0) + (size_t)vc * p.bnx] >= ta.sliprate_thresh;
+            }
+        }
+        raw[(size_t)(i - P.dlo[0]) + (size_t)(j - P.dlo[1]) * ta.nx] = tag;
+    }
+}
+
+// IntVectSet::grow(b) followed by the clip to the domain
+__global__ void k_dilate(const unsigned char* raw, unsigned char* out, int nx, int ny, int b)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int i = t % nx, j = t / nx;
+    unsigned char v = 0;
+    for (int jj = max(0, j - b); jj <= min(ny - 1, j + b) && !v; ++jj)
+        for (int ii = max(0, i - b); ii <= min(nx - 1, i + b) && !v; ++ii) v = raw[(size_t)ii + (size_t)jj * nx];
+    out[t] = v;
+}
+
 __global__ void k_advance_time(Scen* s, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;

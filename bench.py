@@ -47,6 +47,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(nx, ny, tile):
+    """HBM bytes per launch of the FP64 update kernel from the committed counter runs
+    (profiles/traffic_k_godunov_fp64.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    the copy-kernel calibration), if they were taken on this workload and tile shape."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_godunov_fp64.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("nx") != nx or t.get("ny") != ny or t.get("tile") != (list(tile) if tile else "default"):
+        return None
+    return t.get("bytes_per_launch")
+
+
 def run_level(cfg, U, B, precision, warmup, steps, tile):
     ctx = capi.Context(cfg, precision)
     if tile:
@@ -132,7 +146,7 @@ def main():
                    "nx": cfg["nx"], "ny": cfg["ny"], "dx_m": 25.0, "dt_s": scenarios.dt_of(cfg),
                    "normal_predictor": "PPM", "tile": list(tile) if tile else "default"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["nx"], cfg["ny"], tile),
                      "kernel_ms": kms64,
                      "bytes_per_cell_update": LIVE_COMPONENTS * 2 * 8,
                      "bytes_per_cell_update_22comp": ALL_COMPONENTS * 2 * 8,

@@ -25,6 +25,23 @@ def build(verbose=True):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libtetemoko_hip.so")
     os.replace(tmp, out)
+    build_copy_calib(verbose)
+    return out
+
+
+def build_copy_calib(verbose=True):
+    """The copy kernel of known size used to scale HBM byte counters (tetemoko_amd/_lib/copy_calib)."""
+    src = os.path.join(HERE, "csrc", "copy_calib.hip")
+    out = os.path.join(HERE, "_lib", "copy_calib")
+    flags = [f for f in FLAGS if f not in ("-fPIC", "-shared")]
+    cmd = [HIPCC] + flags + ["-o", out + ".tmp", src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building copy_calib")
+    os.replace(out + ".tmp", out)
     return out
 
 

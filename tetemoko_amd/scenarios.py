@@ -143,14 +143,11 @@ def background_vector(cfg):
     return np.array(side + side, dtype=np.float64)
 
 
-def level_arrays(cfg):
-    """Initial state (LEINITF, LinElastPhysicsF.ChF:1215-1263) and fault data
-    (RSIBC::initializeBdry, RSIBC.cpp:115-163; slip-weakening: all zero, rupture 1e10)."""
-    nx, ny = cfg["nx"], cfg["ny"]
-    U = np.empty((NCOMP, ny + 2 * GHOST, nx + 2 * GHOST), dtype=np.float64)
-    U[:] = background_vector(cfg)[:, None, None]
-    nb = NB[cfg["fault"]]
-    B = np.zeros((nb, nx + 2 * GHOST), dtype=np.float64)
+def fault_arrays(cfg):
+    """Initial fault data over the faces with ghosts (RSIBC::initializeBdry, RSIBC.cpp:115-163;
+    slip weakening: all zero, rupture time 1e10)."""
+    nx = cfg["nx"]
+    B = np.zeros((NB[cfg["fault"]], nx + 2 * GHOST), dtype=np.float64)
     if cfg["fault"] == "rs":
         B[9] = 1e10
         psi = cfg.get("faces", {}).get("psi")
@@ -162,7 +159,15 @@ def level_arrays(cfg):
             B[10] = np.asarray(psi)[ii]
     else:
         B[8] = 1e10
-    return U, B
+    return B
+
+
+def level_arrays(cfg):
+    """Initial state (LEINITF, LinElastPhysicsF.ChF:1215-1263) and fault data (fault_arrays)."""
+    nx, ny = cfg["nx"], cfg["ny"]
+    U = np.empty((NCOMP, ny + 2 * GHOST, nx + 2 * GHOST), dtype=np.float64)
+    U[:] = background_vector(cfg)[:, None, None]
+    return U, fault_arrays(cfg)
 
 
 def perturb(U, seed, amp_v=1.0, amp_s=5.0, antiplane=False):

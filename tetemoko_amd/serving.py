@@ -75,23 +75,9 @@ def _start(sc, precision, tile):
         ctx.set_tile(*tile)
     box = (0, 0, cfg["nx"] - 1, cfg["ny"] - 1)
     lev = capi.Level(ctx, [box])
-    _, B = _fault_init(cfg)
+    B = scenarios.fault_arrays(cfg)
     lev.fill(0, scenarios.background_vector(cfg), B)
     return ctx, lev
-
-
-def _fault_init(cfg):
-    nb = scenarios.NB[cfg["fault"]]
-    nx = cfg["nx"]
-    B = np.zeros((nb, nx + 2 * GHOST))
-    if cfg["fault"] == "rs":
-        B[9] = 1e10
-        psi = cfg["faces"].get("psi")
-        ii = np.clip(np.arange(-GHOST, nx + GHOST), 0, nx - 1)
-        B[10] = np.asarray(psi)[ii] if psi is not None else cfg["rs"]["psi"]
-    else:
-        B[8] = 1e10
-    return None, B
 
 
 def _result(r, B):

@@ -78,9 +78,10 @@ def run_level(cfg, U, B, precision, warmup, steps, tile):
     t += steps * dt
     Uo, Bo = np.empty_like(U), np.empty_like(B)
     lev.download(0, Uo, Bo)
-    # per-launch duration of the fused kernel (eager steps, one launch each), measured on the
-    # level's stream; continues the same simulation
-    kms = lev.profile(max(10, steps // 4), t, dt)
+    # per-launch durations of the update kernels (eager steps; interior tiles on the level's
+    # stream, boundary tiles concurrently on its side stream), each from HIP events on the
+    # stream it runs on; continues the same simulation
+    kms = lev.profile_kernels(max(10, steps // 4), t, dt)
     lev.sync()
     lev.close()
     ctx.close()
@@ -126,8 +127,10 @@ def main():
           file=sys.stderr, flush=True)
     ms64, kms64, U64, B64 = run_level(cfg, U0, B0, capi.PREC_FP64, a.warmup, a.steps, tile)
     value = ncell * a.steps / (ms64 * 1e-3)
-    bytes_live = ncell * LIVE_COMPONENTS * 2 * 8
-    achieved = bytes_live / (kms64 * 1e-3) / 1e9
+    # dominant kernel: the interior-tile update; its algorithmic bytes are the live state planes
+    # of the cells it updates, read once and written once
+    bytes_live = kms64["cells_interior"] * LIVE_COMPONENTS * 2 * 8
+    achieved = bytes_live / (kms64["ms_interior"] * 1e-3) / 1e9
     out = {
         "metric": "cell-updates/s/GPU (FP64 plain level step; FP32-bulk reported beside it)",
         "value": value,
@@ -147,10 +150,15 @@ def main():
                    "normal_predictor": "PPM", "tile": list(tile) if tile else "default"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["nx"], cfg["ny"], tile),
-                     "kernel_ms": kms64,
+                     "kernel": "k_godunov_interior (interior tiles)",
+                     "kernel_ms": kms64["ms_interior"],
+                     "cells_per_launch": kms64["cells_interior"],
                      "bytes_per_cell_update": LIVE_COMPONENTS * 2 * 8,
                      "bytes_per_cell_update_22comp": ALL_COMPONENTS * 2 * 8,
-                     "frac_22comp": ncell * ALL_COMPONENTS * 2 * 8 / (kms64 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "frac_22comp": kms64["cells_interior"] * ALL_COMPONENTS * 2 * 8 /
+                                    (kms64["ms_interior"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "boundary_kernel_ms": kms64["ms_boundary"], "boundary_cells": kms64["cells_boundary"],
+                     "step_ms_eager": kms64["ms_step"]},
     }
     if not a.no_fp32:
         print("bench: FP32-bulk leg", file=sys.stderr, flush=True)
@@ -161,9 +169,11 @@ def main():
         out["fp32_bulk"] = {
             "value": ncell * a.steps / (ms32 * 1e-3),
             "ms_per_step": ms32 / a.steps,
-            "kernel_ms": kms32,
+            "kernel_ms": kms32["ms_interior"],
+            "boundary_kernel_ms": kms32["ms_boundary"],
             "speedup_vs_fp64": ms64 / ms32,
-            "roofline_frac": ncell * LIVE_COMPONENTS * 2 * 4 / (kms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "roofline_frac": kms32["cells_interior"] * LIVE_COMPONENTS * 2 * 4 / (kms32["ms_interior"] * 1e-3) / 1e9
+                             / HBM_PEAK_GBS,
             "slip_rel_err": float(np.max(np.abs(slip32 - slip64)) / max(np.max(np.abs(slip64)), 1e-300)),
             "slip_rate_rel_err": float(np.max(np.abs(V32 - V64)) / max(np.max(np.abs(V64)), 1e-300)),
         }

@@ -145,9 +145,24 @@ int tmk_level_run(tmk_level *lev, int nsteps, double time, double dt);
  * waits for completion and returns the elapsed milliseconds in *ms */
 int tmk_level_run_timed(tmk_level *lev, int nsteps, double time, double dt, double *ms);
 /* nsteps eager level steps (host-supplied times, no graph) bracketed by HIP
- * events on the level's stream; *ms_per_step = mean elapsed time per step.  For a
- * single-box level each step is exactly one launch of the fused update kernel. */
+ * events on the level's stream; *ms_per_step = mean elapsed time per step.  A step
+ * of a single-box level is the fused update (one launch over the interior tiles and,
+ * concurrently on a side stream, one over the tiles that meet the domain boundary
+ * or the fault) plus the clock advance. */
 int tmk_level_profile(tmk_level *lev, int nsteps, double time, double dt, double *ms_per_step);
+
+/* Per-kernel timing of level steps.  Measurement aid with no counterpart in the
+ * reference.  Runs nsteps eager steps from (time, dt).  Each launch is bracketed by
+ * HIP events on the stream it runs on: the interior-tile update on the level's
+ * stream, the boundary-tile update on the side stream.  Reports means in ms. */
+typedef struct {
+    double ms_step;      /* whole step (both update launches, exchanges, clock) */
+    double ms_interior;  /* interior-tile update launch */
+    double ms_boundary;  /* boundary-tile update launch (0 when there is none) */
+    long long cells_interior, cells_boundary; /* valid cells each launch updates */
+    int tiles_interior, tiles_boundary;
+} tmk_profile;
+int tmk_level_profile_kernels(tmk_level *lev, int nsteps, double time, double dt, tmk_profile *out);
 /* synchronize and return the first error any launch reported */
 int tmk_level_sync(tmk_level *lev);
 

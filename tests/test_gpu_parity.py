@@ -155,12 +155,14 @@ def test_patch_fluxes():
     assert np.all(_relerr(F1g, F1r) <= TOL)
 
 
-def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=False):
+def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=False, tile=None):
     U, B = _state(cfg, seed)
     Ur, Br = U.copy(), B.copy()
     dt = scenarios.dt_of(cfg)
     assert tmo.level_steps(tmo.OracleParams(cfg), Ur, Br, 0.0, dt, nsteps) == 0
     ctx = capi.Context(cfg, precision)
+    if tile:
+        ctx.set_tile(*tile)
     lev = capi.Level(ctx, boxes)
     for k, b in enumerate(boxes):
         lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
@@ -194,6 +196,45 @@ def test_level_steps(fault, maxbox):
     (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, 6, seed=21)
     assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
     assert np.all(_relerr(Bg, Br) <= TOL), _relerr(Bg, Br)
+
+
+@pytest.mark.parametrize("pred,prim,char,fourth", [("PPM", 1, 0, 1), ("PPM", 0, 1, 1), ("PLM", 1, 0, 1)])
+@pytest.mark.parametrize("tile", [None, (64, 8)])
+def test_level_interior_tiles(pred, prim, char, fourth, tile):
+    """A grid with many interior tiles (the specialised interior kernel) and boundary tiles
+    (the general kernel, on the side stream).  Away from the fault's reach (3 cells a step)
+    the state is bit-identical to the oracle; everywhere within 1e-12."""
+    cfg = _cfg("rs", pred, nx=200, ny=120, prim=prim, char=char, fourth=fourth)
+    nsteps = 4
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], nsteps, seed=31, tile=tile)
+    assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg, Br) <= TOL), _relerr(Bg, Br)
+    far = 3 * nsteps + 3
+    assert np.array_equal(Ug[:, far:, :], Ur[:, far:, :])
+
+
+def test_level_interior_tiles_fp32():
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], 4, seed=32, precision=capi.PREC_FP32_BULK)
+    assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= 1e-4), _relerr(Bg, Br)
+
+
+def test_profile_kernels():
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    U, B = _state(cfg, 5)
+    ctx = capi.Context(cfg, capi.PREC_FP64)
+    lev = capi.Level(ctx, [(0, 0, 199, 119)])
+    lev.upload(0, U, B)
+    p = lev.profile_kernels(3, 0.0, scenarios.dt_of(cfg))
+    lev.sync()
+    lev.close()
+    ctx.close()
+    assert p["cells_interior"] + p["cells_boundary"] == 200 * 120
+    # 30 x 14 tiles; interior: full size with 3 cells of domain around -> x0 = 30..150, y0 = 14..98
+    assert p["tiles_interior"] == 5 * 7
+    assert p["cells_interior"] == 30 * 14 * p["tiles_interior"]
+    assert p["ms_interior"] > 0 and p["ms_boundary"] > 0 and p["ms_step"] > 0
 
 
 def test_level_graph_run_matches_steps():

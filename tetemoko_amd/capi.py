@@ -38,7 +38,14 @@ EXPORTED = (
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
+    "tmk_level_profile_kernels",
 )
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("ms_step", ctypes.c_double), ("ms_interior", ctypes.c_double), ("ms_boundary", ctypes.c_double),
+                ("cells_interior", ctypes.c_longlong), ("cells_boundary", ctypes.c_longlong),
+                ("tiles_interior", ctypes.c_int), ("tiles_boundary", ctypes.c_int)]
 
 
 class Box(ctypes.Structure):
@@ -144,6 +151,7 @@ def lib():
         L.tmk_level_run.argtypes = [vp, c_int, _D, _D]
         L.tmk_level_run_timed.argtypes = [vp, c_int, _D, _D, _PD]
         L.tmk_level_profile.argtypes = [vp, c_int, _D, _D, _PD]
+        L.tmk_level_profile_kernels.argtypes = [vp, c_int, _D, _D, ctypes.POINTER(Profile)]
         L.tmk_level_sync.argtypes = [vp]
         L.tmk_set_tile.argtypes = [vp, c_int, c_int]
         sz = ctypes.c_size_t
@@ -342,6 +350,15 @@ class Level:
         ms = ctypes.c_double()
         _check(lib().tmk_level_profile(self.h, nsteps, time, dt, ctypes.byref(ms)), "tmk_level_profile")
         return ms.value
+
+    def profile_kernels(self, nsteps, time, dt):
+        """Per-launch means (ms) of the interior-tile and boundary-tile update kernels and of
+        the whole step, each timed with HIP events on the stream it runs on, plus the cells
+        and tiles each launch covers."""
+        p = Profile()
+        _check(lib().tmk_level_profile_kernels(self.h, nsteps, time, dt, ctypes.byref(p)),
+               "tmk_level_profile_kernels")
+        return {k: getattr(p, k) for k, _ in Profile._fields_}
 
     def sync(self):
         _check(lib().tmk_level_sync(self.h), "tmk_level_sync")

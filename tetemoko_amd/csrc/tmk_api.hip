@@ -100,6 +100,28 @@ void xcd_order(std::vector<Tile>& t)
     t.swap(o);
 }
 
+// a tile is interior when it is full size and its stencil (+-3 cells) lies inside the domain
+// (the same rule as the kernels' tile_interior)
+bool tile_interior_host(const Tile& t, const DevParams& P, int bx, int by)
+{
+    return t.x1 - t.x0 + 1 == bx - 2 && t.y1 - t.y0 + 1 == by - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
+           t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
+}
+
+// interior tiles first, then boundary tiles, each part in XCD order; returns the interior count
+template <typename F>
+int partition_tiles(std::vector<Tile>& t, F is_interior)
+{
+    std::vector<Tile> in, bd;
+    for (const Tile& x : t) (is_interior(x) ? in : bd).push_back(x);
+    xcd_order(in);
+    xcd_order(bd);
+    const int nint = (int)in.size();
+    t.swap(in);
+    t.insert(t.end(), bd.begin(), bd.end());
+    return nint;
+}
+
 // state planes that sweeps read with a stencil (double-buffered); the others
 // (szz, lam, gam and, in mode II, the anti-plane planes) are updated in place
 bool is_stencil(int c, int ns)
@@ -123,11 +145,16 @@ int stencil_mask(int ns)
 struct Group {
     size_t tsize = 8;
     int ns = 2, bx = 32, by = 16;
+    int pk = PK_RUNTIME;  // predictor kind the kernel is specialised for (all members agree)
     Scen* scen_dev = nullptr;
     int nscen = 0;
     void* patches_dev[2] = {nullptr, nullptr};
-    Tile* tiles_dev = nullptr;
+    Tile* tiles_dev = nullptr;     // interior tiles first, then the boundary tiles
     int ntiles = 0;
+    int nint = 0;                  // interior tiles (k_godunov_interior); the rest run k_godunov
+    long long cells_int = 0, cells_bnd = 0;
+    hipStream_t side = nullptr;    // boundary tiles run here, concurrently with the interior launch
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     CopyRect* rects_dev = nullptr;
     int nrects = 0;
     CopyRect* brects_dev = nullptr;
@@ -146,7 +173,12 @@ struct Group {
         if (rects_dev) (void)hipFree(rects_dev);
         if (brects_dev) (void)hipFree(brects_dev);
         scen_dev = nullptr; tiles_dev = nullptr; rects_dev = nullptr; brects_dev = nullptr;
-        nscen = ntiles = nrects = nbrects = 0;
+        nscen = ntiles = nint = nrects = nbrects = 0;
+        if (side) (void)hipStreamSynchronize(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+        side = nullptr; ev_fork = ev_join = nullptr;
     }
 };
 
@@ -235,6 +267,9 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
     P.lam = (h.cp * h.cp) * P.rho - 2.0 * h.G;
     P.Zs = P.rho * h.cs;
     P.Zp = P.rho * h.cp;
+    P.iZp = 1.0 / P.Zp;
+    P.iZs = 1.0 / P.Zs;
+    P.irho = 1.0 / P.rho;
     P.K = P.lam + 2.0 * h.G / 3.0;
     P.den = 2.0 * h.G + P.lam;
     P.lam2G = P.lam + 2.0 * h.G;
@@ -338,43 +373,100 @@ extern "C" int tmk_set_tile(tmk_ctx* c, int bx, int by)
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int BX, int BY>
-static int launch_tiles_t(hipStream_t st, const KArgs<T>& a)
+// predictor kind a scenario's numerics allow the kernel to be specialised for
+static int pred_kind(const DevParams& P)
+{
+    return (P.pred == 2 && P.prim_lim && !P.char_lim) ? PK_PPM_PRIM : PK_RUNTIME;
+}
+
+template <typename T, int NS, int BX, int BY, int PK>
+static int launch_tiles_t(hipStream_t st, const KArgs<T>& a, bool interior)
 {
     constexpr int NT = BX * BY;
     constexpr int NC = 4 * NS;
     const size_t s1 = (size_t)NC * BY * (BX + 4), s2 = (size_t)NC * (BY + 4) * BX, s3 = (size_t)8 * NS * NT;
     const size_t shm = std::max(s1, std::max(s2, s3)) * sizeof(T);
     if (a.ntiles == 0) return TMK_OK;
-    if (shm > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_godunov<T, NS, BX, BY>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm);
-    hipLaunchKernelGGL((k_godunov<T, NS, BX, BY>), dim3(a.ntiles), dim3(NT), shm, st, a);
+    const void* k = interior ? (const void*)k_godunov_interior<T, NS, BX, BY, PK> : (const void*)k_godunov<T, NS, BX, BY, PK>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (interior)
+        hipLaunchKernelGGL((k_godunov_interior<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(NT), shm, st, a);
+    else
+        hipLaunchKernelGGL((k_godunov<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(NT), shm, st, a);
     return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
 }
 
-template <typename T, int NS>
-static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a)
+template <typename T, int NS, int PK>
+static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a, bool in)
 {
-    if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16>(st, a);
-    if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8>(st, a);
-    if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32>(st, a);
-    if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8>(st, a);
-    if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4>(st, a);
-    if (bx == 64 && by == 16) return launch_tiles_t<T, NS, 64, 16>(st, a);
+    if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16, PK>(st, a, in);
+    if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8, PK>(st, a, in);
+    if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32, PK>(st, a, in);
+    if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8, PK>(st, a, in);
+    if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4, PK>(st, a, in);
+    if (bx == 64 && by == 16) return launch_tiles_t<T, NS, 64, 16, PK>(st, a, in);
     return TMK_E_ARG;
 }
 
+// one launch over a.ntiles tiles; interior: every tile is interior (k_godunov_interior)
 template <typename T>
-static int launch_tiles(hipStream_t st, int ns, int bx, int by, const KArgs<T>& a)
+static int launch_tiles(hipStream_t st, int ns, int pk, int bx, int by, const KArgs<T>& a, bool interior)
 {
-    return ns == 3 ? launch_tiles_ns<T, 3>(st, bx, by, a) : launch_tiles_ns<T, 2>(st, bx, by, a);
+    if (pk == PK_PPM_PRIM)
+        return ns == 3 ? launch_tiles_ns<T, 3, PK_PPM_PRIM>(st, bx, by, a, interior)
+                       : launch_tiles_ns<T, 2, PK_PPM_PRIM>(st, bx, by, a, interior);
+    return ns == 3 ? launch_tiles_ns<T, 3, PK_RUNTIME>(st, bx, by, a, interior)
+                   : launch_tiles_ns<T, 2, PK_RUNTIME>(st, bx, by, a, interior);
+}
+
+// per-kernel timing of one step (profiling only): events around the interior launch on the
+// main stream and around the boundary launch on the side stream
+struct StepTimers {
+    hipEvent_t ia, ib, ba, bb;
+};
+
+// the fused update of one step over a tile list [interior | boundary]: the boundary launch runs
+// on `side` (forked from and joined back into st by events; captured graphs keep the fork), the
+// interior launch on st
+template <typename T>
+static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join, int ns, int pk, int bx,
+                         int by, KArgs<T> a, int nint, const StepTimers* tm)
+{
+    const int nbnd = a.ntiles - nint;
+    const Tile* tiles = a.tiles;
+    int e = TMK_OK;
+    const bool split = nbnd > 0 && nint > 0 && side;
+    if (split) {
+        HIPCHK(hipEventRecord(fork, st));
+        HIPCHK(hipStreamWaitEvent(side, fork, 0));
+    }
+    if (nbnd > 0) {
+        hipStream_t sb = split ? side : st;
+        KArgs<T> b = a;
+        b.tiles = tiles + nint;
+        b.ntiles = nbnd;
+        if (tm) HIPCHK(hipEventRecord(tm->ba, sb));
+        if ((e = launch_tiles<T>(sb, ns, pk, bx, by, b, false))) return e;
+        if (tm) HIPCHK(hipEventRecord(tm->bb, sb));
+    }
+    if (nint > 0) {
+        KArgs<T> b = a;
+        b.ntiles = nint;
+        if (tm) HIPCHK(hipEventRecord(tm->ia, st));
+        if ((e = launch_tiles<T>(st, ns, pk, bx, by, b, true))) return e;
+        if (tm) HIPCHK(hipEventRecord(tm->ib, st));
+    }
+    if (split) {
+        HIPCHK(hipEventRecord(join, side));
+        HIPCHK(hipStreamWaitEvent(st, join, 0));
+    }
+    return TMK_OK;
 }
 
 // one level step of every member of a group: fused update, ghost exchange of the
 // new state, clock advance
 template <typename T>
-static int group_enqueue_step_t(Group& G, hipStream_t st)
+static int group_enqueue_step_t(Group& G, hipStream_t st, const StepTimers* tm = nullptr)
 {
     KArgs<T> a;
     a.scen = G.scen_dev;
@@ -382,7 +474,7 @@ static int group_enqueue_step_t(Group& G, hipStream_t st)
     a.tiles = G.tiles_dev;
     a.ntiles = G.ntiles;
     a.plastic = G.plastic;
-    int e = launch_tiles<T>(st, G.ns, G.bx, G.by, a);
+    int e = launch_update<T>(st, G.side, G.ev_fork, G.ev_join, G.ns, G.pk, G.bx, G.by, a, G.nint, tm);
     if (e) return e;
     const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g];  // its out planes hold the new state
     if (G.nrects > 0) {
@@ -400,9 +492,9 @@ static int group_enqueue_step_t(Group& G, hipStream_t st)
     return TMK_OK;
 }
 
-static int group_enqueue_step(Group& G, hipStream_t st)
+static int group_enqueue_step(Group& G, hipStream_t st, const StepTimers* tm = nullptr)
 {
-    return G.tsize == sizeof(double) ? group_enqueue_step_t<double>(G, st) : group_enqueue_step_t<float>(G, st);
+    return G.tsize == sizeof(double) ? group_enqueue_step_t<double>(G, st, tm) : group_enqueue_step_t<float>(G, st, tm);
 }
 
 // exchange the current state (before the first step after an upload)
@@ -461,6 +553,9 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     tmk_level* L0 = mem[0];
     G.tsize = sizeof(T);
     G.ns = L0->ns;
+    G.pk = PK_PPM_PRIM;
+    for (tmk_level* L : mem)
+        if (pred_kind(L->ctx->P) != PK_PPM_PRIM) G.pk = PK_RUNTIME;
     G.bx = L0->ctx->bx;
     G.by = L0->ctx->by;
     G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
@@ -491,7 +586,21 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         for (CopyRect r : L->brects) { r.src += poff; r.dst += poff; brects.push_back(r); }
         poff += nbox;
     }
-    xcd_order(tiles);
+    // tile -> scenario through the patch index (patches of member m follow those of m - 1)
+    std::vector<int> patch_scen;
+    for (size_t m = 0; m < mem.size(); ++m)
+        for (size_t b = 0; b < mem[m]->boxes.size(); ++b) patch_scen.push_back((int)m);
+    const int bx = G.bx, by = G.by;
+    G.nint = partition_tiles(tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); });
+    G.cells_int = G.cells_bnd = 0;
+    for (size_t k = 0; k < tiles.size(); ++k)
+        ((int)k < G.nint ? G.cells_int : G.cells_bnd) +=
+            (long long)(tiles[k].x1 - tiles[k].x0 + 1) * (tiles[k].y1 - tiles[k].y0 + 1);
+    if (!G.side) {
+        HIPCHK(hipStreamCreate(&G.side));
+        HIPCHK(hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming));
+    }
     G.nscen = (int)sc.size();
     G.ntiles = (int)tiles.size();
     G.nrects = (int)rects.size();
@@ -595,6 +704,8 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         pt.scen = 0;
         std::vector<Tile> tiles;
         make_tiles(valid, 0, c->bx - 2, c->by - 2, tiles);
+        const int bx = c->bx, by = c->by;
+        const int nint = partition_tiles(tiles, [&](const Tile& t) { return tile_interior_host(t, c->P, bx, by); });
         const size_t off_t = ((sizeof(Patch<T>) + 255) / 256) * 256;
         const size_t off_s = off_t + ((sizeof(Tile) * tiles.size() + 255) / 256) * 256;
         if ((e = c->scratch[5].ensure(off_s + sizeof(Scen)))) return e;
@@ -626,7 +737,8 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
                 }
             ns = zero ? 2 : 3;
         }
-        if ((e = launch_tiles<T>(st, ns, c->bx, c->by, a))) return e;
+        if ((e = launch_update<T>(st, nullptr, nullptr, nullptr, ns, pred_kind(c->P), c->bx, c->by, a, nint, nullptr)))
+            return e;
         int herr = 0;
         HIPCHK(hipMemcpyAsync(&herr, meta + off_s + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1028,6 +1140,55 @@ extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double d
     (void)hipEventDestroy(b);
     if (!e) *ms = (double)f / nsteps;
     return e;
+}
+
+extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, double dt, tmk_profile* out)
+{
+    if (!L || !out || nsteps <= 0 || L->batch) return TMK_E_ARG;
+    int e = level_prepare(L, time, dt);
+    if (e) return e;
+    hipStream_t st = L->ctx->stream;
+    std::vector<hipEvent_t> ev((size_t)6 * nsteps, nullptr);
+    for (auto& x : ev)
+        if (hipEventCreate(&x) != hipSuccess) e = TMK_E_HIP;
+    const Group& G = L->own;
+    const bool has_bnd = G.ntiles > G.nint, has_int = G.nint > 0;
+    for (int k = 0; k < nsteps && !e; ++k) {
+        hipEvent_t* x = &ev[(size_t)6 * k];
+        StepTimers tm{x[2], x[3], x[4], x[5]};
+        if (hipEventRecord(x[0], st) != hipSuccess) { e = TMK_E_HIP; break; }
+        e = group_enqueue_step(L->own, st, &tm);
+        if (!e && hipEventRecord(x[1], st) != hipSuccess) e = TMK_E_HIP;
+        L->cur ^= 1;
+        L->time = L->time + dt;
+    }
+    if (!e && hipStreamSynchronize(st) != hipSuccess) e = TMK_E_HIP;
+    double sum[3] = {0, 0, 0};
+    for (int k = 0; k < nsteps && !e; ++k) {
+        const hipEvent_t* x = &ev[(size_t)6 * k];
+        float f = 0.f;
+        if (hipEventElapsedTime(&f, x[0], x[1]) != hipSuccess) { e = TMK_E_HIP; break; }
+        sum[0] += f;
+        if (has_int) {
+            if (hipEventElapsedTime(&f, x[2], x[3]) != hipSuccess) { e = TMK_E_HIP; break; }
+            sum[1] += f;
+        }
+        if (has_bnd) {
+            if (hipEventElapsedTime(&f, x[4], x[5]) != hipSuccess) { e = TMK_E_HIP; break; }
+            sum[2] += f;
+        }
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e) return e;
+    out->ms_step = sum[0] / nsteps;
+    out->ms_interior = sum[1] / nsteps;
+    out->ms_boundary = sum[2] / nsteps;
+    out->cells_interior = G.cells_int;
+    out->cells_boundary = G.cells_bnd;
+    out->tiles_interior = G.nint;
+    out->tiles_boundary = G.ntiles - G.nint;
+    return TMK_OK;
 }
 
 extern "C" int tmk_level_sync(tmk_level* L)

@@ -94,16 +94,13 @@ __device__ __forceinline__ int stage_comp(int d, int c)
     return slot_comp(d, r >> 1, r & 1) + 11 * s;
 }
 
-template <typename T, int NS, int BX, int BY>
-__global__ void __launch_bounds__(BX * BY)
-k_godunov(const KArgs<T> a)
+// INT: the tile is full size and its stencil (+-3 cells) lies inside the domain, so no cell of
+// it is a domain-boundary or fault cell; the body then carries no boundary code at all.
+template <typename T, int NS, int BX, int BY, int PK, bool INT>
+__device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, T* sm)
 {
     constexpr int NT = BX * BY;
     constexpr int NC = 4 * NS;  // staged components per direction
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sm = reinterpret_cast<T*>(smem_raw);
-
-    const Tile tl = a.tiles[blockIdx.x];
     const Patch<T>& pt = a.patches[tl.patch];
     Scen& S = a.scen[pt.scen];
     const DevParams& P = S.P;
@@ -112,10 +109,10 @@ k_godunov(const KArgs<T> a)
     const int tid = threadIdx.x;
     const int lx = tid % BX, ly = tid / BX;
     const int i = tl.x0 - 1 + lx, j = tl.y0 - 1 + ly;
-    const int tnx = tl.x1 - tl.x0 + 1, tny = tl.y1 - tl.y0 + 1;
+    const int tnx = INT ? BX - 2 : tl.x1 - tl.x0 + 1, tny = INT ? BY - 2 : tl.y1 - tl.y0 + 1;
     const int dlo0 = P.dlo[0], dlo1 = P.dlo[1], dhi0 = P.dhi[0], dhi1 = P.dhi[1];
-    const bool colD = i >= dlo0 && i <= dhi0;
-    const bool rowD = j >= dlo1 && j <= dhi1;
+    const bool colD = INT || (i >= dlo0 && i <= dhi0);
+    const bool rowD = INT || (j >= dlo1 && j <= dhi1);
     const bool r1 = colD && rowD && lx <= tnx + 1 && ly <= tny + 1;
     const bool valid = lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny;
     const double time = S.time;
@@ -136,7 +133,7 @@ k_godunov(const KArgs<T> a)
             const int gi = tl.x0 - 3 + xx, gj = tl.y0 - 1 + yy;
             T v = (T)0;
             // only the tile's stencil region (x: tile +-3, y: tile +-1), inside the domain
-            if (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1)
+            if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1))
                 v = pt.in[stage_comp<NS>(0, c)][uidx(pt, gi, gj, 0)];
             sm[e] = v;
         }
@@ -153,7 +150,8 @@ k_godunov(const KArgs<T> a)
 #pragma unroll
                         for (int o = 0; o < 5; ++o) w[s][k][q][o] = sm[(c * BY + ly) * SW + lx + o];
                     }
-            normal_pred<T, NS>(P, m, pred, (T)1, w, i, dlo0, dhi0, dtbydx, WmX, WpX);
+            if (INT) normal_pred<T, NS, PK>(P, m, pred, (T)1, w, 0, -(1 << 20), 1 << 20, dtbydx, WmX, WpX);
+            else normal_pred<T, NS, PK>(P, m, pred, (T)1, w, i, dlo0, dhi0, dtbydx, WmX, WpX);
         }
         __syncthreads();
     }
@@ -167,7 +165,7 @@ k_godunov(const KArgs<T> a)
             const int gi = tl.x0 - 1 + xx, gj = tl.y0 - 3 + yy;
             T v = (T)0;
             // only the tile's stencil region (x: tile +-1, y: tile +-3), inside the domain
-            if (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3)
+            if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3))
                 v = pt.in[stage_comp<NS>(1, c)][uidx(pt, gi, gj, 0)];
             sm[e] = v;
         }
@@ -184,7 +182,8 @@ k_godunov(const KArgs<T> a)
 #pragma unroll
                         for (int o = 0; o < 5; ++o) w[s][k][q][o] = sm[(c * SH + ly + o) * BX + lx];
                     }
-            normal_pred<T, NS>(P, m, pred, rv1, w, j, dlo1, dhi1, dtbydx, WmY, WpY);
+            if (INT) normal_pred<T, NS, PK>(P, m, pred, rv1, w, 0, -(1 << 20), 1 << 20, dtbydx, WmY, WpY);
+            else normal_pred<T, NS, PK>(P, m, pred, rv1, w, j, dlo1, dhi1, dtbydx, WmY, WpY);
         }
         __syncthreads();
     }
@@ -211,9 +210,9 @@ k_godunov(const KArgs<T> a)
         }
         __syncthreads();
         // x faces: lo face of cell i, rows of the ring-grown tile
-        if (lx >= 1 && lx <= tnx + 1 && rowD && ly <= tny + 1 && i >= dlo0 && i <= dhi0 + 1) {
+        if (lx >= 1 && lx <= tnx + 1 && rowD && ly <= tny + 1 && (INT || (i >= dlo0 && i <= dhi0 + 1))) {
             T wpL[2][NS], wmR[2][NS];
-            if (i == dlo0) {
+            if (!INT && i == dlo0) {
                 left_chars<T, NS>(m, (T)1, WmX, wmR);
                 bc_state<T, NS>(P, m, 0, (T)1, P.bc[0], -1, wmR, H0);
             } else {
@@ -221,7 +220,7 @@ k_godunov(const KArgs<T> a)
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
                     for (int k = 0; k < NS; ++k) wpL[s][k] = get(s * NS + k, tid - 1);
-                if (i == dhi0 + 1) {
+                if (!INT && i == dhi0 + 1) {
                     bc_state<T, NS>(P, m, 0, (T)1, P.bc[1], +1, wpL, H0);
                 } else {
                     left_chars<T, NS>(m, (T)1, WmX, wmR);
@@ -230,9 +229,9 @@ k_godunov(const KArgs<T> a)
             }
         }
         // y faces: lo face of cell j
-        if (ly >= 1 && ly <= tny + 1 && colD && lx <= tnx + 1 && j >= dlo1 && j <= dhi1 + 1) {
+        if (ly >= 1 && ly <= tny + 1 && colD && lx <= tnx + 1 && (INT || (j >= dlo1 && j <= dhi1 + 1))) {
             T wpL[2][NS], wmR[2][NS];
-            if (j == dlo1) {
+            if (!INT && j == dlo1) {
                 // fault face (RSIBC::primBC lo-y branch, RSIBC.cpp:220-246)
                 if (!pt.bin) {
                     report(errp, TMK_E_NO_BDRY);
@@ -275,7 +274,7 @@ k_godunov(const KArgs<T> a)
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
                     for (int k = 0; k < NS; ++k) wpL[s][k] = get(2 * NS + s * NS + k, tid - BX);
-                if (j == dhi1 + 1) {
+                if (!INT && j == dhi1 + 1) {
                     bc_state<T, NS>(P, m, 1, rv1, P.bc[3], +1, wpL, H1);
                 } else {
                     left_chars<T, NS>(m, rv1, WmY, wmR);
@@ -340,9 +339,9 @@ k_godunov(const KArgs<T> a)
                 put(2 * NS + s * NS + k, cpy[s][k]);
             }
         __syncthreads();
-        if (lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny && i >= dlo0 && i <= dhi0 + 1) {
+        if (lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny && (INT || (i >= dlo0 && i <= dhi0 + 1))) {
             T wpL[2][NS], wmR[2][NS];
-            if (i == dlo0) {
+            if (!INT && i == dlo0) {
                 left_chars<T, NS>(m, (T)1, WmX, wmR);
                 bc_state<T, NS>(P, m, 0, (T)1, P.bc[0], -1, wmR, F0);
             } else {
@@ -350,7 +349,7 @@ k_godunov(const KArgs<T> a)
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
                     for (int k = 0; k < NS; ++k) wpL[s][k] = get(s * NS + k, tid - 1);
-                if (i == dhi0 + 1) {
+                if (!INT && i == dhi0 + 1) {
                     bc_state<T, NS>(P, m, 0, (T)1, P.bc[1], +1, wpL, F0);
                 } else {
                     left_chars<T, NS>(m, (T)1, WmX, wmR);
@@ -358,9 +357,9 @@ k_godunov(const KArgs<T> a)
                 }
             }
         }
-        if (ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx && j >= dlo1 && j <= dhi1 + 1) {
+        if (ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx && (INT || (j >= dlo1 && j <= dhi1 + 1))) {
             T wpL[2][NS], wmR[2][NS];
-            if (j == dlo1) {
+            if (!INT && j == dlo1) {
                 if (pt.bin) {
                     FaultIn fin;
 #pragma unroll
@@ -432,7 +431,7 @@ k_godunov(const KArgs<T> a)
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
                     for (int k = 0; k < NS; ++k) wpL[s][k] = get(2 * NS + s * NS + k, tid - BX);
-                if (j == dhi1 + 1) {
+                if (!INT && j == dhi1 + 1) {
                     bc_state<T, NS>(P, m, 1, rv1, P.bc[3], +1, wpL, F1);
                 } else {
                     left_chars<T, NS>(m, rv1, WmY, wmR);
@@ -462,19 +461,19 @@ k_godunov(const KArgs<T> a)
             for (int s = 0; s < 2; ++s) {
                 const T r = s ? rv : (T)1;
                 if (s == 0) {
-                    f[s][0] = -(S.s[s][0] / m.rho);
-                    f[s][2] = -(S.s[s][1] / m.rho);
+                    f[s][0] = -m.divrho(S.s[s][0]);
+                    f[s][2] = -m.divrho(S.s[s][1]);
                     f[s][1] = -(S.v[s][0] * m.lam2G);
                     f[s][4] = -(S.v[s][0] * m.lam);
                     f[s][3] = -(S.v[s][1] * m.G);
-                    if (NS == 3) { f[s][5] = -(S.s[s][NS - 1] / m.rho); f[s][6] = -(S.v[s][NS - 1] * m.G); }
+                    if (NS == 3) { f[s][5] = -m.divrho(S.s[s][NS - 1]); f[s][6] = -(S.v[s][NS - 1] * m.G); }
                 } else {
-                    f[s][0] = -(r * S.s[s][0] / m.rho);
-                    f[s][2] = -(r * S.s[s][1] / m.rho);
+                    f[s][0] = -m.divrho(r * S.s[s][0]);
+                    f[s][2] = -m.divrho(r * S.s[s][1]);
                     f[s][1] = -(r * S.v[s][0] * m.lam2G);
                     f[s][4] = -(r * S.v[s][0] * m.lam);
                     f[s][3] = -(r * S.v[s][1] * m.G);
-                    if (NS == 3) { f[s][5] = -(r * S.s[s][NS - 1] / m.rho); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
+                    if (NS == 3) { f[s][5] = -m.divrho(r * S.s[s][NS - 1]); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
                 }
             }
         };
@@ -605,6 +604,53 @@ k_godunov(const KArgs<T> a)
             }
         }
     }
+}
+
+// a tile is interior when it is full size and its stencil (+-3 cells) lies inside the domain
+template <int BX, int BY>
+__device__ __forceinline__ bool tile_interior(const Tile& t, const DevParams& P)
+{
+    return t.x1 - t.x0 + 1 == BX - 2 && t.y1 - t.y0 + 1 == BY - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
+           t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
+}
+
+// any tile: domain boundaries and the fault are handled where the tile meets them
+template <typename T, int NS, int BX, int BY, int PK>
+__global__ void __launch_bounds__(BX * BY)
+k_godunov(const KArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    const Tile tl = a.tiles[blockIdx.x];
+    godunov_tile<T, NS, BX, BY, PK, false>(a, tl, sm);
+}
+
+// Occupancy target (waves per SIMD) of the interior kernel.  FP64 mode II fits 128 VGPRs,
+// so two 512-thread workgroups share a CU and one's barriers overlap the other's work.
+#ifndef TMK_INTERIOR_WAVES_F64
+#define TMK_INTERIOR_WAVES_F64 4
+#endif
+template <typename T, int NS>
+constexpr int interior_waves()
+{
+    return (sizeof(T) == 8 && NS == 2) ? TMK_INTERIOR_WAVES_F64 : 1;
+}
+
+// interior tiles only (the host sorts them out with the same rule); a tile that is not
+// interior reports TMK_E_BOX and is left untouched
+template <typename T, int NS, int BX, int BY, int PK>
+__global__ void __launch_bounds__(BX * BY, (interior_waves<T, NS>()))
+k_godunov_interior(const KArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    const Tile tl = a.tiles[blockIdx.x];
+    Scen& S = a.scen[a.patches[tl.patch].scen];
+    if (!tile_interior<BX, BY>(tl, S.P)) {
+        if (threadIdx.x == 0) report(&S.err, TMK_E_BOX);
+        return;
+    }
+    godunov_tile<T, NS, BX, BY, PK, true>(a, tl, sm);
 }
 
 // LinElastPatchGodunov::plasticUpdate on its own (PLASTICCORRECTION over the valid box)

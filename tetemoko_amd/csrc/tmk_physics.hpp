@@ -26,6 +26,7 @@ namespace tmk {
 // ---------------------------------------------------------------------------
 struct DevParams {
     double cs, cp, G, rho, lam, K, Zp, Zs;
+    double iZp, iZs, irho;  // correctly rounded reciprocals 1/Zp, 1/Zs, 1/rho
     double den;    // 2G + lam   (LinElastPhysicsF.ChF:427 form)
     double lam2G;  // lam + 2G   (LinElastPhysicsF.ChF:318 form)
     double bg[22];
@@ -53,14 +54,41 @@ struct Slots {
     T s[2][NS];
 };
 
+// a / b for b > 0 given y = RN(1/b).  FP64: q0 = RN(a y) is within two ulps of a/b; one
+// Newton-Markstein step q1 = RN(q0 + (a - b q0) y) is faithful (its error is below half an ulp
+// plus |q0 - a/b| u); a second step from a faithful quotient and a correctly rounded reciprocal
+// returns RN(a/b) exactly (Markstein, IBM J. Res. Dev. 34 (1990) 111-119; Muller et al.,
+// Handbook of Floating-Point Arithmetic, 2nd ed., sec. 4.7).  The remainders a - b q are exact
+// in one fma.  copysign keeps the sign of a zero dividend.  The result equals the IEEE quotient
+// bit for bit, at five fma/mul instead of the 10-instruction scaled division sequence.
+// FP32 (bulk arithmetic of the reduced-precision mode): a y, within 1.5 ulps.
+__device__ __forceinline__ double divr(double a, double b, double y)
+{
+    double q = a * y;
+    double r = fma(-q, b, a);
+    q = fma(r, y, q);
+    r = fma(-q, b, a);
+    return copysign(fma(r, y, q), a);
+}
+__device__ __forceinline__ float divr(float a, float, float y) { return a * y; }
+
 // elastic constants in the bulk arithmetic type
 template <typename T>
 struct Mat {
     T rho, lam, G, Zp, Zs, den, lam2G;
+    T iZp, iZs, irho, i2Zp, i2Zs;  // reciprocals (1/(2Z) = (1/Z)/2 exactly)
     __device__ __forceinline__ explicit Mat(const DevParams& p)
         : rho((T)p.rho), lam((T)p.lam), G((T)p.G), Zp((T)p.Zp), Zs((T)p.Zs), den((T)p.den),
-          lam2G((T)p.lam2G) {}
+          lam2G((T)p.lam2G), iZp((T)p.iZp), iZs((T)p.iZs), irho((T)p.irho), i2Zp((T)(0.5 * p.iZp)),
+          i2Zs((T)(0.5 * p.iZs)) {}
     __device__ __forceinline__ T Z(int k) const { return k == 0 ? Zp : Zs; }
+    // x / Z(k), x / (2 Z(k)), x / rho, rounded like the division
+    __device__ __forceinline__ T divZ(T x, int k) const { return k == 0 ? divr(x, Zp, iZp) : divr(x, Zs, iZs); }
+    __device__ __forceinline__ T div2Z(T x, int k) const
+    {
+        return k == 0 ? divr(x, (T)2 * Zp, i2Zp) : divr(x, (T)2 * Zs, i2Zs);
+    }
+    __device__ __forceinline__ T divrho(T x) const { return divr(x, rho, irho); }
 };
 
 // component index of (slot, velocity/stress) for direction d, side 1
@@ -151,11 +179,17 @@ __device__ __forceinline__ void ppm_limit(T& m, T& p)
 // normal predictor for one cell and one direction (LinElastPatchGodunov.cpp:471-702)
 // w[side][slot][v|s][5]: primitive stencil along d; outputs W- and W+ (primitive)
 // ---------------------------------------------------------------------------
-template <typename T, int NS>
-__device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m, int pred, T rv,
+// PK >= 0 fixes the predictor at compile time: PK_PPM_PRIM is PPM with primitive limiting and no
+// characteristic limiting; PK_RUNTIME reads pred and the limiter flags from P.
+constexpr int PK_RUNTIME = -1;
+constexpr int PK_PPM_PRIM = 2;
+
+template <typename T, int NS, int PK = PK_RUNTIME>
+__device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m, int pred_rt, T rv,
                                             const T (&w)[2][NS][2][5], int ci, int dlo, int dhi,
                                             T dtbydx, Slots<T, NS>& Wm, Slots<T, NS>& Wp)
 {
+    const int pred = PK >= 0 ? PK : pred_rt;
     if (pred == 0) {  // CTUNormalPred: W+- = W
 #pragma unroll
         for (int s = 0; s < 2; ++s)
@@ -166,8 +200,8 @@ __device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m,
             }
         return;
     }
-    const bool limit = P.char_lim || P.prim_lim;
-    const bool charl = P.char_lim != 0;
+    const bool limit = PK >= 0 ? true : (P.char_lim || P.prim_lim);
+    const bool charl = PK >= 0 ? false : P.char_lim != 0;
     // characteristic increments: wave moving toward lo (m) / toward hi (p)
     T cm_m[2][NS], cp_m[2][NS];  // chars of the minus-side increment
     T cm_p[2][NS], cp_p[2][NS];  // chars of the plus-side increment
@@ -285,9 +319,12 @@ __device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m,
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const T Z = m.Z(k);
-            T vmi = (cm_m[s][k] - cp_m[s][k]) / (T)2 / Z;
-            T vpi = (cm_p[s][k] - cp_p[s][k]) / (T)2 / Z;
-            if (s) { vmi = rv * (cm_m[s][k] - cp_m[s][k]) / (T)2 / Z; vpi = rv * (cm_p[s][k] - cp_p[s][k]) / (T)2 / Z; }
+            T vmi = m.divZ((cm_m[s][k] - cp_m[s][k]) / (T)2, k);
+            T vpi = m.divZ((cm_p[s][k] - cp_p[s][k]) / (T)2, k);
+            if (s) {
+                vmi = m.divZ(rv * (cm_m[s][k] - cp_m[s][k]) / (T)2, k);
+                vpi = m.divZ(rv * (cm_p[s][k] - cp_p[s][k]) / (T)2, k);
+            }
             const T smi = (cm_m[s][k] + cp_m[s][k]) / (T)2;
             const T spi = (cm_p[s][k] + cp_p[s][k]) / (T)2;
             Wm.v[s][k] = vmi + w[s][k][0][2];
@@ -332,7 +369,7 @@ __device__ __forceinline__ void riemann_state(const Mat<T>& m, T rv, const T (&w
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const T Z = m.Z(k);
-            F.v[s][k] = s ? rv * (wm[s][k] - wp[s][k]) / (T)2 / Z : (wm[s][k] - wp[s][k]) / (T)2 / Z;
+            F.v[s][k] = s ? m.divZ(rv * (wm[s][k] - wp[s][k]) / (T)2, k) : m.divZ((wm[s][k] - wp[s][k]) / (T)2, k);
             F.s[s][k] = (wm[s][k] + wp[s][k]) / (T)2;
         }
 }
@@ -359,7 +396,7 @@ __device__ __forceinline__ void bc_state(const DevParams& P, const Mat<T>& m, in
                 if (lohisign == -1) { wm = cin[s][k]; wp = -wm; }
                 else { wp = cin[s][k]; wm = -wp; }
             }
-            F.v[s][k] = s ? rv * (wm - wp) / ((T)2 * Z) : (wm - wp) / ((T)2 * Z);
+            F.v[s][k] = s ? m.div2Z(rv * (wm - wp), k) : m.div2Z(wm - wp, k);
             F.s[s][k] = (wm + wp) / (T)2;
         }
 }
@@ -375,13 +412,13 @@ __device__ __forceinline__ void transverse_add(const Mat<T>& m, T sc1, T sc2, co
         const T sc = s ? sc2 : sc1;
         const T dNv = Hhi.v[s][0] - Hlo.v[s][0], dNs = Hhi.s[s][0] - Hlo.s[s][0];
         const T dSv = Hhi.v[s][1] - Hlo.v[s][1], dSs = Hhi.s[s][1] - Hlo.s[s][1];
-        W.v[s][0] = W.v[s][0] + (-(sc * dSs / m.rho));
+        W.v[s][0] = W.v[s][0] + (-m.divrho(sc * dSs));
         W.s[s][0] = W.s[s][0] + (-(sc * dNv * m.lam));
-        W.v[s][1] = W.v[s][1] + (-(sc * dNs / m.rho));
+        W.v[s][1] = W.v[s][1] + (-m.divrho(sc * dNs));
         W.s[s][1] = W.s[s][1] + (-(sc * dSv * m.G));
         if (NS == 3) {
             const T dAs = Hhi.s[s][NS - 1] - Hlo.s[s][NS - 1];
-            W.v[s][NS - 1] = W.v[s][NS - 1] + (-(sc * dAs / m.rho));
+            W.v[s][NS - 1] = W.v[s][NS - 1] + (-m.divrho(sc * dAs));
         }
     }
 }

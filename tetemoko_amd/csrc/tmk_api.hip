@@ -16,6 +16,7 @@
 
 #include "tetemoko_hip.h"
 #include "tmk_kernels.hpp"
+#include "tmk_interior.hpp"
 
 using namespace tmk;
 
@@ -385,7 +386,7 @@ static int launch_tiles_t(hipStream_t st, const KArgs<T>& a, bool interior)
     constexpr int NT = BX * BY;
     constexpr int NC = 4 * NS;
     const size_t s1 = (size_t)NC * BY * (BX + 4), s2 = (size_t)NC * (BY + 4) * BX, s3 = (size_t)8 * NS * NT;
-    const size_t shm = std::max(s1, std::max(s2, s3)) * sizeof(T);
+    const size_t shm = (interior ? interior_lds_elems<NS, BX, BY>() : std::max(s1, std::max(s2, s3))) * sizeof(T);
     if (a.ntiles == 0) return TMK_OK;
     const void* k = interior ? (const void*)k_godunov_interior<T, NS, BX, BY, PK> : (const void*)k_godunov<T, NS, BX, BY, PK>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);

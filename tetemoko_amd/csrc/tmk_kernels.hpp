@@ -94,6 +94,152 @@ __device__ __forceinline__ int stage_comp(int d, int c)
     return slot_comp(d, r >> 1, r & 1) + 11 * s;
 }
 
+// GETFLUXF (LinElastPhysicsF.ChF:303-350) from a face state, in slot form:
+// f[s][0]=F(vn) f[s][1]=F(snn) f[s][2]=F(vt) f[s][3]=F(snt) f[s][4]=F(s_transverse normal)
+// f[s][5]=F(va) f[s][6]=F(sna)
+template <typename T, int NS>
+__device__ __forceinline__ void face_flux(const Mat<T>& m, const Slots<T, NS>& S, T rv, T (&f)[2][7])
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const T r = s ? rv : (T)1;
+        if (s == 0) {
+            f[s][0] = -m.divrho(S.s[s][0]);
+            f[s][2] = -m.divrho(S.s[s][1]);
+            f[s][1] = -(S.v[s][0] * m.lam2G);
+            f[s][4] = -(S.v[s][0] * m.lam);
+            f[s][3] = -(S.v[s][1] * m.G);
+            if (NS == 3) { f[s][5] = -m.divrho(S.s[s][NS - 1]); f[s][6] = -(S.v[s][NS - 1] * m.G); }
+        } else {
+            f[s][0] = -m.divrho(r * S.s[s][0]);
+            f[s][2] = -m.divrho(r * S.s[s][1]);
+            f[s][1] = -(r * S.v[s][0] * m.lam2G);
+            f[s][4] = -(r * S.v[s][0] * m.lam);
+            f[s][3] = -(r * S.v[s][1] * m.G);
+            if (NS == 3) { f[s][5] = -m.divrho(r * S.s[s][NS - 1]); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
+        }
+    }
+}
+
+// the 22-component flux of face state Fs on the lo d-face of cell (i, j) into the flux
+// register array of direction d (surroundingNodes(valid box, d))
+template <typename T, int NS>
+__device__ __forceinline__ void store_flux(const Patch<T>& pt, const Mat<T>& m, int d, const Slots<T, NS>& Fs, T rv,
+                                           int i, int j)
+{
+    T f[2][7];
+    face_flux<T, NS>(m, Fs, rv, f);
+    double* Fo = pt.F[d];
+    const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
+    const size_t fp = (size_t)fnx * fny;
+    const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int o = 11 * s;
+        if (d == 0) {
+            Fo[fi + (o + 0) * fp] = (double)f[s][0];
+            Fo[fi + (o + 1) * fp] = (double)f[s][2];
+            Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : -(s ? 1.0 : 1.0) * 0.0 * 0.0;
+            Fo[fi + (o + 3) * fp] = (double)f[s][1];
+            Fo[fi + (o + 4) * fp] = (double)f[s][4];
+            Fo[fi + (o + 5) * fp] = (double)f[s][4];
+            Fo[fi + (o + 6) * fp] = (double)f[s][3];
+            Fo[fi + (o + 7) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
+            Fo[fi + (o + 8) * fp] = 0.0;
+        } else {
+            Fo[fi + (o + 1) * fp] = (double)f[s][0];
+            Fo[fi + (o + 0) * fp] = (double)f[s][2];
+            Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : 0.0;
+            Fo[fi + (o + 4) * fp] = (double)f[s][1];
+            Fo[fi + (o + 5) * fp] = (double)f[s][4];
+            Fo[fi + (o + 3) * fp] = (double)f[s][4];
+            Fo[fi + (o + 6) * fp] = (double)f[s][3];
+            Fo[fi + (o + 8) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
+            Fo[fi + (o + 7) * fp] = 0.0;
+        }
+        Fo[fi + (o + 9) * fp] = 0.0;
+        Fo[fi + (o + 10) * fp] = 0.0;
+    }
+}
+
+// U += dU of valid cell (i, j) from the final states on its four faces (flux difference,
+// LinElastPatchGodunov.cpp:117-126), then the optional Drucker-Prager return
+template <typename T, int NS>
+__device__ __forceinline__ void update_cell(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
+                                            const Mat<T>& m, double dt, int i, int j, const Slots<T, NS>& F0,
+                                            const Slots<T, NS>& F0h, const Slots<T, NS>& F1,
+                                            const Slots<T, NS>& F1h)
+{
+    const T rv1 = (T)-1;
+    T fxl[2][7], fxh[2][7], fyl[2][7], fyh[2][7];
+    face_flux<T, NS>(m, F0, (T)1, fxl);
+    face_flux<T, NS>(m, F0h, (T)1, fxh);
+    face_flux<T, NS>(m, F1, rv1, fyl);
+    face_flux<T, NS>(m, F1h, rv1, fyh);
+    const T sc = (T)(-dt / P.dx);
+    const size_t u = uidx(pt, i, j, 0);
+    // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
+    auto upd = [&](int c, T dx_, T dy_) {
+        const T dU = ((T)0 + dx_) + dy_;
+        pt.out[c][u] = pt.in[c][u] + dU * sc;
+    };
+    T st[2][6];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int o = 11 * s;
+        // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
+        // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
+        const T dvx_x = fxh[s][0] - fxl[s][0], dvx_y = fyh[s][2] - fyl[s][2];
+        const T dvy_x = fxh[s][2] - fxl[s][2], dvy_y = fyh[s][0] - fyl[s][0];
+        const T dsxx_x = fxh[s][1] - fxl[s][1], dsxx_y = fyh[s][4] - fyl[s][4];
+        const T dsyy_x = fxh[s][4] - fxl[s][4], dsyy_y = fyh[s][1] - fyl[s][1];
+        const T dszz_x = fxh[s][4] - fxl[s][4], dszz_y = fyh[s][4] - fyl[s][4];
+        const T dsxy_x = fxh[s][3] - fxl[s][3], dsxy_y = fyh[s][3] - fyl[s][3];
+        upd(o + 0, dvx_x, dvx_y);
+        upd(o + 1, dvy_x, dvy_y);
+        const T sxx = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
+        const T syy = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
+        const T szz = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
+        const T sxy = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
+        st[s][0] = sxx; st[s][1] = syy; st[s][2] = szz; st[s][3] = sxy;
+        if (NS == 3) {
+            const T dvz_x = fxh[s][5] - fxl[s][5], dvz_y = fyh[s][5] - fyl[s][5];
+            const T dsxz_x = fxh[s][6] - fxl[s][6], dsxz_y = (T)0 - (T)0;
+            const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[s][6] - fyl[s][6];
+            upd(o + 2, dvz_x, dvz_y);
+            st[s][4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
+            st[s][5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
+        } else {
+            st[s][4] = (T)0;
+            st[s][5] = (T)0;
+        }
+    }
+    if (P.plast && a.plastic) {
+        // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int o = 11 * s;
+            T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
+            T sxz = NS == 3 ? st[s][4] : pt.in[o + 7][u];
+            T syz = NS == 3 ? st[s][5] : pt.in[o + 8][u];
+            plastic_flow<T>(P, (T)dt, st[s][0], st[s][1], st[s][2], st[s][3], sxz, syz, lam, gam);
+            st[s][4] = sxz; st[s][5] = syz;
+            pt.out[o + 9][u] = lam;
+            pt.out[o + 10][u] = gam;
+            if (NS == 2) { pt.out[o + 7][u] = sxz; pt.out[o + 8][u] = syz; }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int o = 11 * s;
+        pt.out[o + 3][u] = st[s][0];
+        pt.out[o + 4][u] = st[s][1];
+        pt.out[o + 5][u] = st[s][2];
+        pt.out[o + 6][u] = st[s][3];
+        if (NS == 3) { pt.out[o + 7][u] = st[s][4]; pt.out[o + 8][u] = st[s][5]; }
+    }
+}
+
 // INT: the tile is full size and its stencil (+-3 cells) lies inside the domain, so no cell of
 // it is a domain-boundary or fault cell; the body then carries no boundary code at all.
 template <typename T, int NS, int BX, int BY, int PK, bool INT>
@@ -453,77 +599,9 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                 put(6 * NS + s * NS + k, F1.s[s][k]);
             }
         __syncthreads();
-        // GETFLUXF (LinElastPhysicsF.ChF:303-350) from a face state, in slot form:
-        // f[s][0]=F(vn) f[s][1]=F(snn) f[s][2]=F(vt) f[s][3]=F(snt) f[s][4]=F(s_transverse normal)
-        // f[s][5]=F(va) f[s][6]=F(sna)
-        auto flux = [&](const Slots<T, NS>& S, T rv, T (&f)[2][7]) {
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const T r = s ? rv : (T)1;
-                if (s == 0) {
-                    f[s][0] = -m.divrho(S.s[s][0]);
-                    f[s][2] = -m.divrho(S.s[s][1]);
-                    f[s][1] = -(S.v[s][0] * m.lam2G);
-                    f[s][4] = -(S.v[s][0] * m.lam);
-                    f[s][3] = -(S.v[s][1] * m.G);
-                    if (NS == 3) { f[s][5] = -m.divrho(S.s[s][NS - 1]); f[s][6] = -(S.v[s][NS - 1] * m.G); }
-                } else {
-                    f[s][0] = -m.divrho(r * S.s[s][0]);
-                    f[s][2] = -m.divrho(r * S.s[s][1]);
-                    f[s][1] = -(r * S.v[s][0] * m.lam2G);
-                    f[s][4] = -(r * S.v[s][0] * m.lam);
-                    f[s][3] = -(r * S.v[s][1] * m.G);
-                    if (NS == 3) { f[s][5] = -m.divrho(r * S.s[s][NS - 1]); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
-                }
-            }
-        };
         // optional flux output for flux registers (LinElastPatchGodunov.cpp:117-118, LinElastPhysics.cpp:153)
-        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) {
-            T f[2][7];
-            flux(F0, (T)1, f);
-            double* Fo = pt.F[0];
-            const int fnx = pt.vhi0 - pt.vlo0 + 2, fny = pt.vhi1 - pt.vlo1 + 1;
-            const size_t fp = (size_t)fnx * fny;
-            const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int o = 11 * s;
-                Fo[fi + (o + 0) * fp] = (double)f[s][0];
-                Fo[fi + (o + 1) * fp] = (double)f[s][2];
-                Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : -(s ? 1.0 : 1.0) * 0.0 * 0.0;
-                Fo[fi + (o + 3) * fp] = (double)f[s][1];
-                Fo[fi + (o + 4) * fp] = (double)f[s][4];
-                Fo[fi + (o + 5) * fp] = (double)f[s][4];
-                Fo[fi + (o + 6) * fp] = (double)f[s][3];
-                Fo[fi + (o + 7) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
-                Fo[fi + (o + 8) * fp] = 0.0;
-                Fo[fi + (o + 9) * fp] = 0.0;
-                Fo[fi + (o + 10) * fp] = 0.0;
-            }
-        }
-        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) {
-            T f[2][7];
-            flux(F1, rv1, f);
-            double* Fo = pt.F[1];
-            const int fnx = pt.vhi0 - pt.vlo0 + 1, fny = pt.vhi1 - pt.vlo1 + 2;
-            const size_t fp = (size_t)fnx * fny;
-            const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int o = 11 * s;
-                Fo[fi + (o + 1) * fp] = (double)f[s][0];
-                Fo[fi + (o + 0) * fp] = (double)f[s][2];
-                Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : 0.0;
-                Fo[fi + (o + 4) * fp] = (double)f[s][1];
-                Fo[fi + (o + 5) * fp] = (double)f[s][4];
-                Fo[fi + (o + 3) * fp] = (double)f[s][4];
-                Fo[fi + (o + 6) * fp] = (double)f[s][3];
-                Fo[fi + (o + 8) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
-                Fo[fi + (o + 7) * fp] = 0.0;
-                Fo[fi + (o + 9) * fp] = 0.0;
-                Fo[fi + (o + 10) * fp] = 0.0;
-            }
-        }
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
         if (valid) {
             Slots<T, NS> F0h, F1h;
 #pragma unroll
@@ -535,73 +613,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                     F1h.v[s][k] = get(4 * NS + s * NS + k, tid + BX);
                     F1h.s[s][k] = get(6 * NS + s * NS + k, tid + BX);
                 }
-            T fxl[2][7], fxh[2][7], fyl[2][7], fyh[2][7];
-            flux(F0, (T)1, fxl);
-            flux(F0h, (T)1, fxh);
-            flux(F1, rv1, fyl);
-            flux(F1h, rv1, fyh);
-            const T sc = (T)(-dt / P.dx);
-            const size_t u = uidx(pt, i, j, 0);
-            // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
-            auto upd = [&](int c, T dx_, T dy_) {
-                const T dU = ((T)0 + dx_) + dy_;
-                pt.out[c][u] = pt.in[c][u] + dU * sc;
-            };
-            T st[2][6];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int o = 11 * s;
-                // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
-                // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
-                const T dvx_x = fxh[s][0] - fxl[s][0], dvx_y = fyh[s][2] - fyl[s][2];
-                const T dvy_x = fxh[s][2] - fxl[s][2], dvy_y = fyh[s][0] - fyl[s][0];
-                const T dsxx_x = fxh[s][1] - fxl[s][1], dsxx_y = fyh[s][4] - fyl[s][4];
-                const T dsyy_x = fxh[s][4] - fxl[s][4], dsyy_y = fyh[s][1] - fyl[s][1];
-                const T dszz_x = fxh[s][4] - fxl[s][4], dszz_y = fyh[s][4] - fyl[s][4];
-                const T dsxy_x = fxh[s][3] - fxl[s][3], dsxy_y = fyh[s][3] - fyl[s][3];
-                upd(o + 0, dvx_x, dvx_y);
-                upd(o + 1, dvy_x, dvy_y);
-                const T sxx = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
-                const T syy = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
-                const T szz = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
-                const T sxy = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
-                st[s][0] = sxx; st[s][1] = syy; st[s][2] = szz; st[s][3] = sxy;
-                if (NS == 3) {
-                    const T dvz_x = fxh[s][5] - fxl[s][5], dvz_y = fyh[s][5] - fyl[s][5];
-                    const T dsxz_x = fxh[s][6] - fxl[s][6], dsxz_y = (T)0 - (T)0;
-                    const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[s][6] - fyl[s][6];
-                    upd(o + 2, dvz_x, dvz_y);
-                    st[s][4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
-                    st[s][5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
-                } else {
-                    st[s][4] = (T)0;
-                    st[s][5] = (T)0;
-                }
-            }
-            if (P.plast && a.plastic) {
-                // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const int o = 11 * s;
-                    T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
-                    T sxz = NS == 3 ? st[s][4] : pt.in[o + 7][u];
-                    T syz = NS == 3 ? st[s][5] : pt.in[o + 8][u];
-                    plastic_flow<T>(P, (T)dt, st[s][0], st[s][1], st[s][2], st[s][3], sxz, syz, lam, gam);
-                    st[s][4] = sxz; st[s][5] = syz;
-                    pt.out[o + 9][u] = lam;
-                    pt.out[o + 10][u] = gam;
-                    if (NS == 2) { pt.out[o + 7][u] = sxz; pt.out[o + 8][u] = syz; }
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                const int o = 11 * s;
-                pt.out[o + 3][u] = st[s][0];
-                pt.out[o + 4][u] = st[s][1];
-                pt.out[o + 5][u] = st[s][2];
-                pt.out[o + 6][u] = st[s][3];
-                if (NS == 3) { pt.out[o + 7][u] = st[s][4]; pt.out[o + 8][u] = st[s][5]; }
-            }
+            update_cell<T, NS>(a, pt, P, m, dt, i, j, F0, F0h, F1, F1h);
         }
     }
 }
@@ -623,34 +635,6 @@ k_godunov(const KArgs<T> a)
     T* sm = reinterpret_cast<T*>(smem_raw);
     const Tile tl = a.tiles[blockIdx.x];
     godunov_tile<T, NS, BX, BY, PK, false>(a, tl, sm);
-}
-
-// Occupancy target (waves per SIMD) of the interior kernel.  FP64 mode II fits 128 VGPRs,
-// so two 512-thread workgroups share a CU and one's barriers overlap the other's work.
-#ifndef TMK_INTERIOR_WAVES_F64
-#define TMK_INTERIOR_WAVES_F64 4
-#endif
-template <typename T, int NS>
-constexpr int interior_waves()
-{
-    return (sizeof(T) == 8 && NS == 2) ? TMK_INTERIOR_WAVES_F64 : 1;
-}
-
-// interior tiles only (the host sorts them out with the same rule); a tile that is not
-// interior reports TMK_E_BOX and is left untouched
-template <typename T, int NS, int BX, int BY, int PK>
-__global__ void __launch_bounds__(BX * BY, (interior_waves<T, NS>()))
-k_godunov_interior(const KArgs<T> a)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sm = reinterpret_cast<T*>(smem_raw);
-    const Tile tl = a.tiles[blockIdx.x];
-    Scen& S = a.scen[a.patches[tl.patch].scen];
-    if (!tile_interior<BX, BY>(tl, S.P)) {
-        if (threadIdx.x == 0) report(&S.err, TMK_E_BOX);
-        return;
-    }
-    godunov_tile<T, NS, BX, BY, PK, true>(a, tl, sm);
 }
 
 // LinElastPatchGodunov::plasticUpdate on its own (PLASTICCORRECTION over the valid box)

@@ -101,13 +101,16 @@ __host__ __device__ constexpr int slot_comp(int d, int k, int is_s)
 // ---------------------------------------------------------------------------
 // slopes and PPM (restated from the published method; see oracle for the same form)
 // ---------------------------------------------------------------------------
+// van Leer: lim = min(2|l|, 2|r|, |c|), zero at extrema, with the sign of c.  2 min(|l|, |r|)
+// equals min(2|l|, 2|r|) exactly (doubling is exact and monotone) and copysign(lim, c) equals
+// lim * copysign(1, c) for lim >= 0, signed zeros included.
 template <typename T>
 __device__ __forceinline__ T vl_limit(T c, T l, T r)
 {
-    T lim = fmin((T)2 * fabs(l), (T)2 * fabs(r));
+    T lim = (T)2 * fmin(fabs(l), fabs(r));
     lim = fmin(lim, fabs(c));
     if (l * r < (T)0) lim = (T)0;
-    return lim * copysign((T)1, c);
+    return copysign(lim, c);
 }
 
 // side classification of stencil cell: 0 centred, 1 domain lo, 2 domain hi

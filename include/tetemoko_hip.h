@@ -198,8 +198,15 @@ int tmk_batch_remove(tmk_batch *b, tmk_level *lev);
 int tmk_batch_run(tmk_batch *b, int nsteps);
 /* synchronize; first error reported by any member (0 = none) */
 int tmk_batch_sync(tmk_batch *b);
+/* synchronize; *code = the first error member lev reported (0 = none).  No
+ * counterpart in the reference, which runs one scenario per process: a serving loop
+ * uses it to retire only the scenario that failed (a member's error does not touch
+ * the others' data). */
+int tmk_batch_member_error(tmk_batch *b, tmk_level *lev, int *code);
 
-/* Tile shape of the fused update (autotuner hook); 0 = library default. */
+/* Workgroup tile shape of the update kernels (autotuner hook): bx x by threads
+ * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 32x8, 32x12, 32x16,
+ * 32x32, 64x4, 64x8, 64x16.  0, 0 = the library default for the context's precision. */
 int tmk_set_tile(tmk_ctx *ctx, int bx, int by);
 
 #ifdef __cplusplus

@@ -140,7 +140,7 @@ def test_patch_antiplane(where):
         assert np.all(_relerr(Bg, Br) <= TOL)
 
 
-@pytest.mark.parametrize("tile", [(32, 8), (32, 16), (64, 4), (64, 8), (32, 32), (64, 16)])
+@pytest.mark.parametrize("tile", [(32, 8), (32, 12), (32, 16), (64, 4), (64, 8), (32, 32), (64, 16)])
 def test_patch_tile_shapes(tile):
     cfg = _cfg("rs", "PPM")
     (Ug, Ur), (Bg, Br), _, _, _ = _patch_case(cfg, BOXES["whole"], seed=5, tile=tile)
@@ -199,11 +199,12 @@ def test_level_steps(fault, maxbox):
 
 
 @pytest.mark.parametrize("pred,prim,char,fourth", [("PPM", 1, 0, 1), ("PPM", 0, 1, 1), ("PLM", 1, 0, 1)])
-@pytest.mark.parametrize("tile", [None, (64, 8)])
+@pytest.mark.parametrize("tile", [None, (32, 16), (32, 12), (64, 4), (64, 8)])
 def test_level_interior_tiles(pred, prim, char, fourth, tile):
-    """A grid with many interior tiles (the specialised interior kernel) and boundary tiles
-    (the general kernel, on the side stream).  Away from the fault's reach (3 cells a step)
-    the state is bit-identical to the oracle; everywhere within 1e-12."""
+    """A grid with many interior tiles (the specialised interior kernel: one thread per cell and
+    side on the small FP64 tiles, one thread per cell on 32x16 and 64x8) and boundary tiles (the
+    general kernel, on the side stream).  Away from the fault's reach (3 cells a step) the state
+    is bit-identical to the oracle; everywhere within 1e-12."""
     cfg = _cfg("rs", pred, nx=200, ny=120, prim=prim, char=char, fourth=fourth)
     nsteps = 4
     (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], nsteps, seed=31, tile=tile)
@@ -224,6 +225,7 @@ def test_profile_kernels():
     cfg = _cfg("rs", "PPM", nx=200, ny=120)
     U, B = _state(cfg, 5)
     ctx = capi.Context(cfg, capi.PREC_FP64)
+    ctx.set_tile(32, 16)
     lev = capi.Level(ctx, [(0, 0, 199, 119)])
     lev.upload(0, U, B)
     p = lev.profile_kernels(3, 0.0, scenarios.dt_of(cfg))
@@ -365,6 +367,43 @@ def test_batch_member_equals_alone(precision):
     got1 = _download(lev1, U1, B1, bx1)
     assert np.array_equal(got0[0], alone[0][0]) and np.array_equal(got0[1], alone[0][1])
     assert np.array_equal(got1[0], alone[1][0]) and np.array_equal(got1[1], alone[1][1])
+    batch.close()
+    for o in (lev0, lev1, ctx0, ctx1):
+        o.close()
+
+
+def test_batch_member_error_isolated():
+    """A member that fails (NaN psi on one fault face) reports its own error; the other member's
+    result is bit-identical to running alone, and the batch carries on once it is removed."""
+    good = scenarios.config_c1(seed=5, nx=96, ny=48, dx=25.0)
+    bad = scenarios.config_c1(seed=6, nx=96, ny=48, dx=25.0)
+    ctxa, leva, Ua, Ba, bxa = _fresh_level(good, 21, capi.PREC_FP64)
+    leva.run(5, 0.0, scenarios.dt_of(good))
+    leva.sync()
+    alone = _download(leva, Ua, Ba, bxa)
+    leva.close()
+    ctxa.close()
+    ctx0, lev0, U0, B0, bx0 = _fresh_level(good, 21, capi.PREC_FP64)
+    U1, B1 = scenarios.level_arrays(bad)
+    B1[10, G + 40] = np.nan
+    ctx1 = capi.Context(bad, capi.PREC_FP64)
+    lev1 = capi.Level(ctx1, [(0, 0, 95, 47)])
+    lev1.upload(0, U1, B1)
+    batch = capi.Batch()
+    batch.add(lev0, 0.0, scenarios.dt_of(good))
+    batch.add(lev1, 0.0, scenarios.dt_of(bad))
+    batch.run(3)
+    with pytest.raises(capi.TmkError) as ei:
+        batch.sync()
+    assert ei.value.code == capi.E_PSI_NAN
+    assert batch.member_error(lev1) == capi.E_PSI_NAN
+    assert batch.member_error(lev0) == 0
+    batch.remove(lev1)
+    batch.run(2)
+    batch.sync()
+    batch.remove(lev0)
+    got = _download(lev0, U0, B0, bx0)
+    assert np.array_equal(got[0], alone[0]) and np.array_equal(got[1], alone[1])
     batch.close()
     for o in (lev0, lev1, ctx0, ctx1):
         o.close()

@@ -38,7 +38,7 @@ EXPORTED = (
     "tmk_patch_update_state", "tmk_patch_plastic_update",
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
-    "tmk_level_profile_kernels",
+    "tmk_level_profile_kernels", "tmk_batch_member_error",
 )
 
 
@@ -163,6 +163,7 @@ def lib():
         L.tmk_batch_remove.argtypes = [vp, vp]
         L.tmk_batch_run.argtypes = [vp, c_int]
         L.tmk_batch_sync.argtypes = [vp]
+        L.tmk_batch_member_error.argtypes = [vp, vp, ctypes.POINTER(c_int)]
         L.tmk_level_fill.argtypes = [vp, c_int, _PD, _PD]
         L.tmk_level_download_fault.argtypes = [vp, c_int, _PD]
         L.tmk_level_tag.argtypes = [vp, ctypes.POINTER(TagParams), ctypes.POINTER(ctypes.c_ubyte)]
@@ -408,3 +409,9 @@ class Batch:
 
     def sync(self):
         _check(lib().tmk_batch_sync(self.h), "tmk_batch_sync")
+
+    def member_error(self, level):
+        """Error code the member reported (0 = none); other members are unaffected by it."""
+        code = ctypes.c_int()
+        _check(lib().tmk_batch_member_error(self.h, level.h, ctypes.byref(code)), "tmk_batch_member_error")
+        return code.value

@@ -18,6 +18,12 @@
 #include "tmk_kernels.hpp"
 #include "tmk_interior.hpp"
 
+#ifndef TMK_DEFAULT_BY_F64
+#define TMK_DEFAULT_BY_F64 8
+#endif
+
+static void default_tile(int precision, int& bx, int& by);
+
 using namespace tmk;
 
 #define HIPCHK(x)                                                                           \
@@ -109,17 +115,38 @@ bool tile_interior_host(const Tile& t, const DevParams& P, int bx, int by)
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 
-// interior tiles first, then boundary tiles, each part in XCD order; returns the interior count
-template <typename F>
-int partition_tiles(std::vector<Tile>& t, F is_interior)
+// Workgroup shape of the boundary-tile launch, whatever the interior shape: boundary tiles are
+// cut into pieces of at most (BND_BX - 2) x (BND_BY - 2) cells, so that the general kernel's
+// short workgroups start early and fit between the interior ones.
+constexpr int BND_BX = 32, BND_BY = 8;
+
+// interior tiles first (XCD order), then the boundary tiles cut into boundary-shape pieces,
+// fault-row pieces first (they carry the fault solve, the longest workgroups); returns the
+// interior count
+template <typename F, typename G>
+int partition_tiles(std::vector<Tile>& t, F is_interior, G on_fault_row)
 {
-    std::vector<Tile> in, bd;
-    for (const Tile& x : t) (is_interior(x) ? in : bd).push_back(x);
+    std::vector<Tile> in, bf, bo;
+    for (const Tile& x : t) {
+        if (is_interior(x)) {
+            in.push_back(x);
+            continue;
+        }
+        for (int y0 = x.y0; y0 <= x.y1; y0 += BND_BY - 2)
+            for (int x0 = x.x0; x0 <= x.x1; x0 += BND_BX - 2) {
+                Tile p = x;
+                p.x0 = x0;
+                p.y0 = y0;
+                p.x1 = std::min(x0 + BND_BX - 3, x.x1);
+                p.y1 = std::min(y0 + BND_BY - 3, x.y1);
+                (on_fault_row(p) ? bf : bo).push_back(p);
+            }
+    }
     xcd_order(in);
-    xcd_order(bd);
     const int nint = (int)in.size();
     t.swap(in);
-    t.insert(t.end(), bd.begin(), bd.end());
+    t.insert(t.end(), bf.begin(), bf.end());
+    t.insert(t.end(), bo.begin(), bo.end());
     return nint;
 }
 
@@ -317,6 +344,7 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
     if (!c) return TMK_E_NOMEM;
     c->hp = *p;
     c->precision = precision;
+    default_tile(precision, c->bx, c->by);
     fill_dev_params(*p, c->P);
     c->antiplane = !(p->sxz0 == 0.0 && p->syz0 == 0.0);
     if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -357,14 +385,22 @@ extern "C" int tmk_ctx_destroy(tmk_ctx* c)
 
 static bool tile_ok(int bx, int by)
 {
-    return (bx == 32 && (by == 8 || by == 16 || by == 32)) || (bx == 64 && (by == 4 || by == 8 || by == 16));
+    return (bx == 32 && (by == 8 || by == 12 || by == 16 || by == 32)) || (bx == 64 && (by == 4 || by == 8 || by == 16));
+}
+
+// Default tile per precision, from timing the 4096 x 2048 rate-and-state level on MI355X:
+// FP64 runs one thread per (cell, side) on interior tiles and is occupancy-bound, best with
+// small workgroups; FP32 bulk is issue-bound, best with the larger tile (less ring overlap).
+static void default_tile(int precision, int& bx, int& by)
+{
+    bx = 32;
+    by = precision == TMK_PREC_FP64 ? TMK_DEFAULT_BY_F64 : 16;
 }
 
 extern "C" int tmk_set_tile(tmk_ctx* c, int bx, int by)
 {
     if (!c) return TMK_E_ARG;
-    if (bx == 0) bx = 32;
-    if (by == 0) by = 16;
+    if (bx == 0 && by == 0) default_tile(c->precision, bx, by);
     if (!tile_ok(bx, by)) return TMK_E_ARG;
     c->bx = bx;
     c->by = by;
@@ -391,7 +427,8 @@ static int launch_tiles_t(hipStream_t st, const KArgs<T>& a, bool interior)
     const void* k = interior ? (const void*)k_godunov_interior<T, NS, BX, BY, PK> : (const void*)k_godunov<T, NS, BX, BY, PK>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (interior)
-        hipLaunchKernelGGL((k_godunov_interior<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(NT), shm, st, a);
+        hipLaunchKernelGGL((k_godunov_interior<T, NS, BX, BY, PK>), dim3(a.ntiles),
+                           dim3(interior_threads<T, NS, BX, BY>()), shm, st, a);
     else
         hipLaunchKernelGGL((k_godunov<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(NT), shm, st, a);
     return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
@@ -402,6 +439,7 @@ static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a, bo
 {
     if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16, PK>(st, a, in);
     if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8, PK>(st, a, in);
+    if (bx == 32 && by == 12) return launch_tiles_t<T, NS, 32, 12, PK>(st, a, in);
     if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32, PK>(st, a, in);
     if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8, PK>(st, a, in);
     if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4, PK>(st, a, in);
@@ -447,7 +485,7 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         b.tiles = tiles + nint;
         b.ntiles = nbnd;
         if (tm) HIPCHK(hipEventRecord(tm->ba, sb));
-        if ((e = launch_tiles<T>(sb, ns, pk, bx, by, b, false))) return e;
+        if ((e = launch_tiles<T>(sb, ns, pk, BND_BX, BND_BY, b, false))) return e;
         if (tm) HIPCHK(hipEventRecord(tm->bb, sb));
     }
     if (nint > 0) {
@@ -592,7 +630,9 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     for (size_t m = 0; m < mem.size(); ++m)
         for (size_t b = 0; b < mem[m]->boxes.size(); ++b) patch_scen.push_back((int)m);
     const int bx = G.bx, by = G.by;
-    G.nint = partition_tiles(tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); });
+    G.nint = partition_tiles(
+        tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
+        [&](const Tile& t) { return t.y0 == sc[patch_scen[t.patch]].P.dlo[1]; });
     G.cells_int = G.cells_bnd = 0;
     for (size_t k = 0; k < tiles.size(); ++k)
         ((int)k < G.nint ? G.cells_int : G.cells_bnd) +=
@@ -706,7 +746,9 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         std::vector<Tile> tiles;
         make_tiles(valid, 0, c->bx - 2, c->by - 2, tiles);
         const int bx = c->bx, by = c->by;
-        const int nint = partition_tiles(tiles, [&](const Tile& t) { return tile_interior_host(t, c->P, bx, by); });
+        const int nint = partition_tiles(
+            tiles, [&](const Tile& t) { return tile_interior_host(t, c->P, bx, by); },
+            [&](const Tile& t) { return t.y0 == c->P.dlo[1]; });
         const size_t off_t = ((sizeof(Patch<T>) + 255) / 256) * 256;
         const size_t off_s = off_t + ((sizeof(Tile) * tiles.size() + 255) / 256) * 256;
         if ((e = c->scratch[5].ensure(off_s + sizeof(Scen)))) return e;
@@ -1321,6 +1363,17 @@ extern "C" int tmk_batch_run(tmk_batch* B, int nsteps)
     for (tmk_level* L : B->members)
         for (int k = 0; k < nsteps; ++k) L->time = L->time + L->dt;
     return TMK_OK;
+}
+
+extern "C" int tmk_batch_member_error(tmk_batch* B, tmk_level* L, int* code)
+{
+    if (!B || !L || !code || L->batch != B) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(B->stream));
+    *code = 0;
+    if (!B->built) return TMK_OK;
+    const size_t m = std::find(B->members.begin(), B->members.end(), L) - B->members.begin();
+    if (m >= B->members.size()) return TMK_E_ARG;
+    return group_error(B->grp, B->stream, (int)m, code);
 }
 
 extern "C" int tmk_batch_sync(tmk_batch* B)

@@ -251,21 +251,203 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
     }
 }
 
+// The same update with one thread per (cell, side): NT = 2 BX BY threads, side = tid / (BX BY).
+// Inside an interior tile the two sides of the fault never meet (they are coupled only by the
+// fault solve), so each side's thread carries half the state: half the registers per thread and
+// twice the waves per workgroup.  Side 2's y-sweep sign rvy = -1 enters every expression as a
+// product, which is exact, so both sides round exactly as in interior_tile.  LDS layout and
+// barriers are those of interior_tile with a side index in front of every plane.
+template <typename T, int NS, int BX, int BY, int PK>
+__device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Tile& tl, T* sm)
+{
+    static_assert(64 % BX == 0 || BX % 64 == 0, "x lane shifts need whole tile rows per wave");
+    constexpr int NC = BX * BY;  // cells per tile, a multiple of 64: each wave holds one side
+    static_assert(NC % 64 == 0, "a wave must not hold both sides");
+    constexpr int SX = BX + 4, SY = BY + 4, SP = SX * SY;
+    constexpr int NQ1 = stage_nq<NS>();
+    T* const stg = sm;                  // [2][NQ1][SY][SX]
+    T* const exA = sm + 2 * NQ1 * SP;   // [2][NS][NC]
+    T* const exB = sm;                  // [2][2 NS][NC], over the staging box
+
+    const Patch<T>& pt = a.patches[tl.patch];
+    Scen& S = a.scen[pt.scen];
+    const DevParams& P = S.P;
+    const Mat<T> m(P);
+    const int tid = threadIdx.x;
+    const int sd = __builtin_amdgcn_readfirstlane(tid / NC);  // uniform per wave
+    const int c = tid - sd * NC;
+    const int lx = c % BX, ly = c / BX;
+    const int i = tl.x0 - 1 + lx, j = tl.y0 - 1 + ly;
+    const double dt = S.dt;
+    const T dtbydx = (T)(dt / P.dx);
+    const int pred = P.pred;
+    const T rv1 = (T)-1;
+    const T rvy = sd ? rv1 : (T)1;  // velocity sign of this side in y
+    const int q0 = sd * NQ1;        // first staged plane of this side
+
+    // ---------------- staging: this side's stencil planes ----------------
+    {
+        const T* src[NQ1];
+#pragma unroll
+        for (int q = 0; q < NQ1; ++q) src[q] = pt.in[stage_plane_comp<NS>(q0 + q)];
+#pragma unroll
+        for (int r = 0; r < (SP + NC - 1) / NC; ++r) {
+            const int p = c + r * NC;
+            if (p < SP) {
+                const int yy = p / SX, xx = p - yy * SX;
+                const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
+#pragma unroll
+                for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = src[q][g];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---------------- stages 1-2: normal predictors ----------------
+    T WmXv[NS], WmXs[NS], WpXv[NS], WpXs[NS], WmYv[NS], WmYs[NS], WpYv[NS], WpYs[NS];
+    {
+        T w[NS][2][5];
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int o = 0; o < 5; ++o)
+                    w[k][q][o] = stg[(q0 + stage_plane<NS>(0, 0, k, q)) * SP + (ly + 2) * SX + lx + o];
+        normal_pred_side<T, NS, PK>(P, m, pred, (T)1, w, 0, -(1 << 20), 1 << 20, dtbydx, WmXv, WmXs, WpXv, WpXs);
+    }
+    {
+        T w[NS][2][5];
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int o = 0; o < 5; ++o)
+                    w[k][q][o] = stg[(q0 + stage_plane<NS>(1, 0, k, q)) * SP + (ly + o) * SX + lx + 2];
+        normal_pred_side<T, NS, PK>(P, m, pred, rvy, w, 0, -(1 << 20), 1 << 20, dtbydx, WmYv, WmYs, WpYv, WpYs);
+    }
+
+    // ---------------- stage 3: predictor face states ----------------
+    T H0v[NS], H0s[NS], H1v[NS], H1s[NS];
+    {
+        T cpx[NS], cpy[NS], wpL[NS], wmR[NS];
+        right_chars_side<T, NS>(m, (T)1, WpXv, WpXs, cpx);
+        right_chars_side<T, NS>(m, rvy, WpYv, WpYs, cpy);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            exA[(sd * NS + k) * NC + c] = cpy[k];
+            wpL[k] = lane_shift<DPP_FROM_PREV>(cpx[k]);
+        }
+        __syncthreads();
+        left_chars_side<T, NS>(m, (T)1, WmXv, WmXs, wmR);
+        riemann_side<T, NS>(m, (T)1, wpL, wmR, H0v, H0s);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) wpL[k] = exA[(sd * NS + k) * NC + c - BX];
+        left_chars_side<T, NS>(m, rvy, WmYv, WmYs, wmR);
+        riemann_side<T, NS>(m, rvy, wpL, wmR, H1v, H1s);
+    }
+    // ---------------- stage 4: transverse corrections ----------------
+    {
+        T Hv[NS], Hs[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            exB[(sd * 2 * NS + k) * NC + c] = H1v[k];
+            exB[(sd * 2 * NS + NS + k) * NC + c] = H1s[k];
+            Hv[k] = lane_shift<DPP_FROM_NEXT>(H0v[k]);
+            Hs[k] = lane_shift<DPP_FROM_NEXT>(H0s[k]);
+        }
+        __syncthreads();
+        const T scale = (T)(-(1.0 / 2.0) * dt / P.dx);
+        transverse_side<T, NS>(m, scale, H0v, H0s, Hv, Hs, WmYv, WmYs);
+        transverse_side<T, NS>(m, scale, H0v, H0s, Hv, Hs, WpYv, WpYs);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            Hv[k] = exB[(sd * 2 * NS + k) * NC + c + BX];
+            Hs[k] = exB[(sd * 2 * NS + NS + k) * NC + c + BX];
+        }
+        const T scx = sd ? rv1 * scale : scale;
+        transverse_side<T, NS>(m, scx, H1v, H1s, Hv, Hs, WmXv, WmXs);
+        transverse_side<T, NS>(m, scx, H1v, H1s, Hv, Hs, WpXv, WpXs);
+    }
+    // ---------------- stage 5: final face states ----------------
+    Faces1<T, NS> F;
+    {
+        T cpx[NS], cpy[NS], wpL[NS], wmR[NS];
+        right_chars_side<T, NS>(m, (T)1, WpXv, WpXs, cpx);
+        right_chars_side<T, NS>(m, rvy, WpYv, WpYs, cpy);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            exA[(sd * NS + k) * NC + c] = cpy[k];
+            wpL[k] = lane_shift<DPP_FROM_PREV>(cpx[k]);
+        }
+        __syncthreads();
+        left_chars_side<T, NS>(m, (T)1, WmXv, WmXs, wmR);
+        riemann_side<T, NS>(m, (T)1, wpL, wmR, F.xlv, F.xls);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) wpL[k] = exA[(sd * NS + k) * NC + c - BX];
+        left_chars_side<T, NS>(m, rvy, WmYv, WmYs, wmR);
+        riemann_side<T, NS>(m, rvy, wpL, wmR, F.ylv, F.yls);
+    }
+    // ---------------- stage 6: flux difference and update ----------------
+    {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            exB[(sd * 2 * NS + k) * NC + c] = F.ylv[k];
+            exB[(sd * 2 * NS + NS + k) * NC + c] = F.yls[k];
+            F.xhv[k] = lane_shift<DPP_FROM_NEXT>(F.xlv[k]);
+            F.xhs[k] = lane_shift<DPP_FROM_NEXT>(F.xls[k]);
+        }
+        __syncthreads();
+        if (pt.F[0] && lx >= 1 && ly >= 1 && ly <= BY - 2) store_flux_side<T, NS>(pt, m, 0, F.xlv, F.xls, (T)1, i, j, sd);
+        if (pt.F[1] && ly >= 1 && lx >= 1 && lx <= BX - 2) store_flux_side<T, NS>(pt, m, 1, F.ylv, F.yls, rvy, i, j, sd);
+        if (lx >= 1 && lx <= BX - 2 && ly >= 1 && ly <= BY - 2) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                F.yhv[k] = exB[(sd * 2 * NS + k) * NC + c + BX];
+                F.yhs[k] = exB[(sd * 2 * NS + NS + k) * NC + c + BX];
+            }
+            update_cell_side<T, NS>(a, pt, P, m, dt, i, j, sd, F);
+        }
+    }
+}
+
 // Occupancy target (waves per SIMD) of the interior kernel.  FP64 mode II fits 128 VGPRs,
 // so two 512-thread workgroups share a CU and one's barriers overlap the other's work.
 #ifndef TMK_INTERIOR_WAVES_F64
 #define TMK_INTERIOR_WAVES_F64 4
 #endif
-template <typename T, int NS>
+#ifndef TMK_SPLIT_WAVES_F64
+#define TMK_SPLIT_WAVES_F64 0
+#endif
+// one thread per (cell, side) for FP64 mode II on tiles of at most 384 cells (32x8, 32x12,
+// 64x4): three or two workgroups of 8 or 12 waves share a CU, six waves per SIMD at 80 VGPRs.
+// Larger tiles keep one thread per cell (a 1024-thread split workgroup would run alone on its CU).
+template <typename T, int NS, int BX, int BY>
+__host__ __device__ constexpr bool interior_split()
+{
+#ifdef TMK_NO_SPLIT
+    return false;
+#else
+    return sizeof(T) == 8 && NS == 2 && BX * BY <= 384;
+#endif
+}
+template <typename T, int NS, int BX, int BY>
+__host__ __device__ constexpr int interior_threads()
+{
+    return interior_split<T, NS, BX, BY>() ? 2 * BX * BY : BX * BY;
+}
+template <typename T, int NS, int BX, int BY>
 constexpr int interior_waves()
 {
-    return (sizeof(T) == 8 && NS == 2) ? TMK_INTERIOR_WAVES_F64 : 1;
+    return interior_split<T, NS, BX, BY>() ? TMK_SPLIT_WAVES_F64
+                                           : ((sizeof(T) == 8 && NS == 2) ? TMK_INTERIOR_WAVES_F64 : 1);
 }
 
 // interior tiles only (the host sorts them out with the same rule); a tile that is not
 // interior reports TMK_E_BOX and is left untouched
 template <typename T, int NS, int BX, int BY, int PK>
-__global__ void __launch_bounds__(BX * BY, (interior_waves<T, NS>()))
+__global__ void __launch_bounds__((interior_threads<T, NS, BX, BY>()), (interior_waves<T, NS, BX, BY>()))
 k_godunov_interior(const KArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -276,7 +458,10 @@ k_godunov_interior(const KArgs<T> a)
         if (threadIdx.x == 0) report(&S.err, TMK_E_BOX);
         return;
     }
-    interior_tile<T, NS, BX, BY, PK>(a, tl, sm);
+    if constexpr (interior_split<T, NS, BX, BY>())
+        interior_tile_split<T, NS, BX, BY, PK>(a, tl, sm);
+    else
+        interior_tile<T, NS, BX, BY, PK>(a, tl, sm);
 }
 
 }  // namespace tmk

@@ -94,149 +94,172 @@ __device__ __forceinline__ int stage_comp(int d, int c)
     return slot_comp(d, r >> 1, r & 1) + 11 * s;
 }
 
-// GETFLUXF (LinElastPhysicsF.ChF:303-350) from a face state, in slot form:
-// f[s][0]=F(vn) f[s][1]=F(snn) f[s][2]=F(vt) f[s][3]=F(snt) f[s][4]=F(s_transverse normal)
-// f[s][5]=F(va) f[s][6]=F(sna)
+// GETFLUXF (LinElastPhysicsF.ChF:303-350) from the face state of one side, in slot form:
+// f[0]=F(vn) f[1]=F(snn) f[2]=F(vt) f[3]=F(snt) f[4]=F(s_transverse normal) f[5]=F(va) f[6]=F(sna);
+// r is the side's velocity sign in the face direction (a product with r = 1 is exact)
 template <typename T, int NS>
-__device__ __forceinline__ void face_flux(const Mat<T>& m, const Slots<T, NS>& S, T rv, T (&f)[2][7])
+__device__ __forceinline__ void face_flux_side(const Mat<T>& m, const T (&v)[NS], const T (&s)[NS], T r, T (&f)[7])
 {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const T r = s ? rv : (T)1;
-        if (s == 0) {
-            f[s][0] = -m.divrho(S.s[s][0]);
-            f[s][2] = -m.divrho(S.s[s][1]);
-            f[s][1] = -(S.v[s][0] * m.lam2G);
-            f[s][4] = -(S.v[s][0] * m.lam);
-            f[s][3] = -(S.v[s][1] * m.G);
-            if (NS == 3) { f[s][5] = -m.divrho(S.s[s][NS - 1]); f[s][6] = -(S.v[s][NS - 1] * m.G); }
-        } else {
-            f[s][0] = -m.divrho(r * S.s[s][0]);
-            f[s][2] = -m.divrho(r * S.s[s][1]);
-            f[s][1] = -(r * S.v[s][0] * m.lam2G);
-            f[s][4] = -(r * S.v[s][0] * m.lam);
-            f[s][3] = -(r * S.v[s][1] * m.G);
-            if (NS == 3) { f[s][5] = -m.divrho(r * S.s[s][NS - 1]); f[s][6] = -(r * S.v[s][NS - 1] * m.G); }
-        }
+    f[0] = -m.divrho(r * s[0]);
+    f[2] = -m.divrho(r * s[1]);
+    f[1] = -(r * v[0] * m.lam2G);
+    f[4] = -(r * v[0] * m.lam);
+    f[3] = -(r * v[1] * m.G);
+    if (NS == 3) {
+        f[5] = -m.divrho(r * s[NS - 1]);
+        f[6] = -(r * v[NS - 1] * m.G);
     }
 }
 
-// the 22-component flux of face state Fs on the lo d-face of cell (i, j) into the flux
-// register array of direction d (surroundingNodes(valid box, d))
 template <typename T, int NS>
-__device__ __forceinline__ void store_flux(const Patch<T>& pt, const Mat<T>& m, int d, const Slots<T, NS>& Fs, T rv,
-                                           int i, int j)
+__device__ __forceinline__ void face_flux(const Mat<T>& m, const Slots<T, NS>& S, T rv, T (&f)[2][7])
 {
-    T f[2][7];
-    face_flux<T, NS>(m, Fs, rv, f);
+    face_flux_side<T, NS>(m, S.v[0], S.s[0], (T)1, f[0]);
+    face_flux_side<T, NS>(m, S.v[1], S.s[1], rv, f[1]);
+}
+
+// the 11 flux components of side sd from face state (v, s) on the lo d-face of cell (i, j) into
+// the flux register array of direction d (surroundingNodes(valid box, d))
+template <typename T, int NS>
+__device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
+                                                const T (&sv)[NS], T r, int i, int j, int sd)
+{
+    T f[7];
+    face_flux_side<T, NS>(m, v, sv, r, f);
     double* Fo = pt.F[d];
     const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
     const size_t fp = (size_t)fnx * fny;
     const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const int o = 11 * s;
-        if (d == 0) {
-            Fo[fi + (o + 0) * fp] = (double)f[s][0];
-            Fo[fi + (o + 1) * fp] = (double)f[s][2];
-            Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : -(s ? 1.0 : 1.0) * 0.0 * 0.0;
-            Fo[fi + (o + 3) * fp] = (double)f[s][1];
-            Fo[fi + (o + 4) * fp] = (double)f[s][4];
-            Fo[fi + (o + 5) * fp] = (double)f[s][4];
-            Fo[fi + (o + 6) * fp] = (double)f[s][3];
-            Fo[fi + (o + 7) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
-            Fo[fi + (o + 8) * fp] = 0.0;
-        } else {
-            Fo[fi + (o + 1) * fp] = (double)f[s][0];
-            Fo[fi + (o + 0) * fp] = (double)f[s][2];
-            Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[s][5] : 0.0;
-            Fo[fi + (o + 4) * fp] = (double)f[s][1];
-            Fo[fi + (o + 5) * fp] = (double)f[s][4];
-            Fo[fi + (o + 3) * fp] = (double)f[s][4];
-            Fo[fi + (o + 6) * fp] = (double)f[s][3];
-            Fo[fi + (o + 8) * fp] = NS == 3 ? (double)f[s][6] : 0.0;
-            Fo[fi + (o + 7) * fp] = 0.0;
+    const int o = 11 * sd;
+    if (d == 0) {
+        Fo[fi + (o + 0) * fp] = (double)f[0];
+        Fo[fi + (o + 1) * fp] = (double)f[2];
+        Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[5] : -(sd ? 1.0 : 1.0) * 0.0 * 0.0;
+        Fo[fi + (o + 3) * fp] = (double)f[1];
+        Fo[fi + (o + 4) * fp] = (double)f[4];
+        Fo[fi + (o + 5) * fp] = (double)f[4];
+        Fo[fi + (o + 6) * fp] = (double)f[3];
+        Fo[fi + (o + 7) * fp] = NS == 3 ? (double)f[6] : 0.0;
+        Fo[fi + (o + 8) * fp] = 0.0;
+    } else {
+        Fo[fi + (o + 1) * fp] = (double)f[0];
+        Fo[fi + (o + 0) * fp] = (double)f[2];
+        Fo[fi + (o + 2) * fp] = NS == 3 ? (double)f[5] : 0.0;
+        Fo[fi + (o + 4) * fp] = (double)f[1];
+        Fo[fi + (o + 5) * fp] = (double)f[4];
+        Fo[fi + (o + 3) * fp] = (double)f[4];
+        Fo[fi + (o + 6) * fp] = (double)f[3];
+        Fo[fi + (o + 8) * fp] = NS == 3 ? (double)f[6] : 0.0;
+        Fo[fi + (o + 7) * fp] = 0.0;
+    }
+    Fo[fi + (o + 9) * fp] = 0.0;
+    Fo[fi + (o + 10) * fp] = 0.0;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void store_flux(const Patch<T>& pt, const Mat<T>& m, int d, const Slots<T, NS>& Fs, T rv,
+                                           int i, int j)
+{
+    store_flux_side<T, NS>(pt, m, d, Fs.v[0], Fs.s[0], (T)1, i, j, 0);
+    store_flux_side<T, NS>(pt, m, d, Fs.v[1], Fs.s[1], rv, i, j, 1);
+}
+
+// face states of one side on the four faces of a cell: lo/hi x face, lo/hi y face
+template <typename T, int NS>
+struct Faces1 {
+    T xlv[NS], xls[NS], xhv[NS], xhs[NS], ylv[NS], yls[NS], yhv[NS], yhs[NS];
+};
+
+// U += dU of side sd of valid cell (i, j) from the final states on its four faces (flux
+// difference, LinElastPatchGodunov.cpp:117-126), then the optional Drucker-Prager return
+template <typename T, int NS>
+__device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
+                                                 const Mat<T>& m, double dt, int i, int j, int sd,
+                                                 const Faces1<T, NS>& F)
+{
+    const T ry = sd ? (T)-1 : (T)1;
+    T fxl[7], fxh[7], fyl[7], fyh[7];
+    face_flux_side<T, NS>(m, F.xlv, F.xls, (T)1, fxl);
+    face_flux_side<T, NS>(m, F.xhv, F.xhs, (T)1, fxh);
+    face_flux_side<T, NS>(m, F.ylv, F.yls, ry, fyl);
+    face_flux_side<T, NS>(m, F.yhv, F.yhs, ry, fyh);
+    const T sc = (T)(-dt / P.dx);
+    const size_t u = uidx(pt, i, j, 0);
+    const int o = 11 * sd;
+    // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
+    auto upd = [&](int c, T dx_, T dy_) {
+        const T dU = ((T)0 + dx_) + dy_;
+        pt.out[c][u] = pt.in[c][u] + dU * sc;
+    };
+    T st[6];
+    // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
+    // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
+    const T dvx_x = fxh[0] - fxl[0], dvx_y = fyh[2] - fyl[2];
+    const T dvy_x = fxh[2] - fxl[2], dvy_y = fyh[0] - fyl[0];
+    const T dsxx_x = fxh[1] - fxl[1], dsxx_y = fyh[4] - fyl[4];
+    const T dsyy_x = fxh[4] - fxl[4], dsyy_y = fyh[1] - fyl[1];
+    const T dszz_x = fxh[4] - fxl[4], dszz_y = fyh[4] - fyl[4];
+    const T dsxy_x = fxh[3] - fxl[3], dsxy_y = fyh[3] - fyl[3];
+    upd(o + 0, dvx_x, dvx_y);
+    upd(o + 1, dvy_x, dvy_y);
+    st[0] = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
+    st[1] = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
+    st[2] = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
+    st[3] = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
+    if (NS == 3) {
+        const T dvz_x = fxh[5] - fxl[5], dvz_y = fyh[5] - fyl[5];
+        const T dsxz_x = fxh[6] - fxl[6], dsxz_y = (T)0 - (T)0;
+        const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[6] - fyl[6];
+        upd(o + 2, dvz_x, dvz_y);
+        st[4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
+        st[5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
+    } else {
+        st[4] = (T)0;
+        st[5] = (T)0;
+    }
+    if (P.plast && a.plastic) {
+        // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
+        T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
+        T sxz = NS == 3 ? st[4] : pt.in[o + 7][u];
+        T syz = NS == 3 ? st[5] : pt.in[o + 8][u];
+        plastic_flow<T>(P, (T)dt, st[0], st[1], st[2], st[3], sxz, syz, lam, gam);
+        st[4] = sxz;
+        st[5] = syz;
+        pt.out[o + 9][u] = lam;
+        pt.out[o + 10][u] = gam;
+        if (NS == 2) {
+            pt.out[o + 7][u] = sxz;
+            pt.out[o + 8][u] = syz;
         }
-        Fo[fi + (o + 9) * fp] = 0.0;
-        Fo[fi + (o + 10) * fp] = 0.0;
+    }
+    pt.out[o + 3][u] = st[0];
+    pt.out[o + 4][u] = st[1];
+    pt.out[o + 5][u] = st[2];
+    pt.out[o + 6][u] = st[3];
+    if (NS == 3) {
+        pt.out[o + 7][u] = st[4];
+        pt.out[o + 8][u] = st[5];
     }
 }
 
-// U += dU of valid cell (i, j) from the final states on its four faces (flux difference,
-// LinElastPatchGodunov.cpp:117-126), then the optional Drucker-Prager return
+// both sides
 template <typename T, int NS>
 __device__ __forceinline__ void update_cell(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
                                             const Mat<T>& m, double dt, int i, int j, const Slots<T, NS>& F0,
                                             const Slots<T, NS>& F0h, const Slots<T, NS>& F1,
                                             const Slots<T, NS>& F1h)
 {
-    const T rv1 = (T)-1;
-    T fxl[2][7], fxh[2][7], fyl[2][7], fyh[2][7];
-    face_flux<T, NS>(m, F0, (T)1, fxl);
-    face_flux<T, NS>(m, F0h, (T)1, fxh);
-    face_flux<T, NS>(m, F1, rv1, fyl);
-    face_flux<T, NS>(m, F1h, rv1, fyh);
-    const T sc = (T)(-dt / P.dx);
-    const size_t u = uidx(pt, i, j, 0);
-    // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
-    auto upd = [&](int c, T dx_, T dy_) {
-        const T dU = ((T)0 + dx_) + dy_;
-        pt.out[c][u] = pt.in[c][u] + dU * sc;
-    };
-    T st[2][6];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const int o = 11 * s;
-        // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
-        // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
-        const T dvx_x = fxh[s][0] - fxl[s][0], dvx_y = fyh[s][2] - fyl[s][2];
-        const T dvy_x = fxh[s][2] - fxl[s][2], dvy_y = fyh[s][0] - fyl[s][0];
-        const T dsxx_x = fxh[s][1] - fxl[s][1], dsxx_y = fyh[s][4] - fyl[s][4];
-        const T dsyy_x = fxh[s][4] - fxl[s][4], dsyy_y = fyh[s][1] - fyl[s][1];
-        const T dszz_x = fxh[s][4] - fxl[s][4], dszz_y = fyh[s][4] - fyl[s][4];
-        const T dsxy_x = fxh[s][3] - fxl[s][3], dsxy_y = fyh[s][3] - fyl[s][3];
-        upd(o + 0, dvx_x, dvx_y);
-        upd(o + 1, dvy_x, dvy_y);
-        const T sxx = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
-        const T syy = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
-        const T szz = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
-        const T sxy = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
-        st[s][0] = sxx; st[s][1] = syy; st[s][2] = szz; st[s][3] = sxy;
-        if (NS == 3) {
-            const T dvz_x = fxh[s][5] - fxl[s][5], dvz_y = fyh[s][5] - fyl[s][5];
-            const T dsxz_x = fxh[s][6] - fxl[s][6], dsxz_y = (T)0 - (T)0;
-            const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[s][6] - fyl[s][6];
-            upd(o + 2, dvz_x, dvz_y);
-            st[s][4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
-            st[s][5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
-        } else {
-            st[s][4] = (T)0;
-            st[s][5] = (T)0;
+    for (int sd = 0; sd < 2; ++sd) {
+        Faces1<T, NS> F;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            F.xlv[k] = F0.v[sd][k]; F.xls[k] = F0.s[sd][k];
+            F.xhv[k] = F0h.v[sd][k]; F.xhs[k] = F0h.s[sd][k];
+            F.ylv[k] = F1.v[sd][k]; F.yls[k] = F1.s[sd][k];
+            F.yhv[k] = F1h.v[sd][k]; F.yhs[k] = F1h.s[sd][k];
         }
-    }
-    if (P.plast && a.plastic) {
-        // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int o = 11 * s;
-            T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
-            T sxz = NS == 3 ? st[s][4] : pt.in[o + 7][u];
-            T syz = NS == 3 ? st[s][5] : pt.in[o + 8][u];
-            plastic_flow<T>(P, (T)dt, st[s][0], st[s][1], st[s][2], st[s][3], sxz, syz, lam, gam);
-            st[s][4] = sxz; st[s][5] = syz;
-            pt.out[o + 9][u] = lam;
-            pt.out[o + 10][u] = gam;
-            if (NS == 2) { pt.out[o + 7][u] = sxz; pt.out[o + 8][u] = syz; }
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const int o = 11 * s;
-        pt.out[o + 3][u] = st[s][0];
-        pt.out[o + 4][u] = st[s][1];
-        pt.out[o + 5][u] = st[s][2];
-        pt.out[o + 6][u] = st[s][3];
-        if (NS == 3) { pt.out[o + 7][u] = st[s][4]; pt.out[o + 8][u] = st[s][5]; }
+        update_cell_side<T, NS>(a, pt, P, m, dt, i, j, sd, F);
     }
 }
 

@@ -179,8 +179,12 @@ __device__ __forceinline__ void ppm_limit(T& m, T& p)
 }
 
 // ---------------------------------------------------------------------------
-// normal predictor for one cell and one direction (LinElastPatchGodunov.cpp:471-702)
-// w[side][slot][v|s][5]: primitive stencil along d; outputs W- and W+ (primitive)
+// normal predictor for one cell, one direction and one side (LinElastPatchGodunov.cpp:471-702)
+// w[slot][v|s][5]: primitive stencil along d; outputs W- and W+ (primitive) of that side.
+// rvs is the side's velocity sign in this direction (-1 on side 2 in y, else 1: the
+// reference's rv, LinElastPhysicsF.ChF:285-301).  A product with rvs = 1 is exact, so side 1
+// rounds like the reference's unsigned expressions.  Each slot runs PPM/PLM, characteristic
+// analysis, tracing and synthesis to the end before the next starts (fewer live values).
 // ---------------------------------------------------------------------------
 // PK >= 0 fixes the predictor at compile time: PK_PPM_PRIM is PPM with primitive limiting and no
 // characteristic limiting; PK_RUNTIME reads pred and the limiter flags from P.
@@ -188,193 +192,181 @@ constexpr int PK_RUNTIME = -1;
 constexpr int PK_PPM_PRIM = 2;
 
 template <typename T, int NS, int PK = PK_RUNTIME>
-__device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m, int pred_rt, T rv,
-                                            const T (&w)[2][NS][2][5], int ci, int dlo, int dhi,
-                                            T dtbydx, Slots<T, NS>& Wm, Slots<T, NS>& Wp)
+__device__ __forceinline__ void normal_pred_side(const DevParams& P, const Mat<T>& m, int pred_rt, T rvs,
+                                                 const T (&w)[NS][2][5], int ci, int dlo, int dhi, T dtbydx,
+                                                 T (&Wmv)[NS], T (&Wms)[NS], T (&Wpv)[NS], T (&Wps)[NS])
 {
     const int pred = PK >= 0 ? PK : pred_rt;
     if (pred == 0) {  // CTUNormalPred: W+- = W
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                Wm.v[s][k] = Wp.v[s][k] = w[s][k][0][2];
-                Wm.s[s][k] = Wp.s[s][k] = w[s][k][1][2];
-            }
+        for (int k = 0; k < NS; ++k) {
+            Wmv[k] = Wpv[k] = w[k][0][2];
+            Wms[k] = Wps[k] = w[k][1][2];
+        }
         return;
     }
     const bool limit = PK >= 0 ? true : (P.char_lim || P.prim_lim);
     const bool charl = PK >= 0 ? false : P.char_lim != 0;
-    // characteristic increments: wave moving toward lo (m) / toward hi (p)
-    T cm_m[2][NS], cp_m[2][NS];  // chars of the minus-side increment
-    T cm_p[2][NS], cp_p[2][NS];  // chars of the plus-side increment
-    if (pred == 2) {
-        T dvm[2][NS], dsm[2][NS], dvp[2][NS], dsp[2][NS];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                ppm_incr(w[s][k][0], ci, dlo, dhi, limit, dvm[s][k], dvp[s][k]);
-                ppm_incr(w[s][k][1], ci, dlo, dhi, limit, dsm[s][k], dsp[s][k]);
-                if (limit && !charl) {
-                    ppm_limit(dvm[s][k], dvp[s][k]);
-                    ppm_limit(dsm[s][k], dsp[s][k]);
-                }
-            }
-        // CHARANALYSISF (LinElastPhysicsF.ChF:424-447)
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const T Z = m.Z(k);
-                const T vm = s ? rv * dvm[s][k] : dvm[s][k];
-                const T vp = s ? rv * dvp[s][k] : dvp[s][k];
-                cm_m[s][k] = dsm[s][k] + vm * Z;
-                cp_m[s][k] = dsm[s][k] - vm * Z;
-                cm_p[s][k] = dsp[s][k] + vp * Z;
-                cp_p[s][k] = dsp[s][k] - vp * Z;
-                if (charl) {
-                    ppm_limit(cm_m[s][k], cm_p[s][k]);
-                    ppm_limit(cp_m[s][k], cp_p[s][k]);
-                }
-            }
-        // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const T sig = (k == 0 ? (T)P.cp : (T)P.cs) * dtbydx;
-                {
-                    const T mm = cm_m[s][k], pp = cm_p[s][k];
-                    cm_m[s][k] = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
-                }
-                {
-                    const T mm = cp_m[s][k], pp = cp_p[s][k];
-                    cp_p[s][k] = pp + sig * ((mm - pp) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
-                }
-            }
-    } else {  // PLM
-        T dv[2][NS], ds[2][NS], ovl[2][NS], ovr[2][NS], osl[2][NS], osr[2][NS];
-        const bool os = charl || P.fourth;
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const T* ww = w[s][k][q];
-                    const int s0 = cell_side(ci, dlo, dhi);
-                    T d;
-                    if (P.fourth) {
-                        const T vl0 = vl_slope(ww, 2, s0, limit);
-                        if (s0 != 0) {
-                            d = vl0;
-                        } else {
-                            const T vlm = vl_slope(ww, 1, cell_side(ci - 1, dlo, dhi), limit);
-                            const T vlp = vl_slope(ww, 3, cell_side(ci + 1, dlo, dhi), limit);
-                            const T hi = ww[3] - (T)0.25 * vlp;
-                            const T lo = ww[1] + (T)0.25 * vlm;
-                            d = (T)(2.0 / 3.0) * (hi - lo);
-                        }
-                    } else {
-                        d = vl_slope(ww, 2, s0, limit);
-                    }
-                    T C, L, R;
-                    slope_diffs(ww, 2, s0, C, L, R);
-                    if (q == 0) { dv[s][k] = d; ovl[s][k] = os ? L : (T)0; ovr[s][k] = os ? R : (T)0; }
-                    else { ds[s][k] = d; osl[s][k] = os ? L : (T)0; osr[s][k] = os ? R : (T)0; }
-                }
-        T cm_d[2][NS], cp_d[2][NS];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const T Z = m.Z(k);
-                if (limit && !charl && P.fourth) {
-                    dv[s][k] = vl_limit(dv[s][k], ovl[s][k], ovr[s][k]);
-                    ds[s][k] = vl_limit(ds[s][k], osl[s][k], osr[s][k]);
-                }
-                const T vd = s ? rv * dv[s][k] : dv[s][k];
-                cm_d[s][k] = ds[s][k] + vd * Z;
-                cp_d[s][k] = ds[s][k] - vd * Z;
-                if (charl) {
-                    const T vl = s ? rv * ovl[s][k] : ovl[s][k];
-                    const T vr = s ? rv * ovr[s][k] : ovr[s][k];
-                    const T cml = osl[s][k] + vl * Z, cpl = osl[s][k] - vl * Z;
-                    const T cmr = osr[s][k] + vr * Z, cpr = osr[s][k] - vr * Z;
-                    cm_d[s][k] = vl_limit(cm_d[s][k], cml, cmr);
-                    cp_d[s][k] = vl_limit(cp_d[s][k], cpl, cpr);
-                }
-                // PLM characteristic predictor, lmin = -cp, lmax = +cp
-                const T lk = k == 0 ? (T)P.cp : (T)P.cs;
-                const T lmin = -(T)P.cp, lmax = (T)P.cp;
-                // wave k-minus (lambda = -lk < 0)
-                cm_m[s][k] = -((T)0.5 * ((T)1 + dtbydx * (-lk)) * cm_d[s][k]);
-                cm_p[s][k] = (T)0.5 * ((T)1 - dtbydx * lmax) * cm_d[s][k];
-                // wave k-plus (lambda = +lk > 0)
-                cp_m[s][k] = -((T)0.5 * ((T)1 + dtbydx * lmin) * cp_d[s][k]);
-                cp_p[s][k] = (T)0.5 * ((T)1 - dtbydx * lk) * cp_d[s][k];
-            }
-    }
     // CHARSYNTHESISF (LinElastPhysicsF.ChF:526-550), then W+- = increment + W
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
+    auto synth = [&](int k, T cm_m, T cp_m, T cm_p, T cp_p) {
+        const T vmi = m.divZ(rvs * (cm_m - cp_m) / (T)2, k);
+        const T vpi = m.divZ(rvs * (cm_p - cp_p) / (T)2, k);
+        const T smi = (cm_m + cp_m) / (T)2;
+        const T spi = (cm_p + cp_p) / (T)2;
+        Wmv[k] = vmi + w[k][0][2];
+        Wpv[k] = vpi + w[k][0][2];
+        Wms[k] = smi + w[k][1][2];
+        Wps[k] = spi + w[k][1][2];
+    };
+    if (pred == 2) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const T Z = m.Z(k);
-            T vmi = m.divZ((cm_m[s][k] - cp_m[s][k]) / (T)2, k);
-            T vpi = m.divZ((cm_p[s][k] - cp_p[s][k]) / (T)2, k);
-            if (s) {
-                vmi = m.divZ(rv * (cm_m[s][k] - cp_m[s][k]) / (T)2, k);
-                vpi = m.divZ(rv * (cm_p[s][k] - cp_p[s][k]) / (T)2, k);
+            T dvm, dvp, dsm, dsp;
+            ppm_incr(w[k][0], ci, dlo, dhi, limit, dvm, dvp);
+            ppm_incr(w[k][1], ci, dlo, dhi, limit, dsm, dsp);
+            if (limit && !charl) {
+                ppm_limit(dvm, dvp);
+                ppm_limit(dsm, dsp);
             }
-            const T smi = (cm_m[s][k] + cp_m[s][k]) / (T)2;
-            const T spi = (cm_p[s][k] + cp_p[s][k]) / (T)2;
-            Wm.v[s][k] = vmi + w[s][k][0][2];
-            Wp.v[s][k] = vpi + w[s][k][0][2];
-            Wm.s[s][k] = smi + w[s][k][1][2];
-            Wp.s[s][k] = spi + w[s][k][1][2];
+            // CHARANALYSISF (LinElastPhysicsF.ChF:424-447): characteristic increments of the
+            // wave moving toward lo (m) and toward hi (p)
+            const T Z = m.Z(k);
+            const T vm = rvs * dvm;
+            const T vp = rvs * dvp;
+            T cm_m = dsm + vm * Z, cp_m = dsm - vm * Z;
+            T cm_p = dsp + vp * Z, cp_p = dsp - vp * Z;
+            if (charl) {
+                ppm_limit(cm_m, cm_p);
+                ppm_limit(cp_m, cp_p);
+            }
+            // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
+            const T sig = (k == 0 ? (T)P.cp : (T)P.cs) * dtbydx;
+            {
+                const T mm = cm_m, pp = cm_p;
+                cm_m = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+            }
+            {
+                const T mm = cp_m, pp = cp_p;
+                cp_p = pp + sig * ((mm - pp) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+            }
+            synth(k, cm_m, cp_m, cm_p, cp_p);
         }
+    } else {  // PLM
+        const bool os = charl || P.fourth;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            T d2[2], ol[2], orr[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const T* ww = w[k][q];
+                const int s0 = cell_side(ci, dlo, dhi);
+                T d;
+                if (P.fourth) {
+                    const T vl0 = vl_slope(ww, 2, s0, limit);
+                    if (s0 != 0) {
+                        d = vl0;
+                    } else {
+                        const T vlm = vl_slope(ww, 1, cell_side(ci - 1, dlo, dhi), limit);
+                        const T vlp = vl_slope(ww, 3, cell_side(ci + 1, dlo, dhi), limit);
+                        const T hi = ww[3] - (T)0.25 * vlp;
+                        const T lo = ww[1] + (T)0.25 * vlm;
+                        d = (T)(2.0 / 3.0) * (hi - lo);
+                    }
+                } else {
+                    d = vl_slope(ww, 2, s0, limit);
+                }
+                T C, L, R;
+                slope_diffs(ww, 2, s0, C, L, R);
+                d2[q] = d;
+                ol[q] = os ? L : (T)0;
+                orr[q] = os ? R : (T)0;
+            }
+            T dv = d2[0], ds = d2[1];
+            const T Z = m.Z(k);
+            if (limit && !charl && P.fourth) {
+                dv = vl_limit(dv, ol[0], orr[0]);
+                ds = vl_limit(ds, ol[1], orr[1]);
+            }
+            const T vd = rvs * dv;
+            T cm_d = ds + vd * Z;
+            T cp_d = ds - vd * Z;
+            if (charl) {
+                const T vl = rvs * ol[0];
+                const T vr = rvs * orr[0];
+                const T cml = ol[1] + vl * Z, cpl = ol[1] - vl * Z;
+                const T cmr = orr[1] + vr * Z, cpr = orr[1] - vr * Z;
+                cm_d = vl_limit(cm_d, cml, cmr);
+                cp_d = vl_limit(cp_d, cpl, cpr);
+            }
+            // PLM characteristic predictor, lmin = -cp, lmax = +cp
+            const T lk = k == 0 ? (T)P.cp : (T)P.cs;
+            const T lmin = -(T)P.cp, lmax = (T)P.cp;
+            // wave k-minus (lambda = -lk < 0), wave k-plus (lambda = +lk > 0)
+            const T cm_m = -((T)0.5 * ((T)1 + dtbydx * (-lk)) * cm_d);
+            const T cm_p = (T)0.5 * ((T)1 - dtbydx * lmax) * cm_d;
+            const T cp_m = -((T)0.5 * ((T)1 + dtbydx * lmin) * cp_d);
+            const T cp_p = (T)0.5 * ((T)1 - dtbydx * lk) * cp_d;
+            synth(k, cm_m, cp_m, cm_p, cp_p);
+        }
+    }
+}
+
+// both sides (side 2 with velocity sign rv)
+template <typename T, int NS, int PK = PK_RUNTIME>
+__device__ __forceinline__ void normal_pred(const DevParams& P, const Mat<T>& m, int pred_rt, T rv,
+                                            const T (&w)[2][NS][2][5], int ci, int dlo, int dhi,
+                                            T dtbydx, Slots<T, NS>& Wm, Slots<T, NS>& Wp)
+{
+    normal_pred_side<T, NS, PK>(P, m, pred_rt, (T)1, w[0], ci, dlo, dhi, dtbydx, Wm.v[0], Wm.s[0], Wp.v[0], Wp.s[0]);
+    normal_pred_side<T, NS, PK>(P, m, pred_rt, rv, w[1], ci, dlo, dhi, dtbydx, Wm.v[1], Wm.s[1], Wp.v[1], Wp.s[1]);
 }
 
 // outgoing-to-the-right characteristic of a left state (RIEMANNF wxxp, LinElastPhysicsF.ChF:677-679,733-735)
 template <typename T, int NS>
-__device__ __forceinline__ void right_chars(const Mat<T>& m, T rv, const Slots<T, NS>& L, T (&c)[2][NS])
+__device__ __forceinline__ void right_chars_side(const Mat<T>& m, T rvs, const T (&v)[NS], const T (&s)[NS],
+                                                 T (&c)[NS])
 {
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const T rZ = s ? rv * m.Z(k) : m.Z(k);
-            c[s][k] = L.s[s][k] - rZ * L.v[s][k];
-        }
+    for (int k = 0; k < NS; ++k) c[k] = s[k] - (rvs * m.Z(k)) * v[k];
 }
 // left-moving characteristic of a right state (RIEMANNF wxxm, :681-683,738-740)
 template <typename T, int NS>
-__device__ __forceinline__ void left_chars(const Mat<T>& m, T rv, const Slots<T, NS>& R, T (&c)[2][NS])
+__device__ __forceinline__ void left_chars_side(const Mat<T>& m, T rvs, const T (&v)[NS], const T (&s)[NS],
+                                                T (&c)[NS])
 {
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int k = 0; k < NS; ++k) c[k] = s[k] + (rvs * m.Z(k)) * v[k];
+}
+// RIEMANNF face state from the two characteristics (:693-702, :750-759)
+template <typename T, int NS>
+__device__ __forceinline__ void riemann_side(const Mat<T>& m, T rvs, const T (&wp)[NS], const T (&wm)[NS],
+                                             T (&v)[NS], T (&s)[NS])
+{
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const T rZ = s ? rv * m.Z(k) : m.Z(k);
-            c[s][k] = R.s[s][k] + rZ * R.v[s][k];
-        }
+    for (int k = 0; k < NS; ++k) {
+        v[k] = m.divZ(rvs * (wm[k] - wp[k]) / (T)2, k);
+        s[k] = (wm[k] + wp[k]) / (T)2;
+    }
 }
 
-// RIEMANNF face state from the two characteristics (:693-702, :750-759)
+template <typename T, int NS>
+__device__ __forceinline__ void right_chars(const Mat<T>& m, T rv, const Slots<T, NS>& L, T (&c)[2][NS])
+{
+    right_chars_side<T, NS>(m, (T)1, L.v[0], L.s[0], c[0]);
+    right_chars_side<T, NS>(m, rv, L.v[1], L.s[1], c[1]);
+}
+template <typename T, int NS>
+__device__ __forceinline__ void left_chars(const Mat<T>& m, T rv, const Slots<T, NS>& R, T (&c)[2][NS])
+{
+    left_chars_side<T, NS>(m, (T)1, R.v[0], R.s[0], c[0]);
+    left_chars_side<T, NS>(m, rv, R.v[1], R.s[1], c[1]);
+}
 template <typename T, int NS>
 __device__ __forceinline__ void riemann_state(const Mat<T>& m, T rv, const T (&wp)[2][NS], const T (&wm)[2][NS],
                                               Slots<T, NS>& F)
 {
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const T Z = m.Z(k);
-            F.v[s][k] = s ? m.divZ(rv * (wm[s][k] - wp[s][k]) / (T)2, k) : m.divZ((wm[s][k] - wp[s][k]) / (T)2, k);
-            F.s[s][k] = (wm[s][k] + wp[s][k]) / (T)2;
-        }
+    riemann_side<T, NS>(m, (T)1, wp[0], wm[0], F.v[0], F.s[0]);
+    riemann_side<T, NS>(m, rv, wp[1], wm[1], F.v[1], F.s[1]);
 }
 
 // LINELASTOUTBCF / LINELASTFREEBCF (LinElastPhysicsF.ChF:906-1144) on the face state.
@@ -405,25 +397,29 @@ __device__ __forceinline__ void bc_state(const DevParams& P, const Mat<T>& m, in
 }
 
 // GETADWDXF applied as W(d1) += A(d2) (WHalf(d2)[+1] - WHalf(d2)[0]) (LinElastPhysicsF.ChF:791-901,
-// LinElastPatchGodunov.cpp:408-416).  sc = -(1/2) dt/dx, times rv(d2) on side 2.
+// LinElastPatchGodunov.cpp:408-416) on one side.  sc = -(1/2) dt/dx, times rv(d2) on side 2.
+template <typename T, int NS>
+__device__ __forceinline__ void transverse_side(const Mat<T>& m, T sc, const T (&Hlov)[NS], const T (&Hlos)[NS],
+                                                const T (&Hhiv)[NS], const T (&Hhis)[NS], T (&Wv)[NS], T (&Ws)[NS])
+{
+    const T dNv = Hhiv[0] - Hlov[0], dNs = Hhis[0] - Hlos[0];
+    const T dSv = Hhiv[1] - Hlov[1], dSs = Hhis[1] - Hlos[1];
+    Wv[0] = Wv[0] + (-m.divrho(sc * dSs));
+    Ws[0] = Ws[0] + (-(sc * dNv * m.lam));
+    Wv[1] = Wv[1] + (-m.divrho(sc * dNs));
+    Ws[1] = Ws[1] + (-(sc * dSv * m.G));
+    if (NS == 3) {
+        const T dAs = Hhis[NS - 1] - Hlos[NS - 1];
+        Wv[NS - 1] = Wv[NS - 1] + (-m.divrho(sc * dAs));
+    }
+}
+
 template <typename T, int NS>
 __device__ __forceinline__ void transverse_add(const Mat<T>& m, T sc1, T sc2, const Slots<T, NS>& Hlo,
                                                const Slots<T, NS>& Hhi, Slots<T, NS>& W)
 {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const T sc = s ? sc2 : sc1;
-        const T dNv = Hhi.v[s][0] - Hlo.v[s][0], dNs = Hhi.s[s][0] - Hlo.s[s][0];
-        const T dSv = Hhi.v[s][1] - Hlo.v[s][1], dSs = Hhi.s[s][1] - Hlo.s[s][1];
-        W.v[s][0] = W.v[s][0] + (-m.divrho(sc * dSs));
-        W.s[s][0] = W.s[s][0] + (-(sc * dNv * m.lam));
-        W.v[s][1] = W.v[s][1] + (-m.divrho(sc * dNs));
-        W.s[s][1] = W.s[s][1] + (-(sc * dSv * m.G));
-        if (NS == 3) {
-            const T dAs = Hhi.s[s][NS - 1] - Hlo.s[s][NS - 1];
-            W.v[s][NS - 1] = W.v[s][NS - 1] + (-m.divrho(sc * dAs));
-        }
-    }
+    transverse_side<T, NS>(m, sc1, Hlo.v[0], Hlo.s[0], Hhi.v[0], Hhi.s[0], W.v[0], W.s[0]);
+    transverse_side<T, NS>(m, sc2, Hlo.v[1], Hlo.s[1], Hhi.v[1], Hhi.s[1], W.v[1], W.s[1]);
 }
 
 // ---------------------------------------------------------------------------

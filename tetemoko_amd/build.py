@@ -11,19 +11,30 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared", "-ffp
          "-I" + os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument"]
 
 
-def build(verbose=True):
-    src = os.path.join(HERE, "csrc", "tmk_api.hip")
-    out_dir = os.path.join(HERE, "_lib")
-    os.makedirs(out_dir, exist_ok=True)
-    out = os.path.join(out_dir, "libtetemoko_hip.so")
-    tmp = out + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, src]
+def _run(cmd, verbose, what):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libtetemoko_hip.so")
+        raise RuntimeError("hipcc failed building " + what)
+
+
+def build(verbose=True):
+    """tmk_api.hip (host code and every FP64 kernel, no FMA contraction) and tmk_f32.hip (the
+    FP32-bulk update kernels, FMA contraction on), linked into one shared library."""
+    csrc = os.path.join(HERE, "csrc")
+    out_dir = os.path.join(HERE, "_lib")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libtetemoko_hip.so")
+    objs = []
+    for name, contract in (("tmk_api", "off"), ("tmk_f32", "fast")):
+        obj = os.path.join(out_dir, name + ".o")
+        flags = [f if f != "-ffp-contract=off" else "-ffp-contract=" + contract for f in FLAGS if f != "-shared"]
+        _run([HIPCC] + flags + ["-c", "-o", obj, os.path.join(csrc, name + ".hip")], verbose, name + ".hip")
+        objs.append(obj)
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose, "libtetemoko_hip.so")
     os.replace(tmp, out)
     build_copy_calib(verbose)
     return out

@@ -24,6 +24,14 @@
 
 static void default_tile(int precision, int& bx, int& by);
 
+// the FP32-bulk update kernels are instantiated in tmk_f32.hip (built with FMA contraction)
+namespace tmk {
+#define TMK_DECLARE_EXTERN(T, NS, BX, BY, PK)                                         \
+    extern template __global__ void k_godunov<T, NS, BX, BY, PK>(const KArgs<T>);     \
+    extern template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
+TMK_ALL_UPDATE_KERNELS(TMK_DECLARE_EXTERN, float)
+}  // namespace tmk
+
 using namespace tmk;
 
 #define HIPCHK(x)                                                                           \

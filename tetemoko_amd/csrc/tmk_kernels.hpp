@@ -808,7 +808,7 @@ __global__ void k_tag(const Patch<T>* patches, const Scen* scen, TagArgs ta, uns
 }
 
 // IntVectSet::grow(b) followed by the clip to the domain
-__global__ void k_dilate(const unsigned char* raw, unsigned char* out, int nx, int ny, int b)
+static __global__ void k_dilate(const unsigned char* raw, unsigned char* out, int nx, int ny, int b)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nx * ny) return;
@@ -819,7 +819,7 @@ __global__ void k_dilate(const unsigned char* raw, unsigned char* out, int nx, i
     out[t] = v;
 }
 
-__global__ void k_advance_time(Scen* s, int n)
+static __global__ void k_advance_time(Scen* s, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) s[k].time = s[k].time + s[k].dt;

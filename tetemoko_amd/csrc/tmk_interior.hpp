@@ -423,25 +423,28 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
 // one thread per (cell, side) for FP64 mode II on tiles of at most 384 cells (32x8, 32x12,
 // 64x4): three or two workgroups of 8 or 12 waves share a CU, six waves per SIMD at 80 VGPRs.
 // Larger tiles keep one thread per cell (a 1024-thread split workgroup would run alone on its CU).
+// FP32 bulk keeps one thread per cell: it already holds six waves per SIMD at 74 VGPRs, and the
+// split's smaller tiles cost more in ring overlap than the extra waves return (measured on the
+// 4096 x 2048 level: 0.60 ms at 32x8 split against 0.57 ms at 32x16).
 template <typename T, int NS, int BX, int BY>
 __host__ __device__ constexpr bool interior_split()
 {
-#ifdef TMK_NO_SPLIT
-    return false;
-#else
     return sizeof(T) == 8 && NS == 2 && BX * BY <= 384;
-#endif
 }
 template <typename T, int NS, int BX, int BY>
 __host__ __device__ constexpr int interior_threads()
 {
     return interior_split<T, NS, BX, BY>() ? 2 * BX * BY : BX * BY;
 }
+#ifndef TMK_INTERIOR_WAVES_F32
+#define TMK_INTERIOR_WAVES_F32 0
+#endif
 template <typename T, int NS, int BX, int BY>
 constexpr int interior_waves()
 {
     return interior_split<T, NS, BX, BY>() ? TMK_SPLIT_WAVES_F64
-                                           : ((sizeof(T) == 8 && NS == 2) ? TMK_INTERIOR_WAVES_F64 : 1);
+           : sizeof(T) == 8                ? (NS == 2 ? TMK_INTERIOR_WAVES_F64 : 0)
+                                           : TMK_INTERIOR_WAVES_F32;
 }
 
 // interior tiles only (the host sorts them out with the same rule); a tile that is not

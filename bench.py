@@ -88,6 +88,52 @@ def run_level(cfg, U, B, precision, warmup, steps, tile):
     return ms, kms, Uo, Bo
 
 
+def host_cpu():
+    """Core count and model of the host this runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def drop_in_patch_rate(precision, seconds_hint=3.0):
+    """The minimal drop-in: LinElastPatchGodunov::updateState replaced by one
+    tmk_patch_update_state call per box on host FArrayBox data (host -> device -> host each
+    call), over a 512x256 level cut into 32x32 boxes as the reference's default max_grid_size.
+    PCIe-inclusive; reported beside the resident level path, never as its value."""
+    cfg = scenarios.config_c1(seed=0, nx=512, ny=256, dx=25.0)
+    U, B = scenarios.level_arrays(cfg)
+    boxes = scenarios.split_boxes(cfg["nx"], cfg["ny"], 32)
+    g = scenarios.GHOST
+    views = [(b, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None) for b in boxes]
+    ctx = capi.Context(cfg, precision)
+    dt = scenarios.dt_of(cfg)
+
+    def one_step():
+        for b, ub, bb in views:
+            ubox = (b[0] - g, b[1] - g, b[2] + g, b[3] + g)
+            bbox = (b[0] - g, 0, b[2] + g, 0) if bb is not None else None
+            ctx.update_state(b, ub, ubox, bb, bbox, time=0.0, dt=dt)
+
+    one_step()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds_hint:
+        one_step()
+        n += 1
+    el = time.perf_counter() - t0
+    ctx.close()
+    return {"value": cfg["nx"] * cfg["ny"] * n / el, "unit": "cell-updates/s",
+            "sample": "512x256 level, 128 boxes of 32x32, one tmk_patch_update_state call per box on host "
+                      "arrays, %d level steps in %.1f s" % (n, el)}
+
+
 def cpu_baseline(cfg_full, nthreads, seconds_hint):
     """Oracle (scalar C restatement) on a bounded sample: 512 x 256 cells of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -104,10 +150,12 @@ def cpu_baseline(cfg_full, nthreads, seconds_hint):
     t0 = time.perf_counter()
     assert tmo.level_steps(op, U, B, dt, dt, nsteps, nthreads=nthreads, maxbox=32) == 0
     el = time.perf_counter() - t0
+    ncpu, model = host_cpu()
     return {"value": cfg["nx"] * cfg["ny"] * nsteps / el, "unit": "cell-updates/s", "cores": nthreads,
             "kind": "port",
             "sample": "512x256 cells of the same rate-and-state workload, %d level steps, boxes of 32x32, "
-                      "FP64, oracle C restatement, %d thread(s), %.1f s" % (nsteps, nthreads, el)}
+                      "FP64, oracle C restatement (gcc -O3, no FMA contraction), %d thread(s), %.1f s; "
+                      "host: %d logical CPUs, %s" % (nsteps, nthreads, el, ncpu, model)}
 
 
 def main():
@@ -177,6 +225,8 @@ def main():
             "slip_rel_err": float(np.max(np.abs(slip32 - slip64)) / max(np.max(np.abs(slip64)), 1e-300)),
             "slip_rate_rel_err": float(np.max(np.abs(V32 - V64)) / max(np.max(np.abs(V64)), 1e-300)),
         }
+    print("bench: per-patch drop-in leg", file=sys.stderr, flush=True)
+    out["drop_in_patch_fp64"] = drop_in_patch_rate(capi.PREC_FP64)
     if not a.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(cfg, 1, 12.0)

@@ -40,6 +40,7 @@ extern "C" {
 #define TMK_E_HIP (-7)         /* HIP runtime failure */
 #define TMK_E_UNSUPPORTED (-8) /* option the reference aborts on (flattening, artificial viscosity, high-order limiter) */
 #define TMK_E_NOMEM (-9)       /* device memory exhausted */
+#define TMK_E_OVERFLOW (-10)   /* station records dropped: buffer full before it was read */
 
 #define TMK_FAULT_RS 0
 #define TMK_FAULT_SW 1
@@ -203,6 +204,23 @@ int tmk_batch_sync(tmk_batch *b);
  * uses it to retire only the scenario that failed (a member's error does not touch
  * the others' data). */
 int tmk_batch_member_error(tmk_batch *b, tmk_level *lev, int *code);
+
+/* Fault stations of a level (AMRLevelLinElast fault stations: placement
+ * AMRLevelLinElast.cpp:258-278, record AMRLevelLinElast.cpp:1758-1810).  Station k sits at
+ * x[k] metres from the domain's lo x edge (the reference's x_fault_stations plus its domain
+ * centre): face i = floor(x/dx - 0.5), weight d = x - (i + 0.5) dx.  After every level step of
+ * tmk_level_step/run the device appends one record per station -- the new time, then
+ * B[i] (1 - d) + B[i+1] d for each fault-data component, the reference's expression with d in
+ * metres -- inside captured step graphs as well.  capacity = records kept between reads.
+ * Stations of a level that is in a batch record nothing.  A station whose face lies on no
+ * fault box is TMK_E_ARG. */
+typedef struct tmk_stations tmk_stations;
+int tmk_stations_create(tmk_level *lev, int nstations, const double *x, int capacity, tmk_stations **out);
+/* waits for the level, copies the records written since the last read into
+ * records[nrecords][nstations][1 + numBdryVars] (max_records >= capacity) and clears them;
+ * TMK_E_OVERFLOW when records were dropped (the kept ones are still copied) */
+int tmk_stations_read(tmk_stations *st, double *records, int max_records, int *nrecords);
+int tmk_stations_destroy(tmk_stations *st);
 
 /* Workgroup tile shape of the update kernels (autotuner hook): bx x by threads
  * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 32x8, 32x12, 32x16,

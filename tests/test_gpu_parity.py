@@ -434,6 +434,67 @@ def test_batch_member_error_isolated():
         o.close()
 
 
+def test_fault_stations_match_oracle():
+    """Station records written on the device after each level step (graph-captured and eager)
+    equal the reference expression B[i] (1 - d) + B[i+1] d on the oracle's fault data."""
+    from tetemoko_amd import stations as stmod
+    cfg = _cfg("rs", "PPM", nx=96, ny=48)
+    U, B = _state(cfg, 9)
+    dt = scenarios.dt_of(cfg)
+    xs = [12.5 + 25.0 * 10, 25.0 * 40 + 3.0, 25.0 * 70 + 20.0, 25.0 * 95 - 12.5]
+    ctx = capi.Context(cfg, capi.PREC_FP64)
+    lev = capi.Level(ctx, scenarios.split_boxes(96, 48, 32))
+    for k, b in enumerate(lev.boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+    st = capi.Stations(lev, xs, capacity=8)
+    t = 0.0
+    lev.run(3, t, dt)
+    for _ in range(3):
+        t = t + dt
+    lev.step(t, dt)
+    rec = st.read()
+    assert rec.shape == (4, len(xs), 12)
+    op = tmo.OracleParams(cfg)
+    Ur, Br = U.copy(), B.copy()
+    tr = 0.0
+    for k in range(4):
+        assert tmo.level_steps(op, Ur, Br, tr, dt, 1) == 0
+        tr = tr + dt
+        for s_, x in enumerate(xs):
+            i, d = stmod.placement(x, cfg["dx"])
+            want = Br[:, G + i] * (1.0 - d) + Br[:, G + i + 1] * d
+            assert rec[k, s_, 0] == tr
+            scale = np.maximum(np.max(np.abs(Br[:, G:-G]), axis=1), 1e-300)
+            assert np.all(np.abs(rec[k, s_, 1:] - want) <= TOL * scale), (k, s_)
+    assert st.read().shape[0] == 0
+    st.close()
+    lev.close()
+    ctx.close()
+
+
+def test_fault_stations_overflow():
+    cfg = _cfg("sw", "PPM", nx=64, ny=40)
+    U, B = _state(cfg, 3)
+    ctx = capi.Context(cfg, capi.PREC_FP64)
+    lev = capi.Level(ctx, [(0, 0, 63, 39)])
+    lev.upload(0, U, B)
+    st = capi.Stations(lev, [30.0 * 100], capacity=2)
+    lev.run(3, 0.0, scenarios.dt_of(cfg))
+    with pytest.raises(capi.TmkError) as ei:
+        st.read()
+    assert ei.value.code == capi.E_OVERFLOW
+    lev.run(1, 0.0, scenarios.dt_of(cfg))
+    got = st.read()
+    assert got.shape == (1, 1, 10)
+    with pytest.raises(capi.TmkError):
+        capi.Stations(lev, [30.0 * 10], capacity=2)  # a second station set on the same level
+    st.close()
+    with pytest.raises(capi.TmkError):
+        capi.Stations(lev, [1e9], capacity=2)  # a station off the fault
+    lev.close()
+    ctx.close()
+
+
 def test_mem_info_and_footprint():
     free, total = capi.mem_info()
     assert 0 < free <= total

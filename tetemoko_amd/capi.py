@@ -19,6 +19,7 @@ GHOST = 4
 
 OK = 0
 E_PSI_NAN = -1
+E_OVERFLOW = -10
 E_BAD_BC = -2
 E_PRED_ORDER = -3
 E_NO_BDRY = -4
@@ -39,6 +40,7 @@ EXPORTED = (
     "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
     "tmk_level_profile_kernels", "tmk_batch_member_error",
+    "tmk_stations_create", "tmk_stations_read", "tmk_stations_destroy",
 )
 
 
@@ -164,6 +166,9 @@ def lib():
         L.tmk_batch_run.argtypes = [vp, c_int]
         L.tmk_batch_sync.argtypes = [vp]
         L.tmk_batch_member_error.argtypes = [vp, vp, ctypes.POINTER(c_int)]
+        L.tmk_stations_create.argtypes = [vp, c_int, _PD, c_int, ctypes.POINTER(vp)]
+        L.tmk_stations_read.argtypes = [vp, _PD, c_int, ctypes.POINTER(c_int)]
+        L.tmk_stations_destroy.argtypes = [vp]
         L.tmk_level_fill.argtypes = [vp, c_int, _PD, _PD]
         L.tmk_level_download_fault.argtypes = [vp, c_int, _PD]
         L.tmk_level_tag.argtypes = [vp, ctypes.POINTER(TagParams), ctypes.POINTER(ctypes.c_ubyte)]
@@ -415,3 +420,38 @@ class Batch:
         code = ctypes.c_int()
         _check(lib().tmk_batch_member_error(self.h, level.h, ctypes.byref(code)), "tmk_batch_member_error")
         return code.value
+
+
+class Stations:
+    """Fault stations of a level, recorded on the device after every level step
+    (AMRLevelLinElast fault stations, AMRLevelLinElast.cpp:258-309,1758-1810)."""
+
+    def __init__(self, level, x, capacity=1024):
+        self.level = level
+        self.x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        self.capacity = int(capacity)
+        self.nb = NBDRY_RS if level.ctx.cfg["fault"] == "rs" else NBDRY_SW
+        h = ctypes.c_void_p()
+        _check(lib().tmk_stations_create(level.h, len(self.x), _ptr(self.x), self.capacity, ctypes.byref(h)),
+               "tmk_stations_create")
+        self.h = h
+
+    def read(self, allow_overflow=False):
+        """Records since the last read: array [nrecords, nstations, 1 + nb] (time, fault data)."""
+        buf = np.empty((self.capacity, len(self.x), 1 + self.nb))
+        n = ctypes.c_int()
+        code = lib().tmk_stations_read(self.h, _ptr(buf), self.capacity, ctypes.byref(n))
+        if code != 0 and not (allow_overflow and code == E_OVERFLOW):
+            _check(code, "tmk_stations_read")
+        return buf[:n.value].copy()
+
+    def close(self):
+        if self.h:
+            lib().tmk_stations_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

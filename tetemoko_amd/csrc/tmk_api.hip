@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -197,6 +198,7 @@ struct Group {
     int nbrects = 0;
     int nb = 11;
     int plastic = 1;
+    const StationDev* stations = nullptr;  // fault stations recorded after each step (level mode)
     int g = 0;  // parity of the next launch
     hipGraphExec_t gexec = nullptr;
     void release()
@@ -230,6 +232,7 @@ struct tmk_ctx {
 };
 
 struct tmk_batch;
+struct tmk_stations;
 
 struct tmk_level {
     tmk_ctx* ctx = nullptr;
@@ -250,6 +253,13 @@ struct tmk_level {
     bool dirty = true;            // uploaded data not yet exchanged
     tmk_batch* batch = nullptr;
     size_t bytes = 0;             // device bytes held
+    tmk_stations* stations = nullptr;
+};
+
+struct tmk_stations {
+    tmk_level* L = nullptr;
+    StationDev dev{};
+    DevBuf meta, rec, count;
 };
 
 struct tmk_batch {
@@ -269,7 +279,8 @@ static const char* k_errs[] = {"ok",
                                "invalid argument",
                                "HIP runtime failure",
                                "unsupported option",
-                               "device memory exhausted"};
+                               "device memory exhausted",
+                               "station records dropped: buffer full before it was read"};
 
 extern "C" const char* tmk_strerror(int code)
 {
@@ -535,6 +546,10 @@ static int group_enqueue_step_t(Group& G, hipStream_t st, const StepTimers* tm =
     }
     hipLaunchKernelGGL(k_advance_time, dim3((G.nscen + 63) / 64), dim3(64), 0, st, G.scen_dev, G.nscen);
     if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    if (G.stations) {
+        hipLaunchKernelGGL(k_station_record<T>, dim3(1), dim3(256), 0, st, pn, G.scen_dev, *G.stations);
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
     G.g ^= 1;
     return TMK_OK;
 }
@@ -973,6 +988,7 @@ extern "C" int tmk_level_destroy(tmk_level* L)
     if (!L) return TMK_OK;
     if (L->batch) return TMK_E_ARG;
     if (L->ctx && L->ctx->stream) (void)hipStreamSynchronize(L->ctx->stream);
+    if (L->stations) L->stations->L = nullptr;
     L->own.release();
     for (int k = 0; k < 2; ++k) {
         for (void* p : L->ubuf[k]) if (p) (void)hipFree(p);
@@ -1099,6 +1115,7 @@ static int level_own_group(tmk_level* L)
         std::vector<int> ph{L->cur};
         int e = group_build(L->own, m, ph, 1);
         if (e) return e;
+        L->own.stations = L->stations ? &L->stations->dev : nullptr;
         L->own_built = true;
     }
     return TMK_OK;
@@ -1370,6 +1387,96 @@ extern "C" int tmk_batch_run(tmk_batch* B, int nsteps)
     if (e) return e;
     for (tmk_level* L : B->members)
         for (int k = 0; k < nsteps; ++k) L->time = L->time + L->dt;
+    return TMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fault stations
+// ---------------------------------------------------------------------------
+extern "C" int tmk_stations_create(tmk_level* L, int n, const double* x, int capacity, tmk_stations** out)
+{
+    if (!L || !out || n <= 0 || !x || capacity <= 0 || L->stations || L->batch) return TMK_E_ARG;
+    *out = nullptr;
+    tmk_ctx* c = L->ctx;
+    const double dx = c->hp.dx;
+    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    std::vector<int> patch(n), face(n);
+    std::vector<double> d(n);
+    for (int k = 0; k < n; ++k) {
+        // AMRLevelLinElast.cpp:262-270 (x already includes the domain centre)
+        const int i = (int)std::floor(x[k] / dx - 0.5);
+        face[k] = i;
+        d[k] = -(i + 0.5) * dx + x[k];
+        patch[k] = -1;
+        for (size_t b = 0; b < L->boxes.size(); ++b)
+            if (L->has_b[b] && i >= L->boxes[b].lo[0] && i <= L->boxes[b].hi[0]) patch[k] = (int)b;
+        if (patch[k] < 0) return TMK_E_ARG;
+    }
+    tmk_stations* S = new (std::nothrow) tmk_stations;
+    if (!S) return TMK_E_NOMEM;
+    int e = TMK_OK;
+    const size_t mbytes = (size_t)n * (2 * sizeof(int) + sizeof(double));
+    if (!e) e = S->meta.ensure(mbytes);
+    if (!e) e = S->rec.ensure((size_t)capacity * n * (1 + nb) * sizeof(double));
+    if (!e) e = S->count.ensure(2 * sizeof(int));
+    if (!e) {
+        char* m = (char*)S->meta.p;
+        const int zero[2] = {0, 0};
+        if (hipMemcpy(m, d.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m + n * sizeof(double), patch.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m + n * (sizeof(double) + sizeof(int)), face.data(), n * sizeof(int), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(S->count.p, zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
+            e = TMK_E_HIP;
+        S->dev.n = n;
+        S->dev.nb = nb;
+        S->dev.capacity = capacity;
+        S->dev.d = (const double*)m;
+        S->dev.patch = (const int*)(m + n * sizeof(double));
+        S->dev.face = (const int*)(m + n * (sizeof(double) + sizeof(int)));
+        S->dev.rec = (double*)S->rec.p;
+        S->dev.count = (int*)S->count.p;
+    }
+    if (e) {
+        S->meta.release(); S->rec.release(); S->count.release();
+        delete S;
+        return e;
+    }
+    S->L = L;
+    L->stations = S;
+    L->own_built = false;  // the next step rebuilds the group (and its graph) with the recorder
+    *out = S;
+    return TMK_OK;
+}
+
+extern "C" int tmk_stations_read(tmk_stations* S, double* records, int max_records, int* nrecords)
+{
+    if (!S || !S->L || !records || !nrecords || max_records < S->dev.capacity) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(S->L->ctx->stream));
+    int cnt[2] = {0, 0};
+    HIPCHK(hipMemcpy(cnt, S->count.p, sizeof cnt, hipMemcpyDeviceToHost));
+    const int n = std::min(cnt[0], S->dev.capacity);
+    if (n > 0)
+        HIPCHK(hipMemcpy(records, S->rec.p, (size_t)n * S->dev.n * (1 + S->dev.nb) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    const int zero[2] = {0, 0};
+    HIPCHK(hipMemcpy(S->count.p, zero, sizeof zero, hipMemcpyHostToDevice));
+    *nrecords = n;
+    return cnt[1] ? TMK_E_OVERFLOW : TMK_OK;
+}
+
+extern "C" int tmk_stations_destroy(tmk_stations* S)
+{
+    if (!S) return TMK_OK;
+    if (S->L) {
+        (void)hipStreamSynchronize(S->L->ctx->stream);
+        S->L->stations = nullptr;
+        S->L->own_built = false;
+    }
+    S->meta.release();
+    S->rec.release();
+    S->count.release();
+    delete S;
     return TMK_OK;
 }
 

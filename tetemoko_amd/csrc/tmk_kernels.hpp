@@ -720,6 +720,41 @@ __global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int 
     }
 }
 
+// fault stations of a level (AMRLevelLinElast::writeStationLevel, AMRLevelLinElast.cpp:1758-1810)
+struct StationDev {
+    int n, nb, capacity;
+    const int* patch;   // [n] patch whose fault row holds face i
+    const int* face;    // [n] face index i = floor(x/dx - 0.5)
+    const double* d;    // [n] x - (i + 0.5) dx, the reference's weight (metres)
+    double* rec;        // [capacity][n][1 + nb]: time, then the interpolated fault data
+    int* count;         // [0] records written since the last read, [1] set when one was dropped
+};
+
+// one record per station after a level step: the new time and B[i] (1 - d) + B[i+1] d per
+// fault-data component, from the step's output fault data (ghosts exchanged)
+template <typename T>
+__global__ void k_station_record(const Patch<T>* patches, const Scen* scen, StationDev s)
+{
+    __shared__ int base;
+    if (threadIdx.x == 0) base = s.count[0];
+    __syncthreads();
+    const int b = base;
+    if (b >= s.capacity) {
+        if (threadIdx.x == 0) s.count[1] = 1;
+        return;
+    }
+    for (int k = threadIdx.x; k < s.n; k += blockDim.x) {
+        const Patch<T>& p = patches[s.patch[k]];
+        const size_t i = (size_t)(s.face[k] - p.blo0), bp = (size_t)p.bnx;
+        const double d = s.d[k];
+        double* r = s.rec + ((size_t)b * s.n + k) * (size_t)(1 + s.nb);
+        r[0] = scen[p.scen].time;
+        for (int c = 0; c < s.nb; ++c) r[1 + c] = p.bout[i + c * bp] * (1.0 - d) + p.bout[i + 1 + c * bp] * d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s.count[0] = b + 1;
+}
+
 // every scenario's clock advances by its own step (AMRLevelLinElast.cpp:478, m_time += m_dt)
 // fill every plane of a box with a uniform state (LEINITF, LinElastPhysicsF.ChF:1215-1263)
 template <typename T>

@@ -317,9 +317,15 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
     P.iZp = 1.0 / P.Zp;
     P.iZs = 1.0 / P.Zs;
     P.irho = 1.0 / P.rho;
+    P.i2Zp = 0.5 * P.iZp;
+    P.i2Zs = 0.5 * P.iZs;
     P.K = P.lam + 2.0 * h.G / 3.0;
     P.den = 2.0 * h.G + P.lam;
     P.lam2G = P.lam + 2.0 * h.G;
+    P.frho = (float)P.rho; P.flam = (float)P.lam; P.fG = (float)P.G; P.fZp = (float)P.Zp; P.fZs = (float)P.Zs;
+    P.fden = (float)P.den; P.flam2G = (float)P.lam2G; P.fiZp = (float)P.iZp; P.fiZs = (float)P.iZs;
+    P.firho = (float)P.irho; P.fi2Zp = (float)P.i2Zp; P.fi2Zs = (float)P.i2Zs;
+    P.fcp = (float)P.cp; P.fcs = (float)P.cs;
     const double s9[9] = {0.0, 0.0, 0.0, h.sxx0, h.syy0, h.szz0, h.sxy0, h.sxz0, h.syz0};
     for (int s = 0; s < 2; ++s) {
         for (int c = 0; c < 9; ++c) P.bg[11 * s + c] = s9[c];
@@ -632,8 +638,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         tmk_level* L = mem[m];
         memset(&sc[m], 0, sizeof(Scen));
         sc[m].P = L->ctx->P;
-        sc[m].time = L->time;
-        sc[m].dt = L->dt;
+        scen_set_clock(sc[m], L->time, L->dt);
         const int nbox = (int)L->boxes.size();
         for (int g = 0; g < 2; ++g) {
             const Patch<T>* src = (const Patch<T>*)L->hpatch[phase[m] ^ g].data();
@@ -695,10 +700,15 @@ static int group_build(Group& G, const std::vector<tmk_level*>& mem, const std::
                                            : group_build_t<float>(G, mem, phase, plastic);
 }
 
-static int group_set_clock(Group& G, hipStream_t st, int m, double time, double dt)
+// the clock and step constants (Scen fields time .. fmdtdx) of member m
+static int group_set_clock(Group& G, hipStream_t st, int m, double time, double dt, const DevParams& P)
 {
-    double td[2] = {time, dt};
-    HIPCHK(hipMemcpyAsync((char*)(G.scen_dev + m) + offsetof(Scen, time), td, sizeof td, hipMemcpyHostToDevice, st));
+    Scen s;
+    s.P = P;
+    scen_set_clock(s, time, dt);
+    const size_t off = offsetof(Scen, time), len = offsetof(Scen, err) - off;
+    // pageable source: the call returns once s has been staged, so s may go out of scope
+    HIPCHK(hipMemcpyAsync((char*)(G.scen_dev + m) + off, (const char*)&s + off, len, hipMemcpyHostToDevice, st));
     return TMK_OK;
 }
 
@@ -779,8 +789,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         Scen sc;
         memset(&sc, 0, sizeof sc);
         sc.P = c->P;
-        sc.time = time;
-        sc.dt = dt;
+        scen_set_clock(sc, time, dt);
         HIPCHK(hipMemcpyAsync(meta, &pt, sizeof pt, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(meta + off_t, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(meta + off_s, &sc, sizeof sc, hipMemcpyHostToDevice, st));
@@ -1136,7 +1145,7 @@ static int level_prepare(tmk_level* L, double time, double dt)
     // the clock lives in device memory, so captured graphs stay valid across calls
     L->time = time;
     L->dt = dt;
-    return group_set_clock(L->own, st, 0, time, dt);
+    return group_set_clock(L->own, st, 0, time, dt, L->ctx->P);
 }
 
 extern "C" int tmk_level_step(tmk_level* L, double time, double dt, double* dt_new)

@@ -93,7 +93,8 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
     const int lx = tid % BX, ly = tid / BX;
     const int i = tl.x0 - 1 + lx, j = tl.y0 - 1 + ly;
     const double dt = S.dt;
-    const T dtbydx = (T)(dt / P.dx);
+    const StepConst<T> K = step_const<T>(S);
+    const T dtbydx = K.dtbydx;
     const int pred = P.pred;
     const T rv1 = (T)-1;
 
@@ -183,7 +184,7 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
                 Hhi.s[s][k] = lane_shift<DPP_FROM_NEXT>(H0.s[s][k]);
             }
         __syncthreads();
-        const T scale = (T)(-(1.0 / 2.0) * dt / P.dx);
+        const T scale = K.scale;
         // y predictor (used on valid columns, ring rows) corrected with the x-face states
         transverse_add<T, NS>(m, scale, scale, H0, Hhi, WmY);
         transverse_add<T, NS>(m, scale, scale, H0, Hhi, WpY);
@@ -246,7 +247,7 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
                     F1h.v[s][k] = exB[(s * NS + k) * NT + tid + BX];
                     F1h.s[s][k] = exB[(2 * NS + s * NS + k) * NT + tid + BX];
                 }
-            update_cell<T, NS>(a, pt, P, m, dt, i, j, F0, F0h, F1, F1h);
+            update_cell<T, NS>(a, pt, P, m, dt, K.sc, i, j, F0, F0h, F1, F1h);
         }
     }
 }
@@ -279,7 +280,8 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
     const int lx = c % BX, ly = c / BX;
     const int i = tl.x0 - 1 + lx, j = tl.y0 - 1 + ly;
     const double dt = S.dt;
-    const T dtbydx = (T)(dt / P.dx);
+    const StepConst<T> K = step_const<T>(S);
+    const T dtbydx = K.dtbydx;
     const int pred = P.pred;
     const T rv1 = (T)-1;
     const T rvy = sd ? rv1 : (T)1;  // velocity sign of this side in y
@@ -358,7 +360,7 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
             Hs[k] = lane_shift<DPP_FROM_NEXT>(H0s[k]);
         }
         __syncthreads();
-        const T scale = (T)(-(1.0 / 2.0) * dt / P.dx);
+        const T scale = K.scale;
         transverse_side<T, NS>(m, scale, H0v, H0s, Hv, Hs, WmYv, WmYs);
         transverse_side<T, NS>(m, scale, H0v, H0s, Hv, Hs, WpYv, WpYs);
 #pragma unroll
@@ -407,7 +409,7 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
                 F.yhv[k] = exB[(sd * 2 * NS + k) * NC + c + BX];
                 F.yhs[k] = exB[(sd * 2 * NS + NS + k) * NC + c + BX];
             }
-            update_cell_side<T, NS>(a, pt, P, m, dt, i, j, sd, F);
+            update_cell_side<T, NS>(a, pt, P, m, dt, K.sc, i, j, sd, F);
         }
     }
 }

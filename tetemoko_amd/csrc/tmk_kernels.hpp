@@ -55,9 +55,40 @@ struct Scen {
     DevParams P;
     double time;  // time at the start of the current step
     double dt;
+    // step constants, evaluated on the host with the expressions the update uses:
+    // dt/dx, -(1/2) dt/dx (transverse correction), -dt/dx (flux difference), and in FP32
+    double dtdx, hdtdx, mdtdx;
+    float fdtdx, fhdtdx, fmdtdx;
     int err;      // first error reported by any launch (0 = none)
     int pad;
 };
+
+// host: the clock and step constants of a scenario
+__host__ inline void scen_set_clock(Scen& s, double time, double dt)
+{
+    s.time = time;
+    s.dt = dt;
+    s.dtdx = dt / s.P.dx;
+    s.hdtdx = -(1.0 / 2.0) * dt / s.P.dx;
+    s.mdtdx = -dt / s.P.dx;
+    s.fdtdx = (float)s.dtdx;
+    s.fhdtdx = (float)s.hdtdx;
+    s.fmdtdx = (float)s.mdtdx;
+}
+
+// step constants in the bulk arithmetic type
+template <typename T>
+struct StepConst {
+    T dtbydx, scale, sc;  // dt/dx, -(1/2) dt/dx, -dt/dx
+};
+template <typename T>
+__device__ __forceinline__ StepConst<T> step_const(const Scen& S)
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return {(T)S.dtdx, (T)S.hdtdx, (T)S.mdtdx};
+    else
+        return {(T)S.fdtdx, (T)S.fhdtdx, (T)S.fmdtdx};
+}
 
 struct Tile {
     int patch;
@@ -174,7 +205,7 @@ struct Faces1 {
 // difference, LinElastPatchGodunov.cpp:117-126), then the optional Drucker-Prager return
 template <typename T, int NS>
 __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
-                                                 const Mat<T>& m, double dt, int i, int j, int sd,
+                                                 const Mat<T>& m, double dt, T sc, int i, int j, int sd,
                                                  const Faces1<T, NS>& F)
 {
     const T ry = sd ? (T)-1 : (T)1;
@@ -183,7 +214,6 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     face_flux_side<T, NS>(m, F.xhv, F.xhs, (T)1, fxh);
     face_flux_side<T, NS>(m, F.ylv, F.yls, ry, fyl);
     face_flux_side<T, NS>(m, F.yhv, F.yhs, ry, fyh);
-    const T sc = (T)(-dt / P.dx);
     const size_t u = uidx(pt, i, j, 0);
     const int o = 11 * sd;
     // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
@@ -245,7 +275,7 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
 // both sides
 template <typename T, int NS>
 __device__ __forceinline__ void update_cell(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
-                                            const Mat<T>& m, double dt, int i, int j, const Slots<T, NS>& F0,
+                                            const Mat<T>& m, double dt, T sc, int i, int j, const Slots<T, NS>& F0,
                                             const Slots<T, NS>& F0h, const Slots<T, NS>& F1,
                                             const Slots<T, NS>& F1h)
 {
@@ -259,7 +289,7 @@ __device__ __forceinline__ void update_cell(const KArgs<T>& a, const Patch<T>& p
             F.ylv[k] = F1.v[sd][k]; F.yls[k] = F1.s[sd][k];
             F.yhv[k] = F1h.v[sd][k]; F.yhs[k] = F1h.s[sd][k];
         }
-        update_cell_side<T, NS>(a, pt, P, m, dt, i, j, sd, F);
+        update_cell_side<T, NS>(a, pt, P, m, dt, sc, i, j, sd, F);
     }
 }
 
@@ -286,7 +316,8 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
     const bool valid = lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny;
     const double time = S.time;
     const double dt = S.dt;
-    const T dtbydx = (T)(dt / P.dx);
+    const StepConst<T> K = step_const<T>(S);
+    const T dtbydx = K.dtbydx;
     const int pred = P.pred;
     const T rv1 = (T)-1;
 
@@ -465,7 +496,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                 put(6 * NS + s * NS + k, H1.s[s][k]);
             }
         __syncthreads();
-        const T scale = (T)(-(1.0 / 2.0) * dt / P.dx);
+        const T scale = K.scale;
         // x predictor (cells of ccBox[1]: ring columns, valid rows) corrected with WHalf1[1]
         if (colD && lx <= tnx + 1 && ly >= 1 && ly <= tny) {
             Slots<T, NS> Hhi;
@@ -636,7 +667,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                     F1h.v[s][k] = get(4 * NS + s * NS + k, tid + BX);
                     F1h.s[s][k] = get(6 * NS + s * NS + k, tid + BX);
                 }
-            update_cell<T, NS>(a, pt, P, m, dt, i, j, F0, F0h, F1, F1h);
+            update_cell<T, NS>(a, pt, P, m, dt, K.sc, i, j, F0, F0h, F1, F1h);
         }
     }
 }

@@ -27,6 +27,9 @@ namespace tmk {
 struct DevParams {
     double cs, cp, G, rho, lam, K, Zp, Zs;
     double iZp, iZs, irho;  // correctly rounded reciprocals 1/Zp, 1/Zs, 1/rho
+    double i2Zp, i2Zs;      // 1/(2 Zp), 1/(2 Zs): the reciprocals halved (exact)
+    // the same constants rounded to FP32 on the host, for the FP32-bulk kernels
+    float frho, flam, fG, fZp, fZs, fden, flam2G, fiZp, fiZs, firho, fi2Zp, fi2Zs, fcp, fcs;
     double den;    // 2G + lam   (LinElastPhysicsF.ChF:427 form)
     double lam2G;  // lam + 2G   (LinElastPhysicsF.ChF:318 form)
     double bg[22];
@@ -72,15 +75,28 @@ __device__ __forceinline__ double divr(double a, double b, double y)
 }
 __device__ __forceinline__ float divr(float a, float, float y) { return a * y; }
 
+// FP64 value or its host-rounded FP32 copy
+template <typename T>
+__device__ __forceinline__ T pick(double d, float f)
+{
+    if constexpr (sizeof(T) == sizeof(double))
+        return (T)d;
+    else
+        return (T)f;
+}
+
 // elastic constants in the bulk arithmetic type
 template <typename T>
 struct Mat {
     T rho, lam, G, Zp, Zs, den, lam2G;
     T iZp, iZs, irho, i2Zp, i2Zs;  // reciprocals (1/(2Z) = (1/Z)/2 exactly)
+    T cp, cs;
     __device__ __forceinline__ explicit Mat(const DevParams& p)
-        : rho((T)p.rho), lam((T)p.lam), G((T)p.G), Zp((T)p.Zp), Zs((T)p.Zs), den((T)p.den),
-          lam2G((T)p.lam2G), iZp((T)p.iZp), iZs((T)p.iZs), irho((T)p.irho), i2Zp((T)(0.5 * p.iZp)),
-          i2Zs((T)(0.5 * p.iZs)) {}
+        : rho(pick<T>(p.rho, p.frho)), lam(pick<T>(p.lam, p.flam)), G(pick<T>(p.G, p.fG)),
+          Zp(pick<T>(p.Zp, p.fZp)), Zs(pick<T>(p.Zs, p.fZs)), den(pick<T>(p.den, p.fden)),
+          lam2G(pick<T>(p.lam2G, p.flam2G)), iZp(pick<T>(p.iZp, p.fiZp)), iZs(pick<T>(p.iZs, p.fiZs)),
+          irho(pick<T>(p.irho, p.firho)), i2Zp(pick<T>(p.i2Zp, p.fi2Zp)), i2Zs(pick<T>(p.i2Zs, p.fi2Zs)),
+          cp(pick<T>(p.cp, p.fcp)), cs(pick<T>(p.cs, p.fcs)) {}
     __device__ __forceinline__ T Z(int k) const { return k == 0 ? Zp : Zs; }
     // x / Z(k), x / (2 Z(k)), x / rho, rounded like the division
     __device__ __forceinline__ T divZ(T x, int k) const { return k == 0 ? divr(x, Zp, iZp) : divr(x, Zs, iZs); }
@@ -240,7 +256,7 @@ __device__ __forceinline__ void normal_pred_side(const DevParams& P, const Mat<T
                 ppm_limit(cp_m, cp_p);
             }
             // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
-            const T sig = (k == 0 ? (T)P.cp : (T)P.cs) * dtbydx;
+            const T sig = (k == 0 ? m.cp : m.cs) * dtbydx;
             {
                 const T mm = cm_m, pp = cm_p;
                 cm_m = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
@@ -299,8 +315,8 @@ __device__ __forceinline__ void normal_pred_side(const DevParams& P, const Mat<T
                 cp_d = vl_limit(cp_d, cpl, cpr);
             }
             // PLM characteristic predictor, lmin = -cp, lmax = +cp
-            const T lk = k == 0 ? (T)P.cp : (T)P.cs;
-            const T lmin = -(T)P.cp, lmax = (T)P.cp;
+            const T lk = k == 0 ? m.cp : m.cs;
+            const T lmin = -m.cp, lmax = m.cp;
             // wave k-minus (lambda = -lk < 0), wave k-plus (lambda = +lk > 0)
             const T cm_m = -((T)0.5 * ((T)1 + dtbydx * (-lk)) * cm_d);
             const T cm_p = (T)0.5 * ((T)1 - dtbydx * lmax) * cm_d;

@@ -214,6 +214,24 @@ def test_level_interior_tiles(pred, prim, char, fourth, tile):
     assert np.array_equal(Ug[:, far:, :], Ur[:, far:, :])
 
 
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+@pytest.mark.parametrize("tile", [None, (32, 16)])
+def test_level_many_boxes_partial_tiles(precision, tile):
+    """Boxes of 24x24: tiles cut short at box edges run the interior kernel when their stencil lies
+    inside the domain, reading neighbour data from the exchanged ghosts."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    boxes = scenarios.split_boxes(200, 120, 24)
+    nsteps = 4
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, nsteps, seed=33, precision=precision, tile=tile)
+    if precision == capi.PREC_FP64:
+        assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+        assert np.all(_relerr(Bg, Br) <= TOL), _relerr(Bg, Br)
+        far = 3 * nsteps + 3
+        assert np.array_equal(Ug[:, far:, :], Ur[:, far:, :])
+    else:
+        assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+
+
 def test_level_interior_tiles_fp32():
     cfg = _cfg("rs", "PPM", nx=200, ny=120)
     (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], 4, seed=32, precision=capi.PREC_FP32_BULK)
@@ -237,6 +255,65 @@ def test_profile_kernels():
     assert p["tiles_interior"] == 5 * 7
     assert p["cells_interior"] == 30 * 14 * p["tiles_interior"]
     assert p["ms_interior"] > 0 and p["ms_boundary"] > 0 and p["ms_step"] > 0
+
+
+BIG = (0, 0, 199, 119)  # a 200x120 patch: interior tiles in every thread layout, plus boundary pieces
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+@pytest.mark.parametrize("tile", [None, (32, 16), (64, 4)])
+def test_big_patch_antiplane(precision, tile):
+    """Live anti-plane fields (three slots per sweep) on interior and boundary tiles."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120, antiplane=True)
+    (Ug, Ur), (Bg, Br), _, _, _ = _patch_case(cfg, BIG, seed=13, precision=precision, antiplane=True, tile=tile)
+    tol = TOL if precision == capi.PREC_FP64 else 1e-5
+    assert np.all(_relerr(Ug, Ur) <= tol), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= (TOL if tol == TOL else 1e-4))
+    if precision == capi.PREC_FP64:
+        assert np.array_equal(Ug[:, G + 4:, :], Ur[:, G + 4:, :])
+
+
+@pytest.mark.parametrize("pred,prim,char,fourth", [("CTU", 1, 0, 1), ("PLM", 0, 0, 0), ("PLM", 1, 0, 0),
+                                                   ("PLM", 0, 1, 1), ("PPM", 0, 0, 1)])
+@pytest.mark.parametrize("tile", [None, (32, 16)])
+def test_big_patch_predictors(pred, prim, char, fourth, tile):
+    """Every predictor/limiter variant through the interior kernels (runtime predictor kind):
+    bit-identical to the oracle beyond the fault's reach."""
+    cfg = _cfg("sw", pred, nx=200, ny=120, prim=prim, char=char, fourth=fourth)
+    (Ug, Ur), (Bg, Br), _, _, _ = _patch_case(cfg, BIG, seed=14, tile=tile)
+    assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+    assert np.array_equal(Ug[:, G + 4:, :], Ur[:, G + 4:, :])
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+@pytest.mark.parametrize("tile", [None, (32, 16)])
+def test_big_patch_fluxes(precision, tile):
+    """Face fluxes for flux registers written by the interior kernels (both layouts)."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    cfg["rs"]["state_law"] = 1  # the slip law overflows psi on this seeded perturbation, in the oracle too
+    _, _, (F0g, F0r), (F1g, F1r), _ = _patch_case(cfg, BIG, seed=15, precision=precision, with_flux=True,
+                                                  tile=tile)
+    tol = TOL if precision == capi.PREC_FP64 else 1e-5
+    assert np.all(_relerr(F0g, F0r) <= tol), _relerr(F0g, F0r)
+    assert np.all(_relerr(F1g, F1r) <= tol), _relerr(F1g, F1r)
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+@pytest.mark.parametrize("tile", [None, (32, 16)])
+def test_big_level_plasticity(precision, tile):
+    """The fused Drucker-Prager return on interior tiles, one thread per cell and per side."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    cfg["plasticity"] = 1
+    cfg["p_mu"] = 0.6
+    cfg["p_beta"] = 0.3
+    cfg["p_eta"] = 0.03 * scenarios.G_MPA
+    cfg["p_b"] = 1.0
+    cfg["rs"]["state_law"] = 1
+    cfg["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], 4, seed=16, precision=precision, tile=tile)
+    tol = TOL if precision == capi.PREC_FP64 else 1e-5
+    assert np.all(_relerr(Ug, Ur) <= tol), _relerr(Ug, Ur)
+    assert np.max(Ur[9]) > 0.0, "plastic flow expected somewhere"
 
 
 def test_level_graph_run_matches_steps():

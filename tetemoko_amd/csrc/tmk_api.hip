@@ -116,11 +116,11 @@ void xcd_order(std::vector<Tile>& t)
     t.swap(o);
 }
 
-// a tile is interior when it is full size and its stencil (+-3 cells) lies inside the domain
-// (the same rule as the kernels' tile_interior)
+// a tile is interior when its stencil (+-3 cells) lies inside the domain (the same rule as the
+// kernels' tile_interior); partial tiles at box edges qualify
 bool tile_interior_host(const Tile& t, const DevParams& P, int bx, int by)
 {
-    return t.x1 - t.x0 + 1 == bx - 2 && t.y1 - t.y0 + 1 == by - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
+    return t.x1 - t.x0 + 1 <= bx - 2 && t.y1 - t.y0 + 1 <= by - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 

@@ -1,4 +1,4 @@
-// tmk_interior.hpp -- the update of an interior tile (full size, stencil inside the domain).
+// tmk_interior.hpp -- the update of an interior tile (stencil inside the domain).
 //
 // Same arithmetic as godunov_tile (tmk_kernels.hpp) on the same cells, in the same order, so
 // results are bit-identical; only the data movement differs:
@@ -99,6 +99,10 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
     const T rv1 = (T)-1;
 
     // ---------------- staging: both sweeps' stencils ----------------
+    // a partial tile (at a box edge) stages only its own stencil region, tile +-3, which lies in
+    // the box's ghosted array; the rest of the box is zero and feeds only lanes past the tile
+    const int tnx = tl.x1 - tl.x0 + 1, tny = tl.y1 - tl.y0 + 1;
+    const bool full = tnx == BX - 2 && tny == BY - 2;
     {
         const T* src[NQ];
 #pragma unroll
@@ -109,8 +113,13 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
             if (p < SP) {
                 const int yy = p / SX, xx = p - yy * SX;
                 const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
+                if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) stg[q * SP + p] = src[q][g];
+                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = src[q][g];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = (T)0;
+                }
             }
         }
     }
@@ -237,9 +246,9 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
                 F0h.s[s][k] = lane_shift<DPP_FROM_NEXT>(F0.s[s][k]);
             }
         __syncthreads();
-        if (pt.F[0] && lx >= 1 && ly >= 1 && ly <= BY - 2) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
-        if (pt.F[1] && ly >= 1 && lx >= 1 && lx <= BX - 2) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
-        if (lx >= 1 && lx <= BX - 2 && ly >= 1 && ly <= BY - 2) {
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
+        if (lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny) {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -288,6 +297,8 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
     const int q0 = sd * NQ1;        // first staged plane of this side
 
     // ---------------- staging: this side's stencil planes ----------------
+    const int tnx = tl.x1 - tl.x0 + 1, tny = tl.y1 - tl.y0 + 1;
+    const bool full = tnx == BX - 2 && tny == BY - 2;
     {
         const T* src[NQ1];
 #pragma unroll
@@ -298,8 +309,13 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
             if (p < SP) {
                 const int yy = p / SX, xx = p - yy * SX;
                 const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
+                if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = src[q][g];
+                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = src[q][g];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = (T)0;
+                }
             }
         }
     }
@@ -401,9 +417,11 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
             F.xhs[k] = lane_shift<DPP_FROM_NEXT>(F.xls[k]);
         }
         __syncthreads();
-        if (pt.F[0] && lx >= 1 && ly >= 1 && ly <= BY - 2) store_flux_side<T, NS>(pt, m, 0, F.xlv, F.xls, (T)1, i, j, sd);
-        if (pt.F[1] && ly >= 1 && lx >= 1 && lx <= BX - 2) store_flux_side<T, NS>(pt, m, 1, F.ylv, F.yls, rvy, i, j, sd);
-        if (lx >= 1 && lx <= BX - 2 && ly >= 1 && ly <= BY - 2) {
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny)
+            store_flux_side<T, NS>(pt, m, 0, F.xlv, F.xls, (T)1, i, j, sd);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx)
+            store_flux_side<T, NS>(pt, m, 1, F.ylv, F.yls, rvy, i, j, sd);
+        if (lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
                 F.yhv[k] = exB[(sd * 2 * NS + k) * NC + c + BX];
@@ -450,7 +468,8 @@ constexpr int interior_waves()
 }
 
 // interior tiles only (the host sorts them out with the same rule); a tile that is not
-// interior reports TMK_E_BOX and is left untouched
+// interior reports TMK_E_BOX and is left untouched.  Tiles smaller than (BX-2) x (BY-2) (at box
+// edges) are interior too when their stencil lies inside the domain.
 template <typename T, int NS, int BX, int BY, int PK>
 __global__ void __launch_bounds__((interior_threads<T, NS, BX, BY>()), (interior_waves<T, NS, BX, BY>()))
 k_godunov_interior(const KArgs<T> a)

@@ -672,11 +672,12 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
     }
 }
 
-// a tile is interior when it is full size and its stencil (+-3 cells) lies inside the domain
+// a tile is interior when its stencil (+-3 cells) lies inside the domain (and it fits the
+// workgroup): then no cell of it is a domain-boundary or fault cell
 template <int BX, int BY>
 __device__ __forceinline__ bool tile_interior(const Tile& t, const DevParams& P)
 {
-    return t.x1 - t.x0 + 1 == BX - 2 && t.y1 - t.y0 + 1 == BY - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
+    return t.x1 - t.x0 + 1 <= BX - 2 && t.y1 - t.y0 + 1 <= BY - 2 && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] &&
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 

@@ -542,12 +542,12 @@ static int group_enqueue_step_t(Group& G, hipStream_t st, const StepTimers* tm =
     if (e) return e;
     const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g];  // its out planes hold the new state
     if (G.nrects > 0) {
-        hipLaunchKernelGGL(k_exchange<T>, dim3(4, G.nrects), dim3(256), 0, st, pn, G.rects_dev, G.nrects,
+        hipLaunchKernelGGL(k_exchange<T>, dim3(G.nrects), dim3(128), 0, st, pn, G.rects_dev, G.nrects,
                            stencil_mask(G.ns));
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     if (G.nbrects > 0) {
-        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects);
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(G.nbrects), dim3(64), 0, st, pn, G.brects_dev, G.nbrects);
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     hipLaunchKernelGGL(k_advance_time, dim3((G.nscen + 63) / 64), dim3(64), 0, st, G.scen_dev, G.nscen);
@@ -572,12 +572,12 @@ static int group_exchange_now_t(Group& G, hipStream_t st)
     // patches_dev[g ^ 1].out are the planes holding the current state
     const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g ^ 1];
     if (G.nrects > 0) {
-        hipLaunchKernelGGL(k_exchange<T>, dim3(4, G.nrects), dim3(256), 0, st, pn, G.rects_dev, G.nrects,
+        hipLaunchKernelGGL(k_exchange<T>, dim3(G.nrects), dim3(128), 0, st, pn, G.rects_dev, G.nrects,
                            stencil_mask(G.ns));
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     if (G.nbrects > 0) {
-        hipLaunchKernelGGL(k_bexchange<T>, dim3(1, G.nbrects), dim3(256), 0, st, pn, G.brects_dev, G.nbrects);
+        hipLaunchKernelGGL(k_bexchange<T>, dim3(G.nbrects), dim3(64), 0, st, pn, G.brects_dev, G.nbrects);
         if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
     }
     return TMK_OK;
@@ -1537,7 +1537,7 @@ static int level_tag_t(tmk_level* L, const tmk_tagparams* tp, unsigned char* tag
     ta.nx = nx; ta.ny = ny;
     // patches_dev[g].in hold the current state
     const Patch<T>* pd = (const Patch<T>*)L->own.patches_dev[L->own.g];
-    hipLaunchKernelGGL(k_tag<T>, dim3(64, (unsigned)L->boxes.size()), dim3(256), 0, st, pd, L->own.scen_dev, ta, raw);
+    hipLaunchKernelGGL(k_tag<T>, dim3((unsigned)L->boxes.size(), 64), dim3(256), 0, st, pd, L->own.scen_dev, ta, raw);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_dilate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, raw + n, nx, ny,
                        tp->tag_buffer);

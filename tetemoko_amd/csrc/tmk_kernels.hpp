@@ -721,14 +721,15 @@ struct CopyRect {
     int lo0, lo1, hi0, hi1;
 };
 
+// one workgroup per rectangle (most are a few ghost rows of a small box)
 template <typename T>
 __global__ void k_exchange(const Patch<T>* patches, const CopyRect* rects, int nrects, int ncomp_mask)
 {
-    const CopyRect r = rects[blockIdx.y];
+    const CopyRect r = rects[blockIdx.x];
     const Patch<T>& S = patches[r.src];
     const Patch<T>& D = patches[r.dst];
     const int nx = r.hi0 - r.lo0 + 1, ny = r.hi1 - r.lo1 + 1;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nx * ny; t += gridDim.x * blockDim.x) {
+    for (int t = threadIdx.x; t < nx * ny; t += blockDim.x) {
         const int i = r.lo0 + t % nx, j = r.lo1 + t / nx;
         const size_t si = (size_t)(i - S.ulo0) + (size_t)(j - S.ulo1) * S.unx;
         const size_t di = (size_t)(i - D.ulo0) + (size_t)(j - D.ulo1) * D.unx;
@@ -741,7 +742,7 @@ __global__ void k_exchange(const Patch<T>* patches, const CopyRect* rects, int n
 template <typename T>
 __global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int nrects)
 {
-    const CopyRect r = rects[blockIdx.y];
+    const CopyRect r = rects[blockIdx.x];
     const Patch<T>& S = patches[r.src];
     const Patch<T>& D = patches[r.dst];
     const int nx = r.hi0 - r.lo0 + 1;
@@ -824,10 +825,10 @@ __device__ __forceinline__ double undiv(const Patch<T>& p, int c, int i, int j, 
 template <typename T>
 __global__ void k_tag(const Patch<T>* patches, const Scen* scen, TagArgs ta, unsigned char* raw)
 {
-    const Patch<T>& p = patches[blockIdx.y];
+    const Patch<T>& p = patches[blockIdx.x];  // box index on x (no 65535 limit), cell chunks on y
     const DevParams& P = scen[p.scen].P;
     const int vnx = p.vhi0 - p.vlo0 + 1, vny = p.vhi1 - p.vlo1 + 1;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < vnx * vny; t += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < vnx * vny; t += gridDim.y * blockDim.x) {
         const int i = p.vlo0 + t % vnx, j = p.vlo1 + t / vnx;
         // GETFULLGRADF over the side-1 velocities, then MAXF over directions
         double gm = 0.0;

@@ -201,18 +201,26 @@ struct Group {
     const StationDev* stations = nullptr;  // fault stations recorded after each step (level mode)
     int g = 0;  // parity of the next launch
     hipGraphExec_t gexec = nullptr;
-    void release()
+    // grow-only backing of the device tables above: a rebuild (membership change)
+    // rewrites them in place, so no device allocation or free per rebuild
+    DevBuf mem_scen, mem_pt[2], mem_tiles, mem_rects, mem_brects;
+    bool live() const { return nscen > 0; }
+    // forget the layout; keeps the side stream, events and device tables for reuse.
+    // The caller has synchronised the stream the group ran on.
+    void reset()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
-        for (auto& p : patches_dev) { if (p) (void)hipFree(p); p = nullptr; }
-        if (scen_dev) (void)hipFree(scen_dev);
-        if (tiles_dev) (void)hipFree(tiles_dev);
-        if (rects_dev) (void)hipFree(rects_dev);
-        if (brects_dev) (void)hipFree(brects_dev);
         scen_dev = nullptr; tiles_dev = nullptr; rects_dev = nullptr; brects_dev = nullptr;
+        patches_dev[0] = patches_dev[1] = nullptr;
         nscen = ntiles = nint = nrects = nbrects = 0;
+    }
+    void release()
+    {
+        reset();
         if (side) (void)hipStreamSynchronize(side);
+        mem_scen.release(); mem_pt[0].release(); mem_pt[1].release();
+        mem_tiles.release(); mem_rects.release(); mem_brects.release();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (side) (void)hipStreamDestroy(side);
@@ -267,7 +275,9 @@ struct tmk_batch {
     std::vector<tmk_level*> members;
     std::vector<int> phase;       // member parity at group parity 0
     Group grp;
+    Group xgrp;                   // one-member group for the ghost exchange of a newly added member
     bool built = false;
+    bool stale = false;           // membership changed since the group was built
 };
 
 static const char* k_errs[] = {"ok",
@@ -616,7 +626,7 @@ static int group_run(Group& G, hipStream_t st, int nsteps)
 template <typename T>
 static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std::vector<int>& phase, int plastic)
 {
-    G.release();
+    G.reset();
     if (mem.empty()) return TMK_OK;
     tmk_level* L0 = mem[0];
     G.tsize = sizeof(T);
@@ -670,32 +680,32 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         HIPCHK(hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming));
     }
+    // the tables are rewritten in place: every launch that read them has completed
+    // (the callers synchronise the group's stream first; the side stream joins it)
+    auto put = [](DevBuf& buf, const void* src, size_t bytes, void** dst) -> int {
+        int e = buf.ensure(std::max<size_t>(bytes, 1));
+        if (e) return e;
+        *dst = buf.p;
+        if (bytes && hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return TMK_E_HIP;
+        return TMK_OK;
+    };
+    int e;
+    if ((e = put(G.mem_scen, sc.data(), sizeof(Scen) * sc.size(), (void**)&G.scen_dev))) return e;
+    for (int g = 0; g < 2; ++g)
+        if ((e = put(G.mem_pt[g], pts[g].data(), sizeof(Patch<T>) * pts[g].size(), &G.patches_dev[g]))) return e;
+    if ((e = put(G.mem_tiles, tiles.data(), sizeof(Tile) * tiles.size(), (void**)&G.tiles_dev))) return e;
+    if ((e = put(G.mem_rects, rects.data(), sizeof(CopyRect) * rects.size(), (void**)&G.rects_dev))) return e;
+    if ((e = put(G.mem_brects, brects.data(), sizeof(CopyRect) * brects.size(), (void**)&G.brects_dev))) return e;
     G.nscen = (int)sc.size();
     G.ntiles = (int)tiles.size();
     G.nrects = (int)rects.size();
     G.nbrects = (int)brects.size();
-    HIPCHK(hipMalloc(&G.scen_dev, sizeof(Scen) * sc.size()));
-    HIPCHK(hipMemcpy(G.scen_dev, sc.data(), sizeof(Scen) * sc.size(), hipMemcpyHostToDevice));
-    for (int g = 0; g < 2; ++g) {
-        HIPCHK(hipMalloc(&G.patches_dev[g], sizeof(Patch<T>) * pts[g].size()));
-        HIPCHK(hipMemcpy(G.patches_dev[g], pts[g].data(), sizeof(Patch<T>) * pts[g].size(), hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMalloc(&G.tiles_dev, sizeof(Tile) * std::max<size_t>(tiles.size(), 1)));
-    if (!tiles.empty()) HIPCHK(hipMemcpy(G.tiles_dev, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
-    if (!rects.empty()) {
-        HIPCHK(hipMalloc(&G.rects_dev, sizeof(CopyRect) * rects.size()));
-        HIPCHK(hipMemcpy(G.rects_dev, rects.data(), sizeof(CopyRect) * rects.size(), hipMemcpyHostToDevice));
-    }
-    if (!brects.empty()) {
-        HIPCHK(hipMalloc(&G.brects_dev, sizeof(CopyRect) * brects.size()));
-        HIPCHK(hipMemcpy(G.brects_dev, brects.data(), sizeof(CopyRect) * brects.size(), hipMemcpyHostToDevice));
-    }
     return TMK_OK;
 }
 
 static int group_build(Group& G, const std::vector<tmk_level*>& mem, const std::vector<int>& phase, int plastic)
 {
-    if (mem.empty()) { G.release(); return TMK_OK; }
+    if (mem.empty()) { G.reset(); return TMK_OK; }
     return mem[0]->tsize == sizeof(double) ? group_build_t<double>(G, mem, phase, plastic)
                                            : group_build_t<float>(G, mem, phase, plastic);
 }
@@ -1122,6 +1132,8 @@ static int level_own_group(tmk_level* L)
     if (!L->own_built) {
         std::vector<tmk_level*> m{L};
         std::vector<int> ph{L->cur};
+        // the rebuild rewrites the group's device tables in place
+        HIPCHK(hipStreamSynchronize(L->ctx->stream));
         int e = group_build(L->own, m, ph, 1);
         if (e) return e;
         L->own.stations = L->stations ? &L->stations->dev : nullptr;
@@ -1305,6 +1317,7 @@ extern "C" int tmk_batch_destroy(tmk_batch* B)
     (void)hipStreamSynchronize(B->stream);
     for (tmk_level* L : B->members) L->batch = nullptr;
     B->grp.release();
+    B->xgrp.release();
     (void)hipStreamDestroy(B->stream);
     delete B;
     return TMK_OK;
@@ -1325,30 +1338,33 @@ static int batch_detach_all(tmk_batch* B)
         if (sc[m].err) L->own_built = false;
         B->phase[m] = L->cur;
     }
-    B->grp.release();
+    B->grp.reset();
     B->built = false;
     return TMK_OK;
 }
 
+// Build the fused group for the current members.  Called lazily by the first
+// run after membership changes, so adding or removing several members costs one
+// rebuild.  The batch stream is idle here (batch_detach_all synchronised it).
 static int batch_rebuild(tmk_batch* B)
 {
+    B->stale = false;
     for (size_t m = 0; m < B->members.size(); ++m) B->phase[m] = B->members[m]->cur;
     int e = group_build(B->grp, B->members, B->phase, 1);
     if (e) return e;
     B->built = !B->members.empty();
-    // exchange members that were uploaded since their last step
+    // exchange members that were uploaded since their last step (a one-box level
+    // has no same-level copies: its ghosts are filled by the boundary stencils)
     for (tmk_level* L : B->members) {
         if (!L->dirty) continue;
         L->dirty = false;
-        // a one-member exchange on the batch stream
-        Group tmp;
+        if (L->rects.empty() && L->brects.empty()) continue;
         std::vector<tmk_level*> m{L};
         std::vector<int> ph{L->cur};
-        if ((e = group_build(tmp, m, ph, 1))) return e;
-        e = L->tsize == sizeof(double) ? group_exchange_now_t<double>(tmp, B->stream)
-                                       : group_exchange_now_t<float>(tmp, B->stream);
-        (void)hipStreamSynchronize(B->stream);
-        tmp.release();
+        if ((e = group_build(B->xgrp, m, ph, 1))) return e;
+        e = L->tsize == sizeof(double) ? group_exchange_now_t<double>(B->xgrp, B->stream)
+                                       : group_exchange_now_t<float>(B->xgrp, B->stream);
+        HIPCHK(hipStreamSynchronize(B->stream));
         if (e) return e;
     }
     return TMK_OK;
@@ -1370,7 +1386,8 @@ extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
     L->batch = B;
     B->members.push_back(L);
     B->phase.push_back(L->cur);
-    return batch_rebuild(B);
+    B->stale = true;
+    return TMK_OK;
 }
 
 extern "C" int tmk_batch_remove(tmk_batch* B, tmk_level* L)
@@ -1384,15 +1401,18 @@ extern "C" int tmk_batch_remove(tmk_batch* B, tmk_level* L)
     B->phase.erase(B->phase.begin() + m);
     L->batch = nullptr;
     L->own_built = false;  // rebuild its own group at its current parity
-    L->own.release();
-    return batch_rebuild(B);
+    L->own.reset();
+    B->stale = true;
+    return TMK_OK;
 }
 
 extern "C" int tmk_batch_run(tmk_batch* B, int nsteps)
 {
     if (!B || nsteps < 0) return TMK_E_ARG;
+    int e;
+    if (B->stale && (e = batch_rebuild(B))) return e;
     if (!B->built || nsteps == 0) return TMK_OK;
-    int e = group_run(B->grp, B->stream, nsteps);
+    e = group_run(B->grp, B->stream, nsteps);
     if (e) return e;
     for (tmk_level* L : B->members)
         for (int k = 0; k < nsteps; ++k) L->time = L->time + L->dt;

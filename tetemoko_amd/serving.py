@@ -16,7 +16,6 @@ The reference runs one scenario per process with no serving mode
 nothing couples the members of a batch.
 """
 import argparse
-import heapq
 import json
 import math
 import time
@@ -100,93 +99,135 @@ def _result(r, B):
 
 def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reserve=8 << 30, tile=None,
           log=None, keep_fault=False):
-    """Run the arrival stream; returns (results, stats)."""
+    """Run the arrival stream; returns (results, stats).
+
+    The loop keeps the device busy while the host works: chunk k+1 of the shared
+    launches is enqueued before the results of chunk k are handled, and new
+    scenarios are set up (context, level allocation, initial upload) while a chunk
+    runs.  Membership changes take effect at chunk boundaries.
+    """
     rng = np.random.default_rng(12345)
     gaps = rng.exponential(1.0 / rate_hz, size=len(scen_list)) if rate_hz > 0 else np.zeros(len(scen_list))
     arrivals = np.cumsum(gaps) - gaps[0]
     pending = list(zip(arrivals, range(len(scen_list))))
     queue = []
     batch = capi.Batch()
-    running = []
+    running = []      # members of the batch
+    ready = []        # set up, joining the batch at the next chunk boundary
     results, latencies = [], []
     n_failed = 0
     cell_updates = 0
+    footprint = {}    # (fault, nx, ny) -> device bytes of one scenario's level
     t0 = time.perf_counter()
     ip = 0
+    phase = {"admit": 0.0, "launch": 0.0, "sync": 0.0, "retire": 0.0, "idle": 0.0}
+    inflight = 0      # steps of the chunk in flight (0: none)
 
     def now():
         return time.perf_counter() - t0
 
-    while ip < len(pending) or queue or running:
-        t = now()
-        while ip < len(pending) and pending[ip][0] <= t:
-            queue.append(pending[ip])
-            ip += 1
+    class _clock:  # host time spent per phase of the loop
+        def __init__(self, name):
+            self.name = name
+
+        def __enter__(self):
+            self.t = time.perf_counter()
+
+        def __exit__(self, *exc):
+            phase[self.name] += time.perf_counter() - self.t
+
+    def admit():
         # admission by free device memory, in arrival order
         while queue:
             arr, k = queue[0]
             sc = scen_list[k]
             cfg = sc["cfg"]
-            ctx = capi.Context(cfg, precision)
-            need = capi.level_footprint(ctx, [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)])
-            ctx.close()
+            key = (cfg["fault"], cfg["nx"], cfg["ny"])
+            ctx = None
+            if key not in footprint:
+                ctx = capi.Context(cfg, precision)
+                footprint[key] = capi.level_footprint(ctx, [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)])
             free, _ = capi.mem_info()
-            if need + mem_reserve > free:
+            if footprint[key] + mem_reserve > free:
+                if ctx is not None:
+                    ctx.close()
                 break
             queue.pop(0)
+            if ctx is not None:
+                ctx.close()
             ctx, lev = _start(sc, precision, tile)
-            dt = scenarios.dt_of(cfg)
-            batch.add(lev, 0.0, dt)
-            running.append(Running(sc, ctx, lev, arr, now(), dt))
-        if not running:
-            if ip < len(pending):
-                time.sleep(max(0.0, pending[ip][0] - now()))
-            continue
-        n = min(chunk, min(r.sc["steps"] - r.done_steps for r in running))
-        batch.run(n)
-        failed = set()
-        try:
-            batch.sync()
-        except capi.TmkError:
-            # a scenario's own error (e.g. a NaN state variable) retires only that scenario
-            failed = {id(r) for r in running if batch.member_error(r.lev) != 0}
-        for r in running:
-            r.done_steps += n
-            cell_updates += r.sc["cfg"]["nx"] * r.sc["cfg"]["ny"] * n
-        for r in [r for r in running if id(r) in failed]:
-            code = batch.member_error(r.lev)
-            batch.remove(r.lev)
-            res = {"id": r.sc["id"], "failed": capi.strerror(code), "arrival_s": float(r.arrival),
-                   "latency_s": float(now() - r.arrival)}
-            results.append(res)
-            n_failed += 1
-            r.lev.close()
-            r.ctx.close()
-            running.remove(r)
-            if log:
-                log("failed %d: %s" % (res["id"], res["failed"]))
-        finished = [r for r in running if r.done_steps >= r.sc["steps"]]
-        for r in finished:
-            batch.remove(r.lev)
+            ready.append(Running(sc, ctx, lev, arr, now(), scenarios.dt_of(cfg)))
+
+    def retire(r, B, failed_code=None):
+        r.lev.close()
+        r.ctx.close()
+        running.remove(r)
+        if failed_code is not None:
+            return
+        tdone = now()
+        res = _result(r, B)
+        res["arrival_s"] = float(r.arrival)
+        res["latency_s"] = float(tdone - r.arrival)
+        res["queue_s"] = float(r.admitted - r.arrival)
+        if keep_fault:
+            res["fault_data"] = B
+        results.append(res)
+        latencies.append(tdone - r.arrival)
+        if log:
             cfg = r.sc["cfg"]
-            B = np.empty((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * GHOST))
-            r.lev.download_fault(0, B)
-            tdone = now()
-            res = _result(r, B)
-            res["arrival_s"] = float(r.arrival)
-            res["latency_s"] = float(tdone - r.arrival)
-            res["queue_s"] = float(r.admitted - r.arrival)
-            if keep_fault:
-                res["fault_data"] = B
-            results.append(res)
-            latencies.append(tdone - r.arrival)
-            r.lev.close()
-            r.ctx.close()
-            running.remove(r)
-            if log:
-                log("done %d (%dx%d %s, %d steps) latency %.2fs running %d queued %d" %
-                    (res["id"], cfg["nx"], cfg["ny"], cfg["fault"], r.sc["steps"], res["latency_s"],
-                     len(running), len(queue)))
+            log("done %d (%dx%d %s, %d steps) latency %.2fs running %d queued %d" %
+                (res["id"], cfg["nx"], cfg["ny"], cfg["fault"], r.sc["steps"], res["latency_s"],
+                 len(running), len(queue)))
+
+    while ip < len(pending) or queue or running or ready or inflight:
+        t = now()
+        while ip < len(pending) and pending[ip][0] <= t:
+            queue.append(pending[ip])
+            ip += 1
+        with _clock("admit"):
+            admit()
+        if inflight:
+            failed = set()
+            with _clock("sync"):
+                try:
+                    batch.sync()
+                except capi.TmkError:
+                    # a scenario's own error (e.g. a NaN state variable) retires only that scenario
+                    failed = {id(r) for r in running if batch.member_error(r.lev) != 0}
+            n = inflight
+            inflight = 0
+            for r in running:
+                r.done_steps += n
+                cell_updates += r.sc["cfg"]["nx"] * r.sc["cfg"]["ny"] * n
+            with _clock("retire"):
+                for r in [r for r in running if id(r) in failed]:
+                    code = batch.member_error(r.lev)
+                    batch.remove(r.lev)
+                    res = {"id": r.sc["id"], "failed": capi.strerror(code), "arrival_s": float(r.arrival),
+                           "latency_s": float(now() - r.arrival)}
+                    results.append(res)
+                    n_failed += 1
+                    retire(r, None, code)
+                    if log:
+                        log("failed %d: %s" % (res["id"], res["failed"]))
+                for r in [r for r in running if r.done_steps >= r.sc["steps"]]:
+                    batch.remove(r.lev)
+                    cfg = r.sc["cfg"]
+                    B = np.empty((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * GHOST))
+                    r.lev.download_fault(0, B)
+                    retire(r, B)
+        with _clock("launch"):
+            for r in ready:
+                batch.add(r.lev, 0.0, r.dt)
+                running.append(r)
+            ready.clear()
+            if running:
+                n = min(chunk, min(r.sc["steps"] - r.done_steps for r in running))
+                batch.run(n)
+                inflight = n
+        if not running and not ready and not queue and ip < len(pending):
+            with _clock("idle"):
+                time.sleep(max(0.0, pending[ip][0] - now()))
     wall = now()
     batch.close()
     lat = np.array(latencies)
@@ -194,7 +235,8 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
              "scenarios_per_s": (len(results) - n_failed) / wall,
              "cell_updates_per_s": cell_updates / wall,
              "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-             "mean_latency_s": float(np.mean(lat)), "arrival_rate_hz": rate_hz, "chunk_steps": chunk}
+             "mean_latency_s": float(np.mean(lat)), "arrival_rate_hz": rate_hz, "chunk_steps": chunk,
+             "host_seconds": {k: round(v, 3) for k, v in phase.items()}}
     return results, stats
 
 

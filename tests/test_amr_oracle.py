@@ -1,0 +1,238 @@
+"""CPU checks of the multi-level oracle (oracle/amr.py).
+
+The coarse-fine state fill and the flux registers restate Chombo classes that are
+not available here, so they are pinned by properties that a plausible mistake
+breaks: exact reproduction of constant and linear fields (a wrong slope side,
+offset or coefficient fails), neighbourhood bounds where the limiter acts, and
+conservation of the composite sum over a subcycled two- and three-level advance
+(a wrong sign, scale or missing fine substep in the register fails; the same run
+without reflux is checked to break it).  A one-level hierarchy reproduces the
+single-level oracle driver bit for bit.
+"""
+import numpy as np
+import pytest
+
+import amr
+import tmo
+from tetemoko_amd import scenarios
+
+G = amr.G
+
+
+def quiet_sw(nx, ny, dx=100.0):
+    """Slip weakening with every face well below its strength: the fault stays locked."""
+    cfg = scenarios.config_c0(seed=3, nx=nx, ny=ny, dx=dx)
+    cfg["faces"]["tau"] = np.full(nx, 20.0)
+    return cfg
+
+
+def bump(lev, xc, yc, radius, amp=1.0):
+    """Compact cos^2 bump (metres) on velocities and stresses of both sides, background elsewhere."""
+    x = (np.arange(-G, lev.nx + G) + 0.5) * lev.dx
+    y = (np.arange(-G, lev.ny + G) + 0.5) * lev.dx
+    X, Y = np.meshgrid(x, y)
+    R = np.hypot(X - xc, Y - yc)
+    w = np.where(R < radius, np.cos(0.5 * np.pi * R / radius) ** 2, 0.0)
+    U = np.empty_like(lev.U)
+    U[:] = scenarios.background_vector(lev.cfg)[:, None, None]
+    for s in (0, 11):
+        U[s + 0] += amp * w
+        U[s + 1] -= 0.5 * amp * w
+        U[s + 3] += 3.0 * amp * w
+        U[s + 4] -= 2.0 * amp * w
+        U[s + 6] += 1.5 * amp * w
+    return U
+
+
+def composite_sum(h):
+    """Sum of (U - background) dx^2 over the cells not covered by a finer level."""
+    tot = np.zeros(amr.NC)
+    for l, lev in enumerate(h.levels):
+        mask = lev.valid.copy()
+        if l + 1 < len(h.levels):
+            r = h.r
+            for b in h.levels[l + 1].boxes:
+                cb = amr.coarsen_box(b, r)
+                mask[cb[1]:cb[3] + 1, cb[0]:cb[2] + 1] = False
+        bg = scenarios.background_vector(lev.cfg)[:, None]
+        vals = lev.U[:, G:-G, G:-G][:, mask] - bg
+        tot += vals.sum(axis=1) * lev.dx * lev.dx
+    return tot
+
+
+def make_hierarchy(cfg, boxes, r, radius=300.0):
+    h = amr.Hierarchy(cfg, boxes, r)
+    xc, yc = 0.5 * cfg["nx"] * cfg["dx"], 0.5 * cfg["ny"] * cfg["dx"]
+    h.init_state(lambda l: bump(h.levels[l], xc, yc, radius), lambda l: scenarios.fault_arrays(h.levels[l].cfg))
+    h.set_dt(scenarios.dt_of(cfg))
+    return h
+
+
+def test_one_level_matches_level_driver():
+    cfg = scenarios.config_c0(seed=5, nx=48, ny=24, dx=100.0)
+    boxes = scenarios.split_boxes(48, 24, 16)
+    h = amr.Hierarchy(cfg, [boxes], 2)
+    U0, B0 = scenarios.level_arrays(cfg)
+    scenarios.perturb(U0, 9)
+    h.init_state(lambda l: U0, lambda l: B0)
+    dt = scenarios.dt_of(cfg)
+    h.set_dt(dt)
+    assert h.advance(3) == 0
+    U, B = U0.copy(), B0.copy()
+    op = tmo.OracleParams(cfg)
+    assert tmo.level_steps(op, U, B, 0.0, dt, 3, maxbox=16) == 0
+    assert np.array_equal(h.levels[0].U[:, G:-G, G:-G], U[:, G:-G, G:-G])
+    assert np.array_equal(h.levels[0].B[:, G:-G], B[:, G:-G])
+
+
+def _fill_setup(r, fine_boxes, ncx=24, ncy=16):
+    cfg = quiet_sw(ncx, ncy)
+    crs = amr.Level(cfg, [(0, 0, ncx - 1, ncy - 1)])
+    fine = amr.Level(amr.refine_cfg(cfg, r), fine_boxes, r)
+    return crs, fine
+
+
+def _linear(nx, ny, a, bx, by):
+    """(22, ny+2G, nx+2G) field a + bx x + by y with x, y in coarse-cell units at cell centres."""
+    x = np.arange(-G, nx + G) + 0.5
+    y = np.arange(-G, ny + G) + 0.5
+    X, Y = np.meshgrid(x, y)
+    k = np.arange(amr.NC)[:, None, None]
+    return a * (1 + k) + bx * (k - 7.0) * X[None] + by * (3.0 - 0.5 * k) * Y[None]
+
+
+@pytest.mark.parametrize("r", [2, 4])
+def test_state_fill_reproduces_linear_fields(r):
+    # one box on the fault (one-sided y slopes beside it at the domain edge), one inside
+    fb = [(4 * r, 0, 8 * r - 1, 4 * r - 1), (12 * r, 6 * r, 18 * r - 1, 10 * r - 1)]
+    crs, fine = _fill_setup(r, fb)
+    old = _linear(crs.nx, crs.ny, 1.0, 0.25, -0.75)
+    new = _linear(crs.nx, crs.ny, 3.0, -0.5, 0.125)
+    crs.Uold, crs.U = old, new
+    fine.U[:] = np.nan
+    alpha = 0.25
+    amr.fill_state(fine, crs, alpha)
+    xf = (np.arange(-G, fine.nx + G) + 0.5) / r
+    yf = (np.arange(-G, fine.ny + G) + 0.5) / r
+    X, Y = np.meshgrid(xf, yf)
+    k = np.arange(amr.NC)[:, None, None]
+    lin = lambda a, bx, by: a * (1 + k) + bx * (k - 7.0) * X[None] + by * (3.0 - 0.5 * k) * Y[None]
+    want = alpha * lin(3.0, -0.5, 0.125) + (1 - alpha) * lin(1.0, 0.25, -0.75)
+    filled = ~np.isnan(fine.U[0])
+    assert filled.sum() > 0
+    # every ghost cell of every box inside the domain and outside the boxes was filled, nothing else
+    expect = np.zeros_like(filled)
+    for b in fb:
+        gb = amr.grow(b, G)
+        expect[max(gb[1], 0) + G:min(gb[3], fine.ny - 1) + G + 1, max(gb[0], 0) + G:min(gb[2], fine.nx - 1) + G + 1] = True
+    expect[G:-G, G:-G] &= ~fine.valid
+    assert np.array_equal(filled, expect)
+    np.testing.assert_allclose(fine.U[:, filled], want[:, filled], rtol=1e-13, atol=1e-12)
+
+
+def test_state_fill_constant_is_exact():
+    r = 2
+    fb = [(8, 0, 15, 7), (16, 0, 23, 11)]
+    crs, fine = _fill_setup(r, fb)
+    c = np.arange(amr.NC, dtype=float)[:, None, None] * 1.7 - 3.0
+    crs.Uold = np.broadcast_to(c, crs.U.shape).copy()
+    crs.U = crs.Uold.copy()
+    fine.U[:] = np.nan
+    amr.fill_state(fine, crs, 0.5)
+    filled = ~np.isnan(fine.U[0])
+    assert np.array_equal(fine.U[:, filled], np.broadcast_to(c[:, :, 0], (amr.NC, filled.sum())))
+
+
+def test_state_fill_limited_values_stay_in_neighbourhood():
+    r = 4
+    fb = [(24, 24, 47, 47)]
+    crs, fine = _fill_setup(r, fb)
+    rng = np.random.default_rng(4)
+    U = rng.uniform(-1, 1, crs.U.shape)
+    U[:, :, : G + 9] += 5.0          # a step across the fine region
+    crs.Uold, crs.U = U, U.copy()
+    fine.U[:] = np.nan
+    amr.fill_state(fine, crs, 1.0)
+    n = 0
+    for jf in range(fine.ny):
+        for i_f in range(fine.nx):
+            v = fine.U[:, jf + G, i_f + G]
+            if np.isnan(v[0]):
+                continue
+            ic, jc = i_f // r, jf // r
+            if not (1 <= ic <= crs.nx - 2 and 1 <= jc <= crs.ny - 2):
+                continue
+            nb = U[:, jc - 1 + G:jc + 2 + G, ic - 1 + G:ic + 2 + G].reshape(amr.NC, -1)
+            assert np.all(v <= nb.max(axis=1) + 1e-12) and np.all(v >= nb.min(axis=1) - 1e-12)
+            n += 1
+    assert n > 100
+
+
+def test_fault_fill_reproduces_linear_data():
+    r = 2
+    fb = [(8, 0, 15, 7), (24, 0, 31, 7)]
+    crs, fine = _fill_setup(r, fb)
+    x = np.arange(-G, crs.nx + G) + 0.5
+    kk = np.arange(crs.nb)[:, None]
+    crs.Bold = 2.0 + 0.5 * kk * x[None]
+    crs.B = -1.0 + 0.25 * (kk - 4) * x[None]
+    fine.B[:] = np.nan
+    alpha = 0.75
+    amr.fill_fault(fine, crs, alpha)
+    xf = (np.arange(-G, fine.nx + G) + 0.5) / r
+    want = alpha * (-1.0 + 0.25 * (kk - 4) * xf[None]) + (1 - alpha) * (2.0 + 0.5 * kk * xf[None])
+    filled = ~np.isnan(fine.B[0])
+    # ghost faces 4 either side of each box, not covered by the other box
+    assert filled.sum() == 16
+    np.testing.assert_allclose(fine.B[:, filled], want[:, filled], rtol=1e-13, atol=1e-13)
+
+
+def test_average_down_of_linear_field_is_the_cell_centre_value():
+    r = 4
+    fb = [(16, 8, 31, 23)]
+    crs, fine = _fill_setup(r, fb)
+    xf = (np.arange(-G, fine.nx + G) + 0.5) / r
+    yf = (np.arange(-G, fine.ny + G) + 0.5) / r
+    X, Y = np.meshgrid(xf, yf)
+    fine.U[:] = (2.0 + 0.3 * X - 0.7 * Y)[None]
+    crs.U[:] = np.nan
+    amr.average_down(crs, fine)
+    for jc in range(2, 6):
+        for ic in range(4, 8):
+            np.testing.assert_allclose(crs.U[:, jc + G, ic + G], 2.0 + 0.3 * (ic + 0.5) - 0.7 * (jc + 0.5),
+                                       rtol=1e-14)
+    assert np.isnan(crs.U[0, G + 1, G + 1])
+
+
+# the disturbance sits on the domain centre (coarse cell 24, 16); the lo-x edge of the
+# finest region runs through it, so waves cross the coarse-fine interface both ways
+CONSERVATION_CASES = {
+    "two levels r2": (2, [[(0, 0, 47, 31)], [(48, 20, 71, 43)]]),
+    "two levels r4, split coarse": (4, [[(0, 0, 23, 31), (24, 0, 47, 31)], [(96, 40, 143, 87)]]),
+    "three levels r2": (2, [[(0, 0, 47, 31)], [(40, 16, 71, 47)], [(96, 48, 127, 79)]]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CONSERVATION_CASES))
+def test_reflux_conserves_the_composite_sum(case):
+    r, boxes = CONSERVATION_CASES[case]
+    cfg = quiet_sw(48, 32)
+    h = make_hierarchy(cfg, boxes, r)
+    s0 = composite_sum(h)
+    scale = np.abs(s0).max()
+    assert h.advance(2) == 0
+    s1 = composite_sum(h)
+    assert np.abs(s1 - s0).max() <= 1e-12 * scale, (s1 - s0)
+    # the disturbance crossed the coarse-fine interface (the registers did work)
+    assert max(np.abs(reg.reg).max() for reg in h.regs) > 0.0
+
+
+def test_conservation_check_detects_missing_reflux(monkeypatch):
+    r, boxes = CONSERVATION_CASES["two levels r2"]
+    cfg = quiet_sw(48, 32)
+    h = make_hierarchy(cfg, boxes, r)
+    s0 = composite_sum(h)
+    monkeypatch.setattr(amr.Register, "reflux", lambda self, coarse: None)
+    h.advance(2)
+    s1 = composite_sum(h)
+    assert np.abs(s1 - s0).max() > 1e-6 * np.abs(s0).max()

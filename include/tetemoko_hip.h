@@ -41,6 +41,8 @@ extern "C" {
 #define TMK_E_UNSUPPORTED (-8) /* option the reference aborts on (flattening, artificial viscosity, high-order limiter) */
 #define TMK_E_NOMEM (-9)       /* device memory exhausted */
 #define TMK_E_OVERFLOW (-10)   /* station records dropped: buffer full before it was read */
+#define TMK_E_AMR_TIME (-11)   /* LinElastLevelGodunov.cpp:250-258 "alpha < 0.0" / "alpha > 1.0" */
+#define TMK_E_NESTING (-12)    /* a finer level is not aligned to the ratio or not properly nested */
 
 #define TMK_FAULT_RS 0
 #define TMK_FAULT_SW 1
@@ -221,6 +223,32 @@ int tmk_stations_create(tmk_level *lev, int nstations, const double *x, int capa
  * TMK_E_OVERFLOW when records were dropped (the kept ones are still copied) */
 int tmk_stations_read(tmk_stations *st, double *records, int max_records, int *nrecords);
 int tmk_stations_destroy(tmk_stations *st);
+
+/* Hierarchies of levels: the subcycled multi-level step.  Replaces the level loop
+ * of Chombo's AMR::timeStep over AMRLevelLinElast::advance and ::postTimeStep
+ * (AMRLevelLinElast.cpp:372-516) and the coarse-fine parts of
+ * LinElastLevelGodunov::step (LinElastLevelGodunov.cpp:226-264 ghost fill of U and of
+ * the fault data from the coarser level at the interpolated time, :334-355 flux
+ * registers), on one device stream, with two coarse steps captured as one graph.
+ * levels[0] covers its domain (lo corner 0, 0); levels[l] has its own context with
+ * the domain refined by ref_ratio and dx / ref_ratio, boxes aligned to ref_ratio and
+ * nested in levels[l-1] so that every coarse cell the fill reads (the 3x3
+ * neighbourhood of the coarse parent of each fine ghost cell inside the domain) is a
+ * cell of levels[l-1]; otherwise TMK_E_NESTING.  While a level belongs to a hierarchy
+ * it cannot be stepped, uploaded or batched on its own; downloads wait for the
+ * hierarchy.  One coarse step: level 0 steps dt0, then ref_ratio steps of level 1,
+ * each followed by its own finer steps, then reflux (-1/dx) and average-down of the
+ * state and the fault data into the coarser level. */
+typedef struct tmk_amr tmk_amr;
+int tmk_amr_create(int nlevels, tmk_level *const *levels, int ref_ratio, tmk_amr **out);
+int tmk_amr_destroy(tmk_amr *amr);
+/* nsteps coarse steps of dt0 (level l steps dt0 / ref_ratio^l); the level clocks are
+ * set to time when it differs from the hierarchy's current time or dt0 changes */
+int tmk_amr_run(tmk_amr *amr, int nsteps, double time, double dt0);
+/* the same, timed with HIP events on the hierarchy's stream (waits) */
+int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double *ms);
+/* waits; the first error any level reported (0 = none), cleared */
+int tmk_amr_sync(tmk_amr *amr);
 
 /* Workgroup tile shape of the update kernels (autotuner hook): bx x by threads
  * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 32x8, 32x12, 32x16,

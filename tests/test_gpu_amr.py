@@ -1,0 +1,186 @@
+"""Multi-level hierarchies on the GPU (tmk_amr_*) against the multi-level oracle.
+
+The coarse-fine fills, registers, reflux and average-down are integer-indexed gathers and
+fixed-order sums, and the FP64 level update reproduces the oracle bit for bit off the fault,
+so slip-weakening hierarchies (no transcendental functions on the fault) match the oracle
+bit for bit on every level: state on every valid cell and fault data on every valid face.
+Rate-and-state hierarchies stay within 1e-12 relative (per component, against its largest
+magnitude) -- the last-bit differences of exp/log/asinh on fault faces, as on one level.
+Runs of 3 and 4 coarse steps cover both the replayed two-step graph and a direct step.
+"""
+import numpy as np
+import pytest
+
+import amr as oamr
+from tetemoko_amd import amr, capi, scenarios
+
+pytestmark = pytest.mark.gpu
+
+G = scenarios.GHOST
+
+
+def _bump_arrays(cfgs, xc, yc, radius):
+    def arrays(l):
+        cfg = cfgs[l]
+        nx, ny, dx = cfg["nx"], cfg["ny"], cfg["dx"]
+        x = (np.arange(-G, nx + G) + 0.5) * dx
+        y = (np.arange(-G, ny + G) + 0.5) * dx
+        X, Y = np.meshgrid(x, y)
+        R = np.hypot(X - xc, Y - yc)
+        w = np.where(R < radius, np.cos(0.5 * np.pi * R / radius) ** 2, 0.0)
+        U = np.empty((scenarios.NCOMP, ny + 2 * G, nx + 2 * G))
+        U[:] = scenarios.background_vector(cfg)[:, None, None]
+        for s in (0, 11):
+            U[s + 0] += w
+            U[s + 1] -= 0.5 * w
+            U[s + 3] += 3.0 * w
+            U[s + 4] -= 2.0 * w
+            U[s + 6] += 1.5 * w
+        return U, scenarios.fault_arrays(cfg)
+    return arrays
+
+
+def _background_arrays(cfgs):
+    def arrays(l):
+        return scenarios.level_arrays(cfgs[l])
+    return arrays
+
+
+def _cfgs(cfg0, nlev, r):
+    out = [cfg0]
+    for _ in range(1, nlev):
+        out.append(scenarios.refine_cfg(out[-1], r))
+    return out
+
+
+def _compare(cfg0, boxes, r, nsteps, arrays, exact, precision=capi.PREC_FP64):
+    h = oamr.Hierarchy(cfg0, boxes, r)
+    h.init_state(lambda l: arrays(l)[0], lambda l: arrays(l)[1])
+    h.set_dt(scenarios.dt_of(cfg0))
+    assert h.advance(nsteps) == 0
+    H = amr.Hierarchy(cfg0, boxes, r, precision, arrays=arrays)
+    H.run(nsteps)
+    H.sync()
+    for l, lev in enumerate(h.levels):
+        Ug, Bg = H.download(l)
+        mask = lev.valid
+        Ur = lev.U[:, G:-G, G:-G][:, mask]
+        Ugv = Ug[:, G:-G, G:-G][:, mask]
+        fmask = lev.fvalid
+        Br = lev.B[:, G:-G][:, fmask]
+        Bgv = Bg[:, G:-G][:, fmask]
+        if exact:
+            assert np.array_equal(Ugv, Ur), (l, np.nanmax(np.abs(Ugv - Ur)))
+            assert np.array_equal(Bgv, Br), l
+        else:
+            for c in range(scenarios.NCOMP):
+                scale = max(np.max(np.abs(Ur[c])), 1e-300)
+                assert np.max(np.abs(Ugv[c] - Ur[c])) <= 1e-12 * scale, (l, c)
+            for c in range(Br.shape[0]):
+                scale = max(np.max(np.abs(Br[c])), 1e-300)
+                assert np.max(np.abs(Bgv[c] - Br[c])) <= 1e-12 * scale, (l, c)
+    H.close()
+    return h
+
+
+# level 0: 48 x 32 in two boxes; level 1 (ratio 2) two boxes on the fault; level 2 one box
+BOXES_3 = [[(0, 0, 23, 31), (24, 0, 47, 31)],
+           [(24, 0, 55, 23), (56, 0, 71, 23)],
+           [(64, 0, 111, 15)]]
+
+
+@pytest.mark.parametrize("nsteps", [3, 4])
+def test_slip_weakening_rupture_three_levels_bit_identical(nsteps):
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    h = _compare(cfg, BOXES_3, 2, nsteps, _background_arrays(_cfgs(cfg, 3, 2)), exact=True)
+    # the nucleation patch slipped on every level
+    for lev in h.levels:
+        assert np.nanmax(lev.B[7, G:-G][lev.fvalid]) > 0.0
+
+
+def test_disturbance_across_coarse_fine_interface_bit_identical():
+    cfg = scenarios.config_c0(seed=3, nx=48, ny=32, dx=100.0)
+    cfg["faces"]["tau"] = np.full(48, 20.0)
+    boxes = [[(0, 0, 47, 31)], [(48, 20, 71, 43)]]
+    cfgs = _cfgs(cfg, 2, 2)
+    h = _compare(cfg, boxes, 2, 3, _bump_arrays(cfgs, 2400.0, 1600.0, 300.0), exact=True)
+    assert np.abs(h.regs[0].reg).max() > 0.0
+
+
+def test_ratio_four_with_plasticity_bit_identical():
+    cfg = scenarios.config_c0(seed=4, nx=48, ny=32, dx=100.0)
+    cfg["plasticity"] = 1
+    cfg["p_mu"], cfg["p_beta"], cfg["p_eta"], cfg["p_b"] = 0.6, 0.3, 0.03 * scenarios.G_MPA, 1.0
+    boxes = [[(0, 0, 47, 31)], [(64, 0, 127, 31)]]
+    _compare(cfg, boxes, 4, 3, _background_arrays(_cfgs(cfg, 2, 4)), exact=True)
+
+
+@pytest.mark.parametrize("nsteps", [3, 4])
+def test_rate_and_state_three_levels_within_tolerance(nsteps):
+    cfg = scenarios.config_c1(seed=5, nx=48, ny=32, dx=25.0)
+    cfg["rs"]["nuc_R"] = 400.0
+    _compare(cfg, BOXES_3, 2, nsteps, _background_arrays(_cfgs(cfg, 3, 2)), exact=False)
+
+
+def _composite_sum(H, cfgs, boxes, r):
+    tot = np.zeros(scenarios.NCOMP)
+    for l in range(len(boxes)):
+        U, _ = H.download(l)
+        cfg = cfgs[l]
+        mask = ~np.isnan(U[0, G:-G, G:-G])
+        if l + 1 < len(boxes):
+            for b in boxes[l + 1]:
+                mask[b[1] // r:b[3] // r + 1, b[0] // r:b[2] // r + 1] = False
+        vals = U[:, G:-G, G:-G][:, mask] - scenarios.background_vector(cfg)[:, None]
+        tot += vals.sum(axis=1) * cfg["dx"] ** 2
+    return tot
+
+
+def test_composite_sum_conserved_on_device():
+    cfg = scenarios.config_c0(seed=3, nx=48, ny=32, dx=100.0)
+    cfg["faces"]["tau"] = np.full(48, 20.0)
+    boxes = [[(0, 0, 47, 31)], [(40, 16, 71, 47)], [(96, 48, 127, 79)]]
+    cfgs = _cfgs(cfg, 3, 2)
+    H = amr.Hierarchy(cfg, boxes, 2, arrays=_bump_arrays(cfgs, 2400.0, 1600.0, 300.0))
+    s0 = _composite_sum(H, cfgs, boxes, 2)
+    H.run(4)
+    H.sync()
+    s1 = _composite_sum(H, cfgs, boxes, 2)
+    H.close()
+    assert np.abs(s1 - s0).max() <= 1e-12 * np.abs(s0).max()
+
+
+def test_fp32_bulk_hierarchy_close_to_fp64():
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    arrays = _background_arrays(_cfgs(cfg, 3, 2))
+    out = {}
+    for prec in (capi.PREC_FP64, capi.PREC_FP32_BULK):
+        H = amr.Hierarchy(cfg, BOXES_3, 2, prec, arrays=arrays)
+        H.run(6)
+        H.sync()
+        out[prec] = [H.download(l) for l in range(3)]
+        H.close()
+    for l in range(3):
+        B64, B32 = out[capi.PREC_FP64][l][1], out[capi.PREC_FP32_BULK][l][1]
+        ok = ~np.isnan(B64[7])
+        slip64, slip32 = B64[7][ok], B32[7][ok]
+        assert np.max(np.abs(slip32 - slip64)) <= 1e-4 * np.max(np.abs(slip64)) + 1e-12
+
+
+def test_hierarchy_rejects_misaligned_and_unnested_levels():
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    for boxes in ([[(0, 0, 47, 31)], [(25, 0, 56, 23)]],          # not aligned to the ratio
+                  [[(0, 0, 47, 31)], [(24, 0, 55, 23)], [(40, 0, 63, 15)]]):  # level 2 reads outside level 1
+        with pytest.raises(capi.TmkError) as ei:
+            amr.Hierarchy(cfg, boxes, 2)
+        assert ei.value.code == -12
+
+
+def test_levels_of_a_hierarchy_refuse_direct_steps():
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    H = amr.Hierarchy(cfg, [[(0, 0, 47, 31)], [(24, 0, 55, 23)]], 2)
+    with pytest.raises(capi.TmkError):
+        H.levels[1].run(1, 0.0, 1e-3)
+    H.run(1)
+    H.sync()
+    H.close()

@@ -1,0 +1,94 @@
+"""Multi-level rupture runs on one device: a fixed hierarchy of levels stepped with
+subcycling (tmk_amr_*).
+
+The host mirror of Chombo's AMR driver over AMRLevelLinElast for a given box layout
+per level (amrGodunov.cpp:443-461 set-up; AMRLevelLinElast.cpp:372-516 advance and
+postTimeStep).  Level l has its own context (dx / r^l, per-face arrays injected from
+level 0) and steps dt0 / r^l.  Box generation from tags is not part of this module:
+the caller gives the boxes of every level.
+"""
+import numpy as np
+
+from . import capi, scenarios
+
+G = scenarios.GHOST
+
+
+class Hierarchy:
+    def __init__(self, cfg0, boxes_per_level, r, precision=capi.PREC_FP64, tile=None, arrays=None):
+        """arrays(l) -> (U, B) level arrays over the whole level domain with ghosts (the
+        initial state); default: the background state and the initial fault data."""
+        self.r = r
+        self.cfgs = [cfg0]
+        for _ in range(1, len(boxes_per_level)):
+            self.cfgs.append(scenarios.refine_cfg(self.cfgs[-1], r))
+        self.boxes = [[tuple(b) for b in bl] for bl in boxes_per_level]
+        self.ctxs, self.levels = [], []
+        for l, cfg in enumerate(self.cfgs):
+            ctx = capi.Context(cfg, precision)
+            if tile:
+                ctx.set_tile(*tile)
+            lev = capi.Level(ctx, self.boxes[l])
+            if arrays is None:
+                Bf = scenarios.fault_arrays(cfg)
+                for k, b in enumerate(self.boxes[l]):
+                    lev.fill(k, scenarios.background_vector(cfg), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
+            else:
+                U, Bf = arrays(l)
+                for k, b in enumerate(self.boxes[l]):
+                    lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
+            self.ctxs.append(ctx)
+            self.levels.append(lev)
+        self.amr = capi.Amr(self.levels, r)
+        self.dt0 = scenarios.dt_of(cfg0)
+        self.time = 0.0
+        self.steps = 0
+
+    def run(self, nsteps):
+        self.amr.run(nsteps, self.time, self.dt0)
+        for _ in range(nsteps):
+            self.time = self.time + self.dt0
+        self.steps += nsteps
+
+    def run_timed(self, nsteps):
+        ms = self.amr.run_timed(nsteps, self.time, self.dt0)
+        for _ in range(nsteps):
+            self.time = self.time + self.dt0
+        self.steps += nsteps
+        return ms
+
+    def sync(self):
+        self.amr.sync()
+
+    def cell_updates_per_coarse_step(self):
+        """sum over levels of valid cells x steps of that level per coarse step"""
+        tot = 0
+        for l, bl in enumerate(self.boxes):
+            cells = sum((b[2] - b[0] + 1) * (b[3] - b[1] + 1) for b in bl)
+            tot += cells * self.r ** l
+        return tot
+
+    def download(self, l):
+        """(U, B) level arrays of level l: the boxes' valid cells, NaN elsewhere."""
+        cfg = self.cfgs[l]
+        nx, ny = cfg["nx"], cfg["ny"]
+        nb = scenarios.NB[cfg["fault"]]
+        U = np.full((scenarios.NCOMP, ny + 2 * G, nx + 2 * G), np.nan)
+        B = np.full((nb, nx + 2 * G), np.nan)
+        lev = self.levels[l]
+        for k, b in enumerate(self.boxes[l]):
+            x0, y0, x1, y1 = b
+            ub = np.empty((scenarios.NCOMP, y1 - y0 + 1 + 2 * G, x1 - x0 + 1 + 2 * G))
+            bb = np.empty((nb, x1 - x0 + 1 + 2 * G)) if y0 == 0 else None
+            lev.download(k, ub, bb)
+            U[:, y0 + G:y1 + G + 1, x0 + G:x1 + G + 1] = ub[:, G:-G, G:-G]
+            if bb is not None:
+                B[:, x0 + G:x1 + G + 1] = bb[:, G:-G]
+        return U, B
+
+    def close(self):
+        self.amr.close()
+        for lev in self.levels:
+            lev.close()
+        for ctx in self.ctxs:
+            ctx.close()

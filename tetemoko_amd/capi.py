@@ -41,6 +41,7 @@ EXPORTED = (
     "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
     "tmk_level_profile_kernels", "tmk_batch_member_error",
     "tmk_stations_create", "tmk_stations_read", "tmk_stations_destroy",
+    "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed", "tmk_amr_sync",
 )
 
 
@@ -172,6 +173,11 @@ def lib():
         L.tmk_level_fill.argtypes = [vp, c_int, _PD, _PD]
         L.tmk_level_download_fault.argtypes = [vp, c_int, _PD]
         L.tmk_level_tag.argtypes = [vp, ctypes.POINTER(TagParams), ctypes.POINTER(ctypes.c_ubyte)]
+        L.tmk_amr_create.argtypes = [c_int, ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]
+        L.tmk_amr_destroy.argtypes = [vp]
+        L.tmk_amr_run.argtypes = [vp, c_int, _D, _D]
+        L.tmk_amr_run_timed.argtypes = [vp, c_int, _D, _D, _PD]
+        L.tmk_amr_sync.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -368,6 +374,40 @@ class Level:
 
     def sync(self):
         _check(lib().tmk_level_sync(self.h), "tmk_level_sync")
+
+
+class Amr:
+    """Hierarchy of device-resident levels stepped together (subcycled, ratio r)."""
+
+    def __init__(self, levels, r):
+        self.levels = list(levels)
+        arr = (ctypes.c_void_p * len(levels))(*[lv.h for lv in levels])
+        h = ctypes.c_void_p()
+        _check(lib().tmk_amr_create(len(levels), arr, r, ctypes.byref(h)), "tmk_amr_create")
+        self.h = h
+        self.r = r
+
+    def close(self):
+        if self.h:
+            lib().tmk_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps, time, dt0):
+        _check(lib().tmk_amr_run(self.h, nsteps, time, dt0), "tmk_amr_run")
+
+    def run_timed(self, nsteps, time, dt0):
+        ms = ctypes.c_double()
+        _check(lib().tmk_amr_run_timed(self.h, nsteps, time, dt0, ctypes.byref(ms)), "tmk_amr_run_timed")
+        return ms.value
+
+    def sync(self):
+        _check(lib().tmk_amr_sync(self.h), "tmk_amr_sync")
 
 
 def mem_info():

@@ -18,6 +18,7 @@
 #include "tetemoko_hip.h"
 #include "tmk_kernels.hpp"
 #include "tmk_interior.hpp"
+#include "tmk_amr.hpp"
 
 #ifndef TMK_DEFAULT_BY_F64
 #define TMK_DEFAULT_BY_F64 8
@@ -260,6 +261,7 @@ struct tmk_level {
     bool own_built = false;
     bool dirty = true;            // uploaded data not yet exchanged
     tmk_batch* batch = nullptr;
+    hipStream_t amr_stream = nullptr;  // set while the level is part of a hierarchy (tmk_amr)
     size_t bytes = 0;             // device bytes held
     tmk_stations* stations = nullptr;
 };
@@ -290,7 +292,9 @@ static const char* k_errs[] = {"ok",
                                "HIP runtime failure",
                                "unsupported option",
                                "device memory exhausted",
-                               "station records dropped: buffer full before it was read"};
+                               "station records dropped: buffer full before it was read",
+                               "coarse-fine time interpolation weight outside [0, 1]",
+                               "finer level not aligned to the refinement ratio or not properly nested"};
 
 extern "C" const char* tmk_strerror(int code)
 {
@@ -1005,7 +1009,7 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
 extern "C" int tmk_level_destroy(tmk_level* L)
 {
     if (!L) return TMK_OK;
-    if (L->batch) return TMK_E_ARG;
+    if (L->batch || L->amr_stream) return TMK_E_ARG;
     if (L->ctx && L->ctx->stream) (void)hipStreamSynchronize(L->ctx->stream);
     if (L->stations) L->stations->L = nullptr;
     L->own.release();
@@ -1020,7 +1024,7 @@ extern "C" int tmk_level_destroy(tmk_level* L)
 
 extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const double* B)
 {
-    if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch) return TMK_E_ARG;
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch || L->amr_stream) return TMK_E_ARG;
     tmk_ctx* c = L->ctx;
     const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
     const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
@@ -1070,7 +1074,7 @@ static int level_fill_t(tmk_level* L, int b, const double* state)
 
 extern "C" int tmk_level_fill(tmk_level* L, int b, const double* state, const double* B)
 {
-    if (!L || b < 0 || b >= (int)L->boxes.size() || !state || L->batch) return TMK_E_ARG;
+    if (!L || b < 0 || b >= (int)L->boxes.size() || !state || L->batch || L->amr_stream) return TMK_E_ARG;
     if (L->ns == 2 && (state[2] != 0.0 || state[7] != 0.0 || state[8] != 0.0 || state[13] != 0.0 ||
                        state[18] != 0.0 || state[19] != 0.0))
         return TMK_E_ARG;
@@ -1092,7 +1096,7 @@ extern "C" int tmk_level_download_fault(tmk_level* L, int b, double* B)
 {
     if (!L || b < 0 || b >= (int)L->boxes.size() || !B || L->batch) return TMK_E_ARG;
     if (!L->has_b[b]) return TMK_E_NO_BDRY;
-    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    HIPCHK(hipStreamSynchronize(L->amr_stream ? L->amr_stream : L->ctx->stream));
     const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
     HIPCHK(hipMemcpy(B, L->bbuf[L->cur][b], nB * sizeof(double), hipMemcpyDeviceToHost));
@@ -1103,7 +1107,7 @@ extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
 {
     if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch) return TMK_E_ARG;
     tmk_ctx* c = L->ctx;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(L->amr_stream ? L->amr_stream : c->stream));
     const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
     const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
     const size_t nU = plane * TMK_NCOMP;
@@ -1162,7 +1166,7 @@ static int level_prepare(tmk_level* L, double time, double dt)
 
 extern "C" int tmk_level_step(tmk_level* L, double time, double dt, double* dt_new)
 {
-    if (!L || L->batch) return TMK_E_ARG;
+    if (!L || L->batch || L->amr_stream) return TMK_E_ARG;
     int e = level_prepare(L, time, dt);
     if (e) return e;
     if ((e = group_enqueue_step(L->own, L->ctx->stream))) return e;
@@ -1174,7 +1178,7 @@ extern "C" int tmk_level_step(tmk_level* L, double time, double dt, double* dt_n
 
 extern "C" int tmk_level_run(tmk_level* L, int nsteps, double time, double dt)
 {
-    if (!L || nsteps < 0 || L->batch) return TMK_E_ARG;
+    if (!L || nsteps < 0 || L->batch || L->amr_stream) return TMK_E_ARG;
     if (nsteps == 0) return TMK_OK;
     int e = level_prepare(L, time, dt);
     if (e) return e;
@@ -1208,7 +1212,7 @@ extern "C" int tmk_level_run_timed(tmk_level* L, int nsteps, double time, double
 
 extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double dt, double* ms)
 {
-    if (!L || !ms || nsteps <= 0 || L->batch) return TMK_E_ARG;
+    if (!L || !ms || nsteps <= 0 || L->batch || L->amr_stream) return TMK_E_ARG;
     int e = level_prepare(L, time, dt);
     if (e) return e;
     hipStream_t st = L->ctx->stream;
@@ -1233,7 +1237,7 @@ extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double d
 
 extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, double dt, tmk_profile* out)
 {
-    if (!L || !out || nsteps <= 0 || L->batch) return TMK_E_ARG;
+    if (!L || !out || nsteps <= 0 || L->batch || L->amr_stream) return TMK_E_ARG;
     int e = level_prepare(L, time, dt);
     if (e) return e;
     hipStream_t st = L->ctx->stream;
@@ -1282,7 +1286,7 @@ extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, 
 
 extern "C" int tmk_level_sync(tmk_level* L)
 {
-    if (!L || L->batch) return TMK_E_ARG;
+    if (!L || L->batch || L->amr_stream) return TMK_E_ARG;
     HIPCHK(hipStreamSynchronize(L->ctx->stream));
     if (!L->own_built) return TMK_OK;
     int code = 0;
@@ -1372,7 +1376,7 @@ static int batch_rebuild(tmk_batch* B)
 
 extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
 {
-    if (!B || !L || L->batch) return TMK_E_ARG;
+    if (!B || !L || L->batch || L->amr_stream) return TMK_E_ARG;
     if (!B->members.empty()) {
         tmk_level* L0 = B->members[0];
         if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by)
@@ -1424,7 +1428,7 @@ extern "C" int tmk_batch_run(tmk_batch* B, int nsteps)
 // ---------------------------------------------------------------------------
 extern "C" int tmk_stations_create(tmk_level* L, int n, const double* x, int capacity, tmk_stations** out)
 {
-    if (!L || !out || n <= 0 || !x || capacity <= 0 || L->stations || L->batch) return TMK_E_ARG;
+    if (!L || !out || n <= 0 || !x || capacity <= 0 || L->stations || L->batch || L->amr_stream) return TMK_E_ARG;
     *out = nullptr;
     tmk_ctx* c = L->ctx;
     const double dx = c->hp.dx;
@@ -1569,8 +1573,650 @@ static int level_tag_t(tmk_level* L, const tmk_tagparams* tp, unsigned char* tag
 
 extern "C" int tmk_level_tag(tmk_level* L, const tmk_tagparams* tp, unsigned char* tags)
 {
-    if (!L || !tp || !tags || L->batch || tp->tag_buffer < 0) return TMK_E_ARG;
+    if (!L || !tp || !tags || L->batch || L->amr_stream || tp->tag_buffer < 0) return TMK_E_ARG;
     int e = level_prepare(L, L->time, L->dt);
     if (e) return e;
     return L->tsize == sizeof(double) ? level_tag_t<double>(L, tp, tags) : level_tag_t<float>(L, tp, tags);
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy of levels: the subcycled multi-level step
+// (Chombo AMR::timeStep with AMRLevelLinElast::advance / postTimeStep,
+//  AMRLevelLinElast.cpp:372-516, and the coarse-fine parts of
+//  LinElastLevelGodunov::step, LinElastLevelGodunov.cpp:226-355)
+// ---------------------------------------------------------------------------
+struct AmrLevel {
+    tmk_level* L = nullptr;
+    std::vector<double*> F[2];  // per box face fluxes (nullptr: none)
+    std::vector<int> fedge;     // per box: 1 = only the box-edge faces are kept
+    // coarse-fine fill of this level from the next coarser one (levels 1 ..)
+    DevBuf cells, faces, comps;
+    int ncells = 0, nfaces = 0, ncomps = 0;
+    // flux register with the next finer level, and the average-down from it (levels .. n-2)
+    DevBuf regf, reg, rcells, rscale, avgc, avgf;
+    int nreg = 0, nrcells = 0, navgc = 0, navgf = 0;
+};
+
+struct tmk_amr {
+    std::vector<AmrLevel> lv;
+    int r = 2;
+    hipStream_t stream = nullptr;
+    bool clock = false;
+    double dt0 = 0.0;
+    hipGraphExec_t gexec = nullptr;  // two coarse steps (every level back to its parity)
+    int gpar = 0;                    // level-0 group parity the graph was captured at
+};
+
+// offset of cell (i, j) in the ghosted plane of valid box v
+static inline int ghost_off(const tmk_box& v, int i, int j)
+{
+    const int unx = v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST;
+    return (i - (v.lo[0] - TMK_GHOST)) + (j - (v.lo[1] - TMK_GHOST)) * unx;
+}
+
+// face offset in a box's flux array of direction d (full or edge layout); -1 if the edge
+// layout does not hold the face
+static inline int flux_off(const tmk_box& v, int edge, int d, int fi, int fj)
+{
+    const int w = v.hi[0] - v.lo[0] + 1, h = v.hi[1] - v.lo[1] + 1;
+    if (!edge) return (fi - v.lo[0]) + (fj - v.lo[1]) * (w + (d == 0));
+    if (d == 0) {
+        if (fi != v.lo[0] && fi != v.hi[0] + 1) return -1;
+        return (fi != v.lo[0]) + (fj - v.lo[1]) * 2;
+    }
+    (void)h;
+    if (fj != v.lo[1] && fj != v.hi[1] + 1) return -1;
+    return (fi - v.lo[0]) + (fj != v.lo[1]) * w;
+}
+
+template <typename X>
+static int upload_list(DevBuf& buf, const std::vector<X>& v)
+{
+    int e = buf.ensure(std::max<size_t>(sizeof(X) * v.size(), 1));
+    if (e) return e;
+    if (!v.empty()) HIPCHK(hipMemcpy(buf.p, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+    return TMK_OK;
+}
+
+// coverage maps over the coarse domain of the pair (coarse c, fine f)
+struct PairMaps {
+    int ncx = 0, ncy = 0;
+    std::vector<int> own_c;   // coarse box holding the cell
+    std::vector<int> own_f;   // fine box covering the cell (coarsened)
+    std::vector<int> own_cb;  // coarse box with fault data holding fault face i
+    std::vector<int> own_fb;  // fine box with fault data covering coarse fault face i
+};
+
+static int pair_maps(const tmk_level* C, const tmk_level* Fl, int r, PairMaps& m)
+{
+    const tmk_box cd = C->ctx->hp.domain, fd = Fl->ctx->hp.domain;
+    if (cd.lo[0] != 0 || cd.lo[1] != 0 || fd.lo[0] != 0 || fd.lo[1] != 0) return TMK_E_NESTING;
+    if (fd.hi[0] + 1 != (cd.hi[0] + 1) * r || fd.hi[1] + 1 != (cd.hi[1] + 1) * r) return TMK_E_NESTING;
+    if (Fl->ctx->hp.dx * r != C->ctx->hp.dx) return TMK_E_NESTING;
+    m.ncx = cd.hi[0] + 1;
+    m.ncy = cd.hi[1] + 1;
+    const size_t n = (size_t)m.ncx * m.ncy;
+    m.own_c.assign(n, -1);
+    m.own_f.assign(n, -1);
+    m.own_cb.assign(m.ncx, -1);
+    m.own_fb.assign(m.ncx, -1);
+    for (size_t b = 0; b < C->boxes.size(); ++b) {
+        const tmk_box v = C->boxes[b];
+        for (int j = v.lo[1]; j <= v.hi[1]; ++j)
+            for (int i = v.lo[0]; i <= v.hi[0]; ++i) m.own_c[(size_t)j * m.ncx + i] = (int)b;
+        if (v.lo[1] == 0)
+            for (int i = v.lo[0]; i <= v.hi[0]; ++i) m.own_cb[i] = (int)b;
+    }
+    for (size_t b = 0; b < Fl->boxes.size(); ++b) {
+        const tmk_box v = Fl->boxes[b];
+        for (int d = 0; d < 2; ++d)
+            if (v.lo[d] % r != 0 || (v.hi[d] + 1) % r != 0) return TMK_E_NESTING;
+        for (int j = v.lo[1] / r; j <= v.hi[1] / r; ++j)
+            for (int i = v.lo[0] / r; i <= v.hi[0] / r; ++i) {
+                if (m.own_c[(size_t)j * m.ncx + i] < 0) return TMK_E_NESTING;
+                m.own_f[(size_t)j * m.ncx + i] = (int)b;
+            }
+        if (v.lo[1] == 0)
+            for (int i = v.lo[0] / r; i <= v.hi[0] / r; ++i) m.own_fb[i] = (int)b;
+    }
+    return TMK_OK;
+}
+
+
+// register faces of the pair in the order (direction, side, j, i): an uncovered coarse
+// cell with the refined region on that side.  fn(d, side, i, j, coarse box, fine box)
+template <typename Fn>
+static void for_each_reg_face(const PairMaps& m, Fn fn)
+{
+    const int ncx = m.ncx, ncy = m.ncy;
+    for (int d = 0; d < 2; ++d)
+        for (int side = 0; side < 2; ++side)
+            for (int j = 0; j < ncy; ++j)
+                for (int i = 0; i < ncx; ++i) {
+                    const size_t q = (size_t)j * ncx + i;
+                    if (m.own_c[q] < 0 || m.own_f[q] >= 0) continue;
+                    const int sg = side ? 1 : -1;
+                    const int ni = i + (d == 0 ? sg : 0), nj = j + (d == 1 ? sg : 0);
+                    if (ni < 0 || ni >= ncx || nj < 0 || nj >= ncy) continue;
+                    const int fb = m.own_f[(size_t)nj * ncx + ni];
+                    if (fb >= 0) fn(d, side, i, j, m.own_c[q], fb);
+                }
+}
+
+// the device lists of the pair (level l-1, level l); the boxes' flux layouts are set
+static int amr_build_pair(tmk_amr* A, int l)
+{
+    AmrLevel& c = A->lv[l - 1];
+    AmrLevel& f = A->lv[l];
+    const tmk_level* C = c.L;
+    const tmk_level* Fl = f.L;
+    const int r = A->r;
+    PairMaps m;
+    int e = pair_maps(C, Fl, r, m);
+    if (e) return e;
+    const int ncx = m.ncx, ncy = m.ncy;
+    const tmk_box fd = Fl->ctx->hp.domain;
+    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.own_c[(size_t)j * ncx + i] >= 0; };
+    // coarse-fine fill of the state: one entry per (fine box, ghost cell inside the domain
+    // and outside every fine box)
+    std::vector<CfCell> cells;
+    for (size_t b = 0; b < Fl->boxes.size(); ++b) {
+        const tmk_box v = Fl->boxes[b];
+        const int i0 = std::max(v.lo[0] - TMK_GHOST, 0), i1 = std::min(v.hi[0] + TMK_GHOST, fd.hi[0]);
+        const int j0 = std::max(v.lo[1] - TMK_GHOST, 0), j1 = std::min(v.hi[1] + TMK_GHOST, fd.hi[1]);
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                const int ic = i / r, jc = j / r;
+                if (m.own_f[(size_t)jc * ncx + ic] >= 0) continue;  // a cell of the fine level
+                CfCell cc;
+                memset(&cc, 0, sizeof cc);
+                cc.dpatch = (int)b;
+                cc.doff = ghost_off(v, i, j);
+                for (int jj = -1, k = 0; jj <= 1; ++jj)
+                    for (int ii = -1; ii <= 1; ++ii, ++k) {
+                        const int ci = ic + ii, cj = jc + jj;
+                        cc.cpatch[k] = -1;
+                        if (in_c(ci, cj)) {
+                            const int cb = m.own_c[(size_t)cj * ncx + ci];
+                            cc.cpatch[k] = cb;
+                            cc.coff[k] = ghost_off(C->boxes[cb], ci, cj);
+                        }
+                    }
+                if (cc.cpatch[4] < 0) return TMK_E_NESTING;
+                const int lo[2] = {cc.cpatch[3] >= 0, cc.cpatch[1] >= 0};
+                const int hi[2] = {cc.cpatch[5] >= 0, cc.cpatch[7] >= 0};
+                int flags = 0;
+                for (int d = 0; d < 2; ++d) {
+                    int kind;
+                    if (lo[d] && hi[d]) kind = 0;       // central
+                    else if (lo[d]) kind = 1;           // one-sided low
+                    else if (hi[d]) kind = 2;           // one-sided high
+                    else return TMK_E_NESTING;
+                    flags |= kind << (2 * d);
+                }
+                // the limiter covers the coarse cells off the domain's edge layer
+                if (ic >= 1 && ic <= ncx - 2 && jc >= 1 && jc <= ncy - 2) {
+                    for (int k = 0; k < 9; ++k)
+                        if (cc.cpatch[k] < 0) return TMK_E_NESTING;
+                    flags |= 16;
+                }
+                cc.flags = flags;
+                cc.coef0 = -0.5 + ((double)(i - r * ic) + 0.5) / (double)r;
+                cc.coef1 = -0.5 + ((double)(j - r * jc) + 0.5) / (double)r;
+                cells.push_back(cc);
+            }
+    }
+    std::vector<int> comps;
+    for (int k = 0; k < TMK_NCOMP; ++k)
+        if (is_stencil(k, Fl->ns)) comps.push_back(k);
+    // coarse-fine fill of the fault data
+    std::vector<CfFace> faces;
+    for (size_t b = 0; b < Fl->boxes.size(); ++b) {
+        const tmk_box v = Fl->boxes[b];
+        if (!Fl->has_b[b]) continue;
+        const int i0 = std::max(v.lo[0] - TMK_GHOST, 0), i1 = std::min(v.hi[0] + TMK_GHOST, fd.hi[0]);
+        for (int i = i0; i <= i1; ++i) {
+            const int ic = i / r;
+            if (m.own_fb[ic] >= 0) continue;
+            CfFace ff;
+            memset(&ff, 0, sizeof ff);
+            ff.dpatch = (int)b;
+            ff.doff = i - (v.lo[0] - TMK_GHOST);
+            for (int k = 0; k < 3; ++k) {
+                const int ci = ic - 1 + k;
+                const int cb = ci >= 0 && ci < ncx ? m.own_cb[ci] : -1;
+                ff.cpatch[k] = cb;
+                ff.coff[k] = cb >= 0 ? ci - (C->boxes[cb].lo[0] - TMK_GHOST) : 0;
+            }
+            if (ff.cpatch[1] < 0) return TMK_E_NESTING;
+            const bool lo = ff.cpatch[0] >= 0, hi = ff.cpatch[2] >= 0;
+            if (!lo && !hi) return TMK_E_NESTING;
+            ff.kind = lo && hi ? 0 : lo ? 1 : 2;
+            ff.coef = -0.5 + ((double)(i - r * ic) + 0.5) / (double)r;
+            faces.push_back(ff);
+        }
+    }
+    // flux-register faces, grouped by uncovered coarse cell (a stable sort keeps each
+    // cell's faces in the order direction 0 lo, 0 hi, 1 lo, 1 hi)
+    struct Ent {
+        size_t cell;
+        RegFace rf;
+        double s;
+    };
+    std::vector<Ent> ents;
+    const double inv = -1.0 / C->ctx->hp.dx;
+    bool bad = false;
+    for_each_reg_face(m, [&](int d, int side, int i, int j, int cb, int fb) {
+        Ent en;
+        memset(&en, 0, sizeof en);
+        en.cell = (size_t)j * ncx + i;
+        const int fi = d == 0 ? i + side : i, fj = d == 1 ? j + side : j;
+        en.rf.cpatch = cb;
+        en.rf.cface = flux_off(C->boxes[cb], c.fedge[cb], d, fi, fj);
+        en.rf.d = d;
+        en.rf.fpatch = fb;
+        for (int mm = 0; mm < r; ++mm) {
+            const int ffi = d == 0 ? fi * r : fi * r + mm, ffj = d == 0 ? fj * r + mm : fj * r;
+            en.rf.fface[mm] = flux_off(Fl->boxes[fb], f.fedge[fb], d, ffi, ffj);
+            if (en.rf.fface[mm] < 0) bad = true;
+        }
+        if (c.fedge[cb] || !c.F[d][cb] || !f.F[d][fb] || en.rf.cface < 0) bad = true;
+        en.s = inv * (side ? 1.0 : -1.0);
+        ents.push_back(en);
+    });
+    if (bad) return TMK_E_NESTING;
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.cell < b.cell; });
+    std::vector<RegFace> regf(ents.size());
+    std::vector<double> rscale(ents.size());
+    std::vector<RefluxCell> rcells;
+    for (size_t k = 0; k < ents.size(); ++k) {
+        regf[k] = ents[k].rf;
+        rscale[k] = ents[k].s;
+        if (k == 0 || ents[k].cell != ents[k - 1].cell) {
+            RefluxCell rc;
+            const int i = (int)(ents[k].cell % ncx), j = (int)(ents[k].cell / ncx);
+            rc.cpatch = ents[k].rf.cpatch;
+            rc.coff = ghost_off(C->boxes[rc.cpatch], i, j);
+            rc.e0 = (int)k;
+            rc.e1 = (int)k + 1;
+            rcells.push_back(rc);
+        } else {
+            rcells.back().e1 = (int)k + 1;
+        }
+    }
+    // average-down of the state and of the fault data
+    std::vector<AvgCell> avgc;
+    std::vector<AvgFace> avgf;
+    for (size_t b = 0; b < Fl->boxes.size(); ++b) {
+        const tmk_box v = Fl->boxes[b];
+        for (int jc = v.lo[1] / r; jc <= v.hi[1] / r; ++jc)
+            for (int ic = v.lo[0] / r; ic <= v.hi[0] / r; ++ic) {
+                AvgCell a;
+                a.cpatch = m.own_c[(size_t)jc * ncx + ic];
+                a.coff = ghost_off(C->boxes[a.cpatch], ic, jc);
+                a.fpatch = (int)b;
+                a.foff = ghost_off(v, ic * r, jc * r);
+                avgc.push_back(a);
+            }
+        if (!Fl->has_b[b]) continue;
+        for (int ic = v.lo[0] / r; ic <= v.hi[0] / r; ++ic) {
+            AvgFace a;
+            a.cpatch = m.own_cb[ic];
+            if (a.cpatch < 0) return TMK_E_NESTING;
+            a.coff = ic - (C->boxes[a.cpatch].lo[0] - TMK_GHOST);
+            a.fpatch = (int)b;
+            a.foff = ic * r - (v.lo[0] - TMK_GHOST);
+            avgf.push_back(a);
+        }
+    }
+    f.ncells = (int)cells.size();
+    f.nfaces = (int)faces.size();
+    f.ncomps = (int)comps.size();
+    c.nreg = (int)regf.size();
+    c.nrcells = (int)rcells.size();
+    c.navgc = (int)avgc.size();
+    c.navgf = (int)avgf.size();
+    if ((e = upload_list(f.cells, cells)) || (e = upload_list(f.faces, faces)) || (e = upload_list(f.comps, comps)) ||
+        (e = upload_list(c.regf, regf)) || (e = upload_list(c.rcells, rcells)) ||
+        (e = upload_list(c.rscale, rscale)) || (e = upload_list(c.avgc, avgc)) || (e = upload_list(c.avgf, avgf)))
+        return e;
+    return c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * regf.size(), 1));
+}
+
+// flux arrays of every box of every level: full layout for a coarse box holding register
+// cells, box-edge layout for a box of a finer level, none otherwise
+template <typename T>
+static void amr_set_flux_ptrs(AmrLevel& a)
+{
+    tmk_level* L = a.L;
+    for (int par = 0; par < 2; ++par) {
+        Patch<T>* pts = (Patch<T>*)L->hpatch[par].data();
+        for (size_t b = 0; b < L->boxes.size(); ++b) {
+            pts[b].F[0] = a.F[0][b];
+            pts[b].F[1] = a.F[1][b];
+            pts[b].fedge = a.fedge[b];
+        }
+    }
+    L->own_built = false;
+}
+
+// hand the levels back: flux arrays freed, descriptors without fluxes (the level's own
+// group is rebuilt before its next use), level calls allowed again
+static void amr_free(tmk_amr* A)
+{
+    if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
+    A->gexec = nullptr;
+    for (AmrLevel& a : A->lv) {
+        if (!a.L) continue;
+        for (int d = 0; d < 2; ++d)
+            for (double*& p : a.F[d]) {
+                if (p) (void)hipFree(p);
+                p = nullptr;
+            }
+        for (int& x : a.fedge) x = 0;
+        if (!a.F[0].empty()) {
+            if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
+            else amr_set_flux_ptrs<float>(a);
+        }
+        a.L->amr_stream = nullptr;
+    }
+}
+
+static int amr_layouts(tmk_amr* A)
+{
+    const int n = (int)A->lv.size();
+    std::vector<std::vector<char>> full(n), need(n);
+    for (int l = 0; l < n; ++l) {
+        full[l].assign(A->lv[l].L->boxes.size(), 0);
+        need[l].assign(A->lv[l].L->boxes.size(), 0);
+    }
+    for (int l = 1; l < n; ++l) {
+        PairMaps m;
+        int e = pair_maps(A->lv[l - 1].L, A->lv[l].L, A->r, m);
+        if (e) return e;
+        for_each_reg_face(m, [&](int, int, int, int, int cb, int fb) {
+            full[l - 1][cb] = 1;
+            need[l][fb] = 1;
+        });
+    }
+    for (int l = 0; l < n; ++l) {
+        AmrLevel& a = A->lv[l];
+        const size_t nb = a.L->boxes.size();
+        a.F[0].assign(nb, nullptr);
+        a.F[1].assign(nb, nullptr);
+        a.fedge.assign(nb, 0);
+        for (size_t b = 0; b < nb; ++b) {
+            if (!full[l][b] && !need[l][b]) continue;
+            const tmk_box v = a.L->boxes[b];
+            const size_t w = v.hi[0] - v.lo[0] + 1, h = v.hi[1] - v.lo[1] + 1;
+            const int edge = full[l][b] ? 0 : 1;
+            const size_t n0 = edge ? 2 * h : (w + 1) * h, n1 = edge ? 2 * w : w * (h + 1);
+            for (int d = 0; d < 2; ++d) {
+                void* p = nullptr;
+                if (hipMalloc(&p, sizeof(double) * TMK_NCOMP * (d == 0 ? n0 : n1)) != hipSuccess) return TMK_E_NOMEM;
+                a.F[d][b] = (double*)p;
+            }
+            a.fedge[b] = edge;
+        }
+    }
+    return TMK_OK;
+}
+
+extern "C" int tmk_amr_destroy(tmk_amr* A)
+{
+    if (!A) return TMK_OK;
+    if (A->stream) (void)hipStreamSynchronize(A->stream);
+    for (AmrLevel& a : A->lv)
+        if (a.L && a.L->ctx) (void)hipStreamSynchronize(a.L->ctx->stream);
+    amr_free(A);
+    if (A->stream) (void)hipStreamDestroy(A->stream);
+    delete A;
+    return TMK_OK;
+}
+
+extern "C" int tmk_amr_create(int nlevels, tmk_level* const* levels, int ref_ratio, tmk_amr** out)
+{
+    if (!out || nlevels < 1 || !levels || ref_ratio < 2 || ref_ratio > TMK_AMR_MAXR) return TMK_E_ARG;
+    *out = nullptr;
+    for (int l = 0; l < nlevels; ++l) {
+        tmk_level* L = levels[l];
+        if (!L || L->batch || L->amr_stream || L->stations) return TMK_E_ARG;
+        if (L->tsize != levels[0]->tsize) return TMK_E_ARG;
+        for (int k = 0; k < l; ++k)
+            if (levels[k] == L) return TMK_E_ARG;
+    }
+    // level 0 covers its domain
+    {
+        const tmk_level* L0 = levels[0];
+        const tmk_box d = L0->ctx->hp.domain;
+        long long cells = 0;
+        for (const tmk_box& b : L0->boxes) cells += (long long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1);
+        if (cells != (long long)(d.hi[0] - d.lo[0] + 1) * (d.hi[1] - d.lo[1] + 1)) return TMK_E_NESTING;
+    }
+    tmk_amr* A = new (std::nothrow) tmk_amr;
+    if (!A) return TMK_E_NOMEM;
+    A->r = ref_ratio;
+    A->lv.resize(nlevels);
+    for (int l = 0; l < nlevels; ++l) A->lv[l].L = levels[l];
+    int e = TMK_OK;
+    if (hipStreamCreate(&A->stream) != hipSuccess) e = TMK_E_HIP;
+    for (int l = 0; l < nlevels && !e; ++l)
+        if (hipStreamSynchronize(levels[l]->ctx->stream) != hipSuccess) e = TMK_E_HIP;
+    if (!e) e = amr_layouts(A);
+    for (int l = 1; l < nlevels && !e; ++l) e = amr_build_pair(A, l);
+    for (int l = 0; l < nlevels && !e; ++l) {
+        AmrLevel& a = A->lv[l];
+        if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
+        else amr_set_flux_ptrs<float>(a);
+        a.L->amr_stream = A->stream;
+        e = level_own_group(a.L);
+        if (!e && a.L->dirty) {
+            a.L->dirty = false;
+            e = a.L->tsize == sizeof(double) ? group_exchange_now_t<double>(a.L->own, A->stream)
+                                             : group_exchange_now_t<float>(a.L->own, A->stream);
+        }
+    }
+    if (e) {
+        amr_free(A);
+        if (A->stream) (void)hipStreamDestroy(A->stream);
+        delete A;
+        return e;
+    }
+    *out = A;
+    return TMK_OK;
+}
+
+static inline dim3 amr_grid(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// AMRLevelLinElast::advance of level l: coarse-fine fill, the level step (update with
+// face fluxes, exchange, clock), register increments
+template <typename T>
+static int amr_advance_t(tmk_amr* A, int l)
+{
+    AmrLevel& a = A->lv[l];
+    tmk_level* L = a.L;
+    Group& G = L->own;
+    hipStream_t st = A->stream;
+    const int n = (int)A->lv.size();
+    if (l > 0) {
+        Group& GC = A->lv[l - 1].L->own;
+        const Patch<T>* fine = (const Patch<T>*)G.patches_dev[G.g];
+        const Patch<T>* cnew = (const Patch<T>*)GC.patches_dev[GC.g];      // state after the coarse step
+        const Patch<T>* cold = (const Patch<T>*)GC.patches_dev[GC.g ^ 1];  // its input: the state before
+        if (a.ncells > 0) {
+            hipLaunchKernelGGL(k_amr_fill_state<T>, amr_grid((long long)a.ncells * a.ncomps), dim3(256), 0, st,
+                               (const CfCell*)a.cells.p, a.ncells, (const int*)a.comps.p, a.ncomps, fine, cold, cnew,
+                               G.scen_dev, GC.scen_dev, A->r);
+            HIPCHK(hipGetLastError());
+        }
+        if (a.nfaces > 0) {
+            hipLaunchKernelGGL(k_amr_fill_fault<T>, amr_grid((long long)a.nfaces * G.nb), dim3(256), 0, st,
+                               (const CfFace*)a.faces.p, a.nfaces, G.nb, fine, cold, cnew, G.scen_dev, GC.scen_dev,
+                               A->r);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    const Patch<T>* pts = (const Patch<T>*)G.patches_dev[G.g];  // flux pointers are the same at both parities
+    int e = group_enqueue_step(G, st);
+    if (e) return e;
+    L->cur ^= 1;
+    L->time = L->time + L->dt;
+    if (l + 1 < n && a.nreg > 0) {
+        hipLaunchKernelGGL(k_amr_reg_coarse<T>, amr_grid((long long)a.nreg * TMK_NCOMP), dim3(256), 0, st,
+                           (const RegFace*)a.regf.p, a.nreg, (double*)a.reg.p, pts, L->dt);
+        HIPCHK(hipGetLastError());
+    }
+    if (l > 0) {
+        AmrLevel& c = A->lv[l - 1];
+        if (c.nreg > 0) {
+            const double w = L->dt * (1.0 / A->r);
+            hipLaunchKernelGGL(k_amr_reg_fine<T>, amr_grid((long long)c.nreg * TMK_NCOMP), dim3(256), 0, st,
+                               (const RegFace*)c.regf.p, c.nreg, (double*)c.reg.p, pts, A->r, w);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return TMK_OK;
+}
+
+// AMRLevelLinElast::postTimeStep of level l: reflux, average-down of the state and the
+// fault data from level l+1, then the same-level exchange of the changed state
+template <typename T>
+static int amr_post_t(tmk_amr* A, int l)
+{
+    if (l + 1 >= (int)A->lv.size()) return TMK_OK;
+    AmrLevel& a = A->lv[l];
+    Group& G = a.L->own;
+    Group& GF = A->lv[l + 1].L->own;
+    hipStream_t st = A->stream;
+    const Patch<T>* cur = (const Patch<T>*)G.patches_dev[G.g];
+    const Patch<T>* fcur = (const Patch<T>*)GF.patches_dev[GF.g];
+    const int r = A->r;
+    if (a.nrcells > 0) {
+        hipLaunchKernelGGL(k_amr_reflux<T>, amr_grid((long long)a.nrcells * TMK_NCOMP), dim3(256), 0, st,
+                           (const RefluxCell*)a.rcells.p, a.nrcells, (const double*)a.reg.p,
+                           (const double*)a.rscale.p, a.nreg, cur);
+        HIPCHK(hipGetLastError());
+    }
+    if (a.navgc > 0) {
+        hipLaunchKernelGGL(k_amr_avg_state<T>, amr_grid((long long)a.navgc * TMK_NCOMP), dim3(256), 0, st,
+                           (const AvgCell*)a.avgc.p, a.navgc, cur, fcur, r, 1.0 / (r * r));
+        HIPCHK(hipGetLastError());
+    }
+    if (a.navgf > 0) {
+        hipLaunchKernelGGL(k_amr_avg_fault<T>, amr_grid((long long)a.navgf * G.nb), dim3(256), 0, st,
+                           (const AvgFace*)a.avgf.p, a.navgf, G.nb, cur, fcur, r, 1.0 / r);
+        HIPCHK(hipGetLastError());
+    }
+    return group_exchange_now_t<T>(G, st);
+}
+
+template <typename T>
+static int amr_time_step_t(tmk_amr* A, int l)
+{
+    int e = amr_advance_t<T>(A, l);
+    if (e) return e;
+    if (l + 1 < (int)A->lv.size())
+        for (int k = 0; k < A->r; ++k)
+            if ((e = amr_time_step_t<T>(A, l + 1))) return e;
+    return amr_post_t<T>(A, l);
+}
+
+static int amr_time_step(tmk_amr* A)
+{
+    return A->lv[0].L->tsize == sizeof(double) ? amr_time_step_t<double>(A, 0) : amr_time_step_t<float>(A, 0);
+}
+
+// nsteps coarse steps; the clocks are (re)set when time or dt0 is given anew
+extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
+{
+    if (!A || nsteps < 0 || !(dt0 > 0.0)) return TMK_E_ARG;
+    int e;
+    const double t_now = A->lv[0].L->time;
+    if (!A->clock || time != t_now || dt0 != A->dt0) {
+        if (dt0 != A->dt0 && A->gexec) {
+            (void)hipGraphExecDestroy(A->gexec);
+            A->gexec = nullptr;
+        }
+        double dt = dt0;
+        for (AmrLevel& a : A->lv) {
+            a.L->time = time;
+            a.L->dt = dt;
+            if ((e = group_set_clock(a.L->own, A->stream, 0, time, dt, a.L->ctx->P))) return e;
+            dt = dt / A->r;
+        }
+        A->dt0 = dt0;
+        A->clock = true;
+    }
+    Group& G0 = A->lv[0].L->own;
+    while (nsteps > 0) {
+        if (nsteps >= 2 && !A->gexec) {
+            // capture two coarse steps: every level returns to its parity
+            std::vector<double> t0;
+            for (AmrLevel& a : A->lv) t0.push_back(a.L->time);
+            hipGraph_t gr;
+            HIPCHK(hipStreamBeginCapture(A->stream, hipStreamCaptureModeThreadLocal));
+            A->gpar = G0.g;
+            e = amr_time_step(A);
+            if (!e) e = amr_time_step(A);
+            hipError_t ce = hipStreamEndCapture(A->stream, &gr);
+            for (size_t l = 0; l < A->lv.size(); ++l) A->lv[l].L->time = t0[l];  // nothing ran yet
+            if (e) return e;
+            if (ce != hipSuccess) return TMK_E_HIP;
+            hipError_t ie = hipGraphInstantiate(&A->gexec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ie != hipSuccess) return TMK_E_HIP;
+        }
+        if (nsteps >= 2 && G0.g == A->gpar) {
+            HIPCHK(hipGraphLaunch(A->gexec, A->stream));
+            for (size_t l = 0; l < A->lv.size(); ++l) {
+                tmk_level* L = A->lv[l].L;
+                long long k = 2;
+                for (size_t q = 0; q < l; ++q) k *= A->r;
+                for (long long s = 0; s < k; ++s) L->time = L->time + L->dt;
+            }
+            nsteps -= 2;
+            continue;
+        }
+        if ((e = amr_time_step(A))) return e;
+        --nsteps;
+    }
+    return TMK_OK;
+}
+
+extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0, double* ms)
+{
+    if (!A || !ms) return TMK_E_ARG;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int e = TMK_OK;
+    if (hipEventRecord(a, A->stream) != hipSuccess) e = TMK_E_HIP;
+    if (!e) e = tmk_amr_run(A, nsteps, time, dt0);
+    if (!e && hipEventRecord(b, A->stream) != hipSuccess) e = TMK_E_HIP;
+    if (!e && hipEventSynchronize(b) != hipSuccess) e = TMK_E_HIP;
+    float f = 0.f;
+    if (!e && hipEventElapsedTime(&f, a, b) != hipSuccess) e = TMK_E_HIP;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (!e) *ms = f;
+    return e;
+}
+
+// waits for the hierarchy's work; returns (and clears) the first error any level reported
+extern "C" int tmk_amr_sync(tmk_amr* A)
+{
+    if (!A) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(A->stream));
+    int first = 0;
+    for (AmrLevel& a : A->lv) {
+        int code = 0;
+        int e = group_error(a.L->own, A->stream, 0, &code);
+        if (e) return e;
+        if (code) {
+            if (!first) first = code;
+            const int z = 0;
+            HIPCHK(hipMemcpy((char*)(a.L->own.scen_dev) + offsetof(Scen, err), &z, sizeof z, hipMemcpyHostToDevice));
+        }
+    }
+    return first;
 }

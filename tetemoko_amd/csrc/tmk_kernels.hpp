@@ -43,6 +43,7 @@ struct Patch {
     double* F[2];      // optional face fluxes (22 comps, on surroundingNodes(box,d)) or NULL
     int vlo0, vlo1, vhi0, vhi1; // valid box
     int scen;          // index of the scenario (parameters, time, step) this box belongs to
+    int fedge;         // 1: F holds only the box-edge faces (lo and hi face rows of each direction)
 };
 
 struct StateVec {
@@ -155,12 +156,26 @@ template <typename T, int NS>
 __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
                                                 const T (&sv)[NS], T r, int i, int j, int sd)
 {
+    const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
+    size_t fp, fi;
+    if (pt.fedge) {
+        // edge layout: x faces (2, fny) with column 0 = lo face, y faces (2, fnx) with row 0 = lo face
+        if (d == 0) {
+            if (i != pt.vlo0 && i != pt.vhi0 + 1) return;
+            fp = (size_t)2 * fny;
+            fi = (size_t)(i != pt.vlo0) + (size_t)(j - pt.vlo1) * 2;
+        } else {
+            if (j != pt.vlo1 && j != pt.vhi1 + 1) return;
+            fp = (size_t)2 * fnx;
+            fi = (size_t)(i - pt.vlo0) + (size_t)(j != pt.vlo1) * fnx;
+        }
+    } else {
+        fp = (size_t)fnx * fny;
+        fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+    }
     T f[7];
     face_flux_side<T, NS>(m, v, sv, r, f);
     double* Fo = pt.F[d];
-    const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
-    const size_t fp = (size_t)fnx * fny;
-    const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
     const int o = 11 * sd;
     if (d == 0) {
         Fo[fi + (o + 0) * fp] = (double)f[0];

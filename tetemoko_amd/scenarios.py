@@ -133,6 +133,18 @@ def config_c3(seed=0, nx=8192, ny=4096, dx=12.5):
     return cfg
 
 
+def refine_cfg(cfg, r):
+    """The configuration of the next finer level: r times the cells, dx / r.  Per-face
+    parameter arrays are injected (each coarse face's value on its r fine faces), so the
+    seeded values do not depend on the level; positions are in metres and carry over."""
+    out = dict(cfg)
+    out["nx"], out["ny"], out["dx"] = cfg["nx"] * r, cfg["ny"] * r, cfg["dx"] / r
+    faces = cfg.get("faces")
+    if faces:
+        out["faces"] = {k: (v * r if k == "lo" else np.repeat(np.asarray(v), r)) for k, v in faces.items()}
+    return out
+
+
 # ---------------------------------------------------------------------------
 # level arrays
 # ---------------------------------------------------------------------------

@@ -578,7 +578,9 @@ def test_mem_info_and_footprint():
     cfg = scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0)
     ctx = capi.Context(cfg)
     n = capi.level_footprint(ctx, [(0, 0, 95, 47)])
-    assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + 2 * (96 + 8) * 11 * 8
+    # three state arrays over the ghosted box, its cell-owner map, two fault-data rows and their
+    # face-owner map
+    assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + (96 + 8) * (48 + 8) * 4 + 2 * (96 + 8) * 11 * 8 + (96 + 8) * 4
     ctx.close()
 
 

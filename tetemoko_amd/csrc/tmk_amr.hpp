@@ -172,12 +172,11 @@ __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, i
     ((double*)pd.bin)[f.doff + (size_t)c * pd.bnx] = v[1] + f.coef * s;
 }
 
-// plane size of the flux array of direction d of a box (full or edge layout)
+// plane size of the flux array of direction d (faces of the patch's valid box)
 template <typename T>
 __device__ __forceinline__ size_t flux_plane(const Patch<T>& p, int d)
 {
     const int fnx = p.vhi0 - p.vlo0 + 1 + (d == 0), fny = p.vhi1 - p.vlo1 + 1 + (d == 1);
-    if (p.fedge) return (size_t)2 * (d == 0 ? fny : fnx);
     return (size_t)fnx * fny;
 }
 

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <vector>
 
@@ -243,15 +244,34 @@ struct tmk_ctx {
 struct tmk_batch;
 struct tmk_stations;
 
+// Boxes whose ghost regions reach each other share one set of arrays over their union's
+// bounding box (grown by the ghost width): an island.  Neighbouring boxes then read each
+// other's cells in place (no ghost exchange inside an island), and the update tiles cover
+// merged rectangles of the union instead of each box.  An island whose bounding box would
+// be more than twice the boxes' own ghosted area is split into one island per box.
+struct Island {
+    std::vector<int> boxes;
+    tmk_box bbox{}, abox{};      // union bounding box; array box = bbox grown by the ghosts
+    int has_b = 0;               // a box of the island lies on the fault
+    int blo0 = 0, bnx = 0;       // fault-data row: faces blo0 .. blo0 + bnx - 1
+    void* u[2] = {nullptr, nullptr};  // stencil planes (ping-pong)
+    void* w = nullptr;                // planes updated in place
+    double* b[2] = {nullptr, nullptr};
+    int* owner = nullptr;        // device: box owning each array cell, -1 for ghosts
+    int* bowner = nullptr;       // device: box owning each fault face, -1 for ghosts
+    size_t plane() const { return (size_t)(abox.hi[0] - abox.lo[0] + 1) * (abox.hi[1] - abox.lo[1] + 1); }
+    int unx() const { return abox.hi[0] - abox.lo[0] + 1; }
+};
+
 struct tmk_level {
     tmk_ctx* ctx = nullptr;
     std::vector<tmk_box> boxes;
     std::vector<int> has_b;
     int ns = 2;
     size_t tsize = 8;
-    std::vector<void*> ubuf[2];
-    std::vector<void*> upt;
-    std::vector<double*> bbuf[2];
+    std::vector<Island> isl;
+    std::vector<int> box_isl;     // island of each box
+    int npatch = 0;               // descriptors: one per box, then one per island (the tiles')
     int cur = 0;                  // buffer parity holding the current state
     double time = 0.0, dt = 0.0;  // clock of this scenario
     std::vector<char> hpatch[2];  // Patch<T> descriptors for parity 0 / 1 (scen index 0)
@@ -653,10 +673,10 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         memset(&sc[m], 0, sizeof(Scen));
         sc[m].P = L->ctx->P;
         scen_set_clock(sc[m], L->time, L->dt);
-        const int nbox = (int)L->boxes.size();
+        const int np = L->npatch;
         for (int g = 0; g < 2; ++g) {
             const Patch<T>* src = (const Patch<T>*)L->hpatch[phase[m] ^ g].data();
-            for (int b = 0; b < nbox; ++b) {
+            for (int b = 0; b < np; ++b) {
                 Patch<T> p = src[b];
                 p.scen = (int)m;
                 pts[g].push_back(p);
@@ -665,12 +685,12 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         for (Tile t : L->tiles) { t.patch += poff; tiles.push_back(t); }
         for (CopyRect r : L->rects) { r.src += poff; r.dst += poff; rects.push_back(r); }
         for (CopyRect r : L->brects) { r.src += poff; r.dst += poff; brects.push_back(r); }
-        poff += nbox;
+        poff += np;
     }
     // tile -> scenario through the patch index (patches of member m follow those of m - 1)
     std::vector<int> patch_scen;
     for (size_t m = 0; m < mem.size(); ++m)
-        for (size_t b = 0; b < mem[m]->boxes.size(); ++b) patch_scen.push_back((int)m);
+        for (int b = 0; b < mem[m]->npatch; ++b) patch_scen.push_back((int)m);
     const int bx = G.bx, by = G.by;
     G.nint = partition_tiles(
         tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
@@ -890,28 +910,31 @@ extern "C" int tmk_patch_plastic_update(tmk_ctx* c, tmk_box valid, double* U, tm
 template <typename T>
 static void level_fill_patches(tmk_level* L)
 {
-    const int nbox = (int)L->boxes.size();
+    const int nbox = (int)L->boxes.size(), nisl = (int)L->isl.size();
+    L->npatch = nbox + nisl;
     for (int cur = 0; cur < 2; ++cur) {
-        L->hpatch[cur].assign(sizeof(Patch<T>) * nbox, 0);
+        L->hpatch[cur].assign(sizeof(Patch<T>) * L->npatch, 0);
         Patch<T>* pts = (Patch<T>*)L->hpatch[cur].data();
-        for (int b = 0; b < nbox; ++b) {
-            const tmk_box v = L->boxes[b];
-            const tmk_box ub = box_grow(v, TMK_GHOST);
-            const size_t unx = ub.hi[0] - ub.lo[0] + 1, uny = ub.hi[1] - ub.lo[1] + 1, plane = unx * uny;
-            Patch<T>& p = pts[b];
-            T* A = (T*)L->ubuf[cur][b];
-            T* Bn = (T*)L->ubuf[1 - cur][b];
-            T* W = (T*)L->upt[b];
+        for (int q = 0; q < L->npatch; ++q) {
+            const int k = q < nbox ? L->box_isl[q] : q - nbox;
+            const Island& I = L->isl[k];
+            const tmk_box v = q < nbox ? L->boxes[q] : I.bbox;
+            const bool onf = q < nbox ? L->has_b[q] != 0 : I.has_b != 0;
+            const size_t plane = I.plane();
+            Patch<T>& p = pts[q];
+            T* A = (T*)I.u[cur];
+            T* Bn = (T*)I.u[1 - cur];
+            T* W = (T*)I.w;
             for (int c = 0; c < TMK_NCOMP; ++c) {
                 if (is_stencil(c, L->ns)) { p.in[c] = A + c * plane; p.out[c] = Bn + c * plane; }
                 else { p.in[c] = W + c * plane; p.out[c] = W + c * plane; }
             }
-            p.ulo0 = ub.lo[0]; p.ulo1 = ub.lo[1]; p.unx = (int)unx;
-            if (L->has_b[b]) {
-                p.bin = L->bbuf[cur][b];
-                p.bout = L->bbuf[1 - cur][b];
-                p.blo0 = v.lo[0] - TMK_GHOST;
-                p.bnx = v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST;
+            p.ulo0 = I.abox.lo[0]; p.ulo1 = I.abox.lo[1]; p.unx = I.unx();
+            if (onf) {
+                p.bin = I.b[cur];
+                p.bout = I.b[1 - cur];
+                p.blo0 = I.blo0;
+                p.bnx = I.bnx;
                 p.bnb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
             }
             p.vlo0 = v.lo[0]; p.vlo1 = v.lo[1]; p.vhi0 = v.hi[0]; p.vhi1 = v.hi[1];
@@ -920,18 +943,106 @@ static void level_fill_patches(tmk_level* L)
     }
 }
 
+// the islands of a box list (see Island)
+static void build_islands(const std::vector<tmk_box>& boxes, const tmk_box& dom, std::vector<Island>& isl,
+                          std::vector<int>& box_isl)
+{
+    const int n = (int)boxes.size();
+    std::vector<int> par(n);
+    for (int k = 0; k < n; ++k) par[k] = k;
+    std::function<int(int)> root = [&](int k) { return par[k] == k ? k : (par[k] = root(par[k])); };
+    // boxes whose ghost regions reach each other: sort by x and sweep
+    std::vector<int> ord(n);
+    for (int k = 0; k < n; ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return boxes[a].lo[0] < boxes[b].lo[0]; });
+    for (int p = 0; p < n; ++p) {
+        const tmk_box g = box_grow(boxes[ord[p]], TMK_GHOST);
+        for (int q = p + 1; q < n && boxes[ord[q]].lo[0] <= g.hi[0]; ++q)
+            if (!box_empty(box_and(g, boxes[ord[q]]))) par[root(ord[q])] = root(ord[p]);
+    }
+    std::vector<int> comp_of(n, -1);
+    std::vector<std::vector<int>> comps;
+    for (int k = 0; k < n; ++k) {
+        const int r = root(k);
+        if (comp_of[r] < 0) {
+            comp_of[r] = (int)comps.size();
+            comps.emplace_back();
+        }
+        comps[comp_of[r]].push_back(k);
+    }
+    isl.clear();
+    box_isl.assign(n, -1);
+    auto area = [](const tmk_box& b) { return (double)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1); };
+    auto add = [&](const std::vector<int>& members) {
+        Island I;
+        I.boxes = members;
+        I.bbox = boxes[members[0]];
+        for (int k : members) {
+            const tmk_box& b = boxes[k];
+            I.bbox.lo[0] = std::min(I.bbox.lo[0], b.lo[0]); I.bbox.lo[1] = std::min(I.bbox.lo[1], b.lo[1]);
+            I.bbox.hi[0] = std::max(I.bbox.hi[0], b.hi[0]); I.bbox.hi[1] = std::max(I.bbox.hi[1], b.hi[1]);
+            if (b.lo[1] == dom.lo[1]) I.has_b = 1;
+            box_isl[k] = (int)isl.size();
+        }
+        I.abox = box_grow(I.bbox, TMK_GHOST);
+        I.blo0 = I.bbox.lo[0] - TMK_GHOST;
+        I.bnx = I.bbox.hi[0] - I.bbox.lo[0] + 1 + 2 * TMK_GHOST;
+        isl.push_back(I);
+    };
+    for (const auto& cm : comps) {
+        tmk_box bb = boxes[cm[0]];
+        double own = 0.0;
+        for (int k : cm) {
+            bb.lo[0] = std::min(bb.lo[0], boxes[k].lo[0]); bb.lo[1] = std::min(bb.lo[1], boxes[k].lo[1]);
+            bb.hi[0] = std::max(bb.hi[0], boxes[k].hi[0]); bb.hi[1] = std::max(bb.hi[1], boxes[k].hi[1]);
+            own += area(box_grow(boxes[k], TMK_GHOST));
+        }
+        if (cm.size() > 1 && area(box_grow(bb, TMK_GHOST)) > 2.0 * own) {
+            for (int k : cm) add(std::vector<int>{k});
+        } else {
+            add(cm);
+        }
+    }
+}
+
+// the island's boxes merged into larger rectangles (rows of boxes with equal y extent, then
+// columns of those with equal x extent), tiled as the update's work units
+static void island_rects(const std::vector<tmk_box>& boxes, const Island& I, std::vector<tmk_box>& out)
+{
+    std::vector<tmk_box> r;
+    for (int k : I.boxes) r.push_back(boxes[k]);
+    auto merge = [&](int d) {
+        const int o = 1 - d;  // merge along d: equal extent in o, adjacent in d
+        std::sort(r.begin(), r.end(), [&](const tmk_box& a, const tmk_box& b) {
+            if (a.lo[o] != b.lo[o]) return a.lo[o] < b.lo[o];
+            if (a.hi[o] != b.hi[o]) return a.hi[o] < b.hi[o];
+            return a.lo[d] < b.lo[d];
+        });
+        std::vector<tmk_box> m;
+        for (const tmk_box& b : r) {
+            if (!m.empty() && m.back().lo[o] == b.lo[o] && m.back().hi[o] == b.hi[o] && m.back().hi[d] + 1 == b.lo[d])
+                m.back().hi[d] = b.hi[d];
+            else
+                m.push_back(b);
+        }
+        r.swap(m);
+    };
+    merge(0);
+    merge(1);
+    out = r;
+}
+
 static size_t level_bytes(const tmk_ctx* c, int nboxes, const tmk_box* boxes)
 {
     const size_t ts = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    const tmk_box dom = c->hp.domain;
     const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    std::vector<Island> isl;
+    std::vector<int> bi;
+    build_islands(std::vector<tmk_box>(boxes, boxes + nboxes), c->hp.domain, isl, bi);
     size_t tot = 0;
-    for (int b = 0; b < nboxes; ++b) {
-        const tmk_box ub = box_grow(boxes[b], TMK_GHOST);
-        const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
-        tot += 3 * plane * TMK_NCOMP * ts;
-        if (boxes[b].lo[1] == dom.lo[1])
-            tot += 2 * (size_t)(boxes[b].hi[0] - boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb * sizeof(double);
+    for (const Island& I : isl) {
+        tot += 3 * I.plane() * TMK_NCOMP * ts + I.plane() * sizeof(int);
+        if (I.has_b) tot += 2 * (size_t)I.bnx * nb * sizeof(double) + I.bnx * sizeof(int);
     }
     return tot;
 }
@@ -941,6 +1052,23 @@ extern "C" int tmk_level_footprint(const tmk_ctx* c, int nboxes, const tmk_box* 
     if (!c || nboxes <= 0 || !boxes || !bytes) return TMK_E_ARG;
     *bytes = level_bytes(c, nboxes, boxes);
     return TMK_OK;
+}
+
+static void level_free_arrays(tmk_level* L)
+{
+    for (Island& I : L->isl) {
+        for (int k = 0; k < 2; ++k) {
+            if (I.u[k]) (void)hipFree(I.u[k]);
+            if (I.b[k]) (void)hipFree(I.b[k]);
+            I.u[k] = nullptr;
+            I.b[k] = nullptr;
+        }
+        if (I.w) (void)hipFree(I.w);
+        if (I.owner) (void)hipFree(I.owner);
+        if (I.bowner) (void)hipFree(I.bowner);
+        I.w = nullptr;
+        I.owner = I.bowner = nullptr;
+    }
 }
 
 extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tmk_level** out)
@@ -958,40 +1086,59 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     L->tsize = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
     L->bytes = level_bytes(c, nboxes, boxes);
     const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
-    for (int b = 0; b < nboxes; ++b) {
-        const tmk_box v = boxes[b];
-        const tmk_box ub = box_grow(v, TMK_GHOST);
-        const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    for (int b = 0; b < nboxes; ++b) L->has_b.push_back(boxes[b].lo[1] == dom.lo[1]);
+    build_islands(L->boxes, dom, L->isl, L->box_isl);
+    bool ok = true;
+    for (Island& I : L->isl) {
+        const size_t plane = I.plane();
         const size_t bytes = plane * TMK_NCOMP * L->tsize;
-        void *a = nullptr, *bb = nullptr, *w = nullptr;
-        const bool ok = hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&bb, bytes) == hipSuccess &&
-                        hipMalloc(&w, bytes) == hipSuccess;
-        L->ubuf[0].push_back(a);
-        L->ubuf[1].push_back(bb);
-        L->upt.push_back(w);
-        const int hb = v.lo[1] == dom.lo[1];
-        L->has_b.push_back(hb);
-        double *b0 = nullptr, *b1 = nullptr;
-        bool okb = true;
-        if (hb) {
-            const size_t nB = (size_t)(v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST) * nb;
-            okb = hipMalloc(&b0, nB * sizeof(double)) == hipSuccess && hipMalloc(&b1, nB * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&I.u[0], bytes) == hipSuccess && hipMalloc(&I.u[1], bytes) == hipSuccess &&
+             hipMalloc(&I.w, bytes) == hipSuccess && hipMalloc((void**)&I.owner, plane * sizeof(int)) == hipSuccess;
+        if (ok) {
+            std::vector<int> own(plane, -1);
+            const int unx = I.unx();
+            for (int k : I.boxes) {
+                const tmk_box v = L->boxes[k];
+                for (int j = v.lo[1]; j <= v.hi[1]; ++j)
+                    for (int i = v.lo[0]; i <= v.hi[0]; ++i)
+                        own[(size_t)(i - I.abox.lo[0]) + (size_t)(j - I.abox.lo[1]) * unx] = k;
+            }
+            ok = hipMemcpy(I.owner, own.data(), plane * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
         }
-        L->bbuf[0].push_back(b0);
-        L->bbuf[1].push_back(b1);
-        if (!ok || !okb) {
-            tmk_level_destroy(L);
-            return TMK_E_NOMEM;
+        if (ok && I.has_b) {
+            const size_t nB = (size_t)I.bnx * nb;
+            ok = hipMalloc((void**)&I.b[0], nB * sizeof(double)) == hipSuccess &&
+                 hipMalloc((void**)&I.b[1], nB * sizeof(double)) == hipSuccess &&
+                 hipMalloc((void**)&I.bowner, I.bnx * sizeof(int)) == hipSuccess;
+            if (ok) {
+                std::vector<int> own(I.bnx, -1);
+                for (int k : I.boxes)
+                    if (L->has_b[k])
+                        for (int i = L->boxes[k].lo[0]; i <= L->boxes[k].hi[0]; ++i) own[i - I.blo0] = k;
+                ok = hipMemcpy(I.bowner, own.data(), I.bnx * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+            }
         }
+        if (!ok) break;
+    }
+    if (!ok) {
+        level_free_arrays(L);
+        delete L;
+        return TMK_E_NOMEM;
     }
     if (L->tsize == sizeof(double)) level_fill_patches<double>(L);
     else level_fill_patches<float>(L);
-    for (int b = 0; b < nboxes; ++b) make_tiles(boxes[b], b, c->bx - 2, c->by - 2, L->tiles);
-    // exchange plan: ghost cells (inside the domain) of each box covered by another box's valid cells
+    // tiles over each island's merged rectangles; they use the island's descriptor
+    for (size_t k = 0; k < L->isl.size(); ++k) {
+        std::vector<tmk_box> rr;
+        island_rects(L->boxes, L->isl[k], rr);
+        for (const tmk_box& r : rr) make_tiles(r, nboxes + (int)k, c->bx - 2, c->by - 2, L->tiles);
+    }
+    // exchange plan: ghost cells (inside the domain) of each box covered by valid cells of a
+    // box in another island
     for (int d = 0; d < nboxes; ++d) {
         const tmk_box g = box_and(box_grow(boxes[d], TMK_GHOST), dom);
         for (int s = 0; s < nboxes; ++s) {
-            if (s == d) continue;
+            if (s == d || L->box_isl[s] == L->box_isl[d]) continue;
             const tmk_box r = box_and(g, boxes[s]);
             if (box_empty(r)) continue;
             L->rects.push_back(CopyRect{s, d, r.lo[0], r.lo[1], r.hi[0], r.hi[1]});
@@ -1013,12 +1160,114 @@ extern "C" int tmk_level_destroy(tmk_level* L)
     if (L->ctx && L->ctx->stream) (void)hipStreamSynchronize(L->ctx->stream);
     if (L->stations) L->stations->L = nullptr;
     L->own.release();
-    for (int k = 0; k < 2; ++k) {
-        for (void* p : L->ubuf[k]) if (p) (void)hipFree(p);
-        for (double* p : L->bbuf[k]) if (p) (void)hipFree(p);
-    }
-    for (void* p : L->upt) if (p) (void)hipFree(p);
+    level_free_arrays(L);
     delete L;
+    return TMK_OK;
+}
+
+// a box's ghosted region <-> staging array (22 planes of the ghosted box, x fastest); put
+// writes the cells the box owns and the island's ghost cells, never another box's cells
+template <typename T>
+static __global__ void k_box_put(T* u0, T* u1, T* w, const int* owner, int aunx, size_t aplane, int ox, int oy,
+                                 const T* src, int gnx, int gny, int box)
+{
+    const size_t n = (size_t)gnx * gny;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int x = (int)(t % gnx), y = (int)(t / gnx);
+    const size_t a = (size_t)(x + ox) + (size_t)(y + oy) * aunx;
+    const int o = owner[a];
+    if (o != box && o != -1) return;
+    for (int c = 0; c < TMK_NCOMP; ++c) {
+        const T v = src[t + c * n];
+        u0[a + c * aplane] = v;
+        u1[a + c * aplane] = v;
+        w[a + c * aplane] = v;
+    }
+}
+
+template <typename T>
+static __global__ void k_box_const(T* u0, T* u1, T* w, const int* owner, int aunx, size_t aplane, int ox, int oy,
+                                   StateVec st, int gnx, int gny, int box)
+{
+    const size_t n = (size_t)gnx * gny;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int x = (int)(t % gnx), y = (int)(t / gnx);
+    const size_t a = (size_t)(x + ox) + (size_t)(y + oy) * aunx;
+    const int o = owner[a];
+    if (o != box && o != -1) return;
+    for (int c = 0; c < TMK_NCOMP; ++c) {
+        const T v = (T)st.c[c];
+        u0[a + c * aplane] = v;
+        u1[a + c * aplane] = v;
+        w[a + c * aplane] = v;
+    }
+}
+
+template <typename T>
+static __global__ void k_box_get(const T* u, const T* w, int aunx, size_t aplane, int ox, int oy, T* dst, int gnx,
+                                 int gny, int smask)
+{
+    const size_t n = (size_t)gnx * gny;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int x = (int)(t % gnx), y = (int)(t / gnx);
+    const size_t a = (size_t)(x + ox) + (size_t)(y + oy) * aunx;
+    for (int c = 0; c < TMK_NCOMP; ++c) dst[t + c * n] = ((smask >> c) & 1) ? u[a + c * aplane] : w[a + c * aplane];
+}
+
+// fault-data row of a box (with its ghosts) <-> the island's row
+static __global__ void k_row_put(double* b0, double* b1, const int* bowner, int off, int bnx, const double* src, int n,
+                                 int nb, int box)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const int a = off + f;
+    const int o = bowner[a];
+    if (o != box && o != -1) return;
+    for (int c = 0; c < nb; ++c) {
+        b0[a + (size_t)c * bnx] = src[f + (size_t)c * n];
+        b1[a + (size_t)c * bnx] = src[f + (size_t)c * n];
+    }
+}
+
+static __global__ void k_row_get(const double* b, int off, int bnx, double* dst, int n, int nb)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    for (int c = 0; c < nb; ++c) dst[f + (size_t)c * n] = b[off + f + (size_t)c * bnx];
+}
+
+static int level_put_fault(tmk_level* L, int b, const double* B)
+{
+    tmk_ctx* c = L->ctx;
+    const Island& I = L->isl[L->box_isl[b]];
+    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    const int n = L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST;
+    int e;
+    if ((e = c->scratch[0].ensure((size_t)n * nb * sizeof(double)))) return e;
+    HIPCHK(hipMemcpy(c->scratch[0].p, B, (size_t)n * nb * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_row_put, dim3((n + 255) / 256), dim3(256), 0, c->stream, I.b[0], I.b[1], I.bowner,
+                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (const double*)c->scratch[0].p, n, nb, b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TMK_OK;
+}
+
+static int level_get_fault(tmk_level* L, int b, double* B, hipStream_t st)
+{
+    tmk_ctx* c = L->ctx;
+    const Island& I = L->isl[L->box_isl[b]];
+    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    const int n = L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST;
+    int e;
+    if ((e = c->scratch[0].ensure((size_t)n * nb * sizeof(double)))) return e;
+    hipLaunchKernelGGL(k_row_get, dim3((n + 255) / 256), dim3(256), 0, st, I.b[L->cur],
+                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (double*)c->scratch[0].p, n, nb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(B, c->scratch[0].p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost));
     return TMK_OK;
 }
 
@@ -1027,7 +1276,8 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
     if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch || L->amr_stream) return TMK_E_ARG;
     tmk_ctx* c = L->ctx;
     const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
-    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    const int gnx = ub.hi[0] - ub.lo[0] + 1, gny = ub.hi[1] - ub.lo[1] + 1;
+    const size_t plane = (size_t)gnx * gny;
     const size_t nU = plane * TMK_NCOMP;
     if (L->ns == 2) {  // mode II level: the anti-plane planes must be zero
         for (int s = 0; s < 2; ++s)
@@ -1035,6 +1285,7 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
                 for (size_t k = 0; k < plane; ++k)
                     if (U[k + (size_t)(11 * s + cc) * plane] != 0.0) return TMK_E_ARG;
     }
+    if (L->has_b[b] && !B) return TMK_E_NO_BDRY;
     std::vector<float> tmp;
     const void* src = U;
     if (L->tsize == sizeof(float)) {
@@ -1043,32 +1294,22 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
         src = tmp.data();
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    const size_t bytes = nU * L->tsize;
-    for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->ubuf[k][b], src, bytes, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(L->upt[b], src, bytes, hipMemcpyHostToDevice));
-    if (L->has_b[b]) {
-        if (!B) return TMK_E_NO_BDRY;
-        const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
-        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
-        for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->bbuf[k][b], B, nB * sizeof(double), hipMemcpyHostToDevice));
-    }
+    int e;
+    if ((e = c->scratch[7].ensure(nU * L->tsize))) return e;
+    HIPCHK(hipMemcpy(c->scratch[7].p, src, nU * L->tsize, hipMemcpyHostToDevice));
+    const Island& I = L->isl[L->box_isl[b]];
+    const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
+    const dim3 grid((unsigned)((plane + 255) / 256));
+    if (L->tsize == sizeof(double))
+        hipLaunchKernelGGL(k_box_put<double>, grid, dim3(256), 0, c->stream, (double*)I.u[0], (double*)I.u[1],
+                           (double*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const double*)c->scratch[7].p, gnx, gny, b);
+    else
+        hipLaunchKernelGGL(k_box_put<float>, grid, dim3(256), 0, c->stream, (float*)I.u[0], (float*)I.u[1],
+                           (float*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const float*)c->scratch[7].p, gnx, gny, b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (L->has_b[b] && (e = level_put_fault(L, b, B))) return e;
     L->dirty = true;
-    return TMK_OK;
-}
-
-template <typename T>
-static int level_fill_t(tmk_level* L, int b, const double* state)
-{
-    const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
-    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
-    StateVec v;
-    for (int c = 0; c < TMK_NCOMP; ++c) v.c[c] = state[c];
-    const int nblk = (int)std::min<size_t>((plane + 255) / 256, 4096);
-    void* bufs[3] = {L->ubuf[0][b], L->ubuf[1][b], L->upt[b]};
-    for (void* q : bufs) {
-        hipLaunchKernelGGL(k_fill<T>, dim3(nblk), dim3(256), 0, L->ctx->stream, (T*)q, plane, v);
-        HIPCHK(hipGetLastError());
-    }
     return TMK_OK;
 }
 
@@ -1078,15 +1319,26 @@ extern "C" int tmk_level_fill(tmk_level* L, int b, const double* state, const do
     if (L->ns == 2 && (state[2] != 0.0 || state[7] != 0.0 || state[8] != 0.0 || state[13] != 0.0 ||
                        state[18] != 0.0 || state[19] != 0.0))
         return TMK_E_ARG;
-    int e = L->tsize == sizeof(double) ? level_fill_t<double>(L, b, state) : level_fill_t<float>(L, b, state);
-    if (e) return e;
-    HIPCHK(hipStreamSynchronize(L->ctx->stream));
-    if (L->has_b[b]) {
-        if (!B) return TMK_E_NO_BDRY;
-        const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
-        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
-        for (int k = 0; k < 2; ++k) HIPCHK(hipMemcpy(L->bbuf[k][b], B, nB * sizeof(double), hipMemcpyHostToDevice));
-    }
+    if (L->has_b[b] && !B) return TMK_E_NO_BDRY;
+    tmk_ctx* c = L->ctx;
+    const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
+    const int gnx = ub.hi[0] - ub.lo[0] + 1, gny = ub.hi[1] - ub.lo[1] + 1;
+    const size_t plane = (size_t)gnx * gny;
+    StateVec v;
+    for (int k = 0; k < TMK_NCOMP; ++k) v.c[k] = state[k];
+    const Island& I = L->isl[L->box_isl[b]];
+    const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
+    const dim3 grid((unsigned)((plane + 255) / 256));
+    if (L->tsize == sizeof(double))
+        hipLaunchKernelGGL(k_box_const<double>, grid, dim3(256), 0, c->stream, (double*)I.u[0], (double*)I.u[1],
+                           (double*)I.w, I.owner, I.unx(), I.plane(), ox, oy, v, gnx, gny, b);
+    else
+        hipLaunchKernelGGL(k_box_const<float>, grid, dim3(256), 0, c->stream, (float*)I.u[0], (float*)I.u[1],
+                           (float*)I.w, I.owner, I.unx(), I.plane(), ox, oy, v, gnx, gny, b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int e;
+    if (L->has_b[b] && (e = level_put_fault(L, b, B))) return e;
     L->dirty = true;
     return TMK_OK;
 }
@@ -1096,36 +1348,42 @@ extern "C" int tmk_level_download_fault(tmk_level* L, int b, double* B)
 {
     if (!L || b < 0 || b >= (int)L->boxes.size() || !B || L->batch) return TMK_E_ARG;
     if (!L->has_b[b]) return TMK_E_NO_BDRY;
-    HIPCHK(hipStreamSynchronize(L->amr_stream ? L->amr_stream : L->ctx->stream));
-    const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
-    const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
-    HIPCHK(hipMemcpy(B, L->bbuf[L->cur][b], nB * sizeof(double), hipMemcpyDeviceToHost));
-    return TMK_OK;
+    hipStream_t st = L->amr_stream ? L->amr_stream : L->ctx->stream;
+    HIPCHK(hipStreamSynchronize(st));
+    return level_get_fault(L, b, B, st);
 }
 
 extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
 {
     if (!L || b < 0 || b >= (int)L->boxes.size() || !U || L->batch) return TMK_E_ARG;
     tmk_ctx* c = L->ctx;
-    HIPCHK(hipStreamSynchronize(L->amr_stream ? L->amr_stream : c->stream));
+    hipStream_t st = L->amr_stream ? L->amr_stream : c->stream;
+    HIPCHK(hipStreamSynchronize(st));
     const tmk_box ub = box_grow(L->boxes[b], TMK_GHOST);
-    const size_t plane = (size_t)(ub.hi[0] - ub.lo[0] + 1) * (ub.hi[1] - ub.lo[1] + 1);
+    const int gnx = ub.hi[0] - ub.lo[0] + 1, gny = ub.hi[1] - ub.lo[1] + 1;
+    const size_t plane = (size_t)gnx * gny;
     const size_t nU = plane * TMK_NCOMP;
-    std::vector<char> a(nU * L->tsize), w(nU * L->tsize);
-    HIPCHK(hipMemcpy(a.data(), L->ubuf[L->cur][b], nU * L->tsize, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(w.data(), L->upt[b], nU * L->tsize, hipMemcpyDeviceToHost));
-    for (int cc = 0; cc < TMK_NCOMP; ++cc) {
-        const char* src = is_stencil(cc, L->ns) ? a.data() : w.data();
-        for (size_t k = 0; k < plane; ++k) {
-            const size_t q = k + (size_t)cc * plane;
-            U[q] = L->tsize == sizeof(double) ? ((const double*)src)[q] : (double)((const float*)src)[q];
-        }
+    int e;
+    if ((e = c->scratch[7].ensure(nU * L->tsize))) return e;
+    const Island& I = L->isl[L->box_isl[b]];
+    const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
+    const dim3 grid((unsigned)((plane + 255) / 256));
+    if (L->tsize == sizeof(double))
+        hipLaunchKernelGGL(k_box_get<double>, grid, dim3(256), 0, st, (const double*)I.u[L->cur], (const double*)I.w,
+                           I.unx(), I.plane(), ox, oy, (double*)c->scratch[7].p, gnx, gny, stencil_mask(L->ns));
+    else
+        hipLaunchKernelGGL(k_box_get<float>, grid, dim3(256), 0, st, (const float*)I.u[L->cur], (const float*)I.w,
+                           I.unx(), I.plane(), ox, oy, (float*)c->scratch[7].p, gnx, gny, stencil_mask(L->ns));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    if (L->tsize == sizeof(double)) {
+        HIPCHK(hipMemcpy(U, c->scratch[7].p, nU * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> a(nU);
+        HIPCHK(hipMemcpy(a.data(), c->scratch[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nU; ++k) U[k] = (double)a[k];
     }
-    if (L->has_b[b] && B) {
-        const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
-        const size_t nB = (size_t)(L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST) * nb;
-        HIPCHK(hipMemcpy(B, L->bbuf[L->cur][b], nB * sizeof(double), hipMemcpyDeviceToHost));
-    }
+    if (L->has_b[b] && B) return level_get_fault(L, b, B, st);
     return TMK_OK;
 }
 
@@ -1587,8 +1845,8 @@ extern "C" int tmk_level_tag(tmk_level* L, const tmk_tagparams* tp, unsigned cha
 // ---------------------------------------------------------------------------
 struct AmrLevel {
     tmk_level* L = nullptr;
-    std::vector<double*> F[2];  // per box face fluxes (nullptr: none)
-    std::vector<int> fedge;     // per box: 1 = only the box-edge faces are kept
+    std::vector<double*> F[2];          // per island: face fluxes over the island's bounding box
+    std::vector<unsigned char*> fmask;  // per island: which faces the update stores (see Patch::fmask)
     // coarse-fine fill of this level from the next coarser one (levels 1 ..)
     DevBuf cells, faces, comps;
     int ncells = 0, nfaces = 0, ncomps = 0;
@@ -1607,26 +1865,24 @@ struct tmk_amr {
     int gpar = 0;                    // level-0 group parity the graph was captured at
 };
 
-// offset of cell (i, j) in the ghosted plane of valid box v
-static inline int ghost_off(const tmk_box& v, int i, int j)
+// offset of cell (i, j) in the plane of the island array holding box b
+static inline int cell_off(const tmk_level* L, int b, int i, int j)
 {
-    const int unx = v.hi[0] - v.lo[0] + 1 + 2 * TMK_GHOST;
-    return (i - (v.lo[0] - TMK_GHOST)) + (j - (v.lo[1] - TMK_GHOST)) * unx;
+    const Island& I = L->isl[L->box_isl[b]];
+    return (i - I.abox.lo[0]) + (j - I.abox.lo[1]) * I.unx();
 }
 
-// face offset in a box's flux array of direction d (full or edge layout); -1 if the edge
-// layout does not hold the face
-static inline int flux_off(const tmk_box& v, int edge, int d, int fi, int fj)
+// offset of fault face i in the fault-data row of the island holding box b
+static inline int face_off(const tmk_level* L, int b, int i) { return i - L->isl[L->box_isl[b]].blo0; }
+
+// descriptor index of the island holding box b (the tiles' descriptor: its flux arrays)
+static inline int isl_patch(const tmk_level* L, int b) { return (int)L->boxes.size() + L->box_isl[b]; }
+
+// face offset in the flux array of direction d over bounding box v
+static inline int flux_off(const tmk_box& v, int d, int fi, int fj)
 {
-    const int w = v.hi[0] - v.lo[0] + 1, h = v.hi[1] - v.lo[1] + 1;
-    if (!edge) return (fi - v.lo[0]) + (fj - v.lo[1]) * (w + (d == 0));
-    if (d == 0) {
-        if (fi != v.lo[0] && fi != v.hi[0] + 1) return -1;
-        return (fi != v.lo[0]) + (fj - v.lo[1]) * 2;
-    }
-    (void)h;
-    if (fj != v.lo[1] && fj != v.hi[1] + 1) return -1;
-    return (fi - v.lo[0]) + (fj != v.lo[1]) * w;
+    const int w = v.hi[0] - v.lo[0] + 1;
+    return (fi - v.lo[0]) + (fj - v.lo[1]) * (w + (d == 0));
 }
 
 template <typename X>
@@ -1731,7 +1987,7 @@ static int amr_build_pair(tmk_amr* A, int l)
                 CfCell cc;
                 memset(&cc, 0, sizeof cc);
                 cc.dpatch = (int)b;
-                cc.doff = ghost_off(v, i, j);
+                cc.doff = cell_off(Fl, (int)b, i, j);
                 for (int jj = -1, k = 0; jj <= 1; ++jj)
                     for (int ii = -1; ii <= 1; ++ii, ++k) {
                         const int ci = ic + ii, cj = jc + jj;
@@ -1739,7 +1995,7 @@ static int amr_build_pair(tmk_amr* A, int l)
                         if (in_c(ci, cj)) {
                             const int cb = m.own_c[(size_t)cj * ncx + ci];
                             cc.cpatch[k] = cb;
-                            cc.coff[k] = ghost_off(C->boxes[cb], ci, cj);
+                            cc.coff[k] = cell_off(C, cb, ci, cj);
                         }
                     }
                 if (cc.cpatch[4] < 0) return TMK_E_NESTING;
@@ -1781,12 +2037,12 @@ static int amr_build_pair(tmk_amr* A, int l)
             CfFace ff;
             memset(&ff, 0, sizeof ff);
             ff.dpatch = (int)b;
-            ff.doff = i - (v.lo[0] - TMK_GHOST);
+            ff.doff = face_off(Fl, (int)b, i);
             for (int k = 0; k < 3; ++k) {
                 const int ci = ic - 1 + k;
                 const int cb = ci >= 0 && ci < ncx ? m.own_cb[ci] : -1;
                 ff.cpatch[k] = cb;
-                ff.coff[k] = cb >= 0 ? ci - (C->boxes[cb].lo[0] - TMK_GHOST) : 0;
+                ff.coff[k] = cb >= 0 ? face_off(C, cb, ci) : 0;
             }
             if (ff.cpatch[1] < 0) return TMK_E_NESTING;
             const bool lo = ff.cpatch[0] >= 0, hi = ff.cpatch[2] >= 0;
@@ -1802,6 +2058,7 @@ static int amr_build_pair(tmk_amr* A, int l)
         size_t cell;
         RegFace rf;
         double s;
+        int cbox;
     };
     std::vector<Ent> ents;
     const double inv = -1.0 / C->ctx->hp.dx;
@@ -1811,16 +2068,16 @@ static int amr_build_pair(tmk_amr* A, int l)
         memset(&en, 0, sizeof en);
         en.cell = (size_t)j * ncx + i;
         const int fi = d == 0 ? i + side : i, fj = d == 1 ? j + side : j;
-        en.rf.cpatch = cb;
-        en.rf.cface = flux_off(C->boxes[cb], c.fedge[cb], d, fi, fj);
+        en.rf.cpatch = isl_patch(C, cb);
+        en.rf.cface = flux_off(C->isl[C->box_isl[cb]].bbox, d, fi, fj);
         en.rf.d = d;
-        en.rf.fpatch = fb;
+        en.rf.fpatch = isl_patch(Fl, fb);
         for (int mm = 0; mm < r; ++mm) {
             const int ffi = d == 0 ? fi * r : fi * r + mm, ffj = d == 0 ? fj * r + mm : fj * r;
-            en.rf.fface[mm] = flux_off(Fl->boxes[fb], f.fedge[fb], d, ffi, ffj);
-            if (en.rf.fface[mm] < 0) bad = true;
+            en.rf.fface[mm] = flux_off(Fl->isl[Fl->box_isl[fb]].bbox, d, ffi, ffj);
         }
-        if (c.fedge[cb] || !c.F[d][cb] || !f.F[d][fb] || en.rf.cface < 0) bad = true;
+        if (!c.F[d][C->box_isl[cb]] || !f.F[d][Fl->box_isl[fb]]) bad = true;
+        en.cbox = cb;
         en.s = inv * (side ? 1.0 : -1.0);
         ents.push_back(en);
     });
@@ -1835,8 +2092,8 @@ static int amr_build_pair(tmk_amr* A, int l)
         if (k == 0 || ents[k].cell != ents[k - 1].cell) {
             RefluxCell rc;
             const int i = (int)(ents[k].cell % ncx), j = (int)(ents[k].cell / ncx);
-            rc.cpatch = ents[k].rf.cpatch;
-            rc.coff = ghost_off(C->boxes[rc.cpatch], i, j);
+            rc.cpatch = ents[k].cbox;
+            rc.coff = cell_off(C, rc.cpatch, i, j);
             rc.e0 = (int)k;
             rc.e1 = (int)k + 1;
             rcells.push_back(rc);
@@ -1853,9 +2110,9 @@ static int amr_build_pair(tmk_amr* A, int l)
             for (int ic = v.lo[0] / r; ic <= v.hi[0] / r; ++ic) {
                 AvgCell a;
                 a.cpatch = m.own_c[(size_t)jc * ncx + ic];
-                a.coff = ghost_off(C->boxes[a.cpatch], ic, jc);
+                a.coff = cell_off(C, a.cpatch, ic, jc);
                 a.fpatch = (int)b;
-                a.foff = ghost_off(v, ic * r, jc * r);
+                a.foff = cell_off(Fl, (int)b, ic * r, jc * r);
                 avgc.push_back(a);
             }
         if (!Fl->has_b[b]) continue;
@@ -1863,9 +2120,9 @@ static int amr_build_pair(tmk_amr* A, int l)
             AvgFace a;
             a.cpatch = m.own_cb[ic];
             if (a.cpatch < 0) return TMK_E_NESTING;
-            a.coff = ic - (C->boxes[a.cpatch].lo[0] - TMK_GHOST);
+            a.coff = face_off(C, a.cpatch, ic);
             a.fpatch = (int)b;
-            a.foff = ic * r - (v.lo[0] - TMK_GHOST);
+            a.foff = face_off(Fl, (int)b, ic * r);
             avgf.push_back(a);
         }
     }
@@ -1883,18 +2140,21 @@ static int amr_build_pair(tmk_amr* A, int l)
     return c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * regf.size(), 1));
 }
 
-// flux arrays of every box of every level: full layout for a coarse box holding register
-// cells, box-edge layout for a box of a finer level, none otherwise
+// face fluxes of every island of every level of the hierarchy, on the island's descriptor (the
+// tiles'); the per-face mask limits the stores to the faces the registers read
 template <typename T>
 static void amr_set_flux_ptrs(AmrLevel& a)
 {
     tmk_level* L = a.L;
+    const int nbox = (int)L->boxes.size();
     for (int par = 0; par < 2; ++par) {
         Patch<T>* pts = (Patch<T>*)L->hpatch[par].data();
-        for (size_t b = 0; b < L->boxes.size(); ++b) {
-            pts[b].F[0] = a.F[0][b];
-            pts[b].F[1] = a.F[1][b];
-            pts[b].fedge = a.fedge[b];
+        for (int q = 0; q < L->npatch; ++q) {
+            const int k = q - nbox;
+            const bool isl = k >= 0 && k < (int)a.F[0].size();
+            pts[q].F[0] = isl ? a.F[0][k] : nullptr;
+            pts[q].F[1] = isl ? a.F[1][k] : nullptr;
+            pts[q].fmask = isl ? a.fmask[k] : nullptr;
         }
     }
     L->own_built = false;
@@ -1909,54 +2169,59 @@ static void amr_free(tmk_amr* A)
     for (AmrLevel& a : A->lv) {
         if (!a.L) continue;
         for (int d = 0; d < 2; ++d)
-            for (double*& p : a.F[d]) {
+            for (double* p : a.F[d])
                 if (p) (void)hipFree(p);
-                p = nullptr;
-            }
-        for (int& x : a.fedge) x = 0;
-        if (!a.F[0].empty()) {
-            if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
-            else amr_set_flux_ptrs<float>(a);
-        }
+        for (unsigned char* p : a.fmask)
+            if (p) (void)hipFree(p);
+        a.F[0].clear();
+        a.F[1].clear();
+        a.fmask.clear();
+        if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
+        else amr_set_flux_ptrs<float>(a);
         a.L->amr_stream = nullptr;
     }
 }
 
+// allocate the flux arrays and face masks of every island (mask bit 0: a cell of the level,
+// bit 1: under the next finer level), over the island's bounding box grown by one cell
 static int amr_layouts(tmk_amr* A)
 {
     const int n = (int)A->lv.size();
-    std::vector<std::vector<char>> full(n), need(n);
-    for (int l = 0; l < n; ++l) {
-        full[l].assign(A->lv[l].L->boxes.size(), 0);
-        need[l].assign(A->lv[l].L->boxes.size(), 0);
-    }
-    for (int l = 1; l < n; ++l) {
-        PairMaps m;
-        int e = pair_maps(A->lv[l - 1].L, A->lv[l].L, A->r, m);
-        if (e) return e;
-        for_each_reg_face(m, [&](int, int, int, int, int cb, int fb) {
-            full[l - 1][cb] = 1;
-            need[l][fb] = 1;
-        });
-    }
     for (int l = 0; l < n; ++l) {
         AmrLevel& a = A->lv[l];
-        const size_t nb = a.L->boxes.size();
-        a.F[0].assign(nb, nullptr);
-        a.F[1].assign(nb, nullptr);
-        a.fedge.assign(nb, 0);
-        for (size_t b = 0; b < nb; ++b) {
-            if (!full[l][b] && !need[l][b]) continue;
-            const tmk_box v = a.L->boxes[b];
-            const size_t w = v.hi[0] - v.lo[0] + 1, h = v.hi[1] - v.lo[1] + 1;
-            const int edge = full[l][b] ? 0 : 1;
-            const size_t n0 = edge ? 2 * h : (w + 1) * h, n1 = edge ? 2 * w : w * (h + 1);
+        const tmk_level* L = a.L;
+        const size_t ni = L->isl.size();
+        a.F[0].assign(ni, nullptr);
+        a.F[1].assign(ni, nullptr);
+        a.fmask.assign(ni, nullptr);
+        std::vector<tmk_box> finer;
+        if (l + 1 < n)
+            for (const tmk_box& b : A->lv[l + 1].L->boxes)
+                finer.push_back(tmk_box{{b.lo[0] / A->r, b.lo[1] / A->r}, {b.hi[0] / A->r, b.hi[1] / A->r}});
+        for (size_t k = 0; k < ni; ++k) {
+            const Island& I = L->isl[k];
+            const size_t w = I.bbox.hi[0] - I.bbox.lo[0] + 1, h = I.bbox.hi[1] - I.bbox.lo[1] + 1;
             for (int d = 0; d < 2; ++d) {
                 void* p = nullptr;
-                if (hipMalloc(&p, sizeof(double) * TMK_NCOMP * (d == 0 ? n0 : n1)) != hipSuccess) return TMK_E_NOMEM;
-                a.F[d][b] = (double*)p;
+                const size_t nf = d == 0 ? (w + 1) * h : w * (h + 1);
+                if (hipMalloc(&p, sizeof(double) * TMK_NCOMP * nf) != hipSuccess) return TMK_E_NOMEM;
+                a.F[d][k] = (double*)p;
             }
-            a.fedge[b] = edge;
+            const size_t mw = w + 2, mh = h + 2;
+            std::vector<unsigned char> m(mw * mh, 0);
+            auto mark = [&](const tmk_box& b, unsigned char bit) {
+                const int i0 = std::max(b.lo[0], I.bbox.lo[0] - 1), i1 = std::min(b.hi[0], I.bbox.hi[0] + 1);
+                const int j0 = std::max(b.lo[1], I.bbox.lo[1] - 1), j1 = std::min(b.hi[1], I.bbox.hi[1] + 1);
+                for (int j = j0; j <= j1; ++j)
+                    for (int i = i0; i <= i1; ++i)
+                        m[(size_t)(i - I.bbox.lo[0] + 1) + (size_t)(j - I.bbox.lo[1] + 1) * mw] |= bit;
+            };
+            for (const tmk_box& b : L->boxes) mark(b, 1);
+            for (const tmk_box& b : finer) mark(b, 2);
+            void* p = nullptr;
+            if (hipMalloc(&p, m.size()) != hipSuccess) return TMK_E_NOMEM;
+            a.fmask[k] = (unsigned char*)p;
+            HIPCHK(hipMemcpy(p, m.data(), m.size(), hipMemcpyHostToDevice));
         }
     }
     return TMK_OK;

@@ -43,7 +43,10 @@ struct Patch {
     double* F[2];      // optional face fluxes (22 comps, on surroundingNodes(box,d)) or NULL
     int vlo0, vlo1, vhi0, vhi1; // valid box
     int scen;          // index of the scenario (parameters, time, step) this box belongs to
-    int fedge;         // 1: F holds only the box-edge faces (lo and hi face rows of each direction)
+    // with F: per-cell flags over the valid box grown by 1 (bit 0: a cell of the level, bit 1:
+    // covered by the next finer level); NULL keeps every face, else only the faces between a
+    // cell of the level and one outside it, or between covered and uncovered cells of the level
+    const unsigned char* fmask;
 };
 
 struct StateVec {
@@ -157,22 +160,16 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
                                                 const T (&sv)[NS], T r, int i, int j, int sd)
 {
     const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
-    size_t fp, fi;
-    if (pt.fedge) {
-        // edge layout: x faces (2, fny) with column 0 = lo face, y faces (2, fnx) with row 0 = lo face
-        if (d == 0) {
-            if (i != pt.vlo0 && i != pt.vhi0 + 1) return;
-            fp = (size_t)2 * fny;
-            fi = (size_t)(i != pt.vlo0) + (size_t)(j - pt.vlo1) * 2;
-        } else {
-            if (j != pt.vlo1 && j != pt.vhi1 + 1) return;
-            fp = (size_t)2 * fnx;
-            fi = (size_t)(i - pt.vlo0) + (size_t)(j != pt.vlo1) * fnx;
-        }
-    } else {
-        fp = (size_t)fnx * fny;
-        fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+    if (pt.fmask) {
+        const int mw = pt.vhi0 - pt.vlo0 + 3;
+        const unsigned char* mk = pt.fmask + (size_t)(i - pt.vlo0 + 1) + (size_t)(j - pt.vlo1 + 1) * mw;
+        const unsigned a = d == 0 ? mk[-1] : mk[-mw], b = *mk;
+        const bool edge = ((a ^ b) & 1) != 0;                 // level boundary (fine side)
+        const bool cf = (a & b & 1) && ((a ^ b) & 2);        // around the finer level (coarse side)
+        if (!edge && !cf) return;
     }
+    const size_t fp = (size_t)fnx * fny;
+    const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
     T f[7];
     face_flux_side<T, NS>(m, v, sv, r, f);
     double* Fo = pt.F[d];

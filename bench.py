@@ -1,8 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark: cell-updates/s of the level step on one MI355X.
 
-Workload (the plain FP64 drop-in configuration): one level of 4096 x 2048 cells,
-dx = 25 m, regularized rate-and-state fault with aging-law state evolution and
+Workload (the plain FP64 drop-in configuration): one level of 4096 x 2048 cells cut
+into 8192 boxes of 32 x 32 (the reference's default max_grid_size; --max-grid-size 0
+runs it as one box), dx = 25 m, regularized rate-and-state fault with aging-law state
+evolution and
 seeded per-face a, L and normal stress, PPM normal predictor with primitive
 limiting, absorbing outer boundaries.  A cell-update is one valid cell advanced
 one level step; ghosts are not counted.  Inputs are resident in HBM before the
@@ -42,6 +44,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--tile", type=str, default="")
+    ap.add_argument("--max-grid-size", type=int, default=32,
+                    help="boxes of at most this many cells per side (the reference's default max_grid_size "
+                         "is 32); 0 = the level as one box")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     return ap.parse_args()
@@ -61,13 +66,19 @@ def measured_traffic(nx, ny, tile):
     return t.get("bytes_per_launch")
 
 
-def run_level(cfg, U, B, precision, warmup, steps, tile):
+def level_boxes(cfg, max_grid_size):
+    if max_grid_size <= 0:
+        return [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)]
+    return scenarios.split_boxes(cfg["nx"], cfg["ny"], max_grid_size)
+
+
+def run_level(cfg, U, B, precision, warmup, steps, tile, boxes):
     ctx = capi.Context(cfg, precision)
     if tile:
         ctx.set_tile(*tile)
-    box = (0, 0, cfg["nx"] - 1, cfg["ny"] - 1)
-    lev = capi.Level(ctx, [box])
-    lev.upload(0, U, B)
+    lev = capi.Level(ctx, boxes)
+    for k, b in enumerate(boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
     dt = scenarios.dt_of(cfg)
     t = 0.0
     lev.run(warmup, t, dt)
@@ -76,8 +87,16 @@ def run_level(cfg, U, B, precision, warmup, steps, tile):
     ms = lev.run_timed(steps, t, dt)
     lev.sync()
     t += steps * dt
-    Uo, Bo = np.empty_like(U), np.empty_like(B)
-    lev.download(0, Uo, Bo)
+    Uo, Bo = np.full_like(U, np.nan), np.full_like(B, np.nan)
+    g = scenarios.GHOST
+    for k, b in enumerate(boxes):
+        x0, y0, x1, y1 = b
+        ub = np.empty((U.shape[0], y1 - y0 + 1 + 2 * g, x1 - x0 + 1 + 2 * g))
+        bb = np.empty((B.shape[0], x1 - x0 + 1 + 2 * g)) if y0 == 0 else None
+        lev.download(k, ub, bb)
+        Uo[:, y0 + g:y1 + g + 1, x0 + g:x1 + g + 1] = ub[:, g:-g, g:-g]
+        if bb is not None:
+            Bo[:, x0 + g:x1 + g + 1] = bb[:, g:-g]
     # per-launch durations of the update kernels (eager steps; interior tiles on the level's
     # stream, boundary tiles concurrently on its side stream), each from HIP events on the
     # stream it runs on; continues the same simulation
@@ -173,7 +192,8 @@ def main():
     ncell = cfg["nx"] * cfg["ny"]
     print("bench: %dx%d FP64, warmup %d, steps %d" % (cfg["nx"], cfg["ny"], a.warmup, a.steps),
           file=sys.stderr, flush=True)
-    ms64, kms64, U64, B64 = run_level(cfg, U0, B0, capi.PREC_FP64, a.warmup, a.steps, tile)
+    boxes = level_boxes(cfg, a.max_grid_size)
+    ms64, kms64, U64, B64 = run_level(cfg, U0, B0, capi.PREC_FP64, a.warmup, a.steps, tile, boxes)
     value = ncell * a.steps / (ms64 * 1e-3)
     # dominant kernel: the interior-tile update; its algorithmic bytes are the live state planes
     # of the cells it updates, read once and written once
@@ -192,9 +212,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded per-face friction parameters, background prestress)",
-        "config": {"workload": "rate-and-state, aging law, single level %dx%d, dx=25 m, PPM, "
-                               "absorbing boundaries, FP64" % (cfg["nx"], cfg["ny"]),
-                   "nx": cfg["nx"], "ny": cfg["ny"], "dx_m": 25.0, "dt_s": scenarios.dt_of(cfg),
+        "config": {"workload": "rate-and-state, aging law, single level %dx%d in %d boxes, dx=25 m, PPM, "
+                               "absorbing boundaries, FP64" % (cfg["nx"], cfg["ny"], len(boxes)),
+                   "nx": cfg["nx"], "ny": cfg["ny"], "boxes": len(boxes), "max_grid_size": a.max_grid_size,
+                   "dx_m": 25.0, "dt_s": scenarios.dt_of(cfg),
                    "normal_predictor": "PPM", "tile": list(tile) if tile else "default"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["nx"], cfg["ny"], tile),
@@ -210,7 +231,7 @@ def main():
     }
     if not a.no_fp32:
         print("bench: FP32-bulk leg", file=sys.stderr, flush=True)
-        ms32, kms32, U32, B32 = run_level(cfg, U0, B0, capi.PREC_FP32_BULK, a.warmup, a.steps, tile)
+        ms32, kms32, U32, B32 = run_level(cfg, U0, B0, capi.PREC_FP32_BULK, a.warmup, a.steps, tile, boxes)
         g = scenarios.GHOST
         slip64, slip32 = B64[8, g:-g], B32[8, g:-g]
         V64, V32 = B64[2, g:-g], B32[2, g:-g]

@@ -250,6 +250,36 @@ int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double 
 /* waits; the first error any level reported (0 = none), cleared */
 int tmk_amr_sync(tmk_amr *amr);
 
+/* Regrid, step 1: boxes of the next finer level from the tags of a level (Chombo's
+ * MeshRefine::regrid with BRMeshRefine, as AMR::regrid drives it; deck parameters
+ * fill_ratio, block_factor, max_grid_size, ref_ratio, amrGodunov.cpp:263-320).  Tags are
+ * nx x ny bytes over the level's domain (tmk_level_tag).  Blocks of block_factor/ref_ratio
+ * cells of the level are clustered with the Berger-Rigoutsos method, keeping every box
+ * inside the level's proper nesting domain (its boxes shrunk by nest_buffer cells, the
+ * domain edge excepted; ncoarse = 0: the whole domain); the boxes are returned in the
+ * finer level's cells, multiples of block_factor, at most max_grid_size per side.
+ * TMK_E_OVERFLOW with *nout = the count when more than max_out boxes result.  Host only.
+ * Chombo's BRMeshRefine is not available here: layouts are this library's own. */
+typedef struct {
+    double fill_ratio;   /* accepted fraction of tagged cells in a box, (0, 1] */
+    int block_factor;    /* finer-level boxes are multiples of this, a multiple of ref_ratio */
+    int max_grid_size;   /* finer-level boxes have at most this many cells per side */
+    int ref_ratio;
+    int nest_buffer;     /* cells of this level around a finer box that must belong to it;
+                            <= 0: the coarse-fine fill's reach, ceil(4 / ref_ratio) + 1 */
+} tmk_meshparams;
+int tmk_make_boxes(int nx, int ny, const unsigned char *tags, int ncoarse, const tmk_box *coarse_boxes,
+                   const tmk_meshparams *mp, int max_out, tmk_box *out, int *nout);
+
+/* Regrid, step 2: the data of a new layout of a level (AMRLevelLinElast::regrid,
+ * AMRLevelLinElast.cpp:784-951).  The new level's cells take the old level's current data
+ * where the old layout covers them and otherwise are interpolated from the coarser level's
+ * current data: state as Chombo's FineInterp (piecewise linear, central slopes inside the
+ * domain and one-sided at its edges, multidimensionally limited off the domain's edge
+ * layer), fault data as D1FineInterp (D1FineInterp.cpp:61-267).  old may be NULL.  The new
+ * level's other cells keep what tmk_level_fill/upload put there. */
+int tmk_level_regrid(tmk_level *fresh, tmk_level *old, tmk_level *coarse, int ref_ratio);
+
 /* Workgroup tile shape of the update kernels (autotuner hook): bx x by threads
  * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 32x8, 32x12, 32x16,
  * 32x32, 64x4, 64x8, 64x16.  0, 0 = the library default for the context's precision. */

@@ -219,6 +219,85 @@ def fill_fault(fine, coarse, alpha):
             fine.B[:, i_f + G] = c0 + (-0.5 + (i_f - r * ic + 0.5) / r) * s
 
 
+def regrid_level(new, old, coarse):
+    """Data of a new layout of a level (AMRLevelLinElast::regrid, AMRLevelLinElast.cpp:784-951):
+    the new level's arrays already hold the initial fill; every new cell takes the old layout's
+    value where the old layout covers it, else the coarse level's current data interpolated
+    as Chombo's FineInterp does (piecewise constant, then per direction a central slope inside
+    the domain, a one-sided slope on its edge layer -- high side at the lo edge, low side at
+    the hi edge -- limited by the 3x3 coarse neighbourhood off the edge layer, added with
+    coefficient -1/2 + (offset + 1/2)/r).  Fault data: the same in x with D1FineInterp
+    (D1FineInterp.cpp:159-265; its limiter box keeps the faces off the x edges)."""
+    r = new.r
+    ncx, ncy = coarse.nx, coarse.ny
+    C = coarse.U
+
+    def cval(i, j):
+        assert coarse.valid[j, i], "new level not properly nested at %r" % ((i, j),)
+        return C[:, j + G, i + G]
+
+    for b in new.boxes:
+        for jf in range(b[1], b[3] + 1):
+            for i_f in range(b[0], b[2] + 1):
+                if old is not None and old.valid[jf, i_f]:
+                    new.U[:, jf + G, i_f + G] = old.U[:, jf + G, i_f + G]
+                    continue
+                ic, jc = i_f // r, jf // r
+                c0 = cval(ic, jc)
+                s = []
+                for d, (k, n) in enumerate(((ic, ncx), (jc, ncy))):
+                    e = (1, 0) if d == 0 else (0, 1)
+                    if k == 0:
+                        s.append(cval(ic + e[0], jc + e[1]) - c0)
+                    elif k == n - 1:
+                        s.append(c0 - cval(ic - e[0], jc - e[1]))
+                    else:
+                        s.append(0.5 * (cval(ic + e[0], jc + e[1]) - cval(ic - e[0], jc - e[1])))
+                if 1 <= ic <= ncx - 2 and 1 <= jc <= ncy - 2:
+                    smax, smin = c0.copy(), c0.copy()
+                    for jj in (-1, 0, 1):
+                        for ii in (-1, 0, 1):
+                            v = cval(ic + ii, jc + jj)
+                            smax = np.maximum(smax, v)
+                            smin = np.minimum(smin, v)
+                    dsum = 0.5 * (np.abs(s[0]) + np.abs(s[1]))
+                    pos = dsum > 0.0
+                    eta = np.minimum(np.minimum(smax - c0, c0 - smin) / np.where(pos, dsum, 1.0), 1.0)
+                    s = [np.where(pos, eta * s[0], s[0]), np.where(pos, eta * s[1], s[1])]
+                v = c0 + (-0.5 + (i_f - r * ic + 0.5) / r) * s[0]
+                v = v + (-0.5 + (jf - r * jc + 0.5) / r) * s[1]
+                new.U[:, jf + G, i_f + G] = v
+    CB = coarse.B
+
+    def bval(i):
+        assert coarse.fvalid[i], "new fault data not nested at %r" % (i,)
+        return CB[:, i + G]
+
+    for b in new.boxes:
+        if b[1] != 0:
+            continue
+        for i_f in range(b[0], b[2] + 1):
+            if old is not None and old.fvalid[i_f]:
+                new.B[:, i_f + G] = old.B[:, i_f + G]
+                continue
+            ic = i_f // r
+            c0 = bval(ic)
+            if ic == 0:
+                sl = bval(ic + 1) - c0
+            elif ic == ncx - 1:
+                sl = c0 - bval(ic - 1)
+            else:
+                sl = 0.5 * (bval(ic + 1) - bval(ic - 1))
+                lo, hi = bval(ic - 1), bval(ic + 1)
+                smax = np.maximum(np.maximum(lo, c0), hi)
+                smin = np.minimum(np.minimum(lo, c0), hi)
+                dsum = 0.5 * np.abs(sl)
+                pos = dsum > 0.0
+                eta = np.minimum(np.minimum(smax - c0, c0 - smin) / np.where(pos, dsum, 1.0), 1.0)
+                sl = np.where(pos, eta * sl, sl)
+            new.B[:, i_f + G] = c0 + (-0.5 + (i_f - r * ic + 0.5) / r) * sl
+
+
 class Register:
     """Flux register between a coarse level and its finer level (restated LevelFluxRegister)."""
 
@@ -411,6 +490,22 @@ class Hierarchy:
             for _ in range(self.r):
                 self._time_step(l + 1)
         self._post(l)
+
+    def set_layout(self, l, boxes, init_U, init_B):
+        """A new layout of level l >= 1: arrays from init_U/init_B (the initial fill), data by
+        regrid_level from the old layout and level l-1; every level's clock restarts at the
+        coarse level's time (as a new device hierarchy does)."""
+        old = self.levels[l]
+        new = Level(old.cfg, boxes, self.r)
+        new.U[:] = init_U
+        new.B[:] = init_B
+        regrid_level(new, old, self.levels[l - 1])
+        new.dt = old.dt
+        self.levels[l] = new
+        t = self.levels[0].time
+        for lev in self.levels:
+            lev.time = t
+        self.regs = [Register(self.levels[k], self.levels[k + 1]) for k in range(len(self.levels) - 1)]
 
     def advance(self, nsteps=1):
         for _ in range(nsteps):

@@ -184,3 +184,73 @@ def test_levels_of_a_hierarchy_refuse_direct_steps():
     H.run(1)
     H.sync()
     H.close()
+
+
+def _levels_equal(h, H, exact=True):
+    for l, lev in enumerate(h.levels):
+        Ug, Bg = H.download(l)
+        mask = lev.valid
+        Ur, Ugv = lev.U[:, G:-G, G:-G][:, mask], Ug[:, G:-G, G:-G][:, mask]
+        Br, Bgv = lev.B[:, G:-G][:, lev.fvalid], Bg[:, G:-G][:, lev.fvalid]
+        if exact:
+            assert np.array_equal(Ugv, Ur), (l, np.nanmax(np.abs(Ugv - Ur)))
+            assert np.array_equal(Bgv, Br), l
+        else:
+            for c in range(scenarios.NCOMP):
+                assert np.max(np.abs(Ugv[c] - Ur[c])) <= 1e-12 * max(np.max(np.abs(Ur[c])), 1e-300), (l, c)
+
+
+def _pair(cfg, boxes, r, arrays):
+    h = oamr.Hierarchy(cfg, boxes, r)
+    h.init_state(lambda l: arrays(l)[0], lambda l: arrays(l)[1])
+    h.set_dt(scenarios.dt_of(cfg))
+    H = amr.Hierarchy(cfg, boxes, r, arrays=arrays)
+    return h, H
+
+
+def test_new_layout_of_a_level_matches_oracle():
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    cfgs = _cfgs(cfg, 3, 2)
+    arrays = _background_arrays(cfgs)
+    h, H = _pair(cfg, BOXES_3, 2, arrays)
+    assert h.advance(3) == 0
+    H.run(3)
+    H.sync()
+    # level 1 moves: part of it stays, part is new (interpolated from level 0); level 2 stays nested
+    new1 = [(16, 0, 47, 15), (48, 0, 79, 31)]
+    h.set_layout(1, new1, *scenarios.level_arrays(cfgs[1]))
+    H.set_layout(1, new1)
+    _levels_equal(h, H)
+    new2 = [(80, 0, 143, 23)]
+    h.set_layout(2, new2, *scenarios.level_arrays(cfgs[2]))
+    H.set_layout(2, new2)
+    _levels_equal(h, H)
+    assert h.advance(3) == 0
+    H.run(3)
+    H.sync()
+    _levels_equal(h, H)
+    H.close()
+
+
+def test_regrid_from_device_tags_matches_oracle():
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    cfgs = _cfgs(cfg, 3, 2)
+    arrays = _background_arrays(cfgs)
+    h, H = _pair(cfg, BOXES_3, 2, arrays)
+    assert h.advance(4) == 0
+    H.run(4)
+    H.sync()
+    counts = H.regrid(lambda l: capi.TagParams.make(refine_thresh=1e-4, tag_buffer=2, level=l),
+                      fill_ratio=0.7, block_factor=4, max_grid_size=32)
+    assert counts[1] > 0 and counts[2] > 0
+    for l in (1, 2):
+        h.set_layout(l, H.boxes[l], *scenarios.level_arrays(cfgs[l]))
+    _levels_equal(h, H)
+    # the nucleation patch is refined
+    xc = cfg["nx"] // 2 * 4
+    assert any(b[0] <= xc <= b[2] and b[1] == 0 for b in H.boxes[2])
+    assert h.advance(2) == 0
+    H.run(2)
+    H.sync()
+    _levels_equal(h, H)
+    H.close()

@@ -4,8 +4,10 @@ subcycling (tmk_amr_*).
 The host mirror of Chombo's AMR driver over AMRLevelLinElast for a given box layout
 per level (amrGodunov.cpp:443-461 set-up; AMRLevelLinElast.cpp:372-516 advance and
 postTimeStep).  Level l has its own context (dx / r^l, per-face arrays injected from
-level 0) and steps dt0 / r^l.  Box generation from tags is not part of this module:
-the caller gives the boxes of every level.
+level 0) and steps dt0 / r^l.  The caller gives the initial boxes of every level;
+regrid() replaces the finer levels' layouts from the device tags of each level
+(tmk_level_tag, tmk_make_boxes, tmk_level_regrid; AMR::regrid and
+AMRLevelLinElast::regrid, AMRLevelLinElast.cpp:784-951).
 """
 import numpy as np
 
@@ -28,21 +30,57 @@ class Hierarchy:
             ctx = capi.Context(cfg, precision)
             if tile:
                 ctx.set_tile(*tile)
-            lev = capi.Level(ctx, self.boxes[l])
-            if arrays is None:
-                Bf = scenarios.fault_arrays(cfg)
-                for k, b in enumerate(self.boxes[l]):
-                    lev.fill(k, scenarios.background_vector(cfg), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
-            else:
-                U, Bf = arrays(l)
-                for k, b in enumerate(self.boxes[l]):
-                    lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
             self.ctxs.append(ctx)
-            self.levels.append(lev)
+            self.levels.append(self._new_level(l, self.boxes[l], arrays))
         self.amr = capi.Amr(self.levels, r)
         self.dt0 = scenarios.dt_of(cfg0)
         self.time = 0.0
         self.steps = 0
+
+    def _new_level(self, l, boxes, arrays=None):
+        cfg = self.cfgs[l]
+        lev = capi.Level(self.ctxs[l], boxes)
+        if arrays is None:
+            Bf = scenarios.fault_arrays(cfg)
+            for k, b in enumerate(boxes):
+                lev.fill(k, scenarios.background_vector(cfg), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
+        else:
+            U, Bf = arrays(l)
+            for k, b in enumerate(boxes):
+                lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
+        return lev
+
+    def set_layout(self, l, boxes):
+        """Give level l >= 1 a new box layout (boxes in its cells): its data come from the old
+        layout where it covers them and from level l-1 elsewhere (AMRLevelLinElast::regrid)."""
+        self.amr.close()
+        new = self._new_level(l, [tuple(b) for b in boxes])
+        capi.level_regrid(new, self.levels[l], self.levels[l - 1], self.r)
+        self.levels[l].close()
+        self.levels[l] = new
+        self.boxes[l] = [tuple(b) for b in boxes]
+        self.amr = capi.Amr(self.levels, self.r)
+
+    def regrid(self, tag_params, fill_ratio=0.75, block_factor=8, max_grid_size=64):
+        """AMR::regrid from level 0: for l = 0, 1, ... the tags of level l (tag_params(l) ->
+        capi.TagParams) give the boxes of level l+1 (Berger-Rigoutsos, properly nested in
+        level l), whose data come from its old layout and from level l.  Returns the box
+        counts per level; a level left without tags keeps its old layout."""
+        self.amr.close()
+        for l in range(len(self.levels) - 1):
+            cfg = self.cfgs[l]
+            tags = self.levels[l].tag(tag_params(l), cfg["nx"], cfg["ny"])
+            boxes = capi.make_boxes(tags, self.boxes[l] if l > 0 else None, ref_ratio=self.r,
+                                    block_factor=block_factor, max_grid_size=max_grid_size, fill_ratio=fill_ratio)
+            if not boxes:
+                continue
+            new = self._new_level(l + 1, boxes)
+            capi.level_regrid(new, self.levels[l + 1], self.levels[l], self.r)
+            self.levels[l + 1].close()
+            self.levels[l + 1] = new
+            self.boxes[l + 1] = boxes
+        self.amr = capi.Amr(self.levels, self.r)
+        return [len(b) for b in self.boxes]
 
     def run(self, nsteps):
         self.amr.run(nsteps, self.time, self.dt0)

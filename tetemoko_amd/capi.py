@@ -42,6 +42,7 @@ EXPORTED = (
     "tmk_level_profile_kernels", "tmk_batch_member_error",
     "tmk_stations_create", "tmk_stations_read", "tmk_stations_destroy",
     "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed", "tmk_amr_sync",
+    "tmk_make_boxes", "tmk_level_regrid",
 )
 
 
@@ -49,6 +50,11 @@ class Profile(ctypes.Structure):
     _fields_ = [("ms_step", ctypes.c_double), ("ms_interior", ctypes.c_double), ("ms_boundary", ctypes.c_double),
                 ("cells_interior", ctypes.c_longlong), ("cells_boundary", ctypes.c_longlong),
                 ("tiles_interior", ctypes.c_int), ("tiles_boundary", ctypes.c_int)]
+
+
+class MeshParams(ctypes.Structure):
+    _fields_ = [("fill_ratio", ctypes.c_double), ("block_factor", ctypes.c_int), ("max_grid_size", ctypes.c_int),
+                ("ref_ratio", ctypes.c_int), ("nest_buffer", ctypes.c_int)]
 
 
 class Box(ctypes.Structure):
@@ -178,6 +184,9 @@ def lib():
         L.tmk_amr_run.argtypes = [vp, c_int, _D, _D]
         L.tmk_amr_run_timed.argtypes = [vp, c_int, _D, _D, _PD]
         L.tmk_amr_sync.argtypes = [vp]
+        L.tmk_make_boxes.argtypes = [c_int, c_int, ctypes.POINTER(ctypes.c_ubyte), c_int, PBox,
+                                     ctypes.POINTER(MeshParams), c_int, PBox, ctypes.POINTER(c_int)]
+        L.tmk_level_regrid.argtypes = [vp, vp, vp, c_int]
         _lib = L
     return _lib
 
@@ -408,6 +417,34 @@ class Amr:
 
     def sync(self):
         _check(lib().tmk_amr_sync(self.h), "tmk_amr_sync")
+
+
+def make_boxes(tags, coarse_boxes=None, ref_ratio=4, block_factor=8, max_grid_size=64, fill_ratio=0.75,
+               nest_buffer=0):
+    """Boxes of the next finer level (finer-level cells) from a level's tags (ny, nx) uint8."""
+    tags = np.ascontiguousarray(tags, dtype=np.uint8)
+    ny, nx = tags.shape
+    mp = MeshParams(fill_ratio, block_factor, max_grid_size, ref_ratio, nest_buffer)
+    cb = coarse_boxes or []
+    carr = (Box * max(len(cb), 1))(*[Box.make(*b) for b in cb])
+    cap = 1024
+    while True:
+        out = (Box * cap)()
+        n = ctypes.c_int()
+        code = lib().tmk_make_boxes(nx, ny, tags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), len(cb), carr,
+                                    ctypes.byref(mp), cap, out, ctypes.byref(n))
+        if code == E_OVERFLOW:
+            cap = n.value
+            continue
+        _check(code, "tmk_make_boxes")
+        return [out[k].tup() for k in range(n.value)]
+
+
+def level_regrid(fresh, old, coarse, ref_ratio):
+    """The new layout's data: the old layout's cells where it covers them, else interpolated
+    from the coarser level (tmk_level_regrid)."""
+    _check(lib().tmk_level_regrid(fresh.h, old.h if old is not None else None, coarse.h, ref_ratio),
+           "tmk_level_regrid")
 
 
 def mem_info():

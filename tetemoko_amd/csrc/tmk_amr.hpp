@@ -38,7 +38,7 @@ struct CfFace {
     int dpatch, doff;        // fine box, offset in its fault-data row
     int cpatch[3], coff[3];  // coarse faces ic-1, ic, ic+1 (-1: not read)
     int kind;                // 0 central, 1 one-sided low, 2 one-sided high
-    int pad;
+    int limit;               // 1: limit the slope against the three faces (D1FineInterp)
     double coef;
 };
 
@@ -97,17 +97,20 @@ __device__ __forceinline__ T time_interp(T o, T n, double alpha)
 }
 
 // comps: the planes to fill (the stencil planes; the others are never read in ghosts)
+// alpha_fixed >= 0: that weight (regrid: 1, the coarse level's current data) instead of the
+// clocks'
 template <typename T>
 __global__ void k_amr_fill_state(const CfCell* __restrict__ cells, int ncells, const int* __restrict__ comps,
                                  int ncomp, const Patch<T>* __restrict__ fine, const Patch<T>* __restrict__ cold,
-                                 const Patch<T>* __restrict__ cnew, const Scen* fscen, const Scen* cscen, int r)
+                                 const Patch<T>* __restrict__ cnew, const Scen* fscen, const Scen* cscen, int r,
+                                 double alpha_fixed)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)ncells * ncomp) return;
     const int e = (int)(t / ncomp), c = comps[t % ncomp];
     const CfCell cc = cells[e];
-    double alpha;
-    if (!amr_alpha(*fscen, *cscen, r, alpha)) {
+    double alpha = alpha_fixed;
+    if (alpha_fixed < 0.0 && !amr_alpha(*fscen, *cscen, r, alpha)) {
         if (t == 0) report(&((Scen*)fscen)->err, TMK_E_AMR_TIME);
         return;
     }
@@ -148,14 +151,14 @@ __global__ void k_amr_fill_state(const CfCell* __restrict__ cells, int ncells, c
 template <typename T>
 __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, int nb, const Patch<T>* __restrict__ fine,
                                  const Patch<T>* __restrict__ cold, const Patch<T>* __restrict__ cnew,
-                                 const Scen* fscen, const Scen* cscen, int r)
+                                 const Scen* fscen, const Scen* cscen, int r, double alpha_fixed)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfaces * nb) return;
     const int e = t / nb, c = t % nb;
     const CfFace f = faces[e];
-    double alpha;
-    if (!amr_alpha(*fscen, *cscen, r, alpha)) return;
+    double alpha = alpha_fixed;
+    if (alpha_fixed < 0.0 && !amr_alpha(*fscen, *cscen, r, alpha)) return;
     double v[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -167,7 +170,12 @@ __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, i
                                        alpha);
         }
     }
-    const double s = f.kind == 0 ? 0.5 * (v[2] - v[0]) : f.kind == 1 ? v[1] - v[0] : v[2] - v[1];
+    double s = f.kind == 0 ? 0.5 * (v[2] - v[0]) : f.kind == 1 ? v[1] - v[0] : v[2] - v[1];
+    if (f.limit) {
+        const double smax = fmax(fmax(v[0], v[1]), v[2]), smin = fmin(fmin(v[0], v[1]), v[2]);
+        const double dsum = 0.5 * fabs(s);
+        if (dsum > 0.0) s = fmin(fmin(smax - v[1], v[1] - smin) / dsum, 1.0) * s;
+    }
     const Patch<T>& pd = fine[f.dpatch];
     ((double*)pd.bin)[f.doff + (size_t)c * pd.bnx] = v[1] + f.coef * s;
 }

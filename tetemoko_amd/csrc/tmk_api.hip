@@ -2311,13 +2311,13 @@ static int amr_advance_t(tmk_amr* A, int l)
         if (a.ncells > 0) {
             hipLaunchKernelGGL(k_amr_fill_state<T>, amr_grid((long long)a.ncells * a.ncomps), dim3(256), 0, st,
                                (const CfCell*)a.cells.p, a.ncells, (const int*)a.comps.p, a.ncomps, fine, cold, cnew,
-                               G.scen_dev, GC.scen_dev, A->r);
+                               G.scen_dev, GC.scen_dev, A->r, -1.0);
             HIPCHK(hipGetLastError());
         }
         if (a.nfaces > 0) {
             hipLaunchKernelGGL(k_amr_fill_fault<T>, amr_grid((long long)a.nfaces * G.nb), dim3(256), 0, st,
                                (const CfFace*)a.faces.p, a.nfaces, G.nb, fine, cold, cnew, G.scen_dev, GC.scen_dev,
-                               A->r);
+                               A->r, -1.0);
             HIPCHK(hipGetLastError());
         }
     }
@@ -2484,4 +2484,336 @@ extern "C" int tmk_amr_sync(tmk_amr* A)
         }
     }
     return first;
+}
+
+// ---------------------------------------------------------------------------
+// regrid: boxes of the next finer level from tags, and the data of a new level
+// ---------------------------------------------------------------------------
+
+// Berger-Rigoutsos clustering (the method of Chombo's BRMeshRefine, absent here): the
+// tags' bounding box is accepted when the tags fill at least fill_ratio of it; otherwise
+// it is cut at the hole of the signatures nearest its middle, else at the strongest sign
+// change of the signatures' second difference, else in half along its longer side.
+struct TagField {
+    int nx = 0, ny = 0;
+    std::vector<unsigned char> t;
+    bool at(int i, int j) const { return t[(size_t)j * nx + i] != 0; }
+};
+
+static void br_cluster(const TagField& g, tmk_box b, double fill, std::vector<tmk_box>& out)
+{
+    long long cnt = 0;
+    tmk_box tb{{b.hi[0] + 1, b.hi[1] + 1}, {b.lo[0] - 1, b.lo[1] - 1}};
+    for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+            if (g.at(i, j)) {
+                ++cnt;
+                tb.lo[0] = std::min(tb.lo[0], i); tb.lo[1] = std::min(tb.lo[1], j);
+                tb.hi[0] = std::max(tb.hi[0], i); tb.hi[1] = std::max(tb.hi[1], j);
+            }
+    if (cnt == 0) return;
+    b = tb;
+    const int w = b.hi[0] - b.lo[0] + 1, h = b.hi[1] - b.lo[1] + 1;
+    if ((double)cnt >= fill * (double)w * h) {
+        out.push_back(b);
+        return;
+    }
+    std::vector<long long> sig[2] = {std::vector<long long>(w, 0), std::vector<long long>(h, 0)};
+    for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+            if (g.at(i, j)) { ++sig[0][i - b.lo[0]]; ++sig[1][j - b.lo[1]]; }
+    // a cut at k in direction d splits [lo, lo + k - 1] and [lo + k, hi]
+    int best_d = -1, best_k = 0;
+    double best_dist = 1e300;
+    for (int d = 0; d < 2; ++d) {
+        const int n = d == 0 ? w : h;
+        for (int k = 1; k < n - 1; ++k)
+            if (sig[d][k] == 0) {
+                const double dist = fabs((k + 0.5) - 0.5 * n);
+                if (dist < best_dist) { best_dist = dist; best_d = d; best_k = k; }
+            }
+    }
+    if (best_d >= 0) {
+        // split around the hole: the hole row itself holds no tags
+        tmk_box a = b, c = b;
+        a.hi[best_d] = b.lo[best_d] + best_k - 1;
+        c.lo[best_d] = b.lo[best_d] + best_k + 1;
+        br_cluster(g, a, fill, out);
+        br_cluster(g, c, fill, out);
+        return;
+    }
+    long long best_jump = 0;
+    best_dist = 1e300;
+    for (int d = 0; d < 2; ++d) {
+        const int n = d == 0 ? w : h;
+        if (n < 4) continue;
+        std::vector<long long> lap(n, 0);
+        for (int k = 1; k < n - 1; ++k) lap[k] = sig[d][k - 1] - 2 * sig[d][k] + sig[d][k + 1];
+        for (int k = 1; k < n - 2; ++k) {
+            if (!((lap[k] < 0 && lap[k + 1] > 0) || (lap[k] > 0 && lap[k + 1] < 0))) continue;
+            const long long jump = std::llabs(lap[k + 1] - lap[k]);
+            const double dist = fabs((k + 1) - 0.5 * n);
+            if (jump > best_jump || (jump == best_jump && dist < best_dist)) {
+                best_jump = jump; best_dist = dist; best_d = d; best_k = k + 1;
+            }
+        }
+    }
+    if (best_d < 0) {
+        best_d = w >= h ? 0 : 1;
+        best_k = (best_d == 0 ? w : h) / 2;
+    }
+    tmk_box a = b, c = b;
+    a.hi[best_d] = b.lo[best_d] + best_k - 1;
+    c.lo[best_d] = b.lo[best_d] + best_k;
+    br_cluster(g, a, fill, out);
+    br_cluster(g, c, fill, out);
+}
+
+extern "C" int tmk_make_boxes(int nx, int ny, const unsigned char* tags, int ncoarse, const tmk_box* coarse_boxes,
+                              const tmk_meshparams* mp, int max_out, tmk_box* out, int* nout)
+{
+    if (nx <= 0 || ny <= 0 || !tags || !mp || !nout || max_out < 0 || (max_out > 0 && !out)) return TMK_E_ARG;
+    if (ncoarse < 0 || (ncoarse > 0 && !coarse_boxes)) return TMK_E_ARG;
+    const int r = mp->ref_ratio, bf = mp->block_factor, mg = mp->max_grid_size;
+    if (r < 2 || bf < 1 || bf % r != 0 || mg < bf || mg % bf != 0 || !(mp->fill_ratio > 0.0 && mp->fill_ratio <= 1.0))
+        return TMK_E_ARG;
+    const int cf = bf / r;  // tags are clustered in blocks of cf x cf cells of this level
+    const int nb = mp->nest_buffer > 0 ? mp->nest_buffer : (TMK_GHOST + r - 1) / r + 1;
+    // cells of this level that keep a finer box properly nested: covered by the level, with
+    // every cell within nb of them inside the domain covered too
+    std::vector<unsigned char> cov((size_t)nx * ny, ncoarse == 0 ? 1 : 0);
+    for (int k = 0; k < ncoarse; ++k) {
+        const tmk_box& b = coarse_boxes[k];
+        for (int j = std::max(b.lo[1], 0); j <= std::min(b.hi[1], ny - 1); ++j)
+            for (int i = std::max(b.lo[0], 0); i <= std::min(b.hi[0], nx - 1); ++i) cov[(size_t)j * nx + i] = 1;
+    }
+    std::vector<unsigned char> pnd((size_t)nx * ny, 1);
+    if (ncoarse > 0) {
+        // separable erosion: a cell stays when the (2 nb + 1)^2 neighbourhood, clipped to the
+        // domain, is covered
+        std::vector<unsigned char> tmp((size_t)nx * ny, 0);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                bool ok = true;
+                for (int ii = std::max(i - nb, 0); ii <= std::min(i + nb, nx - 1) && ok; ++ii)
+                    ok = cov[(size_t)j * nx + ii] != 0;
+                tmp[(size_t)j * nx + i] = ok;
+            }
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                bool ok = true;
+                for (int jj = std::max(j - nb, 0); jj <= std::min(j + nb, ny - 1) && ok; ++jj)
+                    ok = tmp[(size_t)jj * nx + i] != 0;
+                pnd[(size_t)j * nx + i] = ok;
+            }
+    }
+    // tags and nesting domain coarsened by cf (a block is tagged when any of its cells is; it
+    // is properly nested when all of its cells are)
+    TagField g;
+    g.nx = (nx + cf - 1) / cf;
+    g.ny = (ny + cf - 1) / cf;
+    g.t.assign((size_t)g.nx * g.ny, 0);
+    std::vector<unsigned char> gp((size_t)g.nx * g.ny, 1);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const size_t q = (size_t)(j / cf) * g.nx + i / cf;
+            if (tags[(size_t)j * nx + i]) g.t[q] = 1;
+            if (!pnd[(size_t)j * nx + i]) gp[q] = 0;
+        }
+    for (size_t q = 0; q < g.t.size(); ++q)
+        if (!gp[q]) g.t[q] = 0;
+    std::vector<tmk_box> cl;
+    br_cluster(g, tmk_box{{0, 0}, {g.nx - 1, g.ny - 1}}, mp->fill_ratio, cl);
+    // boxes reaching outside the nesting domain are cut until they lie inside it
+    std::vector<tmk_box> nested;
+    std::function<void(tmk_box)> nest = [&](tmk_box b) {
+        bool inside = true, any = false;
+        for (int j = b.lo[1]; j <= b.hi[1] && inside; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+                if (!gp[(size_t)j * g.nx + i]) { inside = false; break; }
+            }
+        for (int j = b.lo[1]; j <= b.hi[1] && !any; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0] && !any; ++i) any = g.at(i, j);
+        if (!any) return;
+        if (inside) { nested.push_back(b); return; }
+        const int w = b.hi[0] - b.lo[0] + 1, h = b.hi[1] - b.lo[1] + 1;
+        const int d = w >= h ? 0 : 1, k = (d == 0 ? w : h) / 2;
+        tmk_box a = b, c = b;
+        a.hi[d] = b.lo[d] + k - 1;
+        c.lo[d] = b.lo[d] + k;
+        // tight boxes around the tags of each half
+        std::vector<tmk_box> sub;
+        br_cluster(g, a, mp->fill_ratio, sub);
+        br_cluster(g, c, mp->fill_ratio, sub);
+        for (const tmk_box& x : sub) nest(x);
+    };
+    for (const tmk_box& b : cl) nest(b);
+    // to finer-level cells, then pieces of at most max_grid_size, multiples of block_factor
+    std::vector<tmk_box> res;
+    const int fx = cf * r;  // finer cells per clustered block
+    const int fnx = nx * r, fny = ny * r;
+    for (const tmk_box& b : nested) {
+        tmk_box f{{b.lo[0] * fx, b.lo[1] * fx}, {std::min((b.hi[0] + 1) * fx, fnx) - 1, std::min((b.hi[1] + 1) * fx, fny) - 1}};
+        const int nbx = (f.hi[0] - f.lo[0] + 1 + mg - 1) / mg, nby = (f.hi[1] - f.lo[1] + 1 + mg - 1) / mg;
+        const int blk_x = (f.hi[0] - f.lo[0] + bf) / bf, blk_y = (f.hi[1] - f.lo[1] + bf) / bf;
+        for (int py = 0; py < nby; ++py)
+            for (int px = 0; px < nbx; ++px) {
+                tmk_box p;
+                p.lo[0] = f.lo[0] + (blk_x * px / nbx) * bf;
+                p.hi[0] = std::min(f.lo[0] + (blk_x * (px + 1) / nbx) * bf - 1, f.hi[0]);
+                p.lo[1] = f.lo[1] + (blk_y * py / nby) * bf;
+                p.hi[1] = std::min(f.lo[1] + (blk_y * (py + 1) / nby) * bf - 1, f.hi[1]);
+                if (!box_empty(p)) res.push_back(p);
+            }
+    }
+    *nout = (int)res.size();
+    if ((int)res.size() > max_out) return TMK_E_OVERFLOW;
+    for (size_t k = 0; k < res.size(); ++k) out[k] = res[k];
+    return TMK_OK;
+}
+
+template <typename T>
+static int level_regrid_t(tmk_level* N, tmk_level* O, tmk_level* C, int r)
+{
+    PairMaps m;
+    int e = pair_maps(C, N, r, m);
+    if (e) return e;
+    const int ncx = m.ncx, ncy = m.ncy;
+    hipStream_t st = N->ctx->stream;
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamSynchronize(C->ctx->stream));
+    if (O) HIPCHK(hipStreamSynchronize(O->ctx->stream));
+    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.own_c[(size_t)j * ncx + i] >= 0; };
+    const int nN = (int)N->boxes.size();
+    std::vector<CfCell> cells;
+    std::vector<CfFace> faces;
+    std::vector<CopyRect> rects, brects;
+    for (int b = 0; b < nN; ++b) {
+        const tmk_box v = N->boxes[b];
+        const int w = v.hi[0] - v.lo[0] + 1, h = v.hi[1] - v.lo[1] + 1;
+        std::vector<unsigned char> old((size_t)w * h, 0);
+        std::vector<unsigned char> oldf(N->has_b[b] ? w : 0, 0);
+        if (O)
+            for (int o = 0; o < (int)O->boxes.size(); ++o) {
+                const tmk_box x = box_and(v, O->boxes[o]);
+                if (box_empty(x)) continue;
+                rects.push_back(CopyRect{N->npatch + o, b, x.lo[0], x.lo[1], x.hi[0], x.hi[1]});
+                for (int j = x.lo[1]; j <= x.hi[1]; ++j)
+                    for (int i = x.lo[0]; i <= x.hi[0]; ++i) old[(size_t)(i - v.lo[0]) + (size_t)(j - v.lo[1]) * w] = 1;
+                if (N->has_b[b] && O->has_b[o]) {
+                    brects.push_back(CopyRect{N->npatch + o, b, x.lo[0], 0, x.hi[0], 0});
+                    for (int i = x.lo[0]; i <= x.hi[0]; ++i) oldf[i - v.lo[0]] = 1;
+                }
+            }
+        // FineInterp of the cells the old layout does not cover: central slopes inside the
+        // domain, one-sided at its edges, limited off the domain's edge layer
+        for (int j = v.lo[1]; j <= v.hi[1]; ++j)
+            for (int i = v.lo[0]; i <= v.hi[0]; ++i) {
+                if (old[(size_t)(i - v.lo[0]) + (size_t)(j - v.lo[1]) * w]) continue;
+                const int ic = i / r, jc = j / r;
+                CfCell cc;
+                memset(&cc, 0, sizeof cc);
+                cc.dpatch = b;
+                cc.doff = cell_off(N, b, i, j);
+                for (int jj = -1, k = 0; jj <= 1; ++jj)
+                    for (int ii = -1; ii <= 1; ++ii, ++k) {
+                        const int ci = ic + ii, cj = jc + jj;
+                        cc.cpatch[k] = -1;
+                        if (in_c(ci, cj)) {
+                            const int cb = m.own_c[(size_t)cj * ncx + ci];
+                            cc.cpatch[k] = cb;
+                            cc.coff[k] = cell_off(C, cb, ci, cj);
+                        }
+                    }
+                const int kx = ic == 0 ? 2 : ic == ncx - 1 ? 1 : 0, ky = jc == 0 ? 2 : jc == ncy - 1 ? 1 : 0;
+                const bool lim = ic >= 1 && ic <= ncx - 2 && jc >= 1 && jc <= ncy - 2;
+                if (cc.cpatch[4] < 0 || (kx != 1 && cc.cpatch[5] < 0) || (kx != 2 && cc.cpatch[3] < 0) ||
+                    (ky != 1 && cc.cpatch[7] < 0) || (ky != 2 && cc.cpatch[1] < 0))
+                    return TMK_E_NESTING;
+                if (lim)
+                    for (int k = 0; k < 9; ++k)
+                        if (cc.cpatch[k] < 0) return TMK_E_NESTING;
+                cc.flags = kx | (ky << 2) | (lim ? 16 : 0);
+                cc.coef0 = -0.5 + ((double)(i - r * ic) + 0.5) / (double)r;
+                cc.coef1 = -0.5 + ((double)(j - r * jc) + 0.5) / (double)r;
+                cells.push_back(cc);
+            }
+        if (!N->has_b[b]) continue;
+        for (int i = v.lo[0]; i <= v.hi[0]; ++i) {
+            if (oldf[i - v.lo[0]]) continue;
+            const int ic = i / r;
+            CfFace ff;
+            memset(&ff, 0, sizeof ff);
+            ff.dpatch = b;
+            ff.doff = face_off(N, b, i);
+            for (int k = 0; k < 3; ++k) {
+                const int ci = ic - 1 + k;
+                const int cb = ci >= 0 && ci < ncx ? m.own_cb[ci] : -1;
+                ff.cpatch[k] = cb;
+                ff.coff[k] = cb >= 0 ? face_off(C, cb, ci) : 0;
+            }
+            ff.kind = ic == 0 ? 2 : ic == ncx - 1 ? 1 : 0;
+            ff.limit = ic >= 1 && ic <= ncx - 2;
+            if (ff.cpatch[1] < 0 || (ff.kind != 1 && ff.cpatch[2] < 0) || (ff.kind != 2 && ff.cpatch[0] < 0))
+                return TMK_E_NESTING;
+            ff.coef = -0.5 + ((double)(i - r * ic) + 0.5) / (double)r;
+            faces.push_back(ff);
+        }
+    }
+    std::vector<int> comps;
+    for (int k = 0; k < TMK_NCOMP; ++k) comps.push_back(k);
+    // descriptor tables: the new level and the coarse one with .in = their current data; the
+    // copy table (new level, then the old one) with .out = current data, as k_exchange reads it
+    const size_t ps = sizeof(Patch<T>);
+    std::vector<char> copytab(N->hpatch[N->cur ^ 1]);
+    if (O) copytab.insert(copytab.end(), O->hpatch[O->cur ^ 1].begin(), O->hpatch[O->cur ^ 1].end());
+    DevBuf dN, dC, dX, dcells, dfaces, dcomps, drects, dbrects;
+    if ((e = upload_list(dN, N->hpatch[N->cur])) || (e = upload_list(dC, C->hpatch[C->cur])) ||
+        (e = upload_list(dX, copytab)) || (e = upload_list(dcells, cells)) || (e = upload_list(dfaces, faces)) ||
+        (e = upload_list(dcomps, comps)) || (e = upload_list(drects, rects)) || (e = upload_list(dbrects, brects)))
+        return e;
+    (void)ps;
+    const Patch<T>* tN = (const Patch<T>*)dN.p;
+    const Patch<T>* tC = (const Patch<T>*)dC.p;
+    const int nb = N->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    if (!cells.empty()) {
+        hipLaunchKernelGGL(k_amr_fill_state<T>, amr_grid((long long)cells.size() * TMK_NCOMP), dim3(256), 0, st,
+                           (const CfCell*)dcells.p, (int)cells.size(), (const int*)dcomps.p, TMK_NCOMP, tN, tC, tC,
+                           (const Scen*)nullptr, (const Scen*)nullptr, r, 1.0);
+        HIPCHK(hipGetLastError());
+    }
+    if (!faces.empty()) {
+        hipLaunchKernelGGL(k_amr_fill_fault<T>, amr_grid((long long)faces.size() * nb), dim3(256), 0, st,
+                           (const CfFace*)dfaces.p, (int)faces.size(), nb, tN, tC, tC, (const Scen*)nullptr,
+                           (const Scen*)nullptr, r, 1.0);
+        HIPCHK(hipGetLastError());
+    }
+    if (!rects.empty()) {
+        hipLaunchKernelGGL(k_exchange<T>, dim3((unsigned)rects.size()), dim3(128), 0, st, (const Patch<T>*)dX.p,
+                           (const CopyRect*)drects.p, (int)rects.size(), (1 << TMK_NCOMP) - 1);
+        HIPCHK(hipGetLastError());
+    }
+    if (!brects.empty()) {
+        hipLaunchKernelGGL(k_bexchange<T>, dim3((unsigned)brects.size()), dim3(64), 0, st, (const Patch<T>*)dX.p,
+                           (const CopyRect*)dbrects.p, (int)brects.size());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    N->dirty = true;
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_regrid(tmk_level* fresh, tmk_level* old, tmk_level* coarse, int ref_ratio)
+{
+    if (!fresh || !coarse || ref_ratio < 2 || ref_ratio > TMK_AMR_MAXR || fresh == coarse || old == fresh) return TMK_E_ARG;
+    for (tmk_level* L : {fresh, old, coarse})
+        if (L && (L->batch || L->amr_stream)) return TMK_E_ARG;
+    if (fresh->tsize != coarse->tsize || (old && old->tsize != fresh->tsize) || fresh->ns != coarse->ns ||
+        (old && old->ns != fresh->ns))
+        return TMK_E_ARG;
+    if (old && (old->ctx->hp.domain.hi[0] != fresh->ctx->hp.domain.hi[0] ||
+                old->ctx->hp.domain.hi[1] != fresh->ctx->hp.domain.hi[1]))
+        return TMK_E_ARG;
+    return fresh->tsize == sizeof(double) ? level_regrid_t<double>(fresh, old, coarse, ref_ratio)
+                                          : level_regrid_t<float>(fresh, old, coarse, ref_ratio);
 }

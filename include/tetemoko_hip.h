@@ -242,13 +242,29 @@ int tmk_stations_destroy(tmk_stations *st);
 typedef struct tmk_amr tmk_amr;
 int tmk_amr_create(int nlevels, tmk_level *const *levels, int ref_ratio, tmk_amr **out);
 int tmk_amr_destroy(tmk_amr *amr);
-/* nsteps coarse steps of dt0 (level l steps dt0 / ref_ratio^l); the level clocks are
- * set to time when it differs from the hierarchy's current time or dt0 changes */
+/* one-scenario hierarchy: nsteps coarse steps of dt0 (level l steps dt0 / ref_ratio^l); the
+ * level clocks are set to time when it differs from the hierarchy's current time or dt0
+ * changes */
 int tmk_amr_run(tmk_amr *amr, int nsteps, double time, double dt0);
 /* the same, timed with HIP events on the hierarchy's stream (waits) */
 int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double *ms);
 /* waits; the first error any level reported (0 = none), cleared */
 int tmk_amr_sync(tmk_amr *amr);
+
+/* Hierarchy batches: scenarios with the same number of levels and ratio whose level-l boxes
+ * share the level-l update launches (and whose fills, registers and averages share theirs),
+ * each with its own parameters, clock and dt.  Members join and leave between steps.  No
+ * reference counterpart: the reference runs one scenario per process
+ * (amrGodunov.cpp:81-166). */
+int tmk_amr_batch_create(int nlevels, int ref_ratio, tmk_amr **out);
+/* join with levels[0 .. nlevels-1] (coarsest first), the clock of the next coarse step and dt0 */
+int tmk_amr_add(tmk_amr *amr, tmk_level *const *levels, double time, double dt0);
+/* leave; the member's levels are plain levels again */
+int tmk_amr_remove(tmk_amr *amr, tmk_level *level0);
+/* nsteps coarse steps of every member from its own clock */
+int tmk_amr_advance(tmk_amr *amr, int nsteps);
+/* waits; *code = the first error the member reported (0 = none); the others are untouched */
+int tmk_amr_member_error(tmk_amr *amr, tmk_level *level0, int *code);
 
 /* Regrid, step 1: boxes of the next finer level from the tags of a level (Chombo's
  * MeshRefine::regrid with BRMeshRefine, as AMR::regrid drives it; deck parameters

@@ -254,3 +254,58 @@ def test_regrid_from_device_tags_matches_oracle():
     H.sync()
     _levels_equal(h, H)
     H.close()
+
+
+def _download_levels(levels, boxes_per_level, cfgs):
+    out = []
+    for l, lev in enumerate(levels):
+        cfg = cfgs[l]
+        U = np.full((scenarios.NCOMP, cfg["ny"] + 2 * G, cfg["nx"] + 2 * G), np.nan)
+        B = np.full((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * G), np.nan)
+        for k, b in enumerate(boxes_per_level[l]):
+            x0, y0, x1, y1 = b
+            ub = np.empty((scenarios.NCOMP, y1 - y0 + 1 + 2 * G, x1 - x0 + 1 + 2 * G))
+            bb = np.empty((B.shape[0], x1 - x0 + 1 + 2 * G)) if y0 == 0 else None
+            lev.download(k, ub, bb)
+            U[:, y0 + G:y1 + G + 1, x0 + G:x1 + G + 1] = ub[:, G:-G, G:-G]
+            if bb is not None:
+                B[:, x0 + G:x1 + G + 1] = bb[:, G:-G]
+        out.append((U, B))
+    return out
+
+
+def test_hierarchy_batch_members_equal_alone():
+    """Two scenarios of different size and friction law in one hierarchy batch, joining and
+    leaving at different steps, each bit-identical to its hierarchy stepped alone."""
+    cfgA = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    cfgB = scenarios.config_c0(seed=9, nx=64, ny=32, dx=50.0)
+    boxA = [[(0, 0, 47, 31)], [(24, 0, 55, 23), (56, 0, 71, 23)]]
+    boxB = [[(0, 0, 63, 31)], [(32, 0, 95, 15)]]
+    ctxA, levA = amr.make_levels(cfgA, boxA, 2)
+    ctxB, levB = amr.make_levels(cfgB, boxB, 2)
+    ab = capi.AmrBatch(2, 2)
+    ab.add(levA, 0.0, scenarios.dt_of(cfgA))
+    ab.advance(3)
+    ab.add(levB, 0.0, scenarios.dt_of(cfgB))
+    ab.advance(4)
+    ab.sync()
+    ab.remove(levA[0])         # A: 7 coarse steps
+    ab.advance(2)              # B: 6 coarse steps
+    ab.sync()
+    ab.remove(levB[0])
+    ab.close()
+    for cfg, boxes, levels, nsteps in ((cfgA, boxA, levA, 7), (cfgB, boxB, levB, 6)):
+        cfgs = _cfgs(cfg, 2, 2)
+        got = _download_levels(levels, boxes, cfgs)
+        H = amr.Hierarchy(cfg, boxes, 2)
+        H.run(nsteps)
+        H.sync()
+        for l in range(2):
+            Ua, Ba = H.download(l)
+            assert np.array_equal(np.nan_to_num(got[l][0], nan=7.0), np.nan_to_num(Ua, nan=7.0)), l
+            assert np.array_equal(np.nan_to_num(got[l][1], nan=7.0), np.nan_to_num(Ba, nan=7.0)), l
+        H.close()
+    for lev in levA + levB:
+        lev.close()
+    for ctx in ctxA + ctxB:
+        ctx.close()

@@ -16,6 +16,26 @@ from . import capi, scenarios
 G = scenarios.GHOST
 
 
+def make_levels(cfg0, boxes_per_level, r, precision=capi.PREC_FP64, tile=None):
+    """Contexts and levels of a hierarchy (initial state: background and initial fault data),
+    not yet stepped together: for a tmk_amr (Hierarchy) or a hierarchy batch (capi.AmrBatch)."""
+    cfg = cfg0
+    ctxs, levels = [], []
+    for l, boxes in enumerate(boxes_per_level):
+        if l > 0:
+            cfg = scenarios.refine_cfg(cfg, r)
+        ctx = capi.Context(cfg, precision)
+        if tile:
+            ctx.set_tile(*tile)
+        lev = capi.Level(ctx, boxes)
+        Bf = scenarios.fault_arrays(cfg)
+        for k, b in enumerate(boxes):
+            lev.fill(k, scenarios.background_vector(cfg), scenarios.box_bview(Bf, b) if b[1] == 0 else None)
+        ctxs.append(ctx)
+        levels.append(lev)
+    return ctxs, levels
+
+
 class Hierarchy:
     def __init__(self, cfg0, boxes_per_level, r, precision=capi.PREC_FP64, tile=None, arrays=None):
         """arrays(l) -> (U, B) level arrays over the whole level domain with ghosts (the

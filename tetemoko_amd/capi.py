@@ -43,6 +43,7 @@ EXPORTED = (
     "tmk_stations_create", "tmk_stations_read", "tmk_stations_destroy",
     "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed", "tmk_amr_sync",
     "tmk_make_boxes", "tmk_level_regrid",
+    "tmk_amr_batch_create", "tmk_amr_add", "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error",
 )
 
 
@@ -187,6 +188,11 @@ def lib():
         L.tmk_make_boxes.argtypes = [c_int, c_int, ctypes.POINTER(ctypes.c_ubyte), c_int, PBox,
                                      ctypes.POINTER(MeshParams), c_int, PBox, ctypes.POINTER(c_int)]
         L.tmk_level_regrid.argtypes = [vp, vp, vp, c_int]
+        L.tmk_amr_batch_create.argtypes = [c_int, c_int, ctypes.POINTER(vp)]
+        L.tmk_amr_add.argtypes = [vp, ctypes.POINTER(vp), _D, _D]
+        L.tmk_amr_remove.argtypes = [vp, vp]
+        L.tmk_amr_advance.argtypes = [vp, c_int]
+        L.tmk_amr_member_error.argtypes = [vp, vp, ctypes.POINTER(c_int)]
         _lib = L
     return _lib
 
@@ -417,6 +423,45 @@ class Amr:
 
     def sync(self):
         _check(lib().tmk_amr_sync(self.h), "tmk_amr_sync")
+
+
+class AmrBatch:
+    """Hierarchies of several scenarios (same level count and ratio) in shared launches."""
+
+    def __init__(self, nlevels, r):
+        h = ctypes.c_void_p()
+        _check(lib().tmk_amr_batch_create(nlevels, r, ctypes.byref(h)), "tmk_amr_batch_create")
+        self.h = h
+        self.nlevels, self.r = nlevels, r
+
+    def close(self):
+        if self.h:
+            lib().tmk_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, levels, time, dt0):
+        arr = (ctypes.c_void_p * len(levels))(*[lv.h for lv in levels])
+        _check(lib().tmk_amr_add(self.h, arr, time, dt0), "tmk_amr_add")
+
+    def remove(self, level0):
+        _check(lib().tmk_amr_remove(self.h, level0.h), "tmk_amr_remove")
+
+    def advance(self, nsteps):
+        _check(lib().tmk_amr_advance(self.h, nsteps), "tmk_amr_advance")
+
+    def sync(self):
+        _check(lib().tmk_amr_sync(self.h), "tmk_amr_sync")
+
+    def member_error(self, level0):
+        code = ctypes.c_int()
+        _check(lib().tmk_amr_member_error(self.h, level0.h, ctypes.byref(code)), "tmk_amr_member_error")
+        return code.value
 
 
 def make_boxes(tags, coarse_boxes=None, ref_ratio=4, block_factor=8, max_grid_size=64, fill_ratio=0.75,

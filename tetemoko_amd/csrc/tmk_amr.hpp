@@ -29,7 +29,7 @@ struct CfCell {
     int cpatch[9], coff[9];  // coarse 3x3 neighbourhood (jj = -1..1 outer, ii = -1..1 inner); -1: not read
     int flags;               // bits 0-1 x slope, bits 2-3 y slope (0 central, 1 one-sided low,
                              // 2 one-sided high), bit 4: limited
-    int pad;
+    int m;                   // member (scenario) of the hierarchy batch: its clocks
     double coef0, coef1;     // -1/2 + (offset + 1/2) / ref_ratio per direction
 };
 
@@ -39,6 +39,7 @@ struct CfFace {
     int cpatch[3], coff[3];  // coarse faces ic-1, ic, ic+1 (-1: not read)
     int kind;                // 0 central, 1 one-sided low, 2 one-sided high
     int limit;               // 1: limit the slope against the three faces (D1FineInterp)
+    int m, pad;              // member of the hierarchy batch
     double coef;
 };
 
@@ -49,6 +50,7 @@ struct RegFace {
     int d;
     int fpatch;              // fine box whose F[d] holds the fine faces
     int fface[TMK_AMR_MAXR]; // face offsets in that box's F[d]
+    int m;                   // member of the hierarchy batch: its dt on both levels
 };
 
 // one uncovered coarse cell with register faces e0 .. e1-1 (in the order dir 0 lo, dir 0 hi,
@@ -110,8 +112,8 @@ __global__ void k_amr_fill_state(const CfCell* __restrict__ cells, int ncells, c
     const int e = (int)(t / ncomp), c = comps[t % ncomp];
     const CfCell cc = cells[e];
     double alpha = alpha_fixed;
-    if (alpha_fixed < 0.0 && !amr_alpha(*fscen, *cscen, r, alpha)) {
-        if (t == 0) report(&((Scen*)fscen)->err, TMK_E_AMR_TIME);
+    if (alpha_fixed < 0.0 && !amr_alpha(fscen[cc.m], cscen[cc.m], r, alpha)) {
+        report(&((Scen*)fscen)[cc.m].err, TMK_E_AMR_TIME);
         return;
     }
     T v[9];
@@ -157,8 +159,9 @@ __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, i
     if (t >= nfaces * nb) return;
     const int e = t / nb, c = t % nb;
     const CfFace f = faces[e];
+    if (c >= fine[f.dpatch].bnb) return;  // members of a batch may have different fault laws
     double alpha = alpha_fixed;
-    if (alpha_fixed < 0.0 && !amr_alpha(*fscen, *cscen, r, alpha)) return;
+    if (alpha_fixed < 0.0 && !amr_alpha(fscen[f.m], cscen[f.m], r, alpha)) return;
     double v[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -189,15 +192,16 @@ __device__ __forceinline__ size_t flux_plane(const Patch<T>& p, int d)
 }
 
 // reg = -dt_c F_c (the register is cleared by this write: LevelFluxRegister::setToZero
-// then incrementCoarse)
+// then incrementCoarse); dt_c is the member's coarse step
 template <typename T>
 __global__ void k_amr_reg_coarse(const RegFace* __restrict__ faces, int nfaces, double* __restrict__ reg,
-                                 const Patch<T>* __restrict__ crs, double dtc)
+                                 const Patch<T>* __restrict__ crs, const Scen* __restrict__ cscen)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfaces * TMK_NCOMP) return;
     const int c = t / nfaces, e = t % nfaces;
     const RegFace f = faces[e];
+    const double dtc = cscen[f.m].dt;
     const Patch<T>& p = crs[f.cpatch];
     const double F = p.F[f.d][f.cface + (size_t)c * flux_plane(p, f.d)];
     double acc = 0.0;
@@ -205,15 +209,16 @@ __global__ void k_amr_reg_coarse(const RegFace* __restrict__ faces, int nfaces, 
     reg[(size_t)c * nfaces + e] = acc;
 }
 
-// reg += (dt_f / r) F_f over the r fine faces, in order
+// reg += (dt_f / r) F_f over the r fine faces, in order; dt_f is the member's fine step
 template <typename T>
 __global__ void k_amr_reg_fine(const RegFace* __restrict__ faces, int nfaces, double* __restrict__ reg,
-                               const Patch<T>* __restrict__ fine, int r, double w)
+                               const Patch<T>* __restrict__ fine, int r, const Scen* __restrict__ fscen)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nfaces * TMK_NCOMP) return;
     const int c = t / nfaces, e = t % nfaces;
     const RegFace f = faces[e];
+    const double w = fscen[f.m].dt * (1.0 / (double)r);
     const Patch<T>& p = fine[f.fpatch];
     const size_t fp = flux_plane(p, f.d);
     const double* F = p.F[f.d] + (size_t)c * fp;
@@ -265,6 +270,7 @@ __global__ void k_amr_avg_fault(const AvgFace* __restrict__ faces, int nfaces, i
     const AvgFace a = faces[k];
     const Patch<T>& pf = fine[a.fpatch];
     const Patch<T>& pc = crs[a.cpatch];
+    if (c >= pc.bnb) return;  // members of a batch may have different fault laws
     const double* f = pf.bin + a.foff + (size_t)c * pf.bnx;
     double acc = 0.0;
     for (int ii = 0; ii < r; ++ii) acc = acc + f[ii];

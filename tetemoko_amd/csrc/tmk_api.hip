@@ -230,6 +230,9 @@ struct Group {
     }
 };
 
+// Every stream the library creates is non-blocking: a synchronous copy (hipMemcpy, the null
+// stream) does not wait for other scenarios' work in flight.  Every copy that reads device
+// results synchronises the stream that produced them first.
 struct tmk_ctx {
     tmk_params hp;
     DevParams P;
@@ -406,7 +409,7 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
     default_tile(precision, c->bx, c->by);
     fill_dev_params(*p, c->P);
     c->antiplane = !(p->sxz0 == 0.0 && p->syz0 == 0.0);
-    if (hipStreamCreate(&c->stream) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return TMK_E_HIP;
     }
@@ -700,7 +703,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         ((int)k < G.nint ? G.cells_int : G.cells_bnd) +=
             (long long)(tiles[k].x1 - tiles[k].x0 + 1) * (tiles[k].y1 - tiles[k].y0 + 1);
     if (!G.side) {
-        HIPCHK(hipStreamCreate(&G.side));
+        HIPCHK(hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&G.ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&G.ev_join, hipEventDisableTiming));
     }
@@ -1565,7 +1568,7 @@ extern "C" int tmk_batch_create(tmk_batch** out)
     if (!out) return TMK_E_ARG;
     tmk_batch* B = new (std::nothrow) tmk_batch;
     if (!B) return TMK_E_NOMEM;
-    if (hipStreamCreate(&B->stream) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
         delete B;
         return TMK_E_HIP;
     }
@@ -1843,10 +1846,14 @@ extern "C" int tmk_level_tag(tmk_level* L, const tmk_tagparams* tp, unsigned cha
 //  AMRLevelLinElast.cpp:372-516, and the coarse-fine parts of
 //  LinElastLevelGodunov::step, LinElastLevelGodunov.cpp:226-355)
 // ---------------------------------------------------------------------------
+struct AmrMember;
+struct PairLists;
+
+// one level of a hierarchy batch: the members' level-l boxes in one group (shared launches)
 struct AmrLevel {
-    tmk_level* L = nullptr;
-    std::vector<double*> F[2];          // per island: face fluxes over the island's bounding box
-    std::vector<unsigned char*> fmask;  // per island: which faces the update stores (see Patch::fmask)
+    Group grp;
+    std::vector<int> phase;             // member parity at group parity 0
+    std::vector<int> poff;              // first descriptor of member m in the group's tables
     // coarse-fine fill of this level from the next coarser one (levels 1 ..)
     DevBuf cells, faces, comps;
     int ncells = 0, nfaces = 0, ncomps = 0;
@@ -1855,14 +1862,31 @@ struct AmrLevel {
     int nreg = 0, nrcells = 0, navgc = 0, navgf = 0;
 };
 
-struct tmk_amr {
-    std::vector<AmrLevel> lv;
-    int r = 2;
-    hipStream_t stream = nullptr;
-    bool clock = false;
+// one scenario of a hierarchy batch: its levels, and the face fluxes and face masks of their
+// islands (see Patch::fmask), kept while it is a member
+struct AmrMember {
+    std::vector<tmk_level*> L;
+    std::vector<std::vector<double*>> F[2];          // [level][island]
+    std::vector<std::vector<unsigned char*>> fmask;  // [level][island]
+    std::vector<PairLists*> lists;                   // [pair l-1, l], member-local indices
     double dt0 = 0.0;
+    int err = 0;                                     // first error seen at a detach
+    ~AmrMember();
+};
+
+// A hierarchy batch: scenarios with the same number of levels and refinement ratio, their
+// level-l boxes updated in shared launches, each member with its own parameters, clock and
+// dt.  One member is the plain hierarchy of tmk_amr_create.
+struct tmk_amr {
+    int nlev = 0;
+    int r = 2;
+    std::vector<AmrMember*> mem;
+    std::vector<AmrLevel> lv;
+    hipStream_t stream = nullptr;
+    bool built = false;   // groups and lists match the members
     hipGraphExec_t gexec = nullptr;  // two coarse steps (every level back to its parity)
     int gpar = 0;                    // level-0 group parity the graph was captured at
+    Group xgrp;                      // one-level group for the exchange of an uploaded level
 };
 
 // offset of cell (i, j) in the plane of the island array holding box b
@@ -1959,14 +1983,26 @@ static void for_each_reg_face(const PairMaps& m, Fn fn)
                 }
 }
 
-// the device lists of the pair (level l-1, level l); the boxes' flux layouts are set
-static int amr_build_pair(tmk_amr* A, int l)
+struct PairLists {
+    std::vector<CfCell> cells;
+    std::vector<CfFace> faces;
+    std::vector<RegFace> regf;
+    std::vector<double> rscale;
+    std::vector<RefluxCell> rcells;
+    std::vector<AvgCell> avgc;
+    std::vector<AvgFace> avgf;
+};
+
+AmrMember::~AmrMember()
 {
-    AmrLevel& c = A->lv[l - 1];
-    AmrLevel& f = A->lv[l];
-    const tmk_level* C = c.L;
-    const tmk_level* Fl = f.L;
-    const int r = A->r;
+    for (PairLists* p : lists) delete p;
+}
+
+// the lists of member M's pair (level l-1, level l), with the member's own descriptor indices
+static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
+{
+    const tmk_level* C = M.L[l - 1];
+    const tmk_level* Fl = M.L[l];
     PairMaps m;
     int e = pair_maps(C, Fl, r, m);
     if (e) return e;
@@ -2076,7 +2112,7 @@ static int amr_build_pair(tmk_amr* A, int l)
             const int ffi = d == 0 ? fi * r : fi * r + mm, ffj = d == 0 ? fj * r + mm : fj * r;
             en.rf.fface[mm] = flux_off(Fl->isl[Fl->box_isl[fb]].bbox, d, ffi, ffj);
         }
-        if (!c.F[d][C->box_isl[cb]] || !f.F[d][Fl->box_isl[fb]]) bad = true;
+        if (!M.F[l - 1][d][C->box_isl[cb]] || !M.F[l][d][Fl->box_isl[fb]]) bad = true;
         en.cbox = cb;
         en.s = inv * (side ? 1.0 : -1.0);
         ents.push_back(en);
@@ -2126,78 +2162,72 @@ static int amr_build_pair(tmk_amr* A, int l)
             avgf.push_back(a);
         }
     }
-    f.ncells = (int)cells.size();
-    f.nfaces = (int)faces.size();
-    f.ncomps = (int)comps.size();
-    c.nreg = (int)regf.size();
-    c.nrcells = (int)rcells.size();
-    c.navgc = (int)avgc.size();
-    c.navgf = (int)avgf.size();
-    if ((e = upload_list(f.cells, cells)) || (e = upload_list(f.faces, faces)) || (e = upload_list(f.comps, comps)) ||
-        (e = upload_list(c.regf, regf)) || (e = upload_list(c.rcells, rcells)) ||
-        (e = upload_list(c.rscale, rscale)) || (e = upload_list(c.avgc, avgc)) || (e = upload_list(c.avgf, avgf)))
-        return e;
-    return c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * regf.size(), 1));
+    P.cells.swap(cells);
+    P.faces.swap(faces);
+    P.regf.swap(regf);
+    P.rscale.swap(rscale);
+    P.rcells.swap(rcells);
+    P.avgc.swap(avgc);
+    P.avgf.swap(avgf);
+    return TMK_OK;
 }
 
-// face fluxes of every island of every level of the hierarchy, on the island's descriptor (the
-// tiles'); the per-face mask limits the stores to the faces the registers read
+// face fluxes and masks of member M on its levels' descriptors (both parities)
 template <typename T>
-static void amr_set_flux_ptrs(AmrLevel& a)
+static void member_set_flux_ptrs(AmrMember& M, bool on)
 {
-    tmk_level* L = a.L;
-    const int nbox = (int)L->boxes.size();
-    for (int par = 0; par < 2; ++par) {
-        Patch<T>* pts = (Patch<T>*)L->hpatch[par].data();
-        for (int q = 0; q < L->npatch; ++q) {
-            const int k = q - nbox;
-            const bool isl = k >= 0 && k < (int)a.F[0].size();
-            pts[q].F[0] = isl ? a.F[0][k] : nullptr;
-            pts[q].F[1] = isl ? a.F[1][k] : nullptr;
-            pts[q].fmask = isl ? a.fmask[k] : nullptr;
+    for (size_t l = 0; l < M.L.size(); ++l) {
+        tmk_level* L = M.L[l];
+        const int nbox = (int)L->boxes.size();
+        for (int par = 0; par < 2; ++par) {
+            Patch<T>* pts = (Patch<T>*)L->hpatch[par].data();
+            for (int q = 0; q < L->npatch; ++q) {
+                const int k = q - nbox;
+                const bool isl = on && k >= 0 && k < (int)M.F[0][l].size();
+                pts[q].F[0] = isl ? M.F[0][l][k] : nullptr;
+                pts[q].F[1] = isl ? M.F[1][l][k] : nullptr;
+                pts[q].fmask = isl ? M.fmask[l][k] : nullptr;
+            }
         }
+        L->own_built = false;
     }
-    L->own_built = false;
 }
 
-// hand the levels back: flux arrays freed, descriptors without fluxes (the level's own
-// group is rebuilt before its next use), level calls allowed again
-static void amr_free(tmk_amr* A)
+static void member_release(AmrMember& M)
 {
-    if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
-    A->gexec = nullptr;
-    for (AmrLevel& a : A->lv) {
-        if (!a.L) continue;
-        for (int d = 0; d < 2; ++d)
-            for (double* p : a.F[d])
+    for (int d = 0; d < 2; ++d)
+        for (auto& v : M.F[d])
+            for (double* p : v)
                 if (p) (void)hipFree(p);
-        for (unsigned char* p : a.fmask)
+    for (auto& v : M.fmask)
+        for (unsigned char* p : v)
             if (p) (void)hipFree(p);
-        a.F[0].clear();
-        a.F[1].clear();
-        a.fmask.clear();
-        if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
-        else amr_set_flux_ptrs<float>(a);
-        a.L->amr_stream = nullptr;
+    for (int d = 0; d < 2; ++d) M.F[d].assign(M.L.size(), {});
+    M.fmask.assign(M.L.size(), {});
+    if (!M.L.empty()) {
+        if (M.L[0]->tsize == sizeof(double)) member_set_flux_ptrs<double>(M, false);
+        else member_set_flux_ptrs<float>(M, false);
     }
+    for (tmk_level* L : M.L) L->amr_stream = nullptr;
 }
 
-// allocate the flux arrays and face masks of every island (mask bit 0: a cell of the level,
-// bit 1: under the next finer level), over the island's bounding box grown by one cell
-static int amr_layouts(tmk_amr* A)
+// flux arrays and face masks of every island of the member's levels (mask bit 0: a cell of
+// the level, bit 1: under the next finer level), over the island's bounding box grown by one
+static int member_layouts(AmrMember& M, int r)
 {
-    const int n = (int)A->lv.size();
+    const int n = (int)M.L.size();
+    for (int d = 0; d < 2; ++d) M.F[d].assign(n, {});
+    M.fmask.assign(n, {});
     for (int l = 0; l < n; ++l) {
-        AmrLevel& a = A->lv[l];
-        const tmk_level* L = a.L;
+        const tmk_level* L = M.L[l];
         const size_t ni = L->isl.size();
-        a.F[0].assign(ni, nullptr);
-        a.F[1].assign(ni, nullptr);
-        a.fmask.assign(ni, nullptr);
+        M.F[0][l].assign(ni, nullptr);
+        M.F[1][l].assign(ni, nullptr);
+        M.fmask[l].assign(ni, nullptr);
         std::vector<tmk_box> finer;
         if (l + 1 < n)
-            for (const tmk_box& b : A->lv[l + 1].L->boxes)
-                finer.push_back(tmk_box{{b.lo[0] / A->r, b.lo[1] / A->r}, {b.hi[0] / A->r, b.hi[1] / A->r}});
+            for (const tmk_box& b : M.L[l + 1]->boxes)
+                finer.push_back(tmk_box{{b.lo[0] / r, b.lo[1] / r}, {b.hi[0] / r, b.hi[1] / r}});
         for (size_t k = 0; k < ni; ++k) {
             const Island& I = L->isl[k];
             const size_t w = I.bbox.hi[0] - I.bbox.lo[0] + 1, h = I.bbox.hi[1] - I.bbox.lo[1] + 1;
@@ -2205,35 +2235,267 @@ static int amr_layouts(tmk_amr* A)
                 void* p = nullptr;
                 const size_t nf = d == 0 ? (w + 1) * h : w * (h + 1);
                 if (hipMalloc(&p, sizeof(double) * TMK_NCOMP * nf) != hipSuccess) return TMK_E_NOMEM;
-                a.F[d][k] = (double*)p;
+                M.F[d][l][k] = (double*)p;
             }
             const size_t mw = w + 2, mh = h + 2;
-            std::vector<unsigned char> m(mw * mh, 0);
+            std::vector<unsigned char> mk(mw * mh, 0);
             auto mark = [&](const tmk_box& b, unsigned char bit) {
                 const int i0 = std::max(b.lo[0], I.bbox.lo[0] - 1), i1 = std::min(b.hi[0], I.bbox.hi[0] + 1);
                 const int j0 = std::max(b.lo[1], I.bbox.lo[1] - 1), j1 = std::min(b.hi[1], I.bbox.hi[1] + 1);
                 for (int j = j0; j <= j1; ++j)
                     for (int i = i0; i <= i1; ++i)
-                        m[(size_t)(i - I.bbox.lo[0] + 1) + (size_t)(j - I.bbox.lo[1] + 1) * mw] |= bit;
+                        mk[(size_t)(i - I.bbox.lo[0] + 1) + (size_t)(j - I.bbox.lo[1] + 1) * mw] |= bit;
             };
             for (const tmk_box& b : L->boxes) mark(b, 1);
             for (const tmk_box& b : finer) mark(b, 2);
             void* p = nullptr;
-            if (hipMalloc(&p, m.size()) != hipSuccess) return TMK_E_NOMEM;
-            a.fmask[k] = (unsigned char*)p;
-            HIPCHK(hipMemcpy(p, m.data(), m.size(), hipMemcpyHostToDevice));
+            if (hipMalloc(&p, mk.size()) != hipSuccess) return TMK_E_NOMEM;
+            M.fmask[l][k] = (unsigned char*)p;
+            HIPCHK(hipMemcpy(p, mk.data(), mk.size(), hipMemcpyHostToDevice));
         }
     }
+    return TMK_OK;
+}
+
+// pull the members' clocks, parities and errors out of the groups before the layout changes
+static int amr_detach(tmk_amr* A)
+{
+    HIPCHK(hipStreamSynchronize(A->stream));
+    if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
+    A->gexec = nullptr;
+    if (!A->built) return TMK_OK;
+    for (int l = 0; l < A->nlev; ++l) {
+        AmrLevel& a = A->lv[l];
+        std::vector<Scen> sc(A->mem.size());
+        if (!sc.empty())
+            HIPCHK(hipMemcpy(sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+        for (size_t m = 0; m < A->mem.size(); ++m) {
+            tmk_level* L = A->mem[m]->L[l];
+            L->cur = a.phase[m] ^ a.grp.g;
+            L->time = sc[m].time;
+            if (sc[m].err && !A->mem[m]->err) A->mem[m]->err = sc[m].err;
+        }
+        a.grp.reset();
+    }
+    A->built = false;
+    return TMK_OK;
+}
+
+// groups and lists for the current members (lazily, by the first run after a change)
+template <typename T>
+static int amr_build_t(tmk_amr* A)
+{
+    const int nm = (int)A->mem.size();
+    int e;
+    // levels uploaded since their last step: exchange their ghosts first
+    for (AmrMember* M : A->mem)
+        for (tmk_level* L : M->L) {
+            if (!L->dirty) continue;
+            L->dirty = false;
+            std::vector<tmk_level*> one{L};
+            std::vector<int> ph{L->cur};
+            if ((e = group_build(A->xgrp, one, ph, 1))) return e;
+            if ((e = group_exchange_now_t<T>(A->xgrp, A->stream))) return e;
+            HIPCHK(hipStreamSynchronize(A->stream));
+        }
+    for (int l = 0; l < A->nlev; ++l) {
+        AmrLevel& a = A->lv[l];
+        std::vector<tmk_level*> ls;
+        a.phase.assign(nm, 0);
+        a.poff.assign(nm, 0);
+        int off = 0;
+        for (int m = 0; m < nm; ++m) {
+            tmk_level* L = A->mem[m]->L[l];
+            ls.push_back(L);
+            a.phase[m] = L->cur;
+            a.poff[m] = off;
+            off += L->npatch;
+        }
+        if ((e = group_build(a.grp, ls, a.phase, 1))) return e;
+    }
+    for (int l = 1; l < A->nlev; ++l) {
+        AmrLevel& c = A->lv[l - 1];
+        AmrLevel& f = A->lv[l];
+        PairLists all;
+        for (int m = 0; m < nm; ++m) {
+            const PairLists& P = *A->mem[m]->lists[l];  // built when the member joined
+            const int pc = c.poff[m], pf = f.poff[m];
+            for (CfCell x : P.cells) {
+                x.dpatch += pf;
+                for (int k = 0; k < 9; ++k)
+                    if (x.cpatch[k] >= 0) x.cpatch[k] += pc;
+                x.m = m;
+                all.cells.push_back(x);
+            }
+            for (CfFace x : P.faces) {
+                x.dpatch += pf;
+                for (int k = 0; k < 3; ++k)
+                    if (x.cpatch[k] >= 0) x.cpatch[k] += pc;
+                x.m = m;
+                all.faces.push_back(x);
+            }
+            const int e0 = (int)all.regf.size();
+            for (RegFace x : P.regf) {
+                x.cpatch += pc;
+                x.fpatch += pf;
+                x.m = m;
+                all.regf.push_back(x);
+            }
+            all.rscale.insert(all.rscale.end(), P.rscale.begin(), P.rscale.end());
+            for (RefluxCell x : P.rcells) {
+                x.cpatch += pc;
+                x.e0 += e0;
+                x.e1 += e0;
+                all.rcells.push_back(x);
+            }
+            for (AvgCell x : P.avgc) {
+                x.cpatch += pc;
+                x.fpatch += pf;
+                all.avgc.push_back(x);
+            }
+            for (AvgFace x : P.avgf) {
+                x.cpatch += pc;
+                x.fpatch += pf;
+                all.avgf.push_back(x);
+            }
+        }
+        std::vector<int> comps;
+        for (int k = 0; k < TMK_NCOMP; ++k)
+            if (is_stencil(k, A->mem[0]->L[l]->ns)) comps.push_back(k);
+        f.ncells = (int)all.cells.size();
+        f.nfaces = (int)all.faces.size();
+        f.ncomps = (int)comps.size();
+        c.nreg = (int)all.regf.size();
+        c.nrcells = (int)all.rcells.size();
+        c.navgc = (int)all.avgc.size();
+        c.navgf = (int)all.avgf.size();
+        if ((e = upload_list(f.cells, all.cells)) || (e = upload_list(f.faces, all.faces)) ||
+            (e = upload_list(f.comps, comps)) || (e = upload_list(c.regf, all.regf)) ||
+            (e = upload_list(c.rcells, all.rcells)) || (e = upload_list(c.rscale, all.rscale)) ||
+            (e = upload_list(c.avgc, all.avgc)) || (e = upload_list(c.avgf, all.avgf)))
+            return e;
+        if ((e = c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * all.regf.size(), 1)))) return e;
+    }
+    A->built = nm > 0;
+    return TMK_OK;
+}
+
+static int amr_build(tmk_amr* A)
+{
+    if (A->mem.empty()) return TMK_OK;
+    return A->mem[0]->L[0]->tsize == sizeof(double) ? amr_build_t<double>(A) : amr_build_t<float>(A);
+}
+
+extern "C" int tmk_amr_batch_create(int nlevels, int ref_ratio, tmk_amr** out)
+{
+    if (!out || nlevels < 1 || ref_ratio < 2 || ref_ratio > TMK_AMR_MAXR) return TMK_E_ARG;
+    *out = nullptr;
+    tmk_amr* A = new (std::nothrow) tmk_amr;
+    if (!A) return TMK_E_NOMEM;
+    A->nlev = nlevels;
+    A->r = ref_ratio;
+    A->lv.resize(nlevels);
+    if (hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete A;
+        return TMK_E_HIP;
+    }
+    *out = A;
+    return TMK_OK;
+}
+
+// join: one scenario's levels (coarsest first), its clock and coarse step
+extern "C" int tmk_amr_add(tmk_amr* A, tmk_level* const* levels, double time, double dt0)
+{
+    if (!A || !levels || !(dt0 > 0.0)) return TMK_E_ARG;
+    for (int l = 0; l < A->nlev; ++l) {
+        tmk_level* L = levels[l];
+        if (!L || L->batch || L->amr_stream || L->stations) return TMK_E_ARG;
+        if (L->tsize != levels[0]->tsize) return TMK_E_ARG;
+        for (int k = 0; k < l; ++k)
+            if (levels[k] == L) return TMK_E_ARG;
+    }
+    if (!A->mem.empty()) {
+        const tmk_level* a = A->mem[0]->L[0];
+        const tmk_level* b = levels[0];
+        if (a->tsize != b->tsize || a->ns != b->ns || a->ctx->bx != b->ctx->bx || a->ctx->by != b->ctx->by)
+            return TMK_E_ARG;
+    }
+    {   // level 0 covers its domain
+        const tmk_level* L0 = levels[0];
+        const tmk_box d = L0->ctx->hp.domain;
+        long long cells = 0;
+        for (const tmk_box& b : L0->boxes) cells += (long long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1);
+        if (cells != (long long)(d.hi[0] - d.lo[0] + 1) * (d.hi[1] - d.lo[1] + 1)) return TMK_E_NESTING;
+    }
+    for (int l = 1; l < A->nlev; ++l) {
+        PairMaps m;
+        int e = pair_maps(levels[l - 1], levels[l], A->r, m);
+        if (e) return e;
+    }
+    int e = amr_detach(A);
+    if (e) return e;
+    AmrMember* M = new (std::nothrow) AmrMember;
+    if (!M) return TMK_E_NOMEM;
+    M->L.assign(levels, levels + A->nlev);
+    M->dt0 = dt0;
+    for (tmk_level* L : M->L) HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    e = member_layouts(*M, A->r);
+    // the member's lists (which also check nesting), kept while it is a member
+    M->lists.assign(A->nlev, nullptr);
+    for (int l = 1; l < A->nlev && !e; ++l) {
+        M->lists[l] = new (std::nothrow) PairLists;
+        e = M->lists[l] ? member_pair_lists(*M, l, A->r, *M->lists[l]) : TMK_E_NOMEM;
+    }
+    if (e) {
+        member_release(*M);
+        delete M;
+        return e;
+    }
+    if (M->L[0]->tsize == sizeof(double)) member_set_flux_ptrs<double>(*M, true);
+    else member_set_flux_ptrs<float>(*M, true);
+    double dt = dt0;
+    for (tmk_level* L : M->L) {
+        L->amr_stream = A->stream;
+        L->time = time;
+        L->dt = dt;
+        dt = dt / A->r;
+    }
+    A->mem.push_back(M);
+    return TMK_OK;
+}
+
+static int amr_find(tmk_amr* A, const tmk_level* L0)
+{
+    for (size_t m = 0; m < A->mem.size(); ++m)
+        if (A->mem[m]->L[0] == L0) return (int)m;
+    return -1;
+}
+
+// leave: the scenario's levels are plain levels again
+extern "C" int tmk_amr_remove(tmk_amr* A, tmk_level* level0)
+{
+    if (!A || !level0) return TMK_E_ARG;
+    const int m = amr_find(A, level0);
+    if (m < 0) return TMK_E_ARG;
+    int e = amr_detach(A);
+    if (e) return e;
+    AmrMember* M = A->mem[m];
+    A->mem.erase(A->mem.begin() + m);
+    member_release(*M);
+    delete M;
     return TMK_OK;
 }
 
 extern "C" int tmk_amr_destroy(tmk_amr* A)
 {
     if (!A) return TMK_OK;
-    if (A->stream) (void)hipStreamSynchronize(A->stream);
-    for (AmrLevel& a : A->lv)
-        if (a.L && a.L->ctx) (void)hipStreamSynchronize(a.L->ctx->stream);
-    amr_free(A);
+    (void)amr_detach(A);
+    for (AmrMember* M : A->mem) {
+        member_release(*M);
+        delete M;
+    }
+    for (AmrLevel& a : A->lv) a.grp.release();
+    A->xgrp.release();
     if (A->stream) (void)hipStreamDestroy(A->stream);
     delete A;
     return TMK_OK;
@@ -2241,70 +2503,31 @@ extern "C" int tmk_amr_destroy(tmk_amr* A)
 
 extern "C" int tmk_amr_create(int nlevels, tmk_level* const* levels, int ref_ratio, tmk_amr** out)
 {
-    if (!out || nlevels < 1 || !levels || ref_ratio < 2 || ref_ratio > TMK_AMR_MAXR) return TMK_E_ARG;
-    *out = nullptr;
-    for (int l = 0; l < nlevels; ++l) {
-        tmk_level* L = levels[l];
-        if (!L || L->batch || L->amr_stream || L->stations) return TMK_E_ARG;
-        if (L->tsize != levels[0]->tsize) return TMK_E_ARG;
-        for (int k = 0; k < l; ++k)
-            if (levels[k] == L) return TMK_E_ARG;
-    }
-    // level 0 covers its domain
-    {
-        const tmk_level* L0 = levels[0];
-        const tmk_box d = L0->ctx->hp.domain;
-        long long cells = 0;
-        for (const tmk_box& b : L0->boxes) cells += (long long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1);
-        if (cells != (long long)(d.hi[0] - d.lo[0] + 1) * (d.hi[1] - d.lo[1] + 1)) return TMK_E_NESTING;
-    }
-    tmk_amr* A = new (std::nothrow) tmk_amr;
-    if (!A) return TMK_E_NOMEM;
-    A->r = ref_ratio;
-    A->lv.resize(nlevels);
-    for (int l = 0; l < nlevels; ++l) A->lv[l].L = levels[l];
-    int e = TMK_OK;
-    if (hipStreamCreate(&A->stream) != hipSuccess) e = TMK_E_HIP;
-    for (int l = 0; l < nlevels && !e; ++l)
-        if (hipStreamSynchronize(levels[l]->ctx->stream) != hipSuccess) e = TMK_E_HIP;
-    if (!e) e = amr_layouts(A);
-    for (int l = 1; l < nlevels && !e; ++l) e = amr_build_pair(A, l);
-    for (int l = 0; l < nlevels && !e; ++l) {
-        AmrLevel& a = A->lv[l];
-        if (a.L->tsize == sizeof(double)) amr_set_flux_ptrs<double>(a);
-        else amr_set_flux_ptrs<float>(a);
-        a.L->amr_stream = A->stream;
-        e = level_own_group(a.L);
-        if (!e && a.L->dirty) {
-            a.L->dirty = false;
-            e = a.L->tsize == sizeof(double) ? group_exchange_now_t<double>(a.L->own, A->stream)
-                                             : group_exchange_now_t<float>(a.L->own, A->stream);
-        }
-    }
-    if (e) {
-        amr_free(A);
-        if (A->stream) (void)hipStreamDestroy(A->stream);
-        delete A;
+    if (!out || !levels) return TMK_E_ARG;
+    int e = tmk_amr_batch_create(nlevels, ref_ratio, out);
+    if (e) return e;
+    // the clock is set by the first tmk_amr_run
+    if ((e = tmk_amr_add(*out, levels, levels[0]->time, 1.0))) {
+        tmk_amr_destroy(*out);
+        *out = nullptr;
         return e;
     }
-    *out = A;
+    (*out)->mem[0]->dt0 = 0.0;
     return TMK_OK;
 }
 
 static inline dim3 amr_grid(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
-// AMRLevelLinElast::advance of level l: coarse-fine fill, the level step (update with
-// face fluxes, exchange, clock), register increments
+// AMRLevelLinElast::advance of level l for every member: coarse-fine fill, the level step
+// (update with flux capture, exchange, clock), register increments
 template <typename T>
 static int amr_advance_t(tmk_amr* A, int l)
 {
     AmrLevel& a = A->lv[l];
-    tmk_level* L = a.L;
-    Group& G = L->own;
+    Group& G = a.grp;
     hipStream_t st = A->stream;
-    const int n = (int)A->lv.size();
     if (l > 0) {
-        Group& GC = A->lv[l - 1].L->own;
+        Group& GC = A->lv[l - 1].grp;
         const Patch<T>* fine = (const Patch<T>*)G.patches_dev[G.g];
         const Patch<T>* cnew = (const Patch<T>*)GC.patches_dev[GC.g];      // state after the coarse step
         const Patch<T>* cold = (const Patch<T>*)GC.patches_dev[GC.g ^ 1];  // its input: the state before
@@ -2315,28 +2538,30 @@ static int amr_advance_t(tmk_amr* A, int l)
             HIPCHK(hipGetLastError());
         }
         if (a.nfaces > 0) {
-            hipLaunchKernelGGL(k_amr_fill_fault<T>, amr_grid((long long)a.nfaces * G.nb), dim3(256), 0, st,
-                               (const CfFace*)a.faces.p, a.nfaces, G.nb, fine, cold, cnew, G.scen_dev, GC.scen_dev,
-                               A->r, -1.0);
+            hipLaunchKernelGGL(k_amr_fill_fault<T>, amr_grid((long long)a.nfaces * TMK_NBDRY_RS), dim3(256), 0, st,
+                               (const CfFace*)a.faces.p, a.nfaces, TMK_NBDRY_RS, fine, cold, cnew, G.scen_dev,
+                               GC.scen_dev, A->r, -1.0);
             HIPCHK(hipGetLastError());
         }
     }
     const Patch<T>* pts = (const Patch<T>*)G.patches_dev[G.g];  // flux pointers are the same at both parities
     int e = group_enqueue_step(G, st);
     if (e) return e;
-    L->cur ^= 1;
-    L->time = L->time + L->dt;
-    if (l + 1 < n && a.nreg > 0) {
+    for (AmrMember* M : A->mem) {
+        tmk_level* L = M->L[l];
+        L->cur ^= 1;
+        L->time = L->time + L->dt;
+    }
+    if (l + 1 < A->nlev && a.nreg > 0) {
         hipLaunchKernelGGL(k_amr_reg_coarse<T>, amr_grid((long long)a.nreg * TMK_NCOMP), dim3(256), 0, st,
-                           (const RegFace*)a.regf.p, a.nreg, (double*)a.reg.p, pts, L->dt);
+                           (const RegFace*)a.regf.p, a.nreg, (double*)a.reg.p, pts, (const Scen*)G.scen_dev);
         HIPCHK(hipGetLastError());
     }
     if (l > 0) {
         AmrLevel& c = A->lv[l - 1];
         if (c.nreg > 0) {
-            const double w = L->dt * (1.0 / A->r);
             hipLaunchKernelGGL(k_amr_reg_fine<T>, amr_grid((long long)c.nreg * TMK_NCOMP), dim3(256), 0, st,
-                               (const RegFace*)c.regf.p, c.nreg, (double*)c.reg.p, pts, A->r, w);
+                               (const RegFace*)c.regf.p, c.nreg, (double*)c.reg.p, pts, A->r, (const Scen*)G.scen_dev);
             HIPCHK(hipGetLastError());
         }
     }
@@ -2348,10 +2573,10 @@ static int amr_advance_t(tmk_amr* A, int l)
 template <typename T>
 static int amr_post_t(tmk_amr* A, int l)
 {
-    if (l + 1 >= (int)A->lv.size()) return TMK_OK;
+    if (l + 1 >= A->nlev) return TMK_OK;
     AmrLevel& a = A->lv[l];
-    Group& G = a.L->own;
-    Group& GF = A->lv[l + 1].L->own;
+    Group& G = a.grp;
+    Group& GF = A->lv[l + 1].grp;
     hipStream_t st = A->stream;
     const Patch<T>* cur = (const Patch<T>*)G.patches_dev[G.g];
     const Patch<T>* fcur = (const Patch<T>*)GF.patches_dev[GF.g];
@@ -2368,8 +2593,8 @@ static int amr_post_t(tmk_amr* A, int l)
         HIPCHK(hipGetLastError());
     }
     if (a.navgf > 0) {
-        hipLaunchKernelGGL(k_amr_avg_fault<T>, amr_grid((long long)a.navgf * G.nb), dim3(256), 0, st,
-                           (const AvgFace*)a.avgf.p, a.navgf, G.nb, cur, fcur, r, 1.0 / r);
+        hipLaunchKernelGGL(k_amr_avg_fault<T>, amr_grid((long long)a.navgf * TMK_NBDRY_RS), dim3(256), 0, st,
+                           (const AvgFace*)a.avgf.p, a.navgf, TMK_NBDRY_RS, cur, fcur, r, 1.0 / r);
         HIPCHK(hipGetLastError());
     }
     return group_exchange_now_t<T>(G, st);
@@ -2380,7 +2605,7 @@ static int amr_time_step_t(tmk_amr* A, int l)
 {
     int e = amr_advance_t<T>(A, l);
     if (e) return e;
-    if (l + 1 < (int)A->lv.size())
+    if (l + 1 < A->nlev)
         for (int k = 0; k < A->r; ++k)
             if ((e = amr_time_step_t<T>(A, l + 1))) return e;
     return amr_post_t<T>(A, l);
@@ -2388,43 +2613,33 @@ static int amr_time_step_t(tmk_amr* A, int l)
 
 static int amr_time_step(tmk_amr* A)
 {
-    return A->lv[0].L->tsize == sizeof(double) ? amr_time_step_t<double>(A, 0) : amr_time_step_t<float>(A, 0);
+    return A->mem[0]->L[0]->tsize == sizeof(double) ? amr_time_step_t<double>(A, 0) : amr_time_step_t<float>(A, 0);
 }
 
-// nsteps coarse steps; the clocks are (re)set when time or dt0 is given anew
-extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
+// nsteps coarse steps of every member, each from its own clock with its own dt
+extern "C" int tmk_amr_advance(tmk_amr* A, int nsteps)
 {
-    if (!A || nsteps < 0 || !(dt0 > 0.0)) return TMK_E_ARG;
+    if (!A || nsteps < 0) return TMK_E_ARG;
+    if (A->mem.empty() || nsteps == 0) return TMK_OK;
     int e;
-    const double t_now = A->lv[0].L->time;
-    if (!A->clock || time != t_now || dt0 != A->dt0) {
-        if (dt0 != A->dt0 && A->gexec) {
-            (void)hipGraphExecDestroy(A->gexec);
-            A->gexec = nullptr;
-        }
-        double dt = dt0;
-        for (AmrLevel& a : A->lv) {
-            a.L->time = time;
-            a.L->dt = dt;
-            if ((e = group_set_clock(a.L->own, A->stream, 0, time, dt, a.L->ctx->P))) return e;
-            dt = dt / A->r;
-        }
-        A->dt0 = dt0;
-        A->clock = true;
-    }
-    Group& G0 = A->lv[0].L->own;
+    if (!A->built && (e = amr_build(A))) return e;
+    Group& G0 = A->lv[0].grp;
     while (nsteps > 0) {
         if (nsteps >= 2 && !A->gexec) {
             // capture two coarse steps: every level returns to its parity
-            std::vector<double> t0;
-            for (AmrLevel& a : A->lv) t0.push_back(a.L->time);
+            std::vector<std::vector<double>> t0;
+            for (AmrMember* M : A->mem) {
+                t0.emplace_back();
+                for (tmk_level* L : M->L) t0.back().push_back(L->time);
+            }
             hipGraph_t gr;
             HIPCHK(hipStreamBeginCapture(A->stream, hipStreamCaptureModeThreadLocal));
             A->gpar = G0.g;
             e = amr_time_step(A);
             if (!e) e = amr_time_step(A);
             hipError_t ce = hipStreamEndCapture(A->stream, &gr);
-            for (size_t l = 0; l < A->lv.size(); ++l) A->lv[l].L->time = t0[l];  // nothing ran yet
+            for (size_t m = 0; m < A->mem.size(); ++m)  // nothing ran yet
+                for (size_t l = 0; l < A->mem[m]->L.size(); ++l) A->mem[m]->L[l]->time = t0[m][l];
             if (e) return e;
             if (ce != hipSuccess) return TMK_E_HIP;
             hipError_t ie = hipGraphInstantiate(&A->gexec, gr, nullptr, nullptr, 0);
@@ -2433,12 +2648,13 @@ extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
         }
         if (nsteps >= 2 && G0.g == A->gpar) {
             HIPCHK(hipGraphLaunch(A->gexec, A->stream));
-            for (size_t l = 0; l < A->lv.size(); ++l) {
-                tmk_level* L = A->lv[l].L;
-                long long k = 2;
-                for (size_t q = 0; q < l; ++q) k *= A->r;
-                for (long long s = 0; s < k; ++s) L->time = L->time + L->dt;
-            }
+            for (AmrMember* M : A->mem)
+                for (size_t l = 0; l < M->L.size(); ++l) {
+                    tmk_level* L = M->L[l];
+                    long long k = 2;
+                    for (size_t q = 0; q < l; ++q) k *= A->r;
+                    for (long long s = 0; s < k; ++s) L->time = L->time + L->dt;
+                }
             nsteps -= 2;
             continue;
         }
@@ -2448,6 +2664,26 @@ extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
     return TMK_OK;
 }
 
+// one-scenario hierarchy: nsteps coarse steps of dt0; the level clocks are (re)set to time
+// when it differs from the hierarchy's current time or dt0 changes
+extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
+{
+    if (!A || nsteps < 0 || !(dt0 > 0.0) || A->mem.size() != 1) return TMK_E_ARG;
+    AmrMember* M = A->mem[0];
+    if (M->dt0 == 0.0 || time != M->L[0]->time || dt0 != M->dt0) {
+        int e = amr_detach(A);
+        if (e) return e;
+        double dt = dt0;
+        for (tmk_level* L : M->L) {
+            L->time = time;
+            L->dt = dt;
+            dt = dt / A->r;
+        }
+        M->dt0 = dt0;
+    }
+    return tmk_amr_advance(A, nsteps);
+}
+
 extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0, double* ms)
 {
     if (!A || !ms) return TMK_E_ARG;
@@ -2455,7 +2691,11 @@ extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     int e = TMK_OK;
-    if (hipEventRecord(a, A->stream) != hipSuccess) e = TMK_E_HIP;
+    // the first run after a change builds the groups (host work) before any device work
+    if (A->mem.size() == 1 && (A->mem[0]->dt0 == 0.0 || time != A->mem[0]->L[0]->time || dt0 != A->mem[0]->dt0))
+        e = tmk_amr_run(A, 0, time, dt0);
+    if (!e && !A->built) e = amr_build(A);
+    if (!e && hipEventRecord(a, A->stream) != hipSuccess) e = TMK_E_HIP;
     if (!e) e = tmk_amr_run(A, nsteps, time, dt0);
     if (!e && hipEventRecord(b, A->stream) != hipSuccess) e = TMK_E_HIP;
     if (!e && hipEventSynchronize(b) != hipSuccess) e = TMK_E_HIP;
@@ -2467,21 +2707,49 @@ extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0
     return e;
 }
 
+// waits; *code = the first error member level0 reported (0 = none)
+extern "C" int tmk_amr_member_error(tmk_amr* A, tmk_level* level0, int* code)
+{
+    if (!A || !level0 || !code) return TMK_E_ARG;
+    const int m = amr_find(A, level0);
+    if (m < 0) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(A->stream));
+    *code = A->mem[m]->err;
+    if (*code || !A->built) return TMK_OK;
+    for (AmrLevel& a : A->lv) {
+        int c = 0;
+        int e = group_error(a.grp, A->stream, m, &c);
+        if (e) return e;
+        if (c) {
+            *code = c;
+            return TMK_OK;
+        }
+    }
+    return TMK_OK;
+}
+
 // waits for the hierarchy's work; returns (and clears) the first error any level reported
 extern "C" int tmk_amr_sync(tmk_amr* A)
 {
     if (!A) return TMK_E_ARG;
     HIPCHK(hipStreamSynchronize(A->stream));
     int first = 0;
-    for (AmrLevel& a : A->lv) {
-        int code = 0;
-        int e = group_error(a.L->own, A->stream, 0, &code);
-        if (e) return e;
-        if (code) {
-            if (!first) first = code;
-            const int z = 0;
-            HIPCHK(hipMemcpy((char*)(a.L->own.scen_dev) + offsetof(Scen, err), &z, sizeof z, hipMemcpyHostToDevice));
+    for (AmrMember* M : A->mem)
+        if (M->err && !first) {
+            first = M->err;
+            M->err = 0;
         }
+    if (!A->built) return first;
+    for (AmrLevel& a : A->lv) {
+        std::vector<Scen> sc(A->mem.size());
+        HIPCHK(hipMemcpy(sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+        for (size_t m = 0; m < sc.size(); ++m)
+            if (sc[m].err) {
+                if (!first) first = sc[m].err;
+                const int z = 0;
+                HIPCHK(hipMemcpy((char*)(a.grp.scen_dev + m) + offsetof(Scen, err), &z, sizeof z,
+                                 hipMemcpyHostToDevice));
+            }
     }
     return first;
 }

@@ -8,8 +8,14 @@ fused launch as the others (tmk_batch). A scenario that arrives while others are
 running joins at the next chunk boundary. A finished one leaves, its fault data is
 read back, and its memory is freed for the queue.
 
-Reported: scenarios/s, cell-updates/s and p50/p99 time-to-solution
-(completion - arrival, wall clock).
+Scenarios with refined levels (0-2, ratio 2, around the nucleation) join a hierarchy
+batch of their level count (tmk_amr_add): level l of every such scenario is updated in
+the same launches, with the coarse-fine fills, registers and averages of all of them in
+shared launches too, one graph per two coarse steps on the batch's own stream, beside the
+single-level batch.
+
+Reported: scenarios/s, cell-updates/s (every level's cells times its steps) and p50/p99
+time-to-solution (completion - arrival, wall clock).
 
 The reference runs one scenario per process with no serving mode
 (amrGodunov.cpp:81-166). Each result is bit-identical alone and in the mix, because
@@ -22,19 +28,53 @@ import time
 
 import numpy as np
 
-from . import capi, scenarios
+from . import amr, capi, scenarios
 
 GHOST = scenarios.GHOST
 
 
-def draw_scenarios(n, seed, nx_range=(256, 2048), steps_range=(1000, 4000), steps_scale=1.0):
+AMR_RATIO = 2
+
+
+def lens_boxes(nx, ny, xn, half_frac, height_frac, scale, align=8, maxbox=64):
+    """Boxes (in the cells of a level refined by scale) of a region on the fault around coarse
+    cell xn: x within +-half_frac * nx, y below height_frac * ny (coarse cells), edges on
+    multiples of align fine cells, boxes of at most maxbox per side."""
+    fx0 = max(0, int((xn - half_frac * nx) * scale) // align * align)
+    fx1 = min(nx * scale, -(-int((xn + half_frac * nx) * scale) // align) * align)
+    fy1 = min(ny * scale, -(-int(height_frac * ny * scale) // align) * align)
+    boxes = []
+    for y0 in range(0, fy1, maxbox):
+        for x0 in range(fx0, fx1, maxbox):
+            boxes.append((x0, y0, min(x0 + maxbox, fx1) - 1, min(y0 + maxbox, fy1) - 1))
+    return boxes
+
+
+def amr_layout(sc):
+    """Box lists of a scenario with sc["levels"] refined levels (ratio AMR_RATIO): the base as one
+    box, level 1 over +-1/8 of the grid width and 1/8 of its height around the nucleation, level 2
+    over half of that."""
+    cfg = sc["cfg"]
+    nx, ny = cfg["nx"], cfg["ny"]
+    out = [[(0, 0, nx - 1, ny - 1)]]
+    if sc["levels"] >= 1:
+        out.append(lens_boxes(nx, ny, sc["xn"], 1.0 / 8, 1.0 / 8, AMR_RATIO))
+    if sc["levels"] >= 2:
+        out.append(lens_boxes(nx, ny, sc["xn"], 1.0 / 16, 1.0 / 16, AMR_RATIO * AMR_RATIO))
+    return out
+
+
+def draw_scenarios(n, seed, nx_range=(256, 2048), steps_range=(1000, 4000), steps_scale=1.0, max_levels=2):
     """Seeded scenario list.
 
     Grids run from 256x128 to 2048x1024, with the aspect ratio 2:1. Friction is
     slip weakening (config_c0 distributions) or rate-and-state (config_c1
-    distributions), drawn per scenario. The nucleation position is random.
+    distributions), drawn per scenario. The nucleation position is random.  Each
+    scenario has 0 to max_levels refined levels (ratio 2) around its nucleation,
+    drawn from a stream of its own (the other draws do not depend on max_levels).
     """
     rng = np.random.default_rng(seed)
+    lrng = np.random.default_rng(seed + 7777)
     out = []
     for k in range(n):
         lg = rng.uniform(math.log2(nx_range[0]), math.log2(nx_range[1]))
@@ -50,35 +90,68 @@ def draw_scenarios(n, seed, nx_range=(256, 2048), steps_range=(1000, 4000), step
             # move the nucleation patch: shift the per-face shear by a random offset
             shift = int(rng.integers(-nx // 4, nx // 4))
             cfg["faces"]["tau"] = np.roll(cfg["faces"]["tau"], shift)
+            xn = nx // 2 + shift
         else:
             cfg = scenarios.config_c1(seed=seed * 100003 + k, nx=nx, ny=ny, dx=dx)
             cfg["rs"]["nuc_x"] = float(rng.uniform(0.25, 0.75) * nx * dx)
             cfg["rs"]["nuc_R"] = 1500.0 * dx / 25.0
+            xn = int(cfg["rs"]["nuc_x"] / dx)
         steps = int(rng.integers(steps_range[0], steps_range[1] + 1) * steps_scale)
-        out.append({"id": k, "cfg": cfg, "steps": max(steps, 2)})
+        levels = int(lrng.integers(0, max_levels + 1)) if max_levels > 0 else 0
+        out.append({"id": k, "cfg": cfg, "steps": max(steps, 2), "levels": levels, "xn": xn})
     return out
 
 
 class Running:
-    __slots__ = ("sc", "ctx", "lev", "done_steps", "arrival", "admitted", "dt", "peak_V")
+    __slots__ = ("sc", "ctxs", "levels", "done_steps", "arrival", "admitted", "dt", "peak_V")
 
-    def __init__(self, sc, ctx, lev, arrival, admitted, dt):
-        self.sc, self.ctx, self.lev = sc, ctx, lev
+    def __init__(self, sc, ctxs, levels, arrival, admitted, dt):
+        self.sc, self.ctxs, self.levels = sc, ctxs, levels
         self.arrival, self.admitted, self.dt = arrival, admitted, dt
         self.done_steps = 0
         self.peak_V = None
 
+    @property
+    def lev(self):
+        return self.levels[0]
+
+    @property
+    def nlev(self):
+        return len(self.levels)
+
+    def close(self):
+        for lev in self.levels:
+            lev.close()
+        for ctx in self.ctxs:
+            ctx.close()
+
 
 def _start(sc, precision, tile):
+    """Contexts and levels of a scenario: one level, or a hierarchy around its nucleation."""
+    return amr.make_levels(sc["cfg"], amr_layout(sc), AMR_RATIO, precision, tile)
+
+
+def _footprint(sc, precision):
+    """Device bytes of a scenario: its levels' arrays plus, with refined levels, the flux arrays
+    and face masks of each level's islands (bounded by the boxes' bounding box)."""
+    layout = amr_layout(sc)
     cfg = sc["cfg"]
-    ctx = capi.Context(cfg, precision)
-    if tile:
-        ctx.set_tile(*tile)
-    box = (0, 0, cfg["nx"] - 1, cfg["ny"] - 1)
-    lev = capi.Level(ctx, [box])
-    B = scenarios.fault_arrays(cfg)
-    lev.fill(0, scenarios.background_vector(cfg), B)
-    return ctx, lev
+    tot = 0
+    for l, boxes in enumerate(layout):
+        ctx = capi.Context(cfg, precision)
+        tot += capi.level_footprint(ctx, boxes)
+        ctx.close()
+        if len(layout) > 1:
+            w = max(b[2] for b in boxes) - min(b[0] for b in boxes) + 1
+            h = max(b[3] for b in boxes) - min(b[1] for b in boxes) + 1
+            tot += 2 * scenarios.NCOMP * 8 * (w + 1) * (h + 1) + (w + 2) * (h + 2)
+        cfg = scenarios.refine_cfg(cfg, AMR_RATIO)
+    return tot
+
+
+def _cell_updates_per_step(sc):
+    return sum(sum((b[2] - b[0] + 1) * (b[3] - b[1] + 1) for b in boxes) * AMR_RATIO ** l
+               for l, boxes in enumerate(amr_layout(sc)))
 
 
 def _result(r, B):
@@ -91,6 +164,7 @@ def _result(r, B):
     V = B[nb_V, GHOST:-GHOST]
     ruptured = rt < 1e9
     return {"id": r.sc["id"], "fault": cfg["fault"], "nx": cfg["nx"], "ny": cfg["ny"], "steps": r.sc["steps"],
+            "levels": r.sc.get("levels", 0),
             "max_slip": float(np.max(slip)), "mean_slip": float(np.mean(slip)),
             "final_max_slip_rate": float(np.max(V)),
             "ruptured_fraction": float(np.mean(ruptured)),
@@ -111,8 +185,9 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
     arrivals = np.cumsum(gaps) - gaps[0]
     pending = list(zip(arrivals, range(len(scen_list))))
     queue = []
-    batch = capi.Batch()
-    running = []      # members of the batch
+    batch = capi.Batch()                  # single-level scenarios
+    hbatches = {}                         # level count -> hierarchy batch
+    running = []      # members of the batches
     ready = []        # set up, joining the batch at the next chunk boundary
     results, latencies = [], []
     n_failed = 0
@@ -120,7 +195,8 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
     footprint = {}    # (fault, nx, ny) -> device bytes of one scenario's level
     t0 = time.perf_counter()
     ip = 0
-    phase = {"admit": 0.0, "launch": 0.0, "sync": 0.0, "retire": 0.0, "idle": 0.0}
+    phase = {"admit": 0.0, "join": 0.0, "launch_levels": 0.0, "launch": 0.0, "sync": 0.0, "retire": 0.0,
+             "idle": 0.0}
     inflight = 0      # steps of the chunk in flight (0: none)
 
     def now():
@@ -142,25 +218,18 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
             arr, k = queue[0]
             sc = scen_list[k]
             cfg = sc["cfg"]
-            key = (cfg["fault"], cfg["nx"], cfg["ny"])
-            ctx = None
+            key = (cfg["fault"], cfg["nx"], cfg["ny"], sc.get("levels", 0), sc.get("xn", 0))
             if key not in footprint:
-                ctx = capi.Context(cfg, precision)
-                footprint[key] = capi.level_footprint(ctx, [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)])
+                footprint[key] = _footprint(sc, precision)
             free, _ = capi.mem_info()
             if footprint[key] + mem_reserve > free:
-                if ctx is not None:
-                    ctx.close()
                 break
             queue.pop(0)
-            if ctx is not None:
-                ctx.close()
-            ctx, lev = _start(sc, precision, tile)
-            ready.append(Running(sc, ctx, lev, arr, now(), scenarios.dt_of(cfg)))
+            ctxs, levels = _start(sc, precision, tile)
+            ready.append(Running(sc, ctxs, levels, arr, now(), scenarios.dt_of(cfg)))
 
     def retire(r, B, failed_code=None):
-        r.lev.close()
-        r.ctx.close()
+        r.close()
         running.remove(r)
         if failed_code is not None:
             return
@@ -175,9 +244,9 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
         latencies.append(tdone - r.arrival)
         if log:
             cfg = r.sc["cfg"]
-            log("done %d (%dx%d %s, %d steps) latency %.2fs running %d queued %d" %
-                (res["id"], cfg["nx"], cfg["ny"], cfg["fault"], r.sc["steps"], res["latency_s"],
-                 len(running), len(queue)))
+            log("done %d (%dx%d %s, %d refined levels, %d steps) latency %.2fs running %d queued %d" %
+                (res["id"], cfg["nx"], cfg["ny"], cfg["fault"], r.sc.get("levels", 0), r.sc["steps"],
+                 res["latency_s"], len(running), len(queue)))
 
     while ip < len(pending) or queue or running or ready or inflight:
         t = now()
@@ -187,22 +256,31 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
         with _clock("admit"):
             admit()
         if inflight:
-            failed = set()
+            failed = {}
             with _clock("sync"):
+                # a scenario's own error (e.g. a NaN state variable) retires only that scenario
                 try:
                     batch.sync()
                 except capi.TmkError:
-                    # a scenario's own error (e.g. a NaN state variable) retires only that scenario
-                    failed = {id(r) for r in running if batch.member_error(r.lev) != 0}
+                    for r in running:
+                        if r.nlev == 1 and batch.member_error(r.lev) != 0:
+                            failed[id(r)] = batch.member_error(r.lev)
+                for nl, ab in hbatches.items():
+                    try:
+                        ab.sync()
+                    except capi.TmkError:
+                        for r in running:
+                            if r.nlev == nl and ab.member_error(r.lev) != 0:
+                                failed[id(r)] = ab.member_error(r.lev)
             n = inflight
             inflight = 0
             for r in running:
                 r.done_steps += n
-                cell_updates += r.sc["cfg"]["nx"] * r.sc["cfg"]["ny"] * n
+                cell_updates += _cell_updates_per_step(r.sc) * n
             with _clock("retire"):
                 for r in [r for r in running if id(r) in failed]:
-                    code = batch.member_error(r.lev)
-                    batch.remove(r.lev)
+                    code = failed[id(r)]
+                    (batch if r.nlev == 1 else hbatches[r.nlev]).remove(r.lev)
                     res = {"id": r.sc["id"], "failed": capi.strerror(code), "arrival_s": float(r.arrival),
                            "latency_s": float(now() - r.arrival)}
                     results.append(res)
@@ -211,25 +289,39 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
                     if log:
                         log("failed %d: %s" % (res["id"], res["failed"]))
                 for r in [r for r in running if r.done_steps >= r.sc["steps"]]:
-                    batch.remove(r.lev)
+                    (batch if r.nlev == 1 else hbatches[r.nlev]).remove(r.lev)
                     cfg = r.sc["cfg"]
                     B = np.empty((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * GHOST))
                     r.lev.download_fault(0, B)
                     retire(r, B)
-        with _clock("launch"):
+        with _clock("join"):
             for r in ready:
-                batch.add(r.lev, 0.0, r.dt)
+                if r.nlev == 1:
+                    batch.add(r.lev, 0.0, r.dt)
+                else:
+                    if r.nlev not in hbatches:
+                        hbatches[r.nlev] = capi.AmrBatch(r.nlev, AMR_RATIO)
+                    hbatches[r.nlev].add(r.levels, 0.0, r.dt)
                 running.append(r)
             ready.clear()
-            if running:
-                n = min(chunk, min(r.sc["steps"] - r.done_steps for r in running))
+        if running:
+            n = min(chunk, min(r.sc["steps"] - r.done_steps for r in running))
+            # the single-level batch first, so the device works while the host prepares the
+            # hierarchy batches (each shares launches on its own stream; the first call after a
+            # membership change rebuilds its groups and captures its graph)
+            with _clock("launch"):
                 batch.run(n)
-                inflight = n
+            with _clock("launch_levels"):
+                for ab in hbatches.values():
+                    ab.advance(n)
+            inflight = n
         if not running and not ready and not queue and ip < len(pending):
             with _clock("idle"):
                 time.sleep(max(0.0, pending[ip][0] - now()))
     wall = now()
     batch.close()
+    for ab in hbatches.values():
+        ab.close()
     lat = np.array(latencies)
     stats = {"scenarios": len(results) - n_failed, "failed": n_failed, "wall_s": wall,
              "scenarios_per_s": (len(results) - n_failed) / wall,
@@ -250,12 +342,15 @@ def main():
     ap.add_argument("--nx-max", type=int, default=2048)
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--fp64", action="store_true")
+    ap.add_argument("--max-levels", type=int, default=2, help="refined levels per scenario drawn from 0..this")
     a = ap.parse_args()
-    sl = draw_scenarios(a.n, a.seed, nx_range=(256, a.nx_max), steps_scale=a.steps_scale)
+    sl = draw_scenarios(a.n, a.seed, nx_range=(256, a.nx_max), steps_scale=a.steps_scale, max_levels=a.max_levels)
     prec = capi.PREC_FP64 if a.fp64 else capi.PREC_FP32_BULK
     res, stats = serve(sl, a.rate, precision=prec, chunk=a.chunk,
                        log=lambda m: print(m, flush=True))
     stats.update({"n": a.n, "steps_scale": a.steps_scale, "precision": "fp64" if a.fp64 else "fp32-bulk",
+                  "scenarios_by_levels": {str(k): sum(1 for sc in sl if sc.get("levels", 0) == k)
+                                          for k in range(a.max_levels + 1)},
                   "grid_range": "256x128..%dx%d" % (a.nx_max, a.nx_max // 2)})
     print(json.dumps(stats), flush=True)
 

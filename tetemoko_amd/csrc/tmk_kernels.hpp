@@ -800,15 +800,6 @@ __global__ void k_station_record(const Patch<T>* patches, const Scen* scen, Stat
     if (threadIdx.x == 0) s.count[0] = b + 1;
 }
 
-// every scenario's clock advances by its own step (AMRLevelLinElast.cpp:478, m_time += m_dt)
-// fill every plane of a box with a uniform state (LEINITF, LinElastPhysicsF.ChF:1215-1263)
-template <typename T>
-__global__ void k_fill(T* U, size_t plane, StateVec v)
-{
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (size_t)gridDim.x * blockDim.x)
-        for (int c = 0; c < 22; ++c) U[q + c * plane] = (T)v.c[c];
-}
-
 // refinement tags (AMRLevelLinElast::tagCells, AMRLevelLinElast.cpp:577-717; RSIBC::tagCellsInit
 // RSIBC.cpp:341-377; SWIBC::tagCellsInit SWIBC.cpp:336-396) on the current state of a level
 struct TagArgs {
@@ -899,6 +890,7 @@ static __global__ void k_dilate(const unsigned char* raw, unsigned char* out, in
     out[t] = v;
 }
 
+// every scenario's clock advances by its own step (AMRLevelLinElast.cpp:478, m_time += m_dt)
 static __global__ void k_advance_time(Scen* s, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;

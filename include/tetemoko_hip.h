@@ -248,7 +248,8 @@ int tmk_amr_destroy(tmk_amr *amr);
 int tmk_amr_run(tmk_amr *amr, int nsteps, double time, double dt0);
 /* the same, timed with HIP events on the hierarchy's stream (waits) */
 int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double *ms);
-/* waits; the first error any level reported (0 = none), cleared */
+/* waits; the first error any member reported (0 = none); a member's error stays with it
+ * until it leaves (tmk_amr_member_error names the member) */
 int tmk_amr_sync(tmk_amr *amr);
 
 /* Hierarchy batches: scenarios with the same number of levels and ratio whose level-l boxes

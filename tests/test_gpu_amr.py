@@ -309,3 +309,43 @@ def test_hierarchy_batch_members_equal_alone():
         lev.close()
     for ctx in ctxA + ctxB:
         ctx.close()
+
+
+def test_hierarchy_batch_member_error_isolated():
+    """A member whose fine-level fault data holds a NaN psi reports its own error; the other
+    member is bit-identical to running alone and the batch carries on once the failed one
+    leaves."""
+    good = scenarios.config_c1(seed=5, nx=48, ny=32, dx=25.0)
+    bad = scenarios.config_c1(seed=6, nx=48, ny=32, dx=25.0)
+    boxes = [[(0, 0, 47, 31)], [(24, 0, 71, 23)]]
+    ctxg, levg = amr.make_levels(good, boxes, 2)
+    ctxb, levb = amr.make_levels(bad, boxes, 2)
+    cfgb1 = scenarios.refine_cfg(bad, 2)
+    U1, B1 = scenarios.level_arrays(cfgb1)
+    B1[10, G + 40] = np.nan
+    levb[1].upload(0, scenarios.box_view(U1, boxes[1][0]), scenarios.box_bview(B1, boxes[1][0]))
+    ab = capi.AmrBatch(2, 2)
+    ab.add(levg, 0.0, scenarios.dt_of(good))
+    ab.add(levb, 0.0, scenarios.dt_of(bad))
+    ab.advance(3)
+    with pytest.raises(capi.TmkError) as ei:
+        ab.sync()
+    assert ei.value.code == capi.E_PSI_NAN
+    assert ab.member_error(levb[0]) == capi.E_PSI_NAN
+    assert ab.member_error(levg[0]) == 0
+    ab.remove(levb[0])
+    ab.advance(2)
+    ab.sync()
+    ab.remove(levg[0])
+    ab.close()
+    got = _download_levels(levg, boxes, _cfgs(good, 2, 2))
+    H = amr.Hierarchy(good, boxes, 2)
+    H.run(5)
+    H.sync()
+    for l in range(2):
+        Ua, Ba = H.download(l)
+        assert np.array_equal(np.nan_to_num(got[l][0], nan=7.0), np.nan_to_num(Ua, nan=7.0)), l
+        assert np.array_equal(np.nan_to_num(got[l][1], nan=7.0), np.nan_to_num(Ba, nan=7.0)), l
+    H.close()
+    for o in levg + levb + ctxg + ctxb:
+        o.close()

@@ -2728,28 +2728,21 @@ extern "C" int tmk_amr_member_error(tmk_amr* A, tmk_level* level0, int* code)
     return TMK_OK;
 }
 
-// waits for the hierarchy's work; returns (and clears) the first error any level reported
+// waits for the hierarchy's work; returns the first error any member reported.  As in a
+// batch, a member's error stays with it (tmk_amr_member_error) until it leaves.
 extern "C" int tmk_amr_sync(tmk_amr* A)
 {
     if (!A) return TMK_E_ARG;
     HIPCHK(hipStreamSynchronize(A->stream));
     int first = 0;
     for (AmrMember* M : A->mem)
-        if (M->err && !first) {
-            first = M->err;
-            M->err = 0;
-        }
-    if (!A->built) return first;
+        if (M->err && !first) first = M->err;
+    if (!A->built || first) return first;
     for (AmrLevel& a : A->lv) {
         std::vector<Scen> sc(A->mem.size());
         HIPCHK(hipMemcpy(sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
-        for (size_t m = 0; m < sc.size(); ++m)
-            if (sc[m].err) {
-                if (!first) first = sc[m].err;
-                const int z = 0;
-                HIPCHK(hipMemcpy((char*)(a.grp.scen_dev + m) + offsetof(Scen, err), &z, sizeof z,
-                                 hipMemcpyHostToDevice));
-            }
+        for (size_t m = 0; m < sc.size() && !first; ++m)
+            if (sc[m].err) first = sc[m].err;
     }
     return first;
 }

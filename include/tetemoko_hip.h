@@ -43,6 +43,7 @@ extern "C" {
 #define TMK_E_OVERFLOW (-10)   /* station records dropped: buffer full before it was read */
 #define TMK_E_AMR_TIME (-11)   /* LinElastLevelGodunov.cpp:250-258 "alpha < 0.0" / "alpha > 1.0" */
 #define TMK_E_NESTING (-12)    /* a finer level is not aligned to the ratio or not properly nested */
+#define TMK_E_IO (-13)         /* file output failed (boundary plot files, libtmk_h5.so) */
 
 #define TMK_FAULT_RS 0
 #define TMK_FAULT_SW 1

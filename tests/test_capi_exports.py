@@ -4,14 +4,18 @@ import glob
 import os
 import re
 
-from tetemoko_amd import capi
+from tetemoko_amd import bdry_plot, capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# each header's entry points live in one library
+LIBS = {"tetemoko_hip.h": (capi.LIB_PATH, capi.EXPORTED),
+        "tetemoko_h5.h": (bdry_plot.LIB_PATH, bdry_plot.EXPORTED)}
 
 
-def _declared():
+def _declared(header=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    hs = [os.path.join(ROOT, "include", header)] if header else glob.glob(os.path.join(ROOT, "include", "*.h"))
+    for h in hs:
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(tmk_\w+)\s*\(", txt, flags=re.M):
@@ -20,15 +24,19 @@ def _declared():
 
 
 def test_header_declares_entry_points():
-    names = _declared()
+    assert {os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h"))} == set(LIBS)
+    names = _declared("tetemoko_hip.h")
     assert "tmk_patch_update_state" in names and "tmk_level_step" in names
-    assert names == set(capi.EXPORTED), names ^ set(capi.EXPORTED)
+    for h, (_, exported) in LIBS.items():
+        names = _declared(h)
+        assert names == set(exported), (h, names ^ set(exported))
 
 
 def test_library_exports_all_declared():
-    L = ctypes.CDLL(capi.LIB_PATH)
-    for n in _declared():
-        assert hasattr(L, n), n
+    for h, (path, _) in LIBS.items():
+        L = ctypes.CDLL(path)
+        for n in _declared(h):
+            assert hasattr(L, n), (h, n)
 
 
 def test_strerror_without_device():

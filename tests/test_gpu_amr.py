@@ -349,3 +349,47 @@ def test_hierarchy_batch_member_error_isolated():
     H.close()
     for o in levg + levb + ctxg + ctxb:
         o.close()
+
+
+def test_boundary_plot_file_of_a_hierarchy(tmp_path):
+    """The boundary plot file written from a device hierarchy (tetemoko_amd.bdry_plot) holds the
+    oracle's fault data bit for bit on every valid fault face of every level (slip-weakening
+    rupture, three levels), and a box's x-ghost entry equals its neighbour's valid face."""
+    import os
+    from h5dump_util import H5DUMP, h5_attr, h5_data, h5_ints
+    from tetemoko_amd import bdry_plot
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not available")
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    cfgs = _cfgs(cfg, 3, 2)
+    arrays = _background_arrays(cfgs)
+    h = oamr.Hierarchy(cfg, BOXES_3, 2)
+    h.init_state(lambda l: arrays(l)[0], lambda l: arrays(l)[1])
+    h.set_dt(scenarios.dt_of(cfg))
+    assert h.advance(3) == 0
+    H = amr.Hierarchy(cfg, BOXES_3, 2, arrays=arrays)
+    H.run(3)
+    path = bdry_plot.write_hierarchy(str(tmp_path / "run."), H)
+    assert path.endswith("run.boundary.000003.2d.hdf5")
+    assert h5_attr(path, "/iteration") == 3 and h5_attr(path, "/num_levels") == 3
+    assert h5_attr(path, "/time") == H.time
+    for l, lev in enumerate(h.levels):
+        g = "/level_%d" % l
+        assert h5_attr(path, g + "/dx") == cfgs[l]["dx"]
+        assert h5_attr(path, g + "/dt") == H.dt0 / 2 ** l
+        boxes = np.array(h5_ints(path, g + "/boxes")).reshape(-1, 4)
+        assert [tuple(b) for b in boxes] == sorted((b[0], 0, b[2], 0) for b in BOXES_3[l] if b[1] == 0)
+        offs = h5_data(path, g + "/data:offsets=0", "<i8", tmp_path)
+        data = h5_data(path, g + "/data:datatype=0", "<f8", tmp_path)
+        nb = lev.B.shape[0]
+        covered = set()
+        for b in boxes:
+            covered.update(range(b[0], b[2] + 1))
+        for k, b in enumerate(boxes):
+            n = b[2] - b[0] + 1
+            blk = data[offs[k]:offs[k + 1]].reshape(nb, n + 2)
+            assert np.array_equal(blk[:, 1:-1], lev.B[:, G + b[0]:G + b[2] + 1]), (l, k)
+            for col, i in ((0, b[0] - 1), (-1, b[2] + 1)):
+                if i in covered:
+                    assert np.array_equal(blk[:, col], lev.B[:, G + i]), (l, k, i)
+    H.close()

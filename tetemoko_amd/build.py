@@ -37,6 +37,28 @@ def build(verbose=True):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose, "libtetemoko_hip.so")
     os.replace(tmp, out)
     build_copy_calib(verbose)
+    build_h5(verbose)
+    return out
+
+
+HDF5_ROOT = os.environ.get("TMK_HDF5_ROOT", "/opt/conda")
+
+
+def build_h5(verbose=True):
+    """libtmk_h5.so: the boundary plot-file writer (host C++ against the HDF5 C library of
+    HDF5_ROOT, found at run time through the rpath).  Skipped, with a message, when the HDF5
+    headers are absent; tetemoko_amd.bdry_plot then raises when it is used."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print("libtmk_h5.so skipped: no hdf5.h under " + inc, flush=True)
+        return None
+    out = os.path.join(HERE, "_lib", "libtmk_h5.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + inc, "-o", out + ".tmp", os.path.join(HERE, "csrc", "tmk_h5.cpp"),
+           "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir]
+    _run(cmd, verbose, "tmk_h5.cpp")
+    os.replace(out + ".tmp", out)
     return out
 
 

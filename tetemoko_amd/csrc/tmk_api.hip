@@ -317,7 +317,8 @@ static const char* k_errs[] = {"ok",
                                "device memory exhausted",
                                "station records dropped: buffer full before it was read",
                                "coarse-fine time interpolation weight outside [0, 1]",
-                               "finer level not aligned to the refinement ratio or not properly nested"};
+                               "finer level not aligned to the refinement ratio or not properly nested",
+                               "file output failed"};
 
 extern "C" const char* tmk_strerror(int code)
 {

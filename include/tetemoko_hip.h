@@ -243,9 +243,10 @@ int tmk_stations_destroy(tmk_stations *st);
 typedef struct tmk_amr tmk_amr;
 int tmk_amr_create(int nlevels, tmk_level *const *levels, int ref_ratio, tmk_amr **out);
 int tmk_amr_destroy(tmk_amr *amr);
-/* one-scenario hierarchy: nsteps coarse steps of dt0 (level l steps dt0 / ref_ratio^l); the
- * level clocks are set to time when it differs from the hierarchy's current time or dt0
- * changes */
+/* one-scenario hierarchy: nsteps coarse steps of dt0 (level l steps dt0 / ref_ratio^l).  A time
+ * other than the hierarchy's current coarse time (the first call's, or the previous calls'
+ * start plus their steps) sets every level's clock to it; a new dt0 at the current time keeps
+ * the clocks running, each the sum of its level's steps (AMRLevel's m_time). */
 int tmk_amr_run(tmk_amr *amr, int nsteps, double time, double dt0);
 /* the same, timed with HIP events on the hierarchy's stream (waits) */
 int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double *ms);

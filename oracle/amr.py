@@ -404,6 +404,14 @@ class Hierarchy:
             lev.U[:] = U_of_level(l)
             lev.B[:] = B_of_level(l)
 
+    def change_dt(self, dt0):
+        """A new coarse dt between coarse steps: level l steps dt0 / r^l from now on and every
+        level keeps its clock, the sum of its own steps (AMRLevel's m_time)."""
+        dt = dt0
+        for lev in self.levels:
+            lev.dt = dt
+            dt = dt / self.r
+
     def set_dt(self, dt0, t0=0.0):
         dt = dt0
         for lev in self.levels:

@@ -236,3 +236,21 @@ def test_conservation_check_detects_missing_reflux(monkeypatch):
     h.advance(2)
     s1 = composite_sum(h)
     assert np.abs(s1 - s0).max() > 1e-6 * np.abs(s0).max()
+
+
+def test_dt_schedule():
+    """amr.dt_schedule: the first coarse dt is initial_cfl dx / c, each later one grows by at
+    most max_dt_growth up to cfl dx / c; fixed_dt replaces the schedule; the deck's values
+    (cfl = initial_cfl = 0.9, max_dt_growth = 1; bssa_flat_amr.input:140-145) give a constant dt."""
+    from tetemoko_amd import amr as damr
+    cfg = scenarios.config_c0(seed=1, nx=16, ny=8, dx=50.0)
+    c = max(cfg["cs"], cfg["cp"])
+    d = damr.dt_schedule(40, cfg, cfl=0.8, initial_cfl=0.1, max_dt_growth=1.1)
+    ramp = int(np.ceil(np.log(8.0) / np.log(1.1)))
+    want = [min(0.8, 0.1 * 1.1 ** k) * cfg["dx"] / c for k in range(40)]
+    assert np.allclose(d, want, rtol=1e-13, atol=0)
+    assert d[ramp - 1] < 0.8 * cfg["dx"] / c and d[ramp] == 0.8 * cfg["dx"] / c
+    assert all(b <= 1.1 * a * (1 + 1e-15) for a, b in zip(d, d[1:]))
+    assert damr.dt_schedule(5, cfg, fixed_dt=0.002) == [0.002] * 5
+    deck = damr.dt_schedule(5, cfg, cfl=0.9, initial_cfl=0.9, max_dt_growth=1.0)
+    assert deck == [0.9 * cfg["dx"] / c] * 5 == [scenarios.dt_of(dict(cfg, cfl=0.9))] * 5

@@ -393,3 +393,33 @@ def test_boundary_plot_file_of_a_hierarchy(tmp_path):
                 if i in covered:
                     assert np.array_equal(blk[:, col], lev.B[:, G + i]), (l, k, i)
     H.close()
+
+
+def test_dt_schedule_ramp_matches_oracle():
+    """A run whose coarse dt ramps from initial_cfl by max_dt_growth to the CFL dt
+    (amr.dt_schedule) equals the oracle driven by the same dt sequence, bit for bit on every
+    level (slip-weakening, three levels).  Consecutive device calls (one per run of equal dt)
+    continue the clocks, and a change of dt keeps every level's clock running, as the
+    oracle's change_dt does."""
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    arrays = _background_arrays(_cfgs(cfg, 3, 2))
+    dts = amr.dt_schedule(6, cfg, cfl=0.9, initial_cfl=0.3, max_dt_growth=1.5)
+    assert len(set(dts)) == 4
+    h = oamr.Hierarchy(cfg, BOXES_3, 2)
+    h.init_state(lambda l: arrays(l)[0], lambda l: arrays(l)[1])
+    h.set_dt(dts[0])
+    t = 0.0
+    for k, dt in enumerate(dts):
+        if k and dt != dts[k - 1]:
+            h.change_dt(dt)
+        assert h.advance(1) == 0
+        t = t + dt
+    H = amr.Hierarchy(cfg, BOXES_3, 2, arrays=arrays)
+    H.run_schedule(dts)
+    H.sync()
+    assert H.time == t == h.levels[0].time
+    for l, lev in enumerate(h.levels):
+        Ug, Bg = H.download(l)
+        assert np.array_equal(Ug[:, G:-G, G:-G][:, lev.valid], lev.U[:, G:-G, G:-G][:, lev.valid]), l
+        assert np.array_equal(Bg[:, G:-G][:, lev.fvalid], lev.B[:, G:-G][:, lev.fvalid]), l
+    H.close()

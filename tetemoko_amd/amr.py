@@ -36,6 +36,29 @@ def make_levels(cfg0, boxes_per_level, r, precision=capi.PREC_FP64, tile=None):
     return ctxs, levels
 
 
+def dt_schedule(nsteps, cfg0, cfl=None, initial_cfl=None, max_dt_growth=1.1, fixed_dt=-1.0):
+    """Coarse dt of each of nsteps coarse steps as the AMR driver assigns it
+    (amrGodunov.cpp:443-461: cfl, initial_cfl, max_dt_growth, fixed_dt).
+
+    Every level's dt is a CFL number times dx_l / max(cs, cp) (AMRLevelLinElast.cpp:479-480,
+    :1892-1901), and dx_l r^l = dx_0, so the coarse dt the driver takes as the minimum over
+    levels is the same number times dx_0 / max(cs, cp): the first step uses initial_cfl, every
+    later one min(cfl dx_0 / c, max_dt_growth times the previous dt); a positive fixed_dt
+    replaces all of that.  The wave speed is constant, so the schedule ramps geometrically
+    from the initial dt to the CFL dt and stays there.  Defaults of cfl and initial_cfl: the
+    configuration's cfl."""
+    if fixed_dt is not None and fixed_dt > 0:
+        return [float(fixed_dt)] * nsteps
+    c = max(cfg0["cs"], cfg0["cp"])
+    cfl = cfg0.get("cfl", 0.9) if cfl is None else cfl
+    initial_cfl = cfl if initial_cfl is None else initial_cfl
+    dt, out = initial_cfl * cfg0["dx"] / c, []
+    for _ in range(nsteps):
+        out.append(dt)
+        dt = min(cfl * cfg0["dx"] / c, max_dt_growth * dt)
+    return out
+
+
 class Hierarchy:
     def __init__(self, cfg0, boxes_per_level, r, precision=capi.PREC_FP64, tile=None, arrays=None):
         """arrays(l) -> (U, B) level arrays over the whole level domain with ghosts (the
@@ -107,6 +130,18 @@ class Hierarchy:
         for _ in range(nsteps):
             self.time = self.time + self.dt0
         self.steps += nsteps
+
+    def run_schedule(self, dts):
+        """One coarse step per entry of dts (coarse dt of each step, e.g. dt_schedule(...));
+        runs of equal dt go to the device as one call (two coarse steps per graph)."""
+        k = 0
+        while k < len(dts):
+            n = 1
+            while k + n < len(dts) and dts[k + n] == dts[k]:
+                n += 1
+            self.dt0 = float(dts[k])
+            self.run(n)
+            k += n
 
     def run_timed(self, nsteps):
         ms = self.amr.run_timed(nsteps, self.time, self.dt0)

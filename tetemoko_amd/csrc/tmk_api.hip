@@ -1871,6 +1871,7 @@ struct AmrMember {
     std::vector<std::vector<unsigned char*>> fmask;  // [level][island]
     std::vector<PairLists*> lists;                   // [pair l-1, l], member-local indices
     double dt0 = 0.0;
+    double tnext = 0.0;                              // coarse clock after the queued steps (tmk_amr_run)
     int err = 0;                                     // first error seen at a detach
     ~AmrMember();
 };
@@ -2671,18 +2672,25 @@ extern "C" int tmk_amr_run(tmk_amr* A, int nsteps, double time, double dt0)
 {
     if (!A || nsteps < 0 || !(dt0 > 0.0) || A->mem.size() != 1) return TMK_E_ARG;
     AmrMember* M = A->mem[0];
-    if (M->dt0 == 0.0 || time != M->L[0]->time || dt0 != M->dt0) {
-        int e = amr_detach(A);
+    const bool restart = M->dt0 == 0.0 || time != M->tnext;
+    if (restart || dt0 != M->dt0) {
+        int e = amr_detach(A);  // reads the device clocks back
         if (e) return e;
+        // another time restarts every level's clock there; a new dt at the current time keeps
+        // them running, each level's clock the sum of its own steps (AMRLevel's m_time)
         double dt = dt0;
         for (tmk_level* L : M->L) {
-            L->time = time;
+            if (restart) L->time = time;
             L->dt = dt;
             dt = dt / A->r;
         }
         M->dt0 = dt0;
+        if (restart) M->tnext = time;
     }
-    return tmk_amr_advance(A, nsteps);
+    const int e = tmk_amr_advance(A, nsteps);
+    // the coarse clock the device reaches: k_advance_time's additions, in the same order
+    for (int k = 0; k < nsteps; ++k) M->tnext = M->tnext + dt0;
+    return e;
 }
 
 extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0, double* ms)
@@ -2693,7 +2701,7 @@ extern "C" int tmk_amr_run_timed(tmk_amr* A, int nsteps, double time, double dt0
     HIPCHK(hipEventCreate(&b));
     int e = TMK_OK;
     // the first run after a change builds the groups (host work) before any device work
-    if (A->mem.size() == 1 && (A->mem[0]->dt0 == 0.0 || time != A->mem[0]->L[0]->time || dt0 != A->mem[0]->dt0))
+    if (A->mem.size() == 1 && (A->mem[0]->dt0 == 0.0 || time != A->mem[0]->tnext || dt0 != A->mem[0]->dt0))
         e = tmk_amr_run(A, 0, time, dt0);
     if (!e && !A->built) e = amr_build(A);
     if (!e && hipEventRecord(a, A->stream) != hipSuccess) e = TMK_E_HIP;

@@ -826,7 +826,7 @@ static void slope_limiter(tmo_fab *dW, const tmo_fab *dWl, const tmo_fab *dWr, t
 }
 
 /* fourth-order slopes from limited second-order slopes (Colella & Sekora form) */
-static void fourth_order_slopes(tmo_fab *dW, const tmo_fab *W, const tmo_fab *dWvL, int dir,
+void tmo_fourth_order_slopes(tmo_fab *dW, const tmo_fab *W, const tmo_fab *dWvL, int dir,
                                 tmo_box box, tmo_box domain)
 {
     const int io = dir == 0, jo = dir == 1;
@@ -875,7 +875,7 @@ void tmo_ppm_face_values(tmo_fab *WFace, const tmo_fab *W, int nc, int limit, in
 }
 
 /* PPM limiter of Colella & Woodward (1984) eq. 1.10 on the face increments */
-static void ppm_limiter(tmo_fab *dWm, tmo_fab *dWp, tmo_box box)
+void tmo_ppm_limiter(tmo_fab *dWm, tmo_fab *dWp, tmo_box box)
 {
     for (int c = 0; c < TMO_NCOMP; ++c)
         for (int j = box.lo[1]; j <= box.hi[1]; ++j)
@@ -892,7 +892,7 @@ static void ppm_limiter(tmo_fab *dWm, tmo_fab *dWp, tmo_box box)
 }
 
 /* PPM characteristic tracing (Colella & Woodward 1984 eq. 1.12), per field */
-static void ppm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *lambda, double dtbydx, tmo_box box)
+void tmo_ppm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *lambda, double dtbydx, tmo_box box)
 {
     for (int c = 0; c < TMO_NCOMP; ++c)
         for (int j = box.lo[1]; j <= box.hi[1]; ++j)
@@ -907,7 +907,7 @@ static void ppm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *lambda, d
 }
 
 /* PLM characteristic predictor (Colella 1990 reference-state form) */
-static void plm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *dW, const tmo_fab *lambda,
+void tmo_plm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *dW, const tmo_fab *lambda,
                             double dtbydx, tmo_box box)
 {
     for (int j = box.lo[1]; j <= box.hi[1]; ++j)
@@ -1030,7 +1030,7 @@ static void normal_pred(const tmo_params *p, tmo_fab *WM, tmo_fab *WP, const tmo
             tmo_fab dWvL;
             fab_new(&dWvL, boxVL, nc, NAN_FILL);
             tmo_vanleer_slopes(&dWvL, W, nc, limit, dir, boxVL, p->domain);
-            fourth_order_slopes(&dW, W, &dWvL, dir, box, p->domain);
+            tmo_fourth_order_slopes(&dW, W, &dWvL, dir, box, p->domain);
             fab_free(&dWvL);
         } else {
             tmo_vanleer_slopes(&dW, W, nc, limit, dir, box, p->domain);
@@ -1046,7 +1046,7 @@ static void normal_pred(const tmo_params *p, tmo_fab *WM, tmo_fab *WP, const tmo
         if (p->char_limiting || p->prim_limiting)
             if (p->char_limiting || p->fourth_order) slope_limiter(&dW, WM, WP, box);
         if (!p->char_limiting) tmo_charanalysis(&dW, p, dir, box);
-        plm_normal_pred(WM, WP, &dW, &lambda, dt / p->dx, box);
+        tmo_plm_normal_pred(WM, WP, &dW, &lambda, dt / p->dx, box);
         fab_free(&dW);
     } else {
         tmo_box facebox = bnodes(box, dir);
@@ -1067,12 +1067,12 @@ static void normal_pred(const tmo_params *p, tmo_fab *WM, tmo_fab *WP, const tmo
             tmo_charanalysis(WM, p, dir, box);
             tmo_charanalysis(WP, p, dir, box);
         }
-        if (p->char_limiting || p->prim_limiting) ppm_limiter(WM, WP, box);
+        if (p->char_limiting || p->prim_limiting) tmo_ppm_limiter(WM, WP, box);
         if (!p->char_limiting) {
             tmo_charanalysis(WM, p, dir, box);
             tmo_charanalysis(WP, p, dir, box);
         }
-        ppm_normal_pred(WM, WP, &lambda, dt / p->dx, box);
+        tmo_ppm_normal_pred(WM, WP, &lambda, dt / p->dx, box);
     }
     tmo_charsynthesis(WM, p, dir, box);
     tmo_charsynthesis(WP, p, dir, box);

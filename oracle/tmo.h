@@ -109,6 +109,14 @@ void tmo_vanleer_slopes(tmo_fab *dW, const tmo_fab *W, int nc, int limit, int di
                         tmo_box box, tmo_box domain);
 void tmo_ppm_face_values(tmo_fab *WFace, const tmo_fab *W, int nc, int limit, int dir,
                          tmo_box facebox, tmo_box domain);
+/* the remaining restated GodunovUtilities stages, exported for the pin tests; each works on
+ * all TMO_NCOMP components of its arrays over box */
+void tmo_fourth_order_slopes(tmo_fab *dW, const tmo_fab *W, const tmo_fab *dWvL, int dir,
+                             tmo_box box, tmo_box domain);
+void tmo_ppm_limiter(tmo_fab *dWm, tmo_fab *dWp, tmo_box box);
+void tmo_ppm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *lambda, double dtbydx, tmo_box box);
+void tmo_plm_normal_pred(tmo_fab *dWm, tmo_fab *dWp, const tmo_fab *dW, const tmo_fab *lambda,
+                         double dtbydx, tmo_box box);
 
 /* --- patch integrator: LinElastPatchGodunov::updateState ---
  * U   : ghosted state (22 comps), updated in place over U->b

@@ -104,6 +104,10 @@ def lib():
         L.tmo_plastic_correction.argtypes = [PF, PP, _D, Box]
         L.tmo_vanleer_slopes.argtypes = [PF, PF, ctypes.c_int, ctypes.c_int, ctypes.c_int, Box, Box]
         L.tmo_ppm_face_values.argtypes = [PF, PF, ctypes.c_int, ctypes.c_int, ctypes.c_int, Box, Box]
+        L.tmo_fourth_order_slopes.argtypes = [PF, PF, PF, ctypes.c_int, Box, Box]
+        L.tmo_ppm_limiter.argtypes = [PF, PF, Box]
+        L.tmo_ppm_normal_pred.argtypes = [PF, PF, PF, ctypes.c_double, Box]
+        L.tmo_plm_normal_pred.argtypes = [PF, PF, PF, PF, ctypes.c_double, Box]
         L.tmo_update_state.argtypes = [PF, PF, PF, PF, PP, Box, _D, _D]
         L.tmo_update_state.restype = ctypes.c_int
         L.tmo_fullgrad.argtypes = [PF, PF, Box, Box]

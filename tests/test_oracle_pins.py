@@ -12,7 +12,11 @@ operand fails:
     outgoing characteristic, background incoming characteristic);
   * the friction law solved with scipy's brentq;
   * Drucker-Prager consistency Y_after = eta * lambda_dot and sqrt(J2) = tau';
-  * PPM exactness on cubic cell averages;
+  * PPM exactness on cubic cell averages, fourth-order slopes exact on quartic
+    cell averages, the textbook values of the monotonized central limiter;
+  * the PPM limiter against Colella & Woodward's face-value form and the
+    monotone parabola it guarantees; PPM and PLM tracing against the exact
+    average of the reconstruction over each wave's domain of dependence;
   * free-stream preservation, mirror symmetry and box-decomposition invariance.
 """
 import ctypes
@@ -455,6 +459,154 @@ def test_limited_slopes_are_monotone():
         else:
             assert abs(s[i]) <= 2 * min(abs(l), abs(r)) + 1e-15
             assert s[i] * (l + r) >= 0
+
+
+def test_vanleer_slopes_are_the_monotonized_central_limiter():
+    """Textbook values of van Leer's monotonized central limiter, chosen so that minmod,
+    superbee, van Albada and the unlimited slope each give another value in at least one case:
+    (left, right differences) -> slope."""
+    cases = [((1.0, 3.0), 2.0), ((1.0, 5.0), 2.0), ((1.0, 1.5), 1.25), ((2.0, -1.0), 0.0),
+             ((0.0, 3.0), 0.0), ((1.0, 1.0), 1.0), ((-4.0, -1.0), -2.0), ((-0.5, -3.0), -1.0)]
+    for (l, r), want in cases:
+        W = tmo.FabArray((0, 0), (4, 0), 1)
+        W.a[0, 0, :] = [-10.0, 0.0, l, l + r, 100.0]
+        dW = tmo.FabArray((0, 0), (4, 0), 1)
+        L.tmo_vanleer_slopes(dW.fab, W.fab, 1, 1, 0, tmo.Box.make(2, 0, 2, 0), tmo.Box.make(0, 0, 4, 0))
+        assert dW.a[0, 0, 2] == want, ((l, r), dW.a[0, 0, 2], want)
+
+
+def test_fourth_order_slopes_exact_for_quartic_cell_averages():
+    """Unlimited fourth-order slopes of the cell averages of a quartic equal the polynomial's
+    jump across each cell (dx times the cell average of its derivative)."""
+    n = 24
+    dom = tmo.Box.make(0, 0, n - 1, 0)
+    P = np.polynomial.Polynomial([0.4, -1.1, 0.3, 0.08, -0.01])
+    I = P.integ()
+    x = np.arange(n, dtype=float)
+    W = tmo.FabArray((0, 0), (n - 1, 0), 22)
+    W.a[:, 0, :] = (I(x + 0.5) - I(x - 0.5))[None, :] * (1.0 + np.arange(22))[:, None]
+    dWvL = tmo.FabArray((0, 0), (n - 1, 0), 22)
+    L.tmo_vanleer_slopes(dWvL.fab, W.fab, 22, 0, 0, dom, dom)
+    dW = tmo.FabArray((0, 0), (n - 1, 0), 22)
+    box = tmo.Box.make(2, 0, n - 3, 0)
+    L.tmo_fourth_order_slopes(dW.fab, W.fab, dWvL.fab, 0, box, dom)
+    want = (P(x + 0.5) - P(x - 0.5))[None, 2:n - 2] * (1.0 + np.arange(22))[:, None]
+    assert np.allclose(dW.a[:, 0, 2:n - 2], want, rtol=1e-12, atol=1e-12)
+
+
+def _increment_pairs(n, seed):
+    rng = np.random.default_rng(seed)
+    m = rng.normal(size=(22, n)) * 10.0 ** rng.uniform(-3, 3, size=(22, n))
+    p = rng.normal(size=(22, n)) * 10.0 ** rng.uniform(-3, 3, size=(22, n))
+    m[:, ::7] = 0.0           # one-sided flat faces
+    p[:, 3::11] = -2.0 * m[:, 3::11]  # on the limiter's switching line
+    return m, p
+
+
+def _fabs(*arrays):
+    out = []
+    for a in arrays:
+        f = tmo.FabArray((0, 0), (a.shape[1] - 1, 0), 22)
+        f.a[:, 0, :] = a
+        out.append(f)
+    return out
+
+
+def test_ppm_limiter_is_colella_woodward_and_leaves_monotone_parabolas():
+    """The PPM limiter against Colella & Woodward (1984) eq. 1.10 in its face-value form (cell
+    average a, face values aL = a + dWm, aR = a + dWp), and its defining property: afterwards
+    the parabola through aL, a, aR has no extremum inside the cell (|a6| <= |aR - aL|) or is
+    flat."""
+    n = 400
+    m, p = _increment_pairs(n, 11)
+    fm, fp = _fabs(m, p)
+    L.tmo_ppm_limiter(fm.fab, fp.fab, tmo.Box.make(0, 0, n - 1, 0))
+    gm, gp = fm.a[:, 0, :], fp.a[:, 0, :]
+    a = 0.0
+    for c in range(22):
+        for i in range(n):
+            aL, aR = a + m[c, i], a + p[c, i]
+            if (aR - a) * (a - aL) <= 0.0:
+                aL = aR = a
+            elif (aR - aL) * (a - 0.5 * (aL + aR)) > (aR - aL) ** 2 / 6.0:
+                aL = 3.0 * a - 2.0 * aR
+            elif -((aR - aL) ** 2) / 6.0 > (aR - aL) * (a - 0.5 * (aL + aR)):
+                aR = 3.0 * a - 2.0 * aL
+            scale = max(abs(m[c, i]), abs(p[c, i]), 1e-300)
+            assert abs(gm[c, i] - (aL - a)) <= 1e-13 * scale and abs(gp[c, i] - (aR - a)) <= 1e-13 * scale
+            da, a6 = gp[c, i] - gm[c, i], -3.0 * (gm[c, i] + gp[c, i])
+            assert abs(a6) <= abs(da) * (1 + 1e-12) + 1e-300
+
+
+def _parabola_average(m, p, x0, x1):
+    """Average over [x0, x1] of the PPM parabola of a cell of average 0 with face values m
+    (xi = 0) and p (xi = 1), minus the cell average (an increment)."""
+    P = np.polynomial.Polynomial([m, (p - m) - 3.0 * (m + p), 3.0 * (m + p)])  # aL + xi (da + a6 (1 - xi))
+    I = P.integ()
+    return (I(x1) - I(x0)) / (x1 - x0)
+
+
+def test_ppm_tracing_is_the_parabola_average_over_the_domain_of_dependence():
+    """PPM characteristic tracing (Colella & Woodward 1984 eq. 1.12): a field moving right with
+    Courant number s takes at the cell's hi face the average of the cell's parabola over
+    [1 - s, 1]; moving left, at the lo face over [0, s]; the other face keeps its value."""
+    n = 60
+    rng = np.random.default_rng(12)
+    m, p = _increment_pairs(n, 13)
+    lam = rng.uniform(-1.0, 1.0, size=(22, n))
+    lam[:, ::9] = 0.0
+    dtbydx = 0.9
+    fm, fp, fl = _fabs(m, p, lam)
+    L.tmo_ppm_normal_pred(fm.fab, fp.fab, fl.fab, dtbydx, tmo.Box.make(0, 0, n - 1, 0))
+    gm, gp = fm.a[:, 0, :], fp.a[:, 0, :]
+    for c in range(22):
+        for i in range(n):
+            s = lam[c, i] * dtbydx
+            scale = max(abs(m[c, i]), abs(p[c, i]), 1e-300)
+            if s > 0:
+                assert abs(gp[c, i] - _parabola_average(m[c, i], p[c, i], 1.0 - s, 1.0)) <= 1e-12 * scale
+                assert gm[c, i] == m[c, i]
+            elif s < 0:
+                assert abs(gm[c, i] - _parabola_average(m[c, i], p[c, i], 0.0, -s)) <= 1e-12 * scale
+                assert gp[c, i] == p[c, i]
+            else:
+                assert gm[c, i] == m[c, i] and gp[c, i] == p[c, i]
+
+
+def test_plm_tracing_is_the_linear_average_over_the_domain_of_dependence():
+    """PLM characteristic predictor (Colella 1990, reference-state form): with the linear profile
+    a + dW (xi - 1/2), a field moving toward a face takes there the profile's average over its
+    own domain of dependence; a field that does not reach a face takes the average over the
+    domain of dependence of the fastest wave toward that face (lambda_min for the lo face,
+    lambda_max for the hi face), or the face value when no wave moves toward it."""
+    n = 40
+    rng = np.random.default_rng(14)
+    d = rng.normal(size=(22, n))
+    lam = np.sort(rng.uniform(-1.0, 1.0, size=(22, n)), axis=0)  # ordered eigenvalues
+    lam[8:14, :] = 0.0
+    lam[:, 5] = np.abs(lam[:, 5])    # no wave toward the lo face
+    lam[:, 6] = -np.abs(lam[:, 6])   # none toward the hi face
+    lam[:, 6] = np.sort(lam[:, 6])
+    lam[:, 5] = np.sort(lam[:, 5])
+    dtbydx = 0.9
+    fm, fp, fd, fl = _fabs(np.zeros_like(d), np.zeros_like(d), d, lam)
+    L.tmo_plm_normal_pred(fm.fab, fp.fab, fd.fab, fl.fab, dtbydx, tmo.Box.make(0, 0, n - 1, 0))
+    gm, gp = fm.a[:, 0, :], fp.a[:, 0, :]
+
+    def avg(dw, x0, x1):  # average of dw (xi - 1/2) over [x0, x1]
+        return dw * (0.5 * (x0 + x1) - 0.5)
+
+    for i in range(n):
+        smin = min(lam[0, i], 0.0) * dtbydx
+        smax = max(lam[21, i], 0.0) * dtbydx
+        for c in range(22):
+            s = lam[c, i] * dtbydx
+            s_lo = s if s < 0 else smin
+            s_hi = s if s > 0 else smax
+            want_lo = avg(d[c, i], 0.0, -s_lo) if s_lo < 0 else avg(d[c, i], 0.0, 0.0)
+            want_hi = avg(d[c, i], 1.0 - s_hi, 1.0) if s_hi > 0 else avg(d[c, i], 1.0, 1.0)
+            assert abs(gm[c, i] - want_lo) <= 1e-14 * abs(d[c, i]) + 1e-300, (i, c)
+            assert abs(gp[c, i] - want_hi) <= 1e-14 * abs(d[c, i]) + 1e-300, (i, c)
 
 
 # ---------------------------------------------------------------------------

@@ -300,7 +300,7 @@ int tmk_make_boxes(int nx, int ny, const unsigned char *tags, int ncoarse, const
 int tmk_level_regrid(tmk_level *fresh, tmk_level *old, tmk_level *coarse, int ref_ratio);
 
 /* Workgroup tile shape of the update kernels (autotuner hook): bx x by threads
- * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 32x8, 32x12, 32x16,
+ * per side-pair, (bx - 2) x (by - 2) valid cells.  Shapes: 16x16, 32x8, 32x12, 32x16,
  * 32x32, 64x4, 64x8, 64x16.  0, 0 = the library default for the context's precision. */
 int tmk_set_tile(tmk_ctx *ctx, int bx, int by);
 

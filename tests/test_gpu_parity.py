@@ -140,7 +140,7 @@ def test_patch_antiplane(where):
         assert np.all(_relerr(Bg, Br) <= TOL)
 
 
-@pytest.mark.parametrize("tile", [(32, 8), (32, 12), (32, 16), (64, 4), (64, 8), (32, 32), (64, 16)])
+@pytest.mark.parametrize("tile", [(16, 16), (32, 8), (32, 12), (32, 16), (64, 4), (64, 8), (32, 32), (64, 16)])
 def test_patch_tile_shapes(tile):
     cfg = _cfg("rs", "PPM")
     (Ug, Ur), (Bg, Br), _, _, _ = _patch_case(cfg, BOXES["whole"], seed=5, tile=tile)

@@ -10,7 +10,7 @@ import numpy as np
 
 from . import capi, scenarios
 
-SHAPES = ((32, 8), (32, 12), (32, 16), (32, 32), (64, 4), (64, 8), (64, 16))
+SHAPES = ((16, 16), (32, 8), (32, 12), (32, 16), (32, 32), (64, 4), (64, 8), (64, 16))
 
 
 def tune(cfg, boxes=None, precision=capi.PREC_FP64, steps=20, shapes=SHAPES, log=None):

@@ -448,7 +448,8 @@ extern "C" int tmk_ctx_destroy(tmk_ctx* c)
 
 static bool tile_ok(int bx, int by)
 {
-    return (bx == 32 && (by == 8 || by == 12 || by == 16 || by == 32)) || (bx == 64 && (by == 4 || by == 8 || by == 16));
+    return (bx == 16 && by == 16) || (bx == 32 && (by == 8 || by == 12 || by == 16 || by == 32)) ||
+           (bx == 64 && (by == 4 || by == 8 || by == 16));
 }
 
 // Default tile per precision, from timing the 4096 x 2048 rate-and-state level on MI355X:
@@ -501,6 +502,7 @@ template <typename T, int NS, int PK>
 static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a, bool in)
 {
     if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16, PK>(st, a, in);
+    if (bx == 16 && by == 16) return launch_tiles_t<T, NS, 16, 16, PK>(st, a, in);
     if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8, PK>(st, a, in);
     if (bx == 32 && by == 12) return launch_tiles_t<T, NS, 32, 12, PK>(st, a, in);
     if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32, PK>(st, a, in);

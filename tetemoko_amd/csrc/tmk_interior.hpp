@@ -492,8 +492,8 @@ k_godunov_interior(const KArgs<T> a)
 // predictor kind.  Used to declare and to instantiate the FP32-bulk kernels, which are compiled
 // in their own translation unit (tmk_f32.hip).
 #define TMK_TILE_SHAPES(M, T, NS, PK)                                                           \
-    M(T, NS, 32, 8, PK) M(T, NS, 32, 12, PK) M(T, NS, 32, 16, PK) M(T, NS, 32, 32, PK)         \
-    M(T, NS, 64, 4, PK) M(T, NS, 64, 8, PK) M(T, NS, 64, 16, PK)
+    M(T, NS, 16, 16, PK) M(T, NS, 32, 8, PK) M(T, NS, 32, 12, PK) M(T, NS, 32, 16, PK)        \
+    M(T, NS, 32, 32, PK) M(T, NS, 64, 4, PK) M(T, NS, 64, 8, PK) M(T, NS, 64, 16, PK)
 #define TMK_ALL_UPDATE_KERNELS(M, T)                                                            \
     TMK_TILE_SHAPES(M, T, 2, PK_RUNTIME) TMK_TILE_SHAPES(M, T, 3, PK_RUNTIME)                   \
     TMK_TILE_SHAPES(M, T, 2, PK_PPM_PRIM) TMK_TILE_SHAPES(M, T, 3, PK_PPM_PRIM)

@@ -115,7 +115,7 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
                 const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
                 if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = src[q][g];
+                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = gmem(src[q])[g];
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) stg[q * SP + p] = (T)0;
@@ -311,7 +311,7 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
                 const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
                 if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = src[q][g];
+                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = gmem(src[q])[g];
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = (T)0;

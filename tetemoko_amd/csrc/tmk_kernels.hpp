@@ -114,6 +114,20 @@ __device__ __forceinline__ void report(int* err, int code)
     atomicCAS(err, 0, code);
 }
 
+// The plane pointers come out of the patch table in memory, so the compiler cannot tell that
+// they address global memory and would emit flat loads and stores, whose 64-bit addresses cost
+// vector instructions and which also count against the LDS wait counter.  gmem states it.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 template <typename T>
 __device__ __forceinline__ size_t uidx(const Patch<T>& p, int i, int j, size_t plane)
 {
@@ -172,7 +186,7 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
     const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
     T f[7];
     face_flux_side<T, NS>(m, v, sv, r, f);
-    double* Fo = pt.F[d];
+    __attribute__((address_space(1))) double* Fo = gmem(pt.F[d]);
     const int o = 11 * sd;
     if (d == 0) {
         Fo[fi + (o + 0) * fp] = (double)f[0];
@@ -231,7 +245,7 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
     auto upd = [&](int c, T dx_, T dy_) {
         const T dU = ((T)0 + dx_) + dy_;
-        pt.out[c][u] = pt.in[c][u] + dU * sc;
+        gmem(pt.out[c])[u] = gmem(pt.in[c])[u] + dU * sc;
     };
     T st[6];
     // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
@@ -244,43 +258,43 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     const T dsxy_x = fxh[3] - fxl[3], dsxy_y = fyh[3] - fyl[3];
     upd(o + 0, dvx_x, dvx_y);
     upd(o + 1, dvy_x, dvy_y);
-    st[0] = pt.in[o + 3][u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
-    st[1] = pt.in[o + 4][u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
-    st[2] = pt.in[o + 5][u] + (((T)0 + dszz_x) + dszz_y) * sc;
-    st[3] = pt.in[o + 6][u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
+    st[0] = gmem(pt.in[o + 3])[u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
+    st[1] = gmem(pt.in[o + 4])[u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
+    st[2] = gmem(pt.in[o + 5])[u] + (((T)0 + dszz_x) + dszz_y) * sc;
+    st[3] = gmem(pt.in[o + 6])[u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
     if (NS == 3) {
         const T dvz_x = fxh[5] - fxl[5], dvz_y = fyh[5] - fyl[5];
         const T dsxz_x = fxh[6] - fxl[6], dsxz_y = (T)0 - (T)0;
         const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[6] - fyl[6];
         upd(o + 2, dvz_x, dvz_y);
-        st[4] = pt.in[o + 7][u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
-        st[5] = pt.in[o + 8][u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
+        st[4] = gmem(pt.in[o + 7])[u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
+        st[5] = gmem(pt.in[o + 8])[u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
     } else {
         st[4] = (T)0;
         st[5] = (T)0;
     }
     if (P.plast && a.plastic) {
         // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
-        T lam = pt.in[o + 9][u], gam = pt.in[o + 10][u];
-        T sxz = NS == 3 ? st[4] : pt.in[o + 7][u];
-        T syz = NS == 3 ? st[5] : pt.in[o + 8][u];
+        T lam = gmem(pt.in[o + 9])[u], gam = gmem(pt.in[o + 10])[u];
+        T sxz = NS == 3 ? st[4] : gmem(pt.in[o + 7])[u];
+        T syz = NS == 3 ? st[5] : gmem(pt.in[o + 8])[u];
         plastic_flow<T>(P, (T)dt, st[0], st[1], st[2], st[3], sxz, syz, lam, gam);
         st[4] = sxz;
         st[5] = syz;
-        pt.out[o + 9][u] = lam;
-        pt.out[o + 10][u] = gam;
+        gmem(pt.out[o + 9])[u] = lam;
+        gmem(pt.out[o + 10])[u] = gam;
         if (NS == 2) {
-            pt.out[o + 7][u] = sxz;
-            pt.out[o + 8][u] = syz;
+            gmem(pt.out[o + 7])[u] = sxz;
+            gmem(pt.out[o + 8])[u] = syz;
         }
     }
-    pt.out[o + 3][u] = st[0];
-    pt.out[o + 4][u] = st[1];
-    pt.out[o + 5][u] = st[2];
-    pt.out[o + 6][u] = st[3];
+    gmem(pt.out[o + 3])[u] = st[0];
+    gmem(pt.out[o + 4])[u] = st[1];
+    gmem(pt.out[o + 5])[u] = st[2];
+    gmem(pt.out[o + 6])[u] = st[3];
     if (NS == 3) {
-        pt.out[o + 7][u] = st[4];
-        pt.out[o + 8][u] = st[5];
+        gmem(pt.out[o + 7])[u] = st[4];
+        gmem(pt.out[o + 8])[u] = st[5];
     }
 }
 
@@ -346,7 +360,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             T v = (T)0;
             // only the tile's stencil region (x: tile +-3, y: tile +-1), inside the domain
             if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1))
-                v = pt.in[stage_comp<NS>(0, c)][uidx(pt, gi, gj, 0)];
+                v = gmem(pt.in[stage_comp<NS>(0, c)])[uidx(pt, gi, gj, 0)];
             sm[e] = v;
         }
         __syncthreads();
@@ -378,7 +392,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             T v = (T)0;
             // only the tile's stencil region (x: tile +-1, y: tile +-3), inside the domain
             if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3))
-                v = pt.in[stage_comp<NS>(1, c)][uidx(pt, gi, gj, 0)];
+                v = gmem(pt.in[stage_comp<NS>(1, c)])[uidx(pt, gi, gj, 0)];
             sm[e] = v;
         }
         __syncthreads();

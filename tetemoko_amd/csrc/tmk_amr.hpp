@@ -121,7 +121,8 @@ __global__ void k_amr_fill_state(const CfCell* __restrict__ cells, int ncells, c
     for (int k = 0; k < 9; ++k) {
         v[k] = (T)0;
         if (cc.cpatch[k] >= 0)
-            v[k] = time_interp<T>(cold[cc.cpatch[k]].in[c][cc.coff[k]], cnew[cc.cpatch[k]].in[c][cc.coff[k]], alpha);
+            v[k] = time_interp<T>(gmem(cold[cc.cpatch[k]].in[c])[cc.coff[k]], gmem(cnew[cc.cpatch[k]].in[c])[cc.coff[k]],
+                                  alpha);
     }
     const T c0 = v[4];
     T s[2];
@@ -147,7 +148,7 @@ __global__ void k_amr_fill_state(const CfCell* __restrict__ cells, int ncells, c
     }
     T val = c0 + (T)cc.coef0 * s[0];
     val = val + (T)cc.coef1 * s[1];
-    ((T*)fine[cc.dpatch].in[c])[cc.doff] = val;
+    gmem((T*)fine[cc.dpatch].in[c])[cc.doff] = val;
 }
 
 template <typename T>
@@ -169,7 +170,7 @@ __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, i
         if (f.cpatch[k] >= 0) {
             const Patch<T>& po = cold[f.cpatch[k]];
             const Patch<T>& pn = cnew[f.cpatch[k]];
-            v[k] = time_interp<double>(po.bin[f.coff[k] + (size_t)c * po.bnx], pn.bin[f.coff[k] + (size_t)c * pn.bnx],
+            v[k] = time_interp<double>(gmem(po.bin)[f.coff[k] + (size_t)c * po.bnx], gmem(pn.bin)[f.coff[k] + (size_t)c * pn.bnx],
                                        alpha);
         }
     }
@@ -180,7 +181,7 @@ __global__ void k_amr_fill_fault(const CfFace* __restrict__ faces, int nfaces, i
         if (dsum > 0.0) s = fmin(fmin(smax - v[1], v[1] - smin) / dsum, 1.0) * s;
     }
     const Patch<T>& pd = fine[f.dpatch];
-    ((double*)pd.bin)[f.doff + (size_t)c * pd.bnx] = v[1] + f.coef * s;
+    gmem((double*)pd.bin)[f.doff + (size_t)c * pd.bnx] = v[1] + f.coef * s;
 }
 
 // plane size of the flux array of direction d (faces of the patch's valid box)
@@ -203,7 +204,7 @@ __global__ void k_amr_reg_coarse(const RegFace* __restrict__ faces, int nfaces, 
     const RegFace f = faces[e];
     const double dtc = cscen[f.m].dt;
     const Patch<T>& p = crs[f.cpatch];
-    const double F = p.F[f.d][f.cface + (size_t)c * flux_plane(p, f.d)];
+    const double F = gmem(p.F[f.d])[f.cface + (size_t)c * flux_plane(p, f.d)];
     double acc = 0.0;
     acc = acc - dtc * F;
     reg[(size_t)c * nfaces + e] = acc;
@@ -221,7 +222,7 @@ __global__ void k_amr_reg_fine(const RegFace* __restrict__ faces, int nfaces, do
     const double w = fscen[f.m].dt * (1.0 / (double)r);
     const Patch<T>& p = fine[f.fpatch];
     const size_t fp = flux_plane(p, f.d);
-    const double* F = p.F[f.d] + (size_t)c * fp;
+    const __attribute__((address_space(1))) double* F = gmem(p.F[f.d]) + (size_t)c * fp;
     double acc = reg[(size_t)c * nfaces + e];
     for (int m = 0; m < r; ++m) acc = acc + w * F[f.fface[m]];
     reg[(size_t)c * nfaces + e] = acc;
@@ -236,7 +237,7 @@ __global__ void k_amr_reflux(const RefluxCell* __restrict__ cells, int ncells, c
     if (t >= ncells * TMK_NCOMP) return;
     const int c = t / ncells, k = t % ncells;
     const RefluxCell rc = cells[k];
-    T* u = (T*)crs[rc.cpatch].in[c] + rc.coff;
+    __attribute__((address_space(1))) T* u = gmem((T*)crs[rc.cpatch].in[c]) + rc.coff;
     double v = (double)*u;
     for (int e = rc.e0; e < rc.e1; ++e) v = v + scale[e] * reg[(size_t)c * nfaces + e];
     *u = (T)v;
@@ -252,11 +253,11 @@ __global__ void k_amr_avg_state(const AvgCell* __restrict__ cells, int ncells, c
     const int c = t / ncells, k = t % ncells;
     const AvgCell a = cells[k];
     const Patch<T>& pf = fine[a.fpatch];
-    const T* f = pf.in[c] + a.foff;
+    const __attribute__((address_space(1))) T* f = gmem(pf.in[c]) + a.foff;
     double acc = 0.0;
     for (int jj = 0; jj < r; ++jj)
         for (int ii = 0; ii < r; ++ii) acc = acc + (double)f[ii + (size_t)jj * pf.unx];
-    ((T*)crs[a.cpatch].in[c])[a.coff] = (T)(acc * scale);
+    gmem((T*)crs[a.cpatch].in[c])[a.coff] = (T)(acc * scale);
 }
 
 // coarse fault face = (sum of the r fine faces) / r (D1AVERAGE)
@@ -271,10 +272,10 @@ __global__ void k_amr_avg_fault(const AvgFace* __restrict__ faces, int nfaces, i
     const Patch<T>& pf = fine[a.fpatch];
     const Patch<T>& pc = crs[a.cpatch];
     if (c >= pc.bnb) return;  // members of a batch may have different fault laws
-    const double* f = pf.bin + a.foff + (size_t)c * pf.bnx;
+    const __attribute__((address_space(1))) double* f = gmem(pf.bin) + a.foff + (size_t)c * pf.bnx;
     double acc = 0.0;
     for (int ii = 0; ii < r; ++ii) acc = acc + f[ii];
-    ((double*)pc.bin)[a.coff + (size_t)c * pc.bnx] = acc * scale;
+    gmem((double*)pc.bin)[a.coff + (size_t)c * pc.bnx] = acc * scale;
 }
 
 }  // namespace tmk

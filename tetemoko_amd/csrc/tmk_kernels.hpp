@@ -114,20 +114,6 @@ __device__ __forceinline__ void report(int* err, int code)
     atomicCAS(err, 0, code);
 }
 
-// The plane pointers come out of the patch table in memory, so the compiler cannot tell that
-// they address global memory and would emit flat loads and stores, whose 64-bit addresses cost
-// vector instructions and which also count against the LDS wait counter.  gmem states it.
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p)
-{
-    return (const __attribute__((address_space(1))) T*)p;
-}
-template <typename T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
-{
-    return (__attribute__((address_space(1))) T*)p;
-}
-
 template <typename T>
 __device__ __forceinline__ size_t uidx(const Patch<T>& p, int i, int j, size_t plane)
 {
@@ -176,7 +162,8 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
     const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
     if (pt.fmask) {
         const int mw = pt.vhi0 - pt.vlo0 + 3;
-        const unsigned char* mk = pt.fmask + (size_t)(i - pt.vlo0 + 1) + (size_t)(j - pt.vlo1 + 1) * mw;
+        const __attribute__((address_space(1))) unsigned char* mk =
+            gmem(pt.fmask) + (size_t)(i - pt.vlo0 + 1) + (size_t)(j - pt.vlo1 + 1) * mw;
         const unsigned a = d == 0 ? mk[-1] : mk[-mw], b = *mk;
         const bool edge = ((a ^ b) & 1) != 0;                 // level boundary (fine side)
         const bool cf = (a & b & 1) && ((a ^ b) & 2);        // around the finer level (coarse side)
@@ -474,19 +461,19 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                     const size_t bi = (size_t)(i - pt.blo0);
                     const size_t bp = (size_t)pt.bnx;
                     if (P.fault_kind == 0) {
-                        rs_fault(P, i, pt.bin[bi + 10 * bp], fin, fo);
+                        rs_fault(P, i, gmem(pt.bin)[bi + 10 * bp], fin, fo);
                         // RSSETBND into the half-step copy (RSIBC.cpp:325-333): only psi is consumed
                         const double Vx = fo.vx[0] - fo.vx[1];
                         const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
                         const double V = sqrt(Vx * Vx + Vz * Vz);
-                        tmp_state = rs_psi_update(P, i, pt.bin[bi + 10 * bp], V, 0.5 * dt);
+                        tmp_state = rs_psi_update(P, i, gmem(pt.bin)[bi + 10 * bp], V, 0.5 * dt);
                         if (isnan(tmp_state)) report(errp, TMK_E_PSI_NAN);
                     } else {
-                        sw_fault(P, i, pt.bin[bi + 7 * bp], fin, fo);
+                        sw_fault(P, i, gmem(pt.bin)[bi + 7 * bp], fin, fo);
                         const double Vx = fo.vx[0] - fo.vx[1];
                         const double Vz = NS == 3 ? fo.vz[0] - fo.vz[1] : 0.0;
                         const double V = sqrt(Vx * Vx + Vz * Vz);
-                        tmp_state = pt.bin[bi + 7 * bp] + (0.5 * dt) * V;
+                        tmp_state = gmem(pt.bin)[bi + 7 * bp] + (0.5 * dt) * V;
                     }
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
@@ -598,8 +585,8 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                     FaultOut fo;
                     const size_t bi = (size_t)(i - pt.blo0);
                     const size_t bp = (size_t)pt.bnx;
-                    const double* B = pt.bin;
-                    double* Bo = pt.bout;
+                    const __attribute__((address_space(1))) double* B = gmem(pt.bin);
+                    __attribute__((address_space(1))) double* Bo = gmem(pt.bout);
                     const double tnew = time + dt;
                     if (P.fault_kind == 0) {
                         rs_fault(P, i, tmp_state, fin, fo);
@@ -760,7 +747,7 @@ __global__ void k_exchange(const Patch<T>* patches, const CopyRect* rects, int n
         const size_t si = (size_t)(i - S.ulo0) + (size_t)(j - S.ulo1) * S.unx;
         const size_t di = (size_t)(i - D.ulo0) + (size_t)(j - D.ulo1) * D.unx;
         for (int c = 0; c < 22; ++c)
-            if ((ncomp_mask >> c) & 1) D.out[c][di] = S.out[c][si];
+            if ((ncomp_mask >> c) & 1) gmem(D.out[c])[di] = gmem(S.out[c])[si];
     }
 }
 
@@ -775,7 +762,7 @@ __global__ void k_bexchange(const Patch<T>* patches, const CopyRect* rects, int 
     const int nb = D.bnb;
     for (int t = threadIdx.x; t < nx * nb; t += blockDim.x) {
         const int i = r.lo0 + t % nx, c = t / nx;
-        D.bout[(size_t)(i - D.blo0) + (size_t)c * D.bnx] = S.bout[(size_t)(i - S.blo0) + (size_t)c * S.bnx];
+        gmem(D.bout)[(size_t)(i - D.blo0) + (size_t)c * D.bnx] = gmem(S.bout)[(size_t)(i - S.blo0) + (size_t)c * S.bnx];
     }
 }
 
@@ -808,7 +795,7 @@ __global__ void k_station_record(const Patch<T>* patches, const Scen* scen, Stat
         const double d = s.d[k];
         double* r = s.rec + ((size_t)b * s.n + k) * (size_t)(1 + s.nb);
         r[0] = scen[p.scen].time;
-        for (int c = 0; c < s.nb; ++c) r[1 + c] = p.bout[i + c * bp] * (1.0 - d) + p.bout[i + 1 + c * bp] * d;
+        for (int c = 0; c < s.nb; ++c) r[1 + c] = gmem(p.bout)[i + c * bp] * (1.0 - d) + gmem(p.bout)[i + 1 + c * bp] * d;
     }
     __syncthreads();
     if (threadIdx.x == 0) s.count[0] = b + 1;
@@ -830,7 +817,7 @@ __device__ __forceinline__ double undiv(const Patch<T>& p, int c, int i, int j, 
 {
     const int io = d == 0, jo = d == 1;
     const int ci = d == 0 ? i : j;
-    const T* q = p.in[c];
+    const __attribute__((address_space(1))) T* q = gmem(p.in[c]);
     const size_t u = (size_t)(i - p.ulo0) + (size_t)(j - p.ulo1) * p.unx;
     const size_t s = d == 0 ? 1 : (size_t)p.unx;
     (void)io; (void)jo;
@@ -876,7 +863,8 @@ __global__ void k_tag(const Patch<T>* patches, const Scen* scen, TagArgs ta, uns
         unsigned char tag = 0;
         if (ta.level < ta.c2max || (x >= ta.x2c0 && x <= ta.x2c1 && y >= ta.y2c0 && y <= ta.y2c1)) {
             const size_t u = (size_t)(i - p.ulo0) + (size_t)(j - p.ulo1) * p.unx;
-            tag = gm >= thr || (double)p.in[10][u] >= ta.plastic_thresh || (double)p.in[21][u] >= ta.plastic_thresh;
+            tag = gm >= thr || (double)gmem(p.in[10])[u] >= ta.plastic_thresh ||
+                  (double)gmem(p.in[21])[u] >= ta.plastic_thresh;
             if (!tag && ta.stress_thresh > 0.0) {
                 double gs = 0.0;
                 for (int d = 0; d < 2; ++d)
@@ -885,7 +873,7 @@ __global__ void k_tag(const Patch<T>* patches, const Scen* scen, TagArgs ta, uns
             }
             if (!tag && ta.sliprate_thresh > 0.0 && j == P.dlo[1] && p.bin) {
                 const int vc = P.fault_kind == 0 ? 2 : 4;
-                tag = p.bin[(size_t)(i - p.blo0) + (size_t)vc * p.bnx] >= ta.sliprate_thresh;
+                tag = gmem(p.bin)[(size_t)(i - p.blo0) + (size_t)vc * p.bnx] >= ta.sliprate_thresh;
             }
         }
         raw[(size_t)(i - P.dlo[0]) + (size_t)(j - P.dlo[1]) * ta.nx] = tag;

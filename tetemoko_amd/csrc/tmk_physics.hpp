@@ -21,6 +21,20 @@
 
 namespace tmk {
 
+// The plane pointers come out of the patch table in memory, so the compiler cannot tell that
+// they address global memory and would emit flat loads and stores, whose 64-bit addresses cost
+// vector instructions and which also count against the LDS wait counter.  gmem states it.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // parameters (device copy of tmk_params plus derived constants)
 // ---------------------------------------------------------------------------
@@ -446,7 +460,7 @@ __device__ __forceinline__ double face_val(const double* arr, const DevParams& P
     if (!arr) return dflt;
     int k = i - P.face_lo;
     k = k < 0 ? 0 : (k >= P.nface ? P.nface - 1 : k);
-    return arr[k];
+    return gmem(arr)[k];
 }
 
 __device__ __forceinline__ double gaussianfunc(double w, double r) { return exp(-(0.5 * ((r / w) * (r / w)))); }

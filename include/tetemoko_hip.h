@@ -124,7 +124,10 @@ int tmk_patch_plastic_update(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_b
 
 /* Device-resident level: LinElastLevelGodunov (LinElastLevelGodunov.cpp:45-398).
  * boxes: disjoint valid boxes of one level.  Fault data is allocated for every
- * box whose lo-y edge lies on the domain lo-y boundary. */
+ * box whose lo-y edge lies on the domain lo-y boundary.  TMK_E_NOMEM also when one state
+ * plane of an island (boxes whose ghost regions touch share one) would exceed 4 GiB: the
+ * kernels address planes with 32-bit offsets (536 M FP64 cells, more than the device holds
+ * at three copies of 22 planes). */
 int tmk_level_create(tmk_ctx *ctx, int nboxes, const tmk_box *boxes, tmk_level **out);
 /* device bytes tmk_level_create would allocate for these boxes */
 int tmk_level_footprint(const tmk_ctx *ctx, int nboxes, const tmk_box *boxes, size_t *bytes);

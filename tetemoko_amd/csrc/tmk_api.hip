@@ -305,6 +305,10 @@ struct tmk_batch {
     bool stale = false;           // membership changed since the group was built
 };
 
+// largest state plane the kernels address (32-bit byte offsets from the plane's base); one FP64
+// level of planes that size (three copies of 22 planes) would exceed the device's memory anyway
+static constexpr size_t kPlaneMaxBytes = (size_t)1 << 32;
+
 static const char* k_errs[] = {"ok",
                                "NaN state variable psi (dPsi is nan)",
                                "invalid boundary type",
@@ -770,6 +774,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
 {
     const size_t unx = (size_t)(ub.hi[0] - ub.lo[0] + 1), uny = (size_t)(ub.hi[1] - ub.lo[1] + 1);
     const size_t plane = unx * uny;
+    if (plane * sizeof(T) > kPlaneMaxBytes) return TMK_E_NOMEM;  // 32-bit plane offsets (gld/gst)
     const size_t nU = plane * TMK_NCOMP;
     const size_t bnx = B ? (size_t)(bb.hi[0] - bb.lo[0] + 1) : 0;
     const size_t nB = bnx * (size_t)nb;
@@ -1098,6 +1103,8 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     for (Island& I : L->isl) {
         const size_t plane = I.plane();
         const size_t bytes = plane * TMK_NCOMP * L->tsize;
+        // the kernels address a plane with 32-bit byte offsets (gld/gst)
+        ok = ok && plane * L->tsize <= kPlaneMaxBytes;
         ok = ok && hipMalloc(&I.u[0], bytes) == hipSuccess && hipMalloc(&I.u[1], bytes) == hipSuccess &&
              hipMalloc(&I.w, bytes) == hipSuccess && hipMalloc((void**)&I.owner, plane * sizeof(int)) == hipSuccess;
         if (ok) {

@@ -112,10 +112,10 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
             const int p = tid + r * NT;
             if (p < SP) {
                 const int yy = p / SX, xx = p - yy * SX;
-                const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
+                const unsigned g = (unsigned)(tl.x0 - 3 + xx - pt.ulo0) + (unsigned)(tl.y0 - 3 + yy - pt.ulo1) * (unsigned)pt.unx;
                 if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = gmem(src[q])[g];
+                    for (int q = 0; q < NQ; ++q) stg[q * SP + p] = gld(src[q], g);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) stg[q * SP + p] = (T)0;
@@ -308,10 +308,10 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
             const int p = c + r * NC;
             if (p < SP) {
                 const int yy = p / SX, xx = p - yy * SX;
-                const size_t g = (size_t)(tl.x0 - 3 + xx - pt.ulo0) + (size_t)(tl.y0 - 3 + yy - pt.ulo1) * pt.unx;
+                const unsigned g = (unsigned)(tl.x0 - 3 + xx - pt.ulo0) + (unsigned)(tl.y0 - 3 + yy - pt.ulo1) * (unsigned)pt.unx;
                 if (full || (xx <= tnx + 5 && yy <= tny + 5)) {
 #pragma unroll
-                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = gmem(src[q])[g];
+                    for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = gld(src[q], g);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ1; ++q) stg[(q0 + q) * SP + p] = (T)0;

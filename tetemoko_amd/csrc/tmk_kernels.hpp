@@ -227,12 +227,12 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     face_flux_side<T, NS>(m, F.xhv, F.xhs, (T)1, fxh);
     face_flux_side<T, NS>(m, F.ylv, F.yls, ry, fyl);
     face_flux_side<T, NS>(m, F.yhv, F.yhs, ry, fyh);
-    const size_t u = uidx(pt, i, j, 0);
+    const unsigned u = (unsigned)uidx(pt, i, j, 0);
     const int o = 11 * sd;
     // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
     auto upd = [&](int c, T dx_, T dy_) {
         const T dU = ((T)0 + dx_) + dy_;
-        gmem(pt.out[c])[u] = gmem(pt.in[c])[u] + dU * sc;
+        gst(pt.out[c], u, gld(pt.in[c], u) + dU * sc);
     };
     T st[6];
     // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
@@ -245,43 +245,43 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     const T dsxy_x = fxh[3] - fxl[3], dsxy_y = fyh[3] - fyl[3];
     upd(o + 0, dvx_x, dvx_y);
     upd(o + 1, dvy_x, dvy_y);
-    st[0] = gmem(pt.in[o + 3])[u] + (((T)0 + dsxx_x) + dsxx_y) * sc;
-    st[1] = gmem(pt.in[o + 4])[u] + (((T)0 + dsyy_x) + dsyy_y) * sc;
-    st[2] = gmem(pt.in[o + 5])[u] + (((T)0 + dszz_x) + dszz_y) * sc;
-    st[3] = gmem(pt.in[o + 6])[u] + (((T)0 + dsxy_x) + dsxy_y) * sc;
+    st[0] = gld(pt.in[o + 3], u) + (((T)0 + dsxx_x) + dsxx_y) * sc;
+    st[1] = gld(pt.in[o + 4], u) + (((T)0 + dsyy_x) + dsyy_y) * sc;
+    st[2] = gld(pt.in[o + 5], u) + (((T)0 + dszz_x) + dszz_y) * sc;
+    st[3] = gld(pt.in[o + 6], u) + (((T)0 + dsxy_x) + dsxy_y) * sc;
     if (NS == 3) {
         const T dvz_x = fxh[5] - fxl[5], dvz_y = fyh[5] - fyl[5];
         const T dsxz_x = fxh[6] - fxl[6], dsxz_y = (T)0 - (T)0;
         const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[6] - fyl[6];
         upd(o + 2, dvz_x, dvz_y);
-        st[4] = gmem(pt.in[o + 7])[u] + (((T)0 + dsxz_x) + dsxz_y) * sc;
-        st[5] = gmem(pt.in[o + 8])[u] + (((T)0 + dsyz_x) + dsyz_y) * sc;
+        st[4] = gld(pt.in[o + 7], u) + (((T)0 + dsxz_x) + dsxz_y) * sc;
+        st[5] = gld(pt.in[o + 8], u) + (((T)0 + dsyz_x) + dsyz_y) * sc;
     } else {
         st[4] = (T)0;
         st[5] = (T)0;
     }
     if (P.plast && a.plastic) {
         // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
-        T lam = gmem(pt.in[o + 9])[u], gam = gmem(pt.in[o + 10])[u];
-        T sxz = NS == 3 ? st[4] : gmem(pt.in[o + 7])[u];
-        T syz = NS == 3 ? st[5] : gmem(pt.in[o + 8])[u];
+        T lam = gld(pt.in[o + 9], u), gam = gld(pt.in[o + 10], u);
+        T sxz = NS == 3 ? st[4] : gld(pt.in[o + 7], u);
+        T syz = NS == 3 ? st[5] : gld(pt.in[o + 8], u);
         plastic_flow<T>(P, (T)dt, st[0], st[1], st[2], st[3], sxz, syz, lam, gam);
         st[4] = sxz;
         st[5] = syz;
-        gmem(pt.out[o + 9])[u] = lam;
-        gmem(pt.out[o + 10])[u] = gam;
+        gst(pt.out[o + 9], u, lam);
+        gst(pt.out[o + 10], u, gam);
         if (NS == 2) {
-            gmem(pt.out[o + 7])[u] = sxz;
-            gmem(pt.out[o + 8])[u] = syz;
+            gst(pt.out[o + 7], u, sxz);
+            gst(pt.out[o + 8], u, syz);
         }
     }
-    gmem(pt.out[o + 3])[u] = st[0];
-    gmem(pt.out[o + 4])[u] = st[1];
-    gmem(pt.out[o + 5])[u] = st[2];
-    gmem(pt.out[o + 6])[u] = st[3];
+    gst(pt.out[o + 3], u, st[0]);
+    gst(pt.out[o + 4], u, st[1]);
+    gst(pt.out[o + 5], u, st[2]);
+    gst(pt.out[o + 6], u, st[3]);
     if (NS == 3) {
-        gmem(pt.out[o + 7])[u] = st[4];
-        gmem(pt.out[o + 8])[u] = st[5];
+        gst(pt.out[o + 7], u, st[4]);
+        gst(pt.out[o + 8], u, st[5]);
     }
 }
 
@@ -347,7 +347,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             T v = (T)0;
             // only the tile's stencil region (x: tile +-3, y: tile +-1), inside the domain
             if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1))
-                v = gmem(pt.in[stage_comp<NS>(0, c)])[uidx(pt, gi, gj, 0)];
+                v = gld(pt.in[stage_comp<NS>(0, c)], (unsigned)uidx(pt, gi, gj, 0));
             sm[e] = v;
         }
         __syncthreads();
@@ -379,7 +379,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             T v = (T)0;
             // only the tile's stencil region (x: tile +-1, y: tile +-3), inside the domain
             if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3))
-                v = gmem(pt.in[stage_comp<NS>(1, c)])[uidx(pt, gi, gj, 0)];
+                v = gld(pt.in[stage_comp<NS>(1, c)], (unsigned)uidx(pt, gi, gj, 0));
             sm[e] = v;
         }
         __syncthreads();

@@ -35,6 +35,22 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// Element idx of a plane as the plane's base plus a 32-bit byte offset: the hardware adds that
+// offset to a base held in scalar registers, with no 64-bit address arithmetic per access.
+// Every plane the library addresses holds at most 4 GiB (checked where levels and the
+// drop-in path's scratch are sized).
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, unsigned idx)
+{
+    return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)gmem(base) +
+                                                          idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, unsigned idx, T v)
+{
+    *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)gmem(base) + idx * (unsigned)sizeof(T)) = v;
+}
+
 // ---------------------------------------------------------------------------
 // parameters (device copy of tmk_params plus derived constants)
 // ---------------------------------------------------------------------------

@@ -41,7 +41,7 @@ def dt_schedule(nsteps, cfg0, cfl=None, initial_cfl=None, max_dt_growth=1.1, fix
     (amrGodunov.cpp:443-461: cfl, initial_cfl, max_dt_growth, fixed_dt).
 
     Every level's dt is a CFL number times dx_l / max(cs, cp) (AMRLevelLinElast.cpp:479-480,
-    :1892-1901), and dx_l r^l = dx_0, so the coarse dt the driver takes as the minimum over
+    :1892-1901), and dx_l r^l = dx_0, so the coarse dt the AMR driver takes as the minimum over
     levels is the same number times dx_0 / max(cs, cp): the first step uses initial_cfl, every
     later one min(cfl dx_0 / c, max_dt_growth times the previous dt); a positive fixed_dt
     replaces all of that.  The wave speed is constant, so the schedule ramps geometrically

@@ -8,7 +8,7 @@
 //   LinElastLevelGodunov      (srcLinElast/LinElastLevelGodunov.H:91-114)
 // Box and FArrayBox here are minimal stand-ins with Chombo's storage convention
 // (inclusive index box, Fortran order); a Chombo build passes its own
-// FArrayBox::dataPtr()/box() to the same C ABI calls.
+// FArrayBox::dataPtr()/box() to the same C ABI calls (include/tetemoko_hip.h).
 // Errors the reference turns into MayDay::Error / Fortran stop are thrown as
 // tmk::Error carrying the C ABI code.
 #pragma once

@@ -51,7 +51,7 @@ def normal(seed, stream, idx, mu=0.0, sigma=1.0):
 
 
 # ---------------------------------------------------------------------------
-# material (homogeneous): cp = 6000, cs = 3464 m/s, rho = 2670 kg/m^3
+# material (homogeneous, every workload here): cp = 6000, cs = 3464 m/s, rho = 2670 kg/m^3
 # ---------------------------------------------------------------------------
 RHO = 2.67e-3
 CS = 3464.0

@@ -131,6 +131,10 @@ int tmk_patch_plastic_update(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_b
 int tmk_level_create(tmk_ctx *ctx, int nboxes, const tmk_box *boxes, tmk_level **out);
 /* device bytes tmk_level_create would allocate for these boxes */
 int tmk_level_footprint(const tmk_ctx *ctx, int nboxes, const tmk_box *boxes, size_t *bytes);
+/* the same before any context exists, from the parameters a context would be created with
+ * (admission control: no reference counterpart; the reference sizes one run per process) */
+int tmk_level_footprint_params(const tmk_params *params, int precision, int nboxes, const tmk_box *boxes,
+                               size_t *bytes);
 int tmk_level_destroy(tmk_level *lev);
 /* copy one box's ghosted state (grow(box,4), 22 comps) and fault data
  * (x in [lo-4, hi+4], j = 0, nbdry comps; NULL if the box has none) to/from the device */

@@ -42,3 +42,17 @@ def test_library_exports_all_declared():
 def test_strerror_without_device():
     assert capi.strerror(capi.E_PSI_NAN).startswith("NaN")
     assert capi.strerror(-99) == "unknown error"
+
+
+def test_footprint_without_a_context():
+    """tmk_level_footprint_params is host code: the device bytes of a level before any context."""
+    from tetemoko_amd import scenarios
+    cfg = scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0)
+    g = (96 + 8) * (48 + 8)
+    fault = 2 * (96 + 8) * 11 * 8 + (96 + 8) * 4
+    assert capi.level_footprint_params(cfg, [(0, 0, 95, 47)]) == 3 * 22 * g * 8 + g * 4 + fault
+    assert capi.level_footprint_params(cfg, [(0, 0, 95, 47)], capi.PREC_FP32_BULK) == 3 * 22 * g * 4 + g * 4 + fault
+    # two boxes far apart are two islands, each over its own ghosted box; only one has fault data
+    a, b = (16 + 8) * (16 + 8), (32 + 8) * (24 + 8)
+    want = 3 * 22 * (a + b) * 8 + (a + b) * 4 + 2 * (16 + 8) * 11 * 8 + (16 + 8) * 4
+    assert capi.level_footprint_params(cfg, [(0, 0, 15, 15), (64, 24, 95, 47)]) == want

@@ -581,6 +581,9 @@ def test_mem_info_and_footprint():
     # three state arrays over the ghosted box, its cell-owner map, two fault-data rows and their
     # face-owner map
     assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + (96 + 8) * (48 + 8) * 4 + 2 * (96 + 8) * 11 * 8 + (96 + 8) * 4
+    # the context-free form agrees on every layout (one island, split islands, boxes off the fault)
+    for boxes in ([(0, 0, 95, 47)], [(0, 0, 47, 47), (48, 0, 95, 47)], [(0, 0, 15, 15), (64, 24, 95, 47)]):
+        assert capi.level_footprint_params(cfg, boxes) == capi.level_footprint(ctx, boxes), boxes
     ctx.close()
 
 

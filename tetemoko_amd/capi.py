@@ -34,16 +34,15 @@ PREC_FP32_BULK = 1
 
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
-    "tmk_level_footprint", "tmk_batch_create", "tmk_batch_destroy", "tmk_batch_add", "tmk_batch_remove",
-    "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill", "tmk_level_download_fault", "tmk_level_tag",
-    "tmk_patch_update_state", "tmk_patch_plastic_update",
-    "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download",
-    "tmk_level_step", "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
-    "tmk_level_profile_kernels", "tmk_batch_member_error",
-    "tmk_stations_create", "tmk_stations_read", "tmk_stations_destroy",
-    "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed", "tmk_amr_sync",
-    "tmk_make_boxes", "tmk_level_regrid",
-    "tmk_amr_batch_create", "tmk_amr_add", "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error",
+    "tmk_level_footprint", "tmk_level_footprint_params", "tmk_batch_create", "tmk_batch_destroy",
+    "tmk_batch_add", "tmk_batch_remove", "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill",
+    "tmk_level_download_fault", "tmk_level_tag", "tmk_patch_update_state", "tmk_patch_plastic_update",
+    "tmk_level_create", "tmk_level_destroy", "tmk_level_upload", "tmk_level_download", "tmk_level_step",
+    "tmk_level_run", "tmk_level_run_timed", "tmk_level_profile", "tmk_level_sync", "tmk_set_tile",
+    "tmk_level_profile_kernels", "tmk_batch_member_error", "tmk_stations_create", "tmk_stations_read",
+    "tmk_stations_destroy", "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed",
+    "tmk_amr_sync", "tmk_make_boxes", "tmk_level_regrid", "tmk_amr_batch_create", "tmk_amr_add",
+    "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error",
 )
 
 
@@ -167,6 +166,7 @@ def lib():
         sz = ctypes.c_size_t
         L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
+        L.tmk_level_footprint_params.argtypes = [ctypes.POINTER(Params), c_int, c_int, PBox, ctypes.POINTER(sz)]
         L.tmk_batch_create.argtypes = [ctypes.POINTER(vp)]
         L.tmk_batch_destroy.argtypes = [vp]
         L.tmk_batch_add.argtypes = [vp, vp, _D, _D]
@@ -503,6 +503,17 @@ def level_footprint(ctx, boxes):
     arr = (Box * len(boxes))(*[Box.make(*b) for b in boxes])
     n = ctypes.c_size_t()
     _check(lib().tmk_level_footprint(ctx.h, len(boxes), arr, ctypes.byref(n)), "tmk_level_footprint")
+    return n.value
+
+
+def level_footprint_params(cfg, boxes, precision=PREC_FP64):
+    """Device bytes of a level of these boxes for a scenario config, without a context."""
+    p, keep = make_params(cfg)
+    arr = (Box * len(boxes))(*[Box.make(*b) for b in boxes])
+    n = ctypes.c_size_t()
+    _check(lib().tmk_level_footprint_params(ctypes.byref(p), precision, len(boxes), arr, ctypes.byref(n)),
+           "tmk_level_footprint_params")
+    del keep
     return n.value
 
 

@@ -13,10 +13,12 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "tetemoko_hip.h"
+
 #include "tmk_kernels.hpp"
 #include "tmk_interior.hpp"
 #include "tmk_amr.hpp"
@@ -394,6 +396,36 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
     P.dhi[0] = h.domain.hi[0]; P.dhi[1] = h.domain.hi[1];
 }
 
+// Streams of destroyed contexts, idle and synchronized, for the next contexts: creating and
+// destroying a stream per scenario took milliseconds of host time in a serving loop.  Kept
+// until the process ends; every stream is non-blocking at the default priority.
+static std::mutex g_stream_mu;
+static std::vector<hipStream_t> g_stream_pool;
+
+static hipError_t stream_take(hipStream_t* s)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if (!g_stream_pool.empty()) {
+            *s = g_stream_pool.back();
+            g_stream_pool.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+static void stream_give(hipStream_t s)
+{
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if (g_stream_pool.size() < 64) g_stream_pool.push_back(s);
+    else (void)hipStreamDestroy(s);
+}
+
 extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
 {
     if (!out || !p) return TMK_E_ARG;
@@ -414,25 +446,38 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
     default_tile(precision, c->bx, c->by);
     fill_dev_params(*p, c->P);
     c->antiplane = !(p->sxz0 == 0.0 && p->syz0 == 0.0);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (stream_take(&c->stream) != hipSuccess) {
+        c->stream = nullptr;
         delete c;
         return TMK_E_HIP;
     }
     const double* fa[8] = {p->f_a, p->f_b, p->f_L, p->f_fs, p->f_fd, p->f_dc, p->f_syy, p->f_tau};
     const double** fd[8] = {&c->P.f_a, &c->P.f_b, &c->P.f_L, &c->P.f_fs, &c->P.f_fd, &c->P.f_dc, &c->P.f_syy,
                             &c->P.f_tau};
+    // the per-face parameter arrays in one allocation, filled by one copy
+    int nfa = 0;
     for (int k = 0; k < 8; ++k) {
         *fd[k] = nullptr;
-        if (!fa[k]) continue;
+        if (fa[k]) ++nfa;
+    }
+    if (nfa > 0) {
         if (p->nface <= 0) { tmk_ctx_destroy(c); return TMK_E_ARG; }
+        const size_t n = (size_t)p->nface;
+        std::vector<double> host(n * nfa);
         double* d = nullptr;
-        if (hipMalloc(&d, sizeof(double) * p->nface) != hipSuccess) { tmk_ctx_destroy(c); return TMK_E_NOMEM; }
+        if (hipMalloc(&d, sizeof(double) * host.size()) != hipSuccess) { tmk_ctx_destroy(c); return TMK_E_NOMEM; }
         c->face_dev.push_back(d);
-        if (hipMemcpy(d, fa[k], sizeof(double) * p->nface, hipMemcpyHostToDevice) != hipSuccess) {
+        int q = 0;
+        for (int k = 0; k < 8; ++k) {
+            if (!fa[k]) continue;
+            std::copy(fa[k], fa[k] + n, host.begin() + q * n);
+            *fd[k] = d + q * n;
+            ++q;
+        }
+        if (hipMemcpy(d, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
             tmk_ctx_destroy(c);
             return TMK_E_HIP;
         }
-        *fd[k] = d;
     }
     c->hp.f_a = c->hp.f_b = c->hp.f_L = c->hp.f_fs = c->hp.f_fd = c->hp.f_dc = c->hp.f_syy = c->hp.f_tau = nullptr;
     *out = c;
@@ -445,7 +490,7 @@ extern "C" int tmk_ctx_destroy(tmk_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (double* d : c->face_dev) (void)hipFree(d);
     for (auto& b : c->scratch) b.release();
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream) stream_give(c->stream);
     delete c;
     return TMK_OK;
 }
@@ -1045,13 +1090,13 @@ static void island_rects(const std::vector<tmk_box>& boxes, const Island& I, std
     out = r;
 }
 
-static size_t level_bytes(const tmk_ctx* c, int nboxes, const tmk_box* boxes)
+static size_t level_bytes(int precision, int fault_kind, tmk_box domain, int nboxes, const tmk_box* boxes)
 {
-    const size_t ts = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+    const size_t ts = precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const int nb = fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     std::vector<Island> isl;
     std::vector<int> bi;
-    build_islands(std::vector<tmk_box>(boxes, boxes + nboxes), c->hp.domain, isl, bi);
+    build_islands(std::vector<tmk_box>(boxes, boxes + nboxes), domain, isl, bi);
     size_t tot = 0;
     for (const Island& I : isl) {
         tot += 3 * I.plane() * TMK_NCOMP * ts + I.plane() * sizeof(int);
@@ -1063,7 +1108,16 @@ static size_t level_bytes(const tmk_ctx* c, int nboxes, const tmk_box* boxes)
 extern "C" int tmk_level_footprint(const tmk_ctx* c, int nboxes, const tmk_box* boxes, size_t* bytes)
 {
     if (!c || nboxes <= 0 || !boxes || !bytes) return TMK_E_ARG;
-    *bytes = level_bytes(c, nboxes, boxes);
+    *bytes = level_bytes(c->precision, c->hp.fault_kind, c->hp.domain, nboxes, boxes);
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_footprint_params(const tmk_params* p, int precision, int nboxes, const tmk_box* boxes,
+                                          size_t* bytes)
+{
+    if (!p || nboxes <= 0 || !boxes || !bytes) return TMK_E_ARG;
+    if (precision != TMK_PREC_FP64 && precision != TMK_PREC_FP32_BULK) return TMK_E_ARG;
+    *bytes = level_bytes(precision, p->fault_kind, p->domain, nboxes, boxes);
     return TMK_OK;
 }
 
@@ -1097,7 +1151,7 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     L->boxes.assign(boxes, boxes + nboxes);
     L->ns = c->antiplane ? 3 : 2;
     L->tsize = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
-    L->bytes = level_bytes(c, nboxes, boxes);
+    L->bytes = level_bytes(c->precision, c->hp.fault_kind, c->hp.domain, nboxes, boxes);
     const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     for (int b = 0; b < nboxes; ++b) L->has_b.push_back(boxes[b].lo[1] == dom.lo[1]);
     build_islands(L->boxes, dom, L->isl, L->box_isl);

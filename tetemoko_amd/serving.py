@@ -138,9 +138,7 @@ def _footprint(sc, precision):
     cfg = sc["cfg"]
     tot = 0
     for l, boxes in enumerate(layout):
-        ctx = capi.Context(cfg, precision)
-        tot += capi.level_footprint(ctx, boxes)
-        ctx.close()
+        tot += capi.level_footprint_params(cfg, boxes, precision)
         if len(layout) > 1:
             w = max(b[2] for b in boxes) - min(b[0] for b in boxes) + 1
             h = max(b[3] for b in boxes) - min(b[1] for b in boxes) + 1

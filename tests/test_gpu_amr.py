@@ -423,3 +423,38 @@ def test_dt_schedule_ramp_matches_oracle():
         assert np.array_equal(Ug[:, G:-G, G:-G][:, lev.valid], lev.U[:, G:-G, G:-G][:, lev.valid]), l
         assert np.array_equal(Bg[:, G:-G][:, lev.fvalid], lev.B[:, G:-G][:, lev.fvalid]), l
     H.close()
+
+
+def test_regrid_without_tags_removes_finer_levels_and_tags_restore_them():
+    """As Chombo's AMR::regrid: a level without tags ends the hierarchy (its finer levels are
+    removed); a later regrid whose tags reach down again re-creates them, properly nested, and
+    the hierarchy steps on."""
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    arrays = _background_arrays(_cfgs(cfg, 3, 2))
+    H = amr.Hierarchy(cfg, BOXES_3, 2, arrays=arrays)
+    H.run(4)
+
+    def tp(active):
+        return lambda l: capi.TagParams.make(refine_thresh=1e-4 if l in active else 1e30,
+                                             plastic_thresh=1e-5 if l in active else 1e30, tag_buffer=2, level=l)
+
+    counts = H.regrid(tp({0}), fill_ratio=0.7, block_factor=4, max_grid_size=32)
+    assert len(counts) == 2 and counts[1] > 0 and len(H.levels) == 2
+    H.run(2)
+    counts = H.regrid(tp(set()), fill_ratio=0.7, block_factor=4, max_grid_size=32)
+    assert counts == [2] and len(H.levels) == 1
+    H.run(2)
+    counts = H.regrid(tp({0, 1}), fill_ratio=0.7, block_factor=4, max_grid_size=32)
+    assert len(counts) == 3 and counts[1] > 0 and counts[2] > 0
+    for l in (1, 2):  # every fine box lies inside the next coarser level
+        cover = set()
+        for b in H.boxes[l - 1]:
+            cover.update((i, j) for i in range(b[0], b[2] + 1) for j in range(b[1], b[3] + 1))
+        for b in H.boxes[l]:
+            assert all((i // 2, j // 2) in cover for i in (b[0], b[2]) for j in (b[1], b[3])), (l, b)
+    H.run(2)
+    H.sync()
+    for l in range(3):
+        Ug, _ = H.download(l)
+        assert np.isfinite(Ug[:, G:-G, G:-G][~np.isnan(Ug[:, G:-G, G:-G])]).all()
+    H.close()

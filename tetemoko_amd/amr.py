@@ -107,21 +107,36 @@ class Hierarchy:
     def regrid(self, tag_params, fill_ratio=0.75, block_factor=8, max_grid_size=64):
         """AMR::regrid from level 0: for l = 0, 1, ... the tags of level l (tag_params(l) ->
         capi.TagParams) give the boxes of level l+1 (Berger-Rigoutsos, properly nested in
-        level l), whose data come from its old layout and from level l.  Returns the box
-        counts per level; a level left without tags keeps its old layout."""
+        level l), whose data come from its old layout where it had one and from level l.  As in
+        Chombo's AMR::regrid, a level without tags ends the hierarchy there: its finer levels are
+        removed, and a later regrid whose tags reach down again re-creates them (up to the
+        number of levels the hierarchy was made with).  Returns the box counts per level."""
         self.amr.close()
-        for l in range(len(self.levels) - 1):
+        l = 0
+        while l < len(self.cfgs) - 1 and l < len(self.levels):
             cfg = self.cfgs[l]
             tags = self.levels[l].tag(tag_params(l), cfg["nx"], cfg["ny"])
-            boxes = capi.make_boxes(tags, self.boxes[l] if l > 0 else None, ref_ratio=self.r,
-                                    block_factor=block_factor, max_grid_size=max_grid_size, fill_ratio=fill_ratio)
+            boxes = []
+            if tags.any():
+                boxes = capi.make_boxes(tags, self.boxes[l] if l > 0 else None, ref_ratio=self.r,
+                                        block_factor=block_factor, max_grid_size=max_grid_size, fill_ratio=fill_ratio)
             if not boxes:
-                continue
+                for lev in self.levels[l + 1:]:
+                    lev.close()
+                del self.levels[l + 1:]
+                del self.boxes[l + 1:]
+                break
             new = self._new_level(l + 1, boxes)
-            capi.level_regrid(new, self.levels[l + 1], self.levels[l], self.r)
-            self.levels[l + 1].close()
-            self.levels[l + 1] = new
-            self.boxes[l + 1] = boxes
+            old = self.levels[l + 1] if l + 1 < len(self.levels) else None
+            capi.level_regrid(new, old, self.levels[l], self.r)
+            if old is not None:
+                old.close()
+                self.levels[l + 1] = new
+                self.boxes[l + 1] = boxes
+            else:
+                self.levels.append(new)
+                self.boxes.append(boxes)
+            l += 1
         self.amr = capi.Amr(self.levels, self.r)
         return [len(b) for b in self.boxes]
 

@@ -122,9 +122,11 @@ def write_hierarchy(prefix, H, max_level=None):
     hierarchy's time; level l steps dt0 / r^l)."""
     H.sync()
     levels = []
-    for l, cfg in enumerate(H.cfgs):
+    for l in range(len(H.levels)):
+        cfg = H.cfgs[l]
         nb = scenarios.NB[cfg["fault"]]
         fb, data = level_fault_data(H.levels[l], H.boxes[l], nb)
         levels.append({"level": l, "ref_ratio": H.r, "dx": cfg["dx"], "dt": H.dt0 / H.r ** l, "time": H.time,
                        "domain_nx": cfg["nx"], "boxes": fb, "data": data})
+    max_level = len(H.cfgs) - 1 if max_level is None else max_level
     return write(plot_name(prefix, H.steps), levels, H.steps, H.time, BDRY_NAMES[H.cfgs[0]["fault"]], max_level)

@@ -1932,7 +1932,7 @@ struct AmrLevel {
 // islands (see Patch::fmask), kept while it is a member
 struct AmrMember {
     std::vector<tmk_level*> L;
-    std::vector<std::vector<double*>> F[2];          // [level][island]
+    std::vector<std::vector<double*>> F[2];          // [direction][level][island]
     std::vector<std::vector<unsigned char*>> fmask;  // [level][island]
     std::vector<PairLists*> lists;                   // [pair l-1, l], member-local indices
     double dt0 = 0.0;
@@ -2179,7 +2179,7 @@ static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
             const int ffi = d == 0 ? fi * r : fi * r + mm, ffj = d == 0 ? fj * r + mm : fj * r;
             en.rf.fface[mm] = flux_off(Fl->isl[Fl->box_isl[fb]].bbox, d, ffi, ffj);
         }
-        if (!M.F[l - 1][d][C->box_isl[cb]] || !M.F[l][d][Fl->box_isl[fb]]) bad = true;
+        if (!M.F[d][l - 1][C->box_isl[cb]] || !M.F[d][l][Fl->box_isl[fb]]) bad = true;
         en.cbox = cb;
         en.s = inv * (side ? 1.0 : -1.0);
         ents.push_back(en);

@@ -458,3 +458,21 @@ def test_regrid_without_tags_removes_finer_levels_and_tags_restore_them():
         Ug, _ = H.download(l)
         assert np.isfinite(Ug[:, G:-G, G:-G][~np.isnan(Ug[:, G:-G, G:-G])]).all()
     H.close()
+
+
+# level 2 in three separate islands (boxes whose ghost regions do not meet)
+BOXES_ISL = [[(0, 0, 23, 31), (24, 0, 47, 31)],
+             [(16, 0, 79, 31)],
+             [(40, 0, 55, 15), (72, 0, 87, 15), (120, 0, 135, 15)]]
+
+
+def test_separate_fine_islands_build_every_time_and_match_oracle():
+    """A finest level of several separate islands: the hierarchy's lists build every time (their
+    guard reads the flux-array table per direction, level and island), and a slip-weakening run
+    matches the oracle bit for bit on every level."""
+    cfg = scenarios.config_c0(seed=2, nx=48, ny=32, dx=100.0)
+    arrays = _background_arrays(_cfgs(cfg, 3, 2))
+    for _ in range(8):
+        H = amr.Hierarchy(cfg, BOXES_ISL, 2, arrays=arrays)
+        H.close()
+    _compare(cfg, BOXES_ISL, 2, 3, arrays, exact=True)

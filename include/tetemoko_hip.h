@@ -44,6 +44,7 @@ extern "C" {
 #define TMK_E_AMR_TIME (-11)   /* LinElastLevelGodunov.cpp:250-258 "alpha < 0.0" / "alpha > 1.0" */
 #define TMK_E_NESTING (-12)    /* a finer level is not aligned to the ratio or not properly nested */
 #define TMK_E_IO (-13)         /* file output failed (boundary plot files, libtmk_h5.so) */
+#define TMK_E_SCRATCH (-14)    /* tmk_patch_update_state_noalloc: the call needs more than its fixed scratch */
 
 #define TMK_FAULT_RS 0
 #define TMK_FAULT_SW 1
@@ -117,6 +118,22 @@ int tmk_mem_info(size_t *free_bytes, size_t *total_bytes);
 int tmk_patch_update_state(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_box, int ncomp,
                            double *F0, double *F1, double *B, tmk_box B_box, int nbdry,
                            double time, double dt, double *max_wave_speed);
+
+/* tmk_patch_update_state with no device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks, results and error codes as
+ * tmk_patch_update_state (the same kernels on the same tiles), but every device buffer of the
+ * call -- both state copies, the fault data, the fluxes, the FP64 staging of the FP32 bulk and the
+ * launch tables -- lives in one fixed array of TMK_NOALLOC_SCRATCH_BYTES per device, held from
+ * the first context created on that device until the process ends.  TMK_E_SCRATCH when the call
+ * needs more than that (for FP64 with both fluxes, boxes up to about 128 x 128 cells fit) or
+ * more than 1024 tiles.  Every call on a device shares that array: calls that overlap on one
+ * device (from several threads, or contexts used concurrently) race and must be serialised by
+ * the caller. */
+#define TMK_NOALLOC_SCRATCH_BYTES (16u << 20)
+int tmk_patch_update_state_noalloc(tmk_ctx *ctx, tmk_box valid, double *U, tmk_box U_box, int ncomp,
+                                   double *F0, double *F1, double *B, tmk_box B_box, int nbdry,
+                                   double time, double dt, double *max_wave_speed);
 
 /* LinElastPatchGodunov::plasticUpdate (LinElastPatchGodunov.cpp:161-170) ->
  * PLASTICCORRECTION (PlasticPhysicsF.ChF:30-69) on the valid box. */

@@ -28,6 +28,7 @@ E_ARG = -6
 E_HIP = -7
 E_UNSUPPORTED = -8
 E_NOMEM = -9
+E_SCRATCH = -14
 
 PREC_FP64 = 0
 PREC_FP32_BULK = 1
@@ -42,7 +43,7 @@ EXPORTED = (
     "tmk_level_profile_kernels", "tmk_batch_member_error", "tmk_stations_create", "tmk_stations_read",
     "tmk_stations_destroy", "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed",
     "tmk_amr_sync", "tmk_make_boxes", "tmk_level_regrid", "tmk_amr_batch_create", "tmk_amr_add",
-    "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error",
+    "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error", "tmk_patch_update_state_noalloc",
 )
 
 
@@ -151,6 +152,7 @@ def lib():
         L.tmk_strerror.restype = ctypes.c_char_p
         L.tmk_device_ok.argtypes = []
         L.tmk_patch_update_state.argtypes = [vp, Box, _PD, Box, c_int, _PD, _PD, _PD, Box, c_int, _D, _D, _PD]
+        L.tmk_patch_update_state_noalloc.argtypes = L.tmk_patch_update_state.argtypes
         L.tmk_patch_plastic_update.argtypes = [vp, Box, _PD, Box, c_int, _D]
         L.tmk_level_create.argtypes = [vp, c_int, PBox, ctypes.POINTER(vp)]
         L.tmk_level_destroy.argtypes = [vp]
@@ -300,14 +302,16 @@ class Context:
     def set_tile(self, bx, by):
         _check(lib().tmk_set_tile(self.h, bx, by), "tmk_set_tile")
 
-    def update_state(self, valid, U, ubox, B=None, bbox=None, F0=None, F1=None, time=0.0, dt=1.0):
-        """LinElastPatchGodunov::updateState on host FArrayBox arrays (in place). Returns max wave speed."""
+    def update_state(self, valid, U, ubox, B=None, bbox=None, F0=None, F1=None, time=0.0, dt=1.0, noalloc=False):
+        """LinElastPatchGodunov::updateState on host FArrayBox arrays (in place). Returns max wave speed.
+        noalloc: through tmk_patch_update_state_noalloc (no device allocation; fixed per-device scratch)."""
         mws = ctypes.c_double()
         bb = Box.make(*bbox) if bbox is not None else Box.make(0, 0, -1, -1)
         nb = B.shape[0] if B is not None else 0
-        code = lib().tmk_patch_update_state(self.h, Box.make(*valid), _ptr(U), Box.make(*ubox), NCOMP,
-                                            _ptr(F0), _ptr(F1), _ptr(B), bb, nb, time, dt, ctypes.byref(mws))
-        _check(code, "tmk_patch_update_state")
+        fn = lib().tmk_patch_update_state_noalloc if noalloc else lib().tmk_patch_update_state
+        code = fn(self.h, Box.make(*valid), _ptr(U), Box.make(*ubox), NCOMP,
+                  _ptr(F0), _ptr(F1), _ptr(B), bb, nb, time, dt, ctypes.byref(mws))
+        _check(code, "tmk_patch_update_state_noalloc" if noalloc else "tmk_patch_update_state")
         return mws.value
 
     def plastic_update(self, valid, U, ubox, dt):

@@ -28,6 +28,7 @@
 #endif
 
 static void default_tile(int precision, int& bx, int& by);
+static int scratch_device_ptr(void** out);
 
 // the FP32-bulk update kernels are instantiated in tmk_f32.hip (built with FMA contraction)
 namespace tmk {
@@ -241,9 +242,10 @@ struct tmk_ctx {
     int precision = TMK_PREC_FP64;
     hipStream_t stream = nullptr;
     std::vector<double*> face_dev;
-    DevBuf scratch[8];
+    DevBuf bufs[8];  // grow-only device buffers of the drop-in path, uploads and tags
     int bx = 32, by = 16;
     bool antiplane = true;
+    void* noalloc_scratch = nullptr;  // this device's scratch array (tmk_patch_update_state_noalloc)
 };
 
 struct tmk_batch;
@@ -324,7 +326,8 @@ static const char* k_errs[] = {"ok",
                                "station records dropped: buffer full before it was read",
                                "coarse-fine time interpolation weight outside [0, 1]",
                                "finer level not aligned to the refinement ratio or not properly nested",
-                               "file output failed"};
+                               "file output failed",
+                               "box exceeds the fixed scratch of the allocation-free drop-in"};
 
 extern "C" const char* tmk_strerror(int code)
 {
@@ -451,6 +454,9 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
         delete c;
         return TMK_E_HIP;
     }
+    // the allocation-free drop-in's scratch on this device; on failure only that entry point
+    // is unavailable (it returns TMK_E_HIP)
+    if (scratch_device_ptr(&c->noalloc_scratch) != TMK_OK) c->noalloc_scratch = nullptr;
     const double* fa[8] = {p->f_a, p->f_b, p->f_L, p->f_fs, p->f_fd, p->f_dc, p->f_syy, p->f_tau};
     const double** fd[8] = {&c->P.f_a, &c->P.f_b, &c->P.f_L, &c->P.f_fs, &c->P.f_fd, &c->P.f_dc, &c->P.f_syy,
                             &c->P.f_tau};
@@ -489,7 +495,7 @@ extern "C" int tmk_ctx_destroy(tmk_ctx* c)
     if (!c) return TMK_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (double* d : c->face_dev) (void)hipFree(d);
-    for (auto& b : c->scratch) b.release();
+    for (auto& b : c->bufs) b.release();
     if (c->stream) stream_give(c->stream);
     delete c;
     return TMK_OK;
@@ -815,31 +821,123 @@ static int group_error(Group& G, hipStream_t st, int m, int* code)
 // ---------------------------------------------------------------------------
 // per-patch drop-in (host FArrayBox data)
 // ---------------------------------------------------------------------------
+
+// The tiles of one box for the drop-in path, with no heap allocation: interior tiles, then the
+// boundary tiles cut into boundary-shape pieces, fault-row pieces first -- the partition of
+// partition_tiles (without its XCD ordering, which a single box does not need; the order of the
+// tiles does not change any result).  Writes at most cap tiles to out (out may be NULL to
+// count); returns the count, and the interior count in *nint.
+static int patch_tiles(const tmk_box& v, const DevParams& P, int bx, int by, Tile* out, int cap, int* nint)
+{
+    const int tx = bx - 2, ty = by - 2;
+    int n = 0;
+    auto put = [&](const Tile& t) {
+        if (out && n < cap) out[n] = t;
+        ++n;
+    };
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int y0 = v.lo[1]; y0 <= v.hi[1]; y0 += ty)
+            for (int x0 = v.lo[0]; x0 <= v.hi[0]; x0 += tx) {
+                Tile t;
+                t.patch = 0;
+                t.x0 = x0;
+                t.y0 = y0;
+                t.x1 = std::min(x0 + tx - 1, v.hi[0]);
+                t.y1 = std::min(y0 + ty - 1, v.hi[1]);
+                const bool in = tile_interior_host(t, P, bx, by);
+                if (pass == 0) {
+                    if (in) put(t);
+                    continue;
+                }
+                if (in) continue;
+                for (int py = t.y0; py <= t.y1; py += BND_BY - 2)
+                    for (int px = t.x0; px <= t.x1; px += BND_BX - 2) {
+                        Tile p = t;
+                        p.x0 = px;
+                        p.y0 = py;
+                        p.x1 = std::min(px + BND_BX - 3, t.x1);
+                        p.y1 = std::min(py + BND_BY - 3, t.y1);
+                        if ((p.y0 == P.dlo[1]) == (pass == 1)) put(p);
+                    }
+            }
+        if (pass == 0) *nint = n;
+    }
+    return n;
+}
+
+// FP64 <-> FP32 copies of the state for the FP32-bulk drop-in without host buffers
+static __global__ void k_to_f32(const double* __restrict__ s, float* __restrict__ d, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        d[k] = (float)s[k];
+}
+static __global__ void k_to_f64(const float* __restrict__ s, double* __restrict__ d, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        d[k] = (double)s[k];
+}
+
+// Device buffers of one drop-in call.  stage: FP64 image of the state for the FP32 bulk,
+// converted on the device (NULL: converted on the host).  meta: the patch descriptor, the
+// tiles and the scenario block, at the offsets patch_meta gives.
+template <typename T>
+struct PatchWork {
+    T* Uin = nullptr;
+    T* Uout = nullptr;
+    double* Bd = nullptr;
+    double* Fd = nullptr;
+    double* stage = nullptr;
+    char* meta = nullptr;
+};
+
+template <typename T>
+static void patch_meta(int ntiles, size_t* off_t, size_t* off_s, size_t* bytes)
+{
+    *off_t = ((sizeof(Patch<T>) + 255) / 256) * 256;
+    *off_s = *off_t + ((sizeof(Tile) * (size_t)ntiles + 255) / 256) * 256;
+    *bytes = *off_s + sizeof(Scen);
+}
+
+// Sizes of one drop-in call, in elements
+struct PatchSizes {
+    size_t unx, plane, nU, bnx, nB, nF0, nF1;
+};
+static PatchSizes patch_sizes(const tmk_box& valid, const tmk_box& ub, const double* F0, const double* F1,
+                              const double* B, const tmk_box& bb, int nb)
+{
+    PatchSizes z;
+    z.unx = (size_t)(ub.hi[0] - ub.lo[0] + 1);
+    z.plane = z.unx * (size_t)(ub.hi[1] - ub.lo[1] + 1);
+    z.nU = z.plane * TMK_NCOMP;
+    z.bnx = B ? (size_t)(bb.hi[0] - bb.lo[0] + 1) : 0;
+    z.nB = z.bnx * (size_t)nb;
+    z.nF0 = F0 ? (size_t)(valid.hi[0] - valid.lo[0] + 2) * (valid.hi[1] - valid.lo[1] + 1) * TMK_NCOMP : 0;
+    z.nF1 = F1 ? (size_t)(valid.hi[0] - valid.lo[0] + 1) * (valid.hi[1] - valid.lo[1] + 2) * TMK_NCOMP : 0;
+    return z;
+}
+
+// One drop-in call on the buffers of w.  tiles: the box's tiles (patch_tiles), nint interior.
 template <typename T>
 static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, double* F0, double* F1, double* B,
-                          tmk_box bb, int nb, double time, double dt, bool plastic_only)
+                          tmk_box bb, int nb, double time, double dt, bool plastic_only, const PatchWork<T>& w,
+                          const Tile* tiles, int ntiles, int nint)
 {
-    const size_t unx = (size_t)(ub.hi[0] - ub.lo[0] + 1), uny = (size_t)(ub.hi[1] - ub.lo[1] + 1);
-    const size_t plane = unx * uny;
+    const PatchSizes z = patch_sizes(valid, ub, F0, F1, B, bb, nb);
+    const size_t unx = z.unx, plane = z.plane, nU = z.nU, bnx = z.bnx, nB = z.nB, nF0 = z.nF0, nF1 = z.nF1;
     if (plane * sizeof(T) > kPlaneMaxBytes) return TMK_E_NOMEM;  // 32-bit plane offsets (gld/gst)
-    const size_t nU = plane * TMK_NCOMP;
-    const size_t bnx = B ? (size_t)(bb.hi[0] - bb.lo[0] + 1) : 0;
-    const size_t nB = bnx * (size_t)nb;
-    const size_t nF0 = F0 ? (size_t)(valid.hi[0] - valid.lo[0] + 2) * (valid.hi[1] - valid.lo[1] + 1) * TMK_NCOMP : 0;
-    const size_t nF1 = F1 ? (size_t)(valid.hi[0] - valid.lo[0] + 1) * (valid.hi[1] - valid.lo[1] + 2) * TMK_NCOMP : 0;
-    int e;
-    if ((e = c->scratch[1].ensure(nU * sizeof(T)))) return e;
-    if ((e = c->scratch[2].ensure(nU * sizeof(T)))) return e;
-    if ((e = c->scratch[3].ensure(std::max<size_t>(2 * nB, 1) * sizeof(double)))) return e;
-    if ((e = c->scratch[4].ensure(std::max<size_t>(nF0 + nF1, 1) * sizeof(double)))) return e;
     hipStream_t st = c->stream;
-    T* Uin = (T*)c->scratch[1].p;
-    T* Uout = (T*)c->scratch[2].p;
-    double* Bd = (double*)c->scratch[3].p;
-    double* Fd = (double*)c->scratch[4].p;
+    T* Uin = w.Uin;
+    T* Uout = w.Uout;
+    double* Bd = w.Bd;
+    double* Fd = w.Fd;
+    const unsigned cvt_blocks = (unsigned)std::min<size_t>((nU + 255) / 256, 4096);
     std::vector<float> tmp;
     if (sizeof(T) == sizeof(double)) {
         HIPCHK(hipMemcpyAsync(Uin, U, nU * sizeof(double), hipMemcpyHostToDevice, st));
+    } else if (w.stage) {
+        HIPCHK(hipMemcpyAsync(w.stage, U, nU * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_to_f32, dim3(cvt_blocks), dim3(256), 0, st, (const double*)w.stage, (float*)Uin, nU);
+        HIPCHK(hipGetLastError());
     } else {
         tmp.resize(nU);
         for (size_t k = 0; k < nU; ++k) tmp[k] = (float)U[k];
@@ -868,28 +966,21 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         pt.F[1] = F1 ? Fd + nF0 : nullptr;
         pt.vlo0 = valid.lo[0]; pt.vlo1 = valid.lo[1]; pt.vhi0 = valid.hi[0]; pt.vhi1 = valid.hi[1];
         pt.scen = 0;
-        std::vector<Tile> tiles;
-        make_tiles(valid, 0, c->bx - 2, c->by - 2, tiles);
-        const int bx = c->bx, by = c->by;
-        const int nint = partition_tiles(
-            tiles, [&](const Tile& t) { return tile_interior_host(t, c->P, bx, by); },
-            [&](const Tile& t) { return t.y0 == c->P.dlo[1]; });
-        const size_t off_t = ((sizeof(Patch<T>) + 255) / 256) * 256;
-        const size_t off_s = off_t + ((sizeof(Tile) * tiles.size() + 255) / 256) * 256;
-        if ((e = c->scratch[5].ensure(off_s + sizeof(Scen)))) return e;
-        char* meta = (char*)c->scratch[5].p;
+        size_t off_t, off_s, mbytes;
+        patch_meta<T>(ntiles, &off_t, &off_s, &mbytes);
+        char* meta = w.meta;
         Scen sc;
         memset(&sc, 0, sizeof sc);
         sc.P = c->P;
         scen_set_clock(sc, time, dt);
         HIPCHK(hipMemcpyAsync(meta, &pt, sizeof pt, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(meta + off_t, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(meta + off_t, tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(meta + off_s, &sc, sizeof sc, hipMemcpyHostToDevice, st));
         KArgs<T> a;
         a.scen = (Scen*)(meta + off_s);
         a.patches = (const Patch<T>*)meta;
         a.tiles = (const Tile*)(meta + off_t);
-        a.ntiles = (int)tiles.size();
+        a.ntiles = ntiles;
         a.plastic = 0;  // updateState does not include the plastic return
         // the anti-plane planes are skipped only when background and data are zero there
         int ns = 3;
@@ -904,6 +995,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
                 }
             ns = zero ? 2 : 3;
         }
+        int e;
         if ((e = launch_update<T>(st, nullptr, nullptr, nullptr, ns, pred_kind(c->P), c->bx, c->by, a, nint, nullptr)))
             return e;
         int herr = 0;
@@ -917,6 +1009,11 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
     if (sizeof(T) == sizeof(double)) {
         HIPCHK(hipMemcpyAsync(U, Uout, nU * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+    } else if (w.stage) {
+        hipLaunchKernelGGL(k_to_f64, dim3(cvt_blocks), dim3(256), 0, st, (const float*)Uout, w.stage, nU);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(U, w.stage, nU * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
     } else {
         tmp.resize(nU);
         HIPCHK(hipMemcpyAsync(tmp.data(), Uout, nU * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -926,9 +1023,41 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
     return TMK_OK;
 }
 
-extern "C" int tmk_patch_update_state(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp, double* F0,
-                                      double* F1, double* B, tmk_box bb, int nbdry, double time, double dt,
-                                      double* max_wave_speed)
+// drop-in call on the context's grow-only buffers (allocated on first need)
+template <typename T>
+static int patch_update_bufs(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, double* F0, double* F1, double* B,
+                             tmk_box bb, int nb, double time, double dt, bool plastic_only)
+{
+    const PatchSizes z = patch_sizes(valid, ub, F0, F1, B, bb, nb);
+    if (z.plane * sizeof(T) > kPlaneMaxBytes) return TMK_E_NOMEM;
+    int nint = 0;
+    std::vector<Tile> tiles;
+    if (!plastic_only) {
+        const int n = patch_tiles(valid, c->P, c->bx, c->by, nullptr, 0, &nint);
+        tiles.resize(n);
+        patch_tiles(valid, c->P, c->bx, c->by, tiles.data(), n, &nint);
+    }
+    size_t off_t, off_s, mbytes;
+    patch_meta<T>((int)tiles.size(), &off_t, &off_s, &mbytes);
+    int e;
+    if ((e = c->bufs[1].ensure(z.nU * sizeof(T)))) return e;
+    if ((e = c->bufs[2].ensure(z.nU * sizeof(T)))) return e;
+    if ((e = c->bufs[3].ensure(std::max<size_t>(2 * z.nB, 1) * sizeof(double)))) return e;
+    if ((e = c->bufs[4].ensure(std::max<size_t>(z.nF0 + z.nF1, 1) * sizeof(double)))) return e;
+    if ((e = c->bufs[5].ensure(mbytes))) return e;
+    PatchWork<T> w;
+    w.Uin = (T*)c->bufs[1].p;
+    w.Uout = (T*)c->bufs[2].p;
+    w.Bd = (double*)c->bufs[3].p;
+    w.Fd = (double*)c->bufs[4].p;
+    w.meta = (char*)c->bufs[5].p;
+    return patch_update_t<T>(c, valid, U, ub, F0, F1, B, bb, nb, time, dt, plastic_only, w, tiles.data(),
+                             (int)tiles.size(), nint);
+}
+
+// argument checks of tmk_patch_update_state (B is cleared when valid does not touch the fault)
+static int patch_args(tmk_ctx* c, const tmk_box& valid, const double* U, const tmk_box& ub, int ncomp, double*& B,
+                      const tmk_box& bb, int nbdry)
 {
     if (!c || !U || ncomp != TMK_NCOMP) return TMK_E_ARG;
     if (box_empty(valid)) return TMK_E_ARG;
@@ -947,10 +1076,19 @@ extern "C" int tmk_patch_update_state(tmk_ctx* c, tmk_box valid, double* U, tmk_
     } else {
         B = nullptr;
     }
+    return TMK_OK;
+}
+
+extern "C" int tmk_patch_update_state(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp, double* F0,
+                                      double* F1, double* B, tmk_box bb, int nbdry, double time, double dt,
+                                      double* max_wave_speed)
+{
+    int e;
+    if ((e = patch_args(c, valid, U, ub, ncomp, B, bb, nbdry))) return e;
     if (max_wave_speed) *max_wave_speed = std::max(c->hp.cs, c->hp.cp);
     if (c->precision == TMK_PREC_FP64)
-        return patch_update_t<double>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
-    return patch_update_t<float>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
+        return patch_update_bufs<double>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
+    return patch_update_bufs<float>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt, false);
 }
 
 extern "C" int tmk_patch_plastic_update(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp, double dt)
@@ -958,8 +1096,89 @@ extern "C" int tmk_patch_plastic_update(tmk_ctx* c, tmk_box valid, double* U, tm
     if (!c || !U || ncomp != TMK_NCOMP) return TMK_E_ARG;
     if (box_empty(valid) || !box_contains(ub, valid)) return TMK_E_BOX;
     if (c->precision == TMK_PREC_FP64)
-        return patch_update_t<double>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
-    return patch_update_t<float>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
+        return patch_update_bufs<double>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
+    return patch_update_bufs<float>(c, valid, U, ub, nullptr, nullptr, nullptr, ub, 0, 0.0, dt, true);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free drop-in
+// ---------------------------------------------------------------------------
+// Device scratch of tmk_patch_update_state_noalloc: the two state copies, the fault data (read
+// and written copies), the fluxes, the FP64 staging of the FP32 bulk and the launch tables of
+// one call.  One array per device for the life of the process; every call on a device shares
+// it, so calls that overlap on one device race.  Host code never passes this name to a kernel
+// or a copy: it is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress (scratch_device_ptr).
+static __device__ __align__(256) unsigned char scratch[TMK_NOALLOC_SCRATCH_BYTES];
+
+// the most tiles one allocation-free call launches (they are listed in a stack array)
+static constexpr int kNoAllocMaxTiles = 1024;
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];  // per device ordinal, resolved once
+
+// device address of scratch on the current device (resolved once per device, at context set-up)
+static int scratch_device_ptr(void** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return TMK_E_HIP;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "tetemoko_hip: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return TMK_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TMK_OK;
+}
+
+static size_t align256(size_t b) { return ((b + 255) / 256) * 256; }
+
+template <typename T>
+static int patch_update_noalloc_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, double* F0, double* F1,
+                                  double* B, tmk_box bb, int nb, double time, double dt)
+{
+    if (!c->noalloc_scratch) return TMK_E_HIP;
+    const PatchSizes z = patch_sizes(valid, ub, F0, F1, B, bb, nb);
+    int nint = 0;
+    const int ntiles = patch_tiles(valid, c->P, c->bx, c->by, nullptr, 0, &nint);
+    if (ntiles > kNoAllocMaxTiles) return TMK_E_SCRATCH;
+    size_t off_t, off_s, mbytes;
+    patch_meta<T>(ntiles, &off_t, &off_s, &mbytes);
+    // layout: Uin, Uout, fault data (2 copies), fluxes, FP64 staging (FP32 bulk only), tables
+    const size_t oUout = align256(z.nU * sizeof(T));
+    const size_t oB = oUout + align256(z.nU * sizeof(T));
+    const size_t oF = oB + align256(2 * z.nB * sizeof(double));
+    const size_t oS = oF + align256((z.nF0 + z.nF1) * sizeof(double));
+    const size_t oM = oS + (sizeof(T) == sizeof(double) ? 0 : align256(z.nU * sizeof(double)));
+    if (oM + mbytes > (size_t)TMK_NOALLOC_SCRATCH_BYTES) return TMK_E_SCRATCH;
+    Tile tiles[kNoAllocMaxTiles];
+    patch_tiles(valid, c->P, c->bx, c->by, tiles, kNoAllocMaxTiles, &nint);
+    char* base = (char*)c->noalloc_scratch;
+    PatchWork<T> w;
+    w.Uin = (T*)base;
+    w.Uout = (T*)(base + oUout);
+    w.Bd = (double*)(base + oB);
+    w.Fd = (double*)(base + oF);
+    w.stage = sizeof(T) == sizeof(double) ? nullptr : (double*)(base + oS);
+    w.meta = base + oM;
+    return patch_update_t<T>(c, valid, U, ub, F0, F1, B, bb, nb, time, dt, false, w, tiles, ntiles, nint);
+}
+
+extern "C" int tmk_patch_update_state_noalloc(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, int ncomp,
+                                              double* F0, double* F1, double* B, tmk_box bb, int nbdry, double time,
+                                              double dt, double* max_wave_speed)
+{
+    int e;
+    if ((e = patch_args(c, valid, U, ub, ncomp, B, bb, nbdry))) return e;
+    if (max_wave_speed) *max_wave_speed = std::max(c->hp.cs, c->hp.cp);
+    if (c->precision == TMK_PREC_FP64)
+        return patch_update_noalloc_t<double>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt);
+    return patch_update_noalloc_t<float>(c, valid, U, ub, F0, F1, B, bb, nbdry, time, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1315,10 +1534,10 @@ static int level_put_fault(tmk_level* L, int b, const double* B)
     const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     const int n = L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST;
     int e;
-    if ((e = c->scratch[0].ensure((size_t)n * nb * sizeof(double)))) return e;
-    HIPCHK(hipMemcpy(c->scratch[0].p, B, (size_t)n * nb * sizeof(double), hipMemcpyHostToDevice));
+    if ((e = c->bufs[0].ensure((size_t)n * nb * sizeof(double)))) return e;
+    HIPCHK(hipMemcpy(c->bufs[0].p, B, (size_t)n * nb * sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_row_put, dim3((n + 255) / 256), dim3(256), 0, c->stream, I.b[0], I.b[1], I.bowner,
-                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (const double*)c->scratch[0].p, n, nb, b);
+                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (const double*)c->bufs[0].p, n, nb, b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return TMK_OK;
@@ -1331,12 +1550,12 @@ static int level_get_fault(tmk_level* L, int b, double* B, hipStream_t st)
     const int nb = c->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     const int n = L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST;
     int e;
-    if ((e = c->scratch[0].ensure((size_t)n * nb * sizeof(double)))) return e;
+    if ((e = c->bufs[0].ensure((size_t)n * nb * sizeof(double)))) return e;
     hipLaunchKernelGGL(k_row_get, dim3((n + 255) / 256), dim3(256), 0, st, I.b[L->cur],
-                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (double*)c->scratch[0].p, n, nb);
+                       L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (double*)c->bufs[0].p, n, nb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(B, c->scratch[0].p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(B, c->bufs[0].p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost));
     return TMK_OK;
 }
 
@@ -1364,17 +1583,17 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     int e;
-    if ((e = c->scratch[7].ensure(nU * L->tsize))) return e;
-    HIPCHK(hipMemcpy(c->scratch[7].p, src, nU * L->tsize, hipMemcpyHostToDevice));
+    if ((e = c->bufs[7].ensure(nU * L->tsize))) return e;
+    HIPCHK(hipMemcpy(c->bufs[7].p, src, nU * L->tsize, hipMemcpyHostToDevice));
     const Island& I = L->isl[L->box_isl[b]];
     const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
     const dim3 grid((unsigned)((plane + 255) / 256));
     if (L->tsize == sizeof(double))
         hipLaunchKernelGGL(k_box_put<double>, grid, dim3(256), 0, c->stream, (double*)I.u[0], (double*)I.u[1],
-                           (double*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const double*)c->scratch[7].p, gnx, gny, b);
+                           (double*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const double*)c->bufs[7].p, gnx, gny, b);
     else
         hipLaunchKernelGGL(k_box_put<float>, grid, dim3(256), 0, c->stream, (float*)I.u[0], (float*)I.u[1],
-                           (float*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const float*)c->scratch[7].p, gnx, gny, b);
+                           (float*)I.w, I.owner, I.unx(), I.plane(), ox, oy, (const float*)c->bufs[7].p, gnx, gny, b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (L->has_b[b] && (e = level_put_fault(L, b, B))) return e;
@@ -1433,23 +1652,23 @@ extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
     const size_t plane = (size_t)gnx * gny;
     const size_t nU = plane * TMK_NCOMP;
     int e;
-    if ((e = c->scratch[7].ensure(nU * L->tsize))) return e;
+    if ((e = c->bufs[7].ensure(nU * L->tsize))) return e;
     const Island& I = L->isl[L->box_isl[b]];
     const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
     const dim3 grid((unsigned)((plane + 255) / 256));
     if (L->tsize == sizeof(double))
         hipLaunchKernelGGL(k_box_get<double>, grid, dim3(256), 0, st, (const double*)I.u[L->cur], (const double*)I.w,
-                           I.unx(), I.plane(), ox, oy, (double*)c->scratch[7].p, gnx, gny, stencil_mask(L->ns));
+                           I.unx(), I.plane(), ox, oy, (double*)c->bufs[7].p, gnx, gny, stencil_mask(L->ns));
     else
         hipLaunchKernelGGL(k_box_get<float>, grid, dim3(256), 0, st, (const float*)I.u[L->cur], (const float*)I.w,
-                           I.unx(), I.plane(), ox, oy, (float*)c->scratch[7].p, gnx, gny, stencil_mask(L->ns));
+                           I.unx(), I.plane(), ox, oy, (float*)c->bufs[7].p, gnx, gny, stencil_mask(L->ns));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     if (L->tsize == sizeof(double)) {
-        HIPCHK(hipMemcpy(U, c->scratch[7].p, nU * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(U, c->bufs[7].p, nU * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         std::vector<float> a(nU);
-        HIPCHK(hipMemcpy(a.data(), c->scratch[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(a.data(), c->bufs[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < nU; ++k) U[k] = (double)a[k];
     }
     if (L->has_b[b] && B) return level_get_fault(L, b, B, st);
@@ -1875,8 +2094,8 @@ static int level_tag_t(tmk_level* L, const tmk_tagparams* tp, unsigned char* tag
     const int nx = dom.hi[0] - dom.lo[0] + 1, ny = dom.hi[1] - dom.lo[1] + 1;
     const size_t n = (size_t)nx * ny;
     int e;
-    if ((e = c->scratch[6].ensure(2 * n))) return e;
-    unsigned char* raw = (unsigned char*)c->scratch[6].p;
+    if ((e = c->bufs[6].ensure(2 * n))) return e;
+    unsigned char* raw = (unsigned char*)c->bufs[6].p;
     HIPCHK(hipMemsetAsync(raw, 0, 2 * n, st));
     TagArgs ta;
     ta.refine_thresh = tp->refine_thresh; ta.plastic_thresh = tp->plastic_thresh;

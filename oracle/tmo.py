@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libtmo.so")
+# TMO_LIB names another build of tmo.c (the mutation tests load deliberately broken copies)
+LIB_PATH = os.environ.get("TMO_LIB") or os.path.join(_HERE, "_build", "libtmo.so")
 
 NCOMP = 22
 NB_RS = 11

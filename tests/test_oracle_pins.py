@@ -23,6 +23,7 @@ import ctypes
 
 import numpy as np
 import pytest
+from scipy.integrate import solve_ivp
 from scipy.optimize import brentq
 
 import tmo
@@ -751,3 +752,292 @@ def test_tagging_rules():
     # coarsen region raises the threshold away from it (cell corners x = i dx)
     t = tmo.tag_level(op, U, B, dict(base, x_coarsen=(0.0, 500.0), slope_coarsen=10.0, width_coarsen=100.0))
     assert not t[15, 19]
+
+
+# ---------------------------------------------------------------------------
+# oblique plane waves: the CTU transverse coupling (LinElastPatchGodunov.cpp:397-452)
+# ---------------------------------------------------------------------------
+def _sinc(z):
+    z = np.asarray(z, dtype=float)
+    out = np.ones_like(z)
+    nz = np.abs(z) > 0
+    out[nz] = np.sin(z[nz]) / z[nz]
+    return out
+
+
+def _plane_wave(kind, theta, p, x, y, t, h, lam_w):
+    """Cell averages of an exact plane P or S wave of unit velocity amplitude travelling along
+    n = (cos theta, sin theta), from rho v_t = div sigma and sigma_t = lam tr(grad v) I +
+    G (grad v + grad v^T) (plane strain): P: v = n g, sigma = -(lam I + 2G n n^T) g / cp,
+    szz = -lam g / cp; S: v = t g with t = (-n_y, n_x), sigma = -G (n t^T + t n^T) g / cs.
+    g = sin(k (n.x - c t)); the average of g over a cell is g at its centre times
+    sinc(k n_x h / 2) sinc(k n_y h / 2)."""
+    n = np.array([np.cos(theta), np.sin(theta)])
+    k = 2 * np.pi / lam_w
+    c = p.cp if kind == "P" else p.cs
+    g = np.sin(k * (n[0] * x + n[1] * y) - k * c * t) * _sinc(k * n[0] * h / 2) * _sinc(k * n[1] * h / 2)
+    if kind == "P":
+        v, S, szz = n, -(p.lam * np.eye(2) + 2 * p.G * np.outer(n, n)) / p.cp, -p.lam / p.cp
+    else:
+        tv = np.array([-n[1], n[0]])
+        v, S, szz = tv, -p.G * (np.outer(n, tv) + np.outer(tv, n)) / p.cs, 0.0
+    out = np.zeros((11,) + g.shape)
+    out[0], out[1] = v[0] * g, v[1] * g
+    out[3], out[4], out[5], out[6] = S[0, 0] * g, S[1, 1] * g, szz * g, S[0, 1] * g
+    return out
+
+
+def _oblique_errors(pred, N, kinds, limit, theta=np.pi / 6):
+    """Max and mean error after N/4 steps at CFL 0.9 of a patch of N x N cells whose ghost cells
+    hold the exact solution before every step.  Side 1 carries a plane wave of kinds[0]; side 2
+    (the mirror image y -> -y, whose y derivatives change sign) the plane wave of kinds[1]
+    evaluated at (x, -y).  The patch lies 8 cells inside the domain: no boundary condition and
+    no fault face is involved."""
+    lam_w = 6400.0
+    h = lam_w / N
+    cfg = scenarios.config_c1(seed=1, nx=N + 16, ny=N + 16, dx=h)
+    cfg["background"] = (0.0,) * 6
+    cfg["numerics"] = {"normal_predictor": pred, "fourth_order": 1, "prim_limiting": int(limit),
+                       "char_limiting": 0}
+    op = tmo.OracleParams(cfg)
+    p = op.p
+    g = scenarios.GHOST
+    box = (8, 8, N + 7, N + 7)
+    ub = (box[0] - g, box[1] - g, box[2] + g, box[3] + g)
+    X, Y = np.meshgrid((np.arange(ub[0], ub[2] + 1) + 0.5) * h, (np.arange(ub[1], ub[3] + 1) + 0.5) * h)
+
+    def exact(t):
+        E = np.zeros((22,) + X.shape)
+        E[:11] = _plane_wave(kinds[0], theta, p, X, Y, t, h, lam_w)
+        E[11:] = _plane_wave(kinds[1], theta, p, X, -Y, t, h, lam_w)
+        return E
+
+    inside = np.zeros(X.shape, dtype=bool)
+    inside[g:-g, g:-g] = True
+    U = exact(0.0)
+    dt = 0.9 * h / p.cp
+    t = 0.0
+    for _ in range(N // 4):
+        U = np.ascontiguousarray(np.where(inside[None], U, exact(t)))
+        Uf = tmo.FabArray(ub[:2], ub[2:], 22, data=U)
+        assert L.tmo_update_state(Uf.fab, None, None, None, op.ref, tmo.Box.make(*box), t, dt) == 0
+        t += dt
+    w = np.array([1.0] * 3 + [1.0 / p.Zp] * 6 + [0.0] * 2)  # velocities; stresses / Zp
+    d = (U - exact(t))[:, g:-g, g:-g] * np.concatenate([w, w])[:, None, None]
+    return np.max(np.abs(d)), np.mean(np.abs(d))
+
+
+@pytest.mark.parametrize("kinds", [("P", "S"), ("S", "P")])
+@pytest.mark.parametrize("pred", ["CTU", "PLM", "PPM"])
+def test_oblique_plane_waves_converge(pred, kinds):
+    """Plane waves at 30 degrees to the grid move every state component in both sweeps, so the
+    transverse corrections W(d1) += -(1/2) dt/dx A(d2) dWHalf(d2) carry leading-order terms.
+    With them, CTU (W+- = W, first-order normal predictor) converges at first order and the
+    unlimited PLM and PPM predictors at second order; without them (or with a wrong sign or
+    scale) the unsplit update is inconsistent at this order and unstable at CFL 0.9."""
+    errs = np.array([_oblique_errors(pred, n, kinds, limit=False) for n in (32, 64, 128)])
+    orders = np.log2(errs[:-1, 0] / errs[1:, 0])
+    if pred == "CTU":
+        assert np.all((orders > 0.9) & (orders < 1.2)), (errs, orders)
+        assert errs[-1, 0] < 0.01
+    else:
+        assert np.all(orders > 1.85), (errs, orders)
+        assert errs[-1, 0] < 2e-4
+
+
+def test_oblique_plane_waves_limited_ppm_converge_in_l1():
+    """With the deck's primitive limiting the limiter clips the wave's extrema, so only the mean
+    error keeps second order."""
+    errs = np.array([_oblique_errors("PPM", n, ("P", "S"), limit=True) for n in (32, 64, 128)])
+    orders = np.log2(errs[:-1, 1] / errs[1:, 1])
+    assert np.all(orders > 1.9), (errs, orders)
+
+
+# ---------------------------------------------------------------------------
+# rate-and-state fault evolution (RSIBCF.ChF:116-132; predictor call at dt/2,
+# LinElastPatchGodunov.cpp:419-420 and LinElastPhysics.cpp:386-395; final call
+# at time + dt, LinElastPhysics.cpp:174)
+# ---------------------------------------------------------------------------
+# A fault whose parameters, prestress and state are uniform in x loads itself only through the
+# waves it radiates.  Those leave the fault (the incoming characteristics stay at their
+# initial values), so the face obeys the radiation-damping law
+#     tau0 - (Zs/2) V = sn a asinh(V exp(psi/a) / (2 V0)),
+# with psi evolving by the state law and slip by dslip/dt = V.  Velocity strengthening
+# (a > b) keeps the slip rate rising smoothly to a steady value, through the rupture
+# threshold 1e-3.
+RS_A, RS_B, RS_L, RS_V0, RS_F0, RS_SN = 0.015, 0.010, 0.01, 1e-6, 0.6, 120.0
+RS_ZS = scenarios.RHO * scenarios.CS
+
+
+def _rs_fric(V, psi):
+    return RS_A * np.arcsinh(V * np.exp(psi / RS_A) / (2 * RS_V0))
+
+
+def _rs_slip_rate(psi, tau):
+    g = lambda V: RS_SN * _rs_fric(V, psi) + 0.5 * RS_ZS * V - tau
+    return brentq(g, 0.0, tau / (0.5 * RS_ZS), xtol=1e-300, rtol=1e-15, maxiter=500)
+
+
+RS_PSI0 = RS_F0 - RS_B * np.log(1e-4 / RS_V0)
+RS_TAU = RS_SN * _rs_fric(1e-2, RS_F0 - RS_B * np.log(1e-2 / RS_V0)) + 0.5 * RS_ZS * 1e-2
+
+
+def _rs_ode(law, t_end):
+    """The radiation-damping problem integrated by scipy (DOP853, rtol 1e-13): the slip law of
+    RSSETBND, dpsi/dt = -(V/L)(f(V, psi) - fss(V)), fss = f0 - (b - a) ln(V/V0), or the aging
+    law dpsi/dt = (b V0/L)(exp((f0 - psi)/b) - V/V0).  y = (psi, slip)."""
+    def rhs(t, y):
+        V = _rs_slip_rate(y[0], RS_TAU)
+        if law == 0:
+            d = -(V / RS_L) * (_rs_fric(V, y[0]) - (RS_F0 - (RS_B - RS_A) * np.log(V / RS_V0)))
+        else:
+            d = (RS_B * RS_V0 / RS_L) * (np.exp((RS_F0 - y[0]) / RS_B) - V / RS_V0)
+        return [d, V]
+    ev = lambda t, y: _rs_slip_rate(y[0], RS_TAU) - 1e-3
+    return solve_ivp(rhs, (0.0, t_end), [RS_PSI0, 0.0], method="DOP853", rtol=1e-13, atol=1e-16,
+                     dense_output=True, events=ev)
+
+
+def _uniform_fault_cfg(law, cfl, nx, dx):
+    cfg = scenarios.config_c1(seed=1, nx=nx, ny=6, dx=dx)
+    cfg["cfl"] = cfl
+    cfg["rs"].update({"a": RS_A, "b": RS_B, "L": RS_L, "V0": RS_V0, "f0": RS_F0, "psi": RS_PSI0,
+                      "nuc_S": 0.0, "rupture_thr": 1e-3, "state_law": law})
+    cfg["faces"] = {"lo": 0, "tau": np.full(nx, RS_TAU), "psi": np.full(nx, RS_PSI0)}
+    return cfg
+
+
+def _uniform_fault_run(law, cfl, t_end, nx=256, dx=2000.0):
+    """Centre-face fault data after t_end of a uniform fault on a level wide enough that the
+    disturbance of its x edges (travelling at cp) has not reached the centre."""
+    cfg = _uniform_fault_cfg(law, cfl, nx, dx)
+    U, B = scenarios.level_arrays(cfg)
+    dt = scenarios.dt_of(cfg)
+    n = int(round(t_end / dt))
+    # the edges' disturbance travels cp t_end: it stays a quarter of the half-width from the centre
+    assert abs(n * dt - t_end) < 1e-9 * t_end and scenarios.CP * t_end <= 0.75 * dx * nx / 2
+    assert tmo.level_steps(tmo.OracleParams(cfg), U, B, 0.0, dt, n, nthreads=4) == 0
+    return B[:, scenarios.GHOST + nx // 2], dt
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_uniform_fault_follows_the_radiation_damping_ode(law):
+    """Slip rate, slip and psi converge at first order in dt to the ODE solution (the
+    reference's explicit state update is first order), and the rupture time lies within a
+    step and a half of the ODE's threshold crossing.  Using V0/L for V/L in the slip law's
+    rate, or dropping b from the aging law's, converges to another ODE."""
+    T = 30.0
+    sol = _rs_ode(law, T)
+    psi_x, slip_x = sol.sol(T)
+    V_x = _rs_slip_rate(psi_x, RS_TAU)
+    t_rup = sol.t_events[0][0]
+    assert 0.0 < t_rup < T
+    errs = []
+    for cfl in (0.9, 0.45, 0.225):
+        b, dt = _uniform_fault_run(law, cfl, T)
+        errs.append([abs(b[2] - V_x) / V_x, abs(b[8] - slip_x) / slip_x, abs(b[10] - psi_x)])
+        assert abs(b[9] - t_rup) <= 1.5 * dt, (cfl, b[9], t_rup)
+    errs = np.array(errs)
+    ratios = errs[:-1] / errs[1:]
+    assert np.all((ratios > 1.8) & (ratios < 2.3)), (errs, ratios)
+    assert np.all(errs[-1] < [0.02, 0.02, 5e-4]), errs
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_fault_face_state_is_time_centred(law):
+    """One step from rest of the loading: the final Riemann state of the fault face, and so
+    the slip rate the step records and slips with, is the time-centred value V(t + dt/2) to
+    second order -- the predictor's fault update runs the state law over dt/2 before the
+    final solve.  Relative to V(t + dt) it is only first order, and the one-step slip is
+    dt V(t + dt/2), exact to third order."""
+    e_mid, e_end, e_slip = [], [], []
+    dx = 10000.0
+    for k in range(4):
+        cfl = 0.9 / 2 ** k
+        dt = cfl * dx / scenarios.CP
+        b, dt_run = _uniform_fault_run(law, cfl, dt, nx=64, dx=dx)
+        assert dt_run == dt
+        sol = _rs_ode(law, dt)
+        V_mid = _rs_slip_rate(sol.sol(dt / 2)[0], RS_TAU)
+        V_end = _rs_slip_rate(sol.sol(dt)[0], RS_TAU)
+        e_mid.append(abs(b[2] - V_mid) / V_mid)
+        e_end.append(abs(b[2] - V_end) / V_end)
+        e_slip.append(abs(b[8] - sol.sol(dt)[1]) / sol.sol(dt)[1])
+    r_mid = np.array(e_mid[:-1]) / np.array(e_mid[1:])
+    r_end = np.array(e_end[:-1]) / np.array(e_end[1:])
+    r_slip = np.array(e_slip[:-1]) / np.array(e_slip[1:])
+    assert np.all(r_mid > 3.6) and np.all(r_slip > 3.6), (e_mid, e_slip)
+    assert np.all(r_end < 2.3), e_end
+    assert e_mid[0] < 1e-3
+
+
+def _rupture_clock_check(cfg, nsteps, comp_V, comp_rt):
+    """Step a level one step at a time; every face's rupture time must be 1e10 until the end of
+    the first step whose recorded slip rate exceeds the threshold, then that step's end time
+    (the clock the step's fault data describe), and never change afterwards."""
+    U, B = scenarios.level_arrays(cfg)
+    op = tmo.OracleParams(cfg)
+    dt = scenarios.dt_of(cfg)
+    thr = cfg[cfg["fault"]]["rupture_thr"]
+    g = scenarios.GHOST
+    t = 0.0
+    stamp = np.full(cfg["nx"], 1e10)
+    for _ in range(nsteps):
+        assert tmo.level_steps(op, U, B, t, dt, 1) == 0
+        t_end = t + dt
+        first = (stamp == 1e10) & (B[comp_V, g:-g] > thr)
+        stamp = np.where(first, t_end, stamp)
+        assert np.array_equal(B[comp_rt, g:-g], stamp), t_end
+        t = t_end
+    return stamp
+
+
+def test_rupture_time_is_the_end_of_the_first_step_over_threshold():
+    """RSSETBND / SWSETBND stamp the time they are called with, time + dt in the final call:
+    the rupture time is the clock of the fault data in which the slip rate first exceeds the
+    threshold."""
+    cfg = _uniform_fault_cfg(1, 0.9, 64, 10000.0)
+    stamp = _rupture_clock_check(cfg, 20, 2, 9)
+    assert np.all(stamp < 1e10)
+    cfg = scenarios.config_c0(seed=3, nx=120, ny=40, dx=100.0)
+    stamp = _rupture_clock_check(cfg, 40, 4, 8)
+    assert 10 <= np.count_nonzero(stamp < 1e10) < 120
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_state_laws_relax_in_closed_form_at_constant_slip_rate(law):
+    """RSSETBND's update iterated at a fixed slip rate V is forward Euler on the state law, so
+    it converges at first order to the law's solution at constant V:
+      aging: theta = (L/V0) exp((psi - f0)/b) obeys dtheta/dt = 1 - V theta/L, so
+             theta(t) = L/V + (theta0 - L/V) exp(-V t/L);
+      slip:  with f = a asinh(V exp(psi/a)/(2V0)) = a ln(V/V0) + psi + O(exp(-2 psi/a)),
+             dpsi/dt = -(V/L)(psi - psi_ss), psi_ss = f0 - b ln(V/V0), so
+             psi(t) = psi_ss + (psi0 - psi_ss) exp(-V t/L)."""
+    op, cfg = _params()
+    p = op.p
+    p.state_law = law
+    i = 5
+    a, b, Lc = cfg["faces"]["a"][i], cfg["faces"]["b"][i], cfg["faces"]["L"][i]
+    V, T = 2e-3, 25.0
+    psi0 = p.rs_f0 - b * np.log(V / p.rs_V0) + 0.05
+    if law == 1:
+        th0 = Lc / p.rs_V0 * np.exp((psi0 - p.rs_f0) / b)
+        th = Lc / V + (th0 - Lc / V) * np.exp(-V * T / Lc)
+        psi_T = p.rs_f0 + b * np.log(p.rs_V0 * th / Lc)
+    else:
+        pss = p.rs_f0 - b * np.log(V / p.rs_V0)
+        psi_T = pss + (psi0 - pss) * np.exp(-V * T / Lc)
+    errs = []
+    for n in (100, 200, 400):
+        U = tmo.FabArray((i, 0), (i, 0), 22)
+        U.a[0, 0, 0] = V
+        B = tmo.FabArray((i, 0), (i, 0), 11)
+        B.a[10, 0, 0] = psi0
+        B.a[9, 0, 0] = 1e10
+        for k in range(n):
+            L.tmo_rssetbnd(B.fab, B.fab, U.fab, op.ref, T / n, (k + 1) * T / n, tmo.Box.make(i, 0, i, 0))
+        errs.append(abs(B.a[10, 0, 0] - psi_T))
+    assert abs(psi_T - psi0) > 0.02
+    r = np.array(errs[:-1]) / np.array(errs[1:])
+    assert np.all((r > 1.8) & (r < 2.2)), errs
+    assert errs[-1] < 2e-4

@@ -1,20 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: cell-updates/s of the level step on one MI355X.
+"""Benchmark: cell-updates/s of the level step on one MI355X, FP32 bulk against FP64.
 
-Workload (the plain FP64 drop-in configuration): one level of 4096 x 2048 cells cut
-into 8192 boxes of 32 x 32 (the reference's default max_grid_size; --max-grid-size 0
-runs it as one box), dx = 25 m, regularized rate-and-state fault with aging-law state
-evolution and
-seeded per-face a, L and normal stress, PPM normal predictor with primitive
-limiting, absorbing outer boundaries.  A cell-update is one valid cell advanced
-one level step; ghosts are not counted.  Inputs are resident in HBM before the
-timed region starts.
+Workload (the precision configuration the metric is quoted on): one level of 8192 x 4096 cells
+(33.5 M), dx = 12.5 m, cut into boxes of 32 x 32 (the reference's default max_grid_size;
+--max-grid-size 0 runs it as one box); regularized rate-and-state fault with aging-law state
+evolution, seeded per-face a, L and normal stress and a 25 MPa Gaussian nucleation bump;
+Drucker-Prager viscoplasticity (mu 0.6, beta 0.3, relaxation time 0.03 s, cohesion 1 MPa);
+PPM normal predictor with primitive limiting; absorbing outer boundaries.  A cell-update is one
+valid cell advanced one level step; ghosts are not counted.  Inputs are resident in HBM (set on
+the device) before the timed region starts.
 
-Printed as one JSON line.  The same workload is also run with FP32 bulk
-arithmetic (fault solve, state, slip and time in FP64) and its slip error
-against the FP64 run is reported beside it.  The CPU baseline is the oracle
-(a scalar C restatement of the reference) on a bounded sample of the same
-workload, run on this host.
+`value` is the FP32-bulk rate (bulk update and plastic return in FP32; fault solve, state
+exponential, slip and time in FP64); the FP64 rate and the ratio are reported beside it.  The
+accuracy leg runs both precisions from the same start for --acc-steps steps (4000 = 7.5 s
+simulated, the rupture has spread over about 9 km) and reports, at 64 seeded fault stations
+inside the ruptured region, the error of FP32 bulk against FP64 in final slip, peak slip rate
+and rupture arrival time (in cell-crossing times dx/cs).  A secondary leg times the plain FP64
+drop-in configuration (4096 x 2048, dx = 25 m, no plasticity).  The CPU baseline is the oracle (a
+scalar C restatement of the reference) on a bounded sample of the same workload, on this host.
+
+Printed as one JSON line.
 """
 import argparse
 import json
@@ -30,55 +35,78 @@ sys.path.insert(0, ROOT)
 from tetemoko_amd import capi, scenarios  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# bytes one cell-update must move with the state layout the kernel touches:
-# mode-II stencil planes (vx vy sxx syy sxy) + szz, read and written, both sides
-LIVE_COMPONENTS = 12
-ALL_COMPONENTS = 22
+G = scenarios.GHOST
+METRIC = "cell-updates/s/GPU, FP32-bulk vs FP64 at ≤1e-4 slip err; scenario p50/p99 latency"
+
+
+def stored_bytes(plastic, tsize):
+    """Bytes one cell-update moves in the stored layout the update kernel touches: per side it
+    reads vx vy sxx syy sxy szz (+ gam with plasticity) and writes them (+ lam and gam), both
+    sides; lam/gam in the bulk type."""
+    per_side = 12 + (3 if plastic else 0)
+    return per_side * 2 * tsize
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--tile", type=str, default="")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=8192)
+    ap.add_argument("--ny", type=int, default=4096)
+    ap.add_argument("--dx", type=float, default=12.5)
+    ap.add_argument("--tile", type=str, default="", help="update tile, e.g. 32x16 (default: the library's)")
     ap.add_argument("--max-grid-size", type=int, default=32,
                     help="boxes of at most this many cells per side (the reference's default max_grid_size "
                          "is 32); 0 = the level as one box")
+    ap.add_argument("--acc-steps", type=int, default=4000, help="steps of the accuracy leg (0: skip it)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the secondary 4096x2048 FP64 drop-in leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the per-patch drop-in leg")
     return ap.parse_args()
 
 
-def measured_traffic(nx, ny, tile):
-    """HBM bytes per launch of the FP64 update kernel from the committed counter runs
-    (profiles/traffic_k_godunov_fp64.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
-    the copy-kernel calibration), if they were taken on this workload and tile shape."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_godunov_fp64.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        t = json.load(f)
-    if t.get("nx") != nx or t.get("ny") != ny or t.get("tile") != (list(tile) if tile else "default"):
-        return None
-    return t.get("bytes_per_launch")
+def measured_traffic(workload, precision, tile):
+    """HBM bytes per launch of the interior update kernel from the committed counter runs
+    (profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    copy-kernel calibration), if one was taken on this workload, precision and tile."""
+    d = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        if not (name.startswith("traffic_") and name.endswith(".json")):
+            continue
+        with open(os.path.join(d, name)) as f:
+            t = json.load(f)
+        if t.get("workload") == workload and t.get("precision") == precision and \
+                t.get("tile") == (list(tile) if tile else "default"):
+            return t.get("bytes_per_launch"), "profiles/" + name
+    return None, None
 
 
-def level_boxes(cfg, max_grid_size):
+def level_boxes(nx, ny, max_grid_size):
     if max_grid_size <= 0:
-        return [(0, 0, cfg["nx"] - 1, cfg["ny"] - 1)]
-    return scenarios.split_boxes(cfg["nx"], cfg["ny"], max_grid_size)
+        return [(0, 0, nx - 1, ny - 1)]
+    return scenarios.split_boxes(nx, ny, max_grid_size)
 
 
-def run_level(cfg, U, B, precision, warmup, steps, tile, boxes):
+def make_level(cfg, precision, boxes, tile):
+    """A device-resident level holding the configuration's initial state (uniform background,
+    seeded fault data), set on the device box by box (no host copy of the state)."""
     ctx = capi.Context(cfg, precision)
     if tile:
         ctx.set_tile(*tile)
     lev = capi.Level(ctx, boxes)
+    B = scenarios.fault_arrays(cfg)
+    bg = scenarios.background_vector(cfg)
     for k, b in enumerate(boxes):
-        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+        lev.fill(k, bg, scenarios.box_bview(B, b) if b[1] == 0 else None)
+    return ctx, lev
+
+
+def timed_leg(cfg, precision, boxes, warmup, steps, tile):
+    """W untimed graph-replayed steps, then exactly K graph-replayed steps bracketed by HIP
+    events on the level's stream (after a device synchronisation), then per-launch durations of
+    the update kernels inside a replayed graph (HIP events on each kernel's stream)."""
+    ctx, lev = make_level(cfg, precision, boxes, tile)
     dt = scenarios.dt_of(cfg)
     t = 0.0
     lev.run(warmup, t, dt)
@@ -87,24 +115,67 @@ def run_level(cfg, U, B, precision, warmup, steps, tile, boxes):
     ms = lev.run_timed(steps, t, dt)
     lev.sync()
     t += steps * dt
-    Uo, Bo = np.full_like(U, np.nan), np.full_like(B, np.nan)
-    g = scenarios.GHOST
-    for k, b in enumerate(boxes):
-        x0, y0, x1, y1 = b
-        ub = np.empty((U.shape[0], y1 - y0 + 1 + 2 * g, x1 - x0 + 1 + 2 * g))
-        bb = np.empty((B.shape[0], x1 - x0 + 1 + 2 * g)) if y0 == 0 else None
-        lev.download(k, ub, bb)
-        Uo[:, y0 + g:y1 + g + 1, x0 + g:x1 + g + 1] = ub[:, g:-g, g:-g]
-        if bb is not None:
-            Bo[:, x0 + g:x1 + g + 1] = bb[:, g:-g]
-    # per-launch durations of the update kernels (eager steps; interior tiles on the level's
-    # stream, boundary tiles concurrently on its side stream), each from HIP events on the
-    # stream it runs on; continues the same simulation
-    kms = lev.profile_kernels(max(10, steps // 4), t, dt)
+    prof = lev.profile_kernels(max(10, min(steps, 50)), t, dt)
     lev.sync()
     lev.close()
     ctx.close()
-    return ms, kms, Uo, Bo
+    ncell = cfg["nx"] * cfg["ny"]
+    return {"ms_per_step": ms / steps, "value": ncell * steps / (ms * 1e-3), "profile": prof}
+
+
+def roofline(prof, plastic, tsize, workload, precision, tile):
+    bpc = stored_bytes(plastic, tsize)
+    achieved = prof["cells_interior"] * bpc / (prof["ms_interior"] * 1e-3) / 1e9
+    traffic, src = measured_traffic(workload, precision, tile)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
+            "kernel": "k_godunov_interior (interior tiles)", "kernel_ms": prof["ms_interior"],
+            "kernel_ms_timed_in_graph": bool(prof["in_graph"]),
+            "cells_per_launch": prof["cells_interior"], "bytes_per_cell_update": bpc,
+            "bytes_per_cell_update_22comp": 22 * 2 * tsize,
+            "frac_22comp": prof["cells_interior"] * 22 * 2 * tsize / (prof["ms_interior"] * 1e-3) / 1e9
+                           / HBM_PEAK_GBS,
+            "boundary_kernel_ms": prof["ms_boundary"], "boundary_cells": prof["cells_boundary"],
+            "step_ms_in_profile": prof["ms_step"]}
+
+
+def station_faces(cfg, n=64, half_width_m=4500.0, seed=1234):
+    """n distinct seeded fault faces within +-half_width_m of the nucleation centre; each
+    station sits on its face centre, x = (i + 1/2) dx, where the reference's station
+    interpolation (weight in metres, AMRLevelLinElast.cpp:262-270) returns the face value."""
+    dx = cfg["dx"]
+    xc = cfg["rs"]["nuc_x"]
+    lo, hi = int(np.ceil((xc - half_width_m) / dx)), int(np.floor((xc + half_width_m) / dx))
+    faces = np.sort(np.random.default_rng(seed).choice(np.arange(lo, hi), size=n, replace=False))
+    return faces, (faces + 0.5) * dx
+
+
+def accuracy_leg(cfg, precision, boxes, nsteps, xs, tile, chunk=500):
+    """nsteps steps from t = 0 with fault stations recorded on the device after every step:
+    final slip, peak slip rate (over every step) and rupture time at each station."""
+    ctx, lev = make_level(cfg, precision, boxes, tile)
+    st = capi.Stations(lev, xs, capacity=chunk)
+    dt = scenarios.dt_of(cfg)
+    peak = np.zeros(len(xs))
+    last = None
+    t, done = 0.0, 0
+    t0 = time.perf_counter()
+    while done < nsteps:
+        n = min(chunk, nsteps - done)
+        lev.run(n, t, dt)
+        for _ in range(n):
+            t = t + dt
+        done += n
+        rec = st.read()
+        assert rec.shape[0] == n
+        peak = np.maximum(peak, np.max(rec[:, :, 1 + 2], axis=0))
+        last = rec[-1]
+    lev.sync()
+    wall = time.perf_counter() - t0
+    st.close()
+    lev.close()
+    ctx.close()
+    return {"slip": last[:, 1 + 8], "peak": peak, "rupture": last[:, 1 + 9], "wall_s": wall, "time": last[0, 0]}
 
 
 def host_cpu():
@@ -129,15 +200,14 @@ def drop_in_patch_rate(precision, seconds_hint=3.0):
     cfg = scenarios.config_c1(seed=0, nx=512, ny=256, dx=25.0)
     U, B = scenarios.level_arrays(cfg)
     boxes = scenarios.split_boxes(cfg["nx"], cfg["ny"], 32)
-    g = scenarios.GHOST
     views = [(b, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None) for b in boxes]
     ctx = capi.Context(cfg, precision)
     dt = scenarios.dt_of(cfg)
 
     def one_step():
         for b, ub, bb in views:
-            ubox = (b[0] - g, b[1] - g, b[2] + g, b[3] + g)
-            bbox = (b[0] - g, 0, b[2] + g, 0) if bb is not None else None
+            ubox = (b[0] - G, b[1] - G, b[2] + G, b[3] + G)
+            bbox = (b[0] - G, 0, b[2] + G, 0) if bb is not None else None
             ctx.update_state(b, ub, ubox, bb, bbox, time=0.0, dt=dt)
 
     one_step()
@@ -157,7 +227,7 @@ def cpu_baseline(cfg_full, nthreads, seconds_hint):
     """Oracle (scalar C restatement) on a bounded sample: 512 x 256 cells of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import tmo
-    cfg = scenarios.config_c1(seed=0, nx=512, ny=256, dx=25.0)
+    cfg = scenarios.config_c3(seed=0, nx=512, ny=256, dx=cfg_full["dx"])
     U, B = scenarios.level_arrays(cfg)
     op = tmo.OracleParams(cfg)
     dt = scenarios.dt_of(cfg)
@@ -172,9 +242,10 @@ def cpu_baseline(cfg_full, nthreads, seconds_hint):
     ncpu, model = host_cpu()
     return {"value": cfg["nx"] * cfg["ny"] * nsteps / el, "unit": "cell-updates/s", "cores": nthreads,
             "kind": "port",
-            "sample": "512x256 cells of the same rate-and-state workload, %d level steps, boxes of 32x32, "
-                      "FP64, oracle C restatement (gcc -O3, no FMA contraction), %d thread(s), %.1f s; "
-                      "host: %d logical CPUs, %s" % (nsteps, nthreads, el, ncpu, model)}
+            "sample": "512x256 cells of the same rate-and-state + Drucker-Prager workload (dx %.1f m), %d level "
+                      "steps, boxes of 32x32, FP64, oracle C restatement (gcc -O3, no FMA contraction), %d "
+                      "thread(s), %.1f s; host: %d logical CPUs, %s" % (cfg["dx"], nsteps, nthreads, el, ncpu,
+                                                                          model)}
 
 
 def main():
@@ -187,73 +258,90 @@ def main():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(3)
     tile = tuple(int(x) for x in a.tile.split("x")) if a.tile else None
-    cfg = scenarios.config_c1(seed=0, nx=a.nx, ny=a.ny, dx=25.0)
-    U0, B0 = scenarios.level_arrays(cfg)
-    ncell = cfg["nx"] * cfg["ny"]
-    print("bench: %dx%d FP64, warmup %d, steps %d" % (cfg["nx"], cfg["ny"], a.warmup, a.steps),
+    cfg = scenarios.config_c3(seed=0, nx=a.nx, ny=a.ny, dx=a.dx)
+    boxes = level_boxes(cfg["nx"], cfg["ny"], a.max_grid_size)
+    wl = "rs-aging+drucker-prager %dx%d dx=%g" % (cfg["nx"], cfg["ny"], cfg["dx"])
+    print("bench: %s, %d boxes, FP32 bulk, warmup %d, steps %d" % (wl, len(boxes), a.warmup, a.steps),
           file=sys.stderr, flush=True)
-    boxes = level_boxes(cfg, a.max_grid_size)
-    ms64, kms64, U64, B64 = run_level(cfg, U0, B0, capi.PREC_FP64, a.warmup, a.steps, tile, boxes)
-    value = ncell * a.steps / (ms64 * 1e-3)
-    # dominant kernel: the interior-tile update; its algorithmic bytes are the live state planes
-    # of the cells it updates, read once and written once
-    bytes_live = kms64["cells_interior"] * LIVE_COMPONENTS * 2 * 8
-    achieved = bytes_live / (kms64["ms_interior"] * 1e-3) / 1e9
+    r32 = timed_leg(cfg, capi.PREC_FP32_BULK, boxes, a.warmup, a.steps, tile)
+    print("bench: FP64, warmup %d, steps %d" % (a.warmup, a.steps), file=sys.stderr, flush=True)
+    r64 = timed_leg(cfg, capi.PREC_FP64, boxes, a.warmup, a.steps, tile)
     out = {
-        "metric": "cell-updates/s/GPU (FP64 plain level step; FP32-bulk reported beside it)",
-        "value": value,
+        "metric": METRIC,
+        "value": r32["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms64 / a.steps,
+        "ms_per_step": r32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded per-face friction parameters, background prestress)",
-        "config": {"workload": "rate-and-state, aging law, single level %dx%d in %d boxes, dx=25 m, PPM, "
-                               "absorbing boundaries, FP64" % (cfg["nx"], cfg["ny"], len(boxes)),
+        "dtype": "f32 bulk (f64 fault solve, state exponential, slip, time)",
+        "data": "synthetic (seeded per-face friction parameters and normal stress, background prestress)",
+        "config": {"workload": "rate-and-state (aging law) + Drucker-Prager viscoplasticity, single level "
+                               "%dx%d (%.1f M cells) in %d boxes, dx=%g m, PPM, absorbing boundaries, FP32 bulk "
+                               "with FP64 fault; FP64 beside it" % (cfg["nx"], cfg["ny"], cfg["nx"] * cfg["ny"] / 1e6,
+                                                                     len(boxes), cfg["dx"]),
                    "nx": cfg["nx"], "ny": cfg["ny"], "boxes": len(boxes), "max_grid_size": a.max_grid_size,
-                   "dx_m": 25.0, "dt_s": scenarios.dt_of(cfg),
-                   "normal_predictor": "PPM", "tile": list(tile) if tile else "default"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(cfg["nx"], cfg["ny"], tile),
-                     "kernel": "k_godunov_interior (interior tiles)",
-                     "kernel_ms": kms64["ms_interior"],
-                     "cells_per_launch": kms64["cells_interior"],
-                     "bytes_per_cell_update": LIVE_COMPONENTS * 2 * 8,
-                     "bytes_per_cell_update_22comp": ALL_COMPONENTS * 2 * 8,
-                     "frac_22comp": kms64["cells_interior"] * ALL_COMPONENTS * 2 * 8 /
-                                    (kms64["ms_interior"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "boundary_kernel_ms": kms64["ms_boundary"], "boundary_cells": kms64["cells_boundary"],
-                     "step_ms_eager": kms64["ms_step"]},
+                   "dx_m": cfg["dx"], "dt_s": scenarios.dt_of(cfg), "normal_predictor": "PPM",
+                   "plasticity": {"mu": cfg["p_mu"], "beta": cfg["p_beta"], "eta_MPa_s": cfg["p_eta"],
+                                  "cohesion_MPa": cfg["p_b"]},
+                   "tile": list(tile) if tile else "default"},
+        "roofline": roofline(r32["profile"], True, 4, wl, "fp32_bulk", tile),
+        "fp64": {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                 "roofline": roofline(r64["profile"], True, 8, wl, "fp64", tile)},
+        "speedup_fp32_bulk_vs_fp64": r64["ms_per_step"] / r32["ms_per_step"],
     }
-    if not a.no_fp32:
-        print("bench: FP32-bulk leg", file=sys.stderr, flush=True)
-        ms32, kms32, U32, B32 = run_level(cfg, U0, B0, capi.PREC_FP32_BULK, a.warmup, a.steps, tile, boxes)
-        g = scenarios.GHOST
-        slip64, slip32 = B64[8, g:-g], B32[8, g:-g]
-        V64, V32 = B64[2, g:-g], B32[2, g:-g]
-        out["fp32_bulk"] = {
-            "value": ncell * a.steps / (ms32 * 1e-3),
-            "ms_per_step": ms32 / a.steps,
-            "kernel_ms": kms32["ms_interior"],
-            "boundary_kernel_ms": kms32["ms_boundary"],
-            "speedup_vs_fp64": ms64 / ms32,
-            "roofline_frac": kms32["cells_interior"] * LIVE_COMPONENTS * 2 * 4 / (kms32["ms_interior"] * 1e-3) / 1e9
-                             / HBM_PEAK_GBS,
-            "slip_rel_err": float(np.max(np.abs(slip32 - slip64)) / max(np.max(np.abs(slip64)), 1e-300)),
-            "slip_rate_rel_err": float(np.max(np.abs(V32 - V64)) / max(np.max(np.abs(V64)), 1e-300)),
+    if a.acc_steps > 0:
+        faces, xs = station_faces(cfg)
+        print("bench: accuracy leg, %d steps per precision, 64 stations" % a.acc_steps, file=sys.stderr, flush=True)
+        s64 = accuracy_leg(cfg, capi.PREC_FP64, boxes, a.acc_steps, xs, tile)
+        s32 = accuracy_leg(cfg, capi.PREC_FP32_BULK, boxes, a.acc_steps, xs, tile)
+        r64t, r32t = s64["rupture"], s32["rupture"]
+        both = (r64t < 1e9) & (r32t < 1e9)
+        cells = np.abs(r64t[both] - r32t[both]) * cfg["cs"] / cfg["dx"]
+        slip_err = float(np.max(np.abs(s32["slip"] - s64["slip"])) / max(np.max(np.abs(s64["slip"])), 1e-300))
+        arrival = float(np.max(cells)) if cells.size else None
+        out["accuracy"] = {
+            "steps": a.acc_steps, "simulated_s": s64["time"], "stations": len(xs),
+            "station_faces": [int(faces[0]), int(faces[-1])],
+            "slip_rel_err_max": slip_err,
+            "peak_slip_rate_rel_err_max": float(np.max(np.abs(s32["peak"] - s64["peak"])) /
+                                                max(np.max(np.abs(s64["peak"])), 1e-300)),
+            "rupture_arrival_err_cells_max": arrival,
+            "ruptured_stations_fp64": int(np.sum(r64t < 1e9)), "ruptured_stations_fp32": int(np.sum(r32t < 1e9)),
+            "max_slip_fp64_m": float(np.max(s64["slip"])), "max_peak_slip_rate_fp64": float(np.max(s64["peak"])),
+            "slip_err_le_1e-4": slip_err <= 1e-4,
+            "arrival_err_le_1_cell": arrival is not None and arrival <= 1.0 and
+                                     int(np.sum(r64t < 1e9)) == int(np.sum(r32t < 1e9)),
+            "wall_s_fp64": s64["wall_s"], "wall_s_fp32": s32["wall_s"],
         }
-    print("bench: per-patch drop-in leg", file=sys.stderr, flush=True)
-    out["drop_in_patch_fp64"] = drop_in_patch_rate(capi.PREC_FP64)
+    if not a.no_c1:
+        c1 = scenarios.config_c1(seed=0, nx=4096, ny=2048, dx=25.0)
+        print("bench: secondary leg, 4096x2048 FP64 drop-in configuration", file=sys.stderr, flush=True)
+        b1 = level_boxes(c1["nx"], c1["ny"], a.max_grid_size)
+        wl1 = "rs-aging 4096x2048 dx=25"
+        q64 = timed_leg(c1, capi.PREC_FP64, b1, a.warmup, a.steps, tile)
+        q32 = timed_leg(c1, capi.PREC_FP32_BULK, b1, a.warmup, a.steps, tile)
+        out["secondary_4096x2048_rs"] = {
+            "workload": "rate-and-state (aging law), single level 4096x2048 in %d boxes, dx=25 m, PPM, no "
+                        "plasticity" % len(b1),
+            "fp64": {"value": q64["value"], "ms_per_step": q64["ms_per_step"],
+                     "roofline": roofline(q64["profile"], False, 8, wl1, "fp64", tile)},
+            "fp32_bulk": {"value": q32["value"], "ms_per_step": q32["ms_per_step"],
+                          "roofline": roofline(q32["profile"], False, 4, wl1, "fp32_bulk", tile)},
+        }
+    if not a.no_dropin:
+        print("bench: per-patch drop-in leg", file=sys.stderr, flush=True)
+        out["drop_in_patch_fp64"] = drop_in_patch_rate(capi.PREC_FP64)
     if not a.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(cfg, 1, 12.0)
         nthr = min(16, os.cpu_count() or 1)
         cb_all = cpu_baseline(cfg, nthr, 6.0)
         cb["all_threads"] = {"value": cb_all["value"], "cores": nthr, "sample": cb_all["sample"]}
+        cb["gpu_fp32_bulk_over_cpu_1_thread"] = out["value"] / cb["value"]
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

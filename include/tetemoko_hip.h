@@ -180,15 +180,18 @@ int tmk_level_run_timed(tmk_level *lev, int nsteps, double time, double dt, doub
 int tmk_level_profile(tmk_level *lev, int nsteps, double time, double dt, double *ms_per_step);
 
 /* Per-kernel timing of level steps.  Measurement aid with no counterpart in the
- * reference.  Runs nsteps eager steps from (time, dt).  Each launch is bracketed by
- * HIP events on the stream it runs on: the interior-tile update on the level's
- * stream, the boundary-tile update on the side stream.  Reports means in ms. */
+ * reference.  Runs nsteps steps from (time, dt), captured as one graph and replayed once
+ * (eagerly where capture or event timing inside a graph is unavailable; in_graph says
+ * which).  Each launch is bracketed by HIP events on the stream it runs on: the
+ * interior-tile update on the level's stream, the boundary-tile update on the side
+ * stream.  Reports means in ms. */
 typedef struct {
     double ms_step;      /* whole step (both update launches, exchanges, clock) */
     double ms_interior;  /* interior-tile update launch */
     double ms_boundary;  /* boundary-tile update launch (0 when there is none) */
     long long cells_interior, cells_boundary; /* valid cells each launch updates */
     int tiles_interior, tiles_boundary;
+    int in_graph;        /* 1: timed inside a replayed graph; 0: eager steps */
 } tmk_profile;
 int tmk_level_profile_kernels(tmk_level *lev, int nsteps, double time, double dt, tmk_profile *out);
 /* synchronize and return the first error any launch reported */

@@ -80,6 +80,40 @@ int main()
     printf("patch: state rel err %.3e, fault %.3e, max wave speed %.1f\n", eu, eb, mws);
     if (!(eu <= 1e-12 && eb <= 1e-12 && mws == 6000.0)) return 1;
 
+    // --- plastic pass: LinElastPatchGodunov::plasticUpdate (LinElastPatchGodunov.cpp:161-170) on a
+    // box where about half the cells yield (Drucker-Prager with 1 MPa cohesion) ---
+    {
+        tmk_params pp = p;
+        tmo_params qp = q;
+        pp.use_plasticity = qp.use_plasticity = 1;
+        pp.p_mu = qp.p_mu = 0.6;
+        pp.p_beta = qp.p_beta = 0.3;
+        pp.p_eta = qp.p_eta = 0.03 * pp.G;
+        pp.p_b = qp.p_b = 1.0;
+        pp.p_h = qp.p_h = 0.0;
+        tmk::RSIBC pibc(pp);
+        tmk::LinElastPhysics pphys(&pibc);
+        tmk::LinElastPatchGodunov ppatch;
+        ppatch.define(domain, pp.dx, &pphys);
+        const tmk::Box pbox(4, 6, 35, 27);
+        tmk::FArrayBox P(tmk::grow(pbox, 4), 22);
+        for (int c = 0; c < 22; ++c)
+            for (int j = P.box().lo[1]; j <= P.box().hi[1]; ++j)
+                for (int i = P.box().lo[0]; i <= P.box().hi[0]; ++i)
+                    P(i, j, c) = q.background[c] + (c % 11 == 6 ? 150.0 : (c % 11 >= 3 && c % 11 <= 5) ? 40.0 : 0.0) * U1(rng);
+        tmk::FArrayBox Pr = P;
+        ppatch.plasticUpdate(P, dt, pbox);
+        tmo_fab pf{Pr.dataPtr(), {{Pr.box().lo[0], Pr.box().lo[1]}, {Pr.box().hi[0], Pr.box().hi[1]}}, 22};
+        tmo_plastic_correction(&pf, &qp, dt, tmo_box{{pbox.lo[0], pbox.lo[1]}, {pbox.hi[0], pbox.hi[1]}});
+        size_t yielded = 0, differ = 0;
+        const size_t nP = (size_t)P.box().size(0) * P.box().size(1);
+        for (size_t k = 0; k < nP * 22; ++k) differ += P.dataPtr()[k] != Pr.dataPtr()[k];
+        for (size_t k = 0; k < nP; ++k) yielded += Pr.dataPtr()[k + 9 * nP] > 0.0;
+        printf("plastic: %zu of %d cells yield, %zu values differ from the oracle\n", yielded,
+               pbox.size(0) * pbox.size(1), differ);
+        if (differ != 0 || yielded < (size_t)(pbox.size(0) * pbox.size(1)) / 4) return 1;
+    }
+
     // --- level: LinElastLevelGodunov::step over a 2x2 box layout ---
     std::vector<tmk::Box> grids = {tmk::Box(0, 0, 23, 15), tmk::Box(24, 0, 47, 15), tmk::Box(0, 16, 23, 31),
                                    tmk::Box(24, 16, 47, 31)};

@@ -5,13 +5,15 @@ fixed-order sums, and the FP64 level update reproduces the oracle bit for bit of
 so slip-weakening hierarchies (no transcendental functions on the fault) match the oracle
 bit for bit on every level: state on every valid cell and fault data on every valid face.
 Rate-and-state hierarchies stay within 1e-12 relative (per component, against its largest
-magnitude) -- the last-bit differences of exp/log/asinh on fault faces, as on one level.
+magnitude; rupture times per face against their own value, unruptured faces holding the 1e10
+sentinel exactly) -- the last-bit differences of exp/log/asinh on fault faces, as on one level.
 Runs of 3 and 4 coarse steps cover both the replayed two-step graph and a direct step.
 """
 import numpy as np
 import pytest
 
 import amr as oamr
+from faultcheck import SENTINEL, fault_errors
 from tetemoko_amd import amr, capi, scenarios
 
 pytestmark = pytest.mark.gpu
@@ -76,9 +78,8 @@ def _compare(cfg0, boxes, r, nsteps, arrays, exact, precision=capi.PREC_FP64):
             for c in range(scenarios.NCOMP):
                 scale = max(np.max(np.abs(Ur[c])), 1e-300)
                 assert np.max(np.abs(Ugv[c] - Ur[c])) <= 1e-12 * scale, (l, c)
-            for c in range(Br.shape[0]):
-                scale = max(np.max(np.abs(Br[c])), 1e-300)
-                assert np.max(np.abs(Bgv[c] - Br[c])) <= 1e-12 * scale, (l, c)
+            e = fault_errors(Bgv, Br)
+            assert np.all(e <= 1e-12), (l, e)
     H.close()
     return h
 
@@ -119,7 +120,10 @@ def test_ratio_four_with_plasticity_bit_identical():
 def test_rate_and_state_three_levels_within_tolerance(nsteps):
     cfg = scenarios.config_c1(seed=5, nx=48, ny=32, dx=25.0)
     cfg["rs"]["nuc_R"] = 400.0
-    _compare(cfg, BOXES_3, 2, nsteps, _background_arrays(_cfgs(cfg, 3, 2)), exact=False)
+    cfg["rs"]["rupture_thr"] = 1e-6  # faces of every level rupture inside the window
+    h = _compare(cfg, BOXES_3, 2, nsteps, _background_arrays(_cfgs(cfg, 3, 2)), exact=False)
+    for lev in h.levels:
+        assert np.count_nonzero(lev.B[9, G:-G][lev.fvalid] != SENTINEL) >= 8
 
 
 def _composite_sum(H, cfgs, boxes, r):
@@ -198,6 +202,8 @@ def _levels_equal(h, H, exact=True):
         else:
             for c in range(scenarios.NCOMP):
                 assert np.max(np.abs(Ugv[c] - Ur[c])) <= 1e-12 * max(np.max(np.abs(Ur[c])), 1e-300), (l, c)
+            e = fault_errors(Bgv, Br)
+            assert np.all(e <= 1e-12), (l, e)
 
 
 def _pair(cfg, boxes, r, arrays):

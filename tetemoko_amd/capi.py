@@ -50,7 +50,7 @@ EXPORTED = (
 class Profile(ctypes.Structure):
     _fields_ = [("ms_step", ctypes.c_double), ("ms_interior", ctypes.c_double), ("ms_boundary", ctypes.c_double),
                 ("cells_interior", ctypes.c_longlong), ("cells_boundary", ctypes.c_longlong),
-                ("tiles_interior", ctypes.c_int), ("tiles_boundary", ctypes.c_int)]
+                ("tiles_interior", ctypes.c_int), ("tiles_boundary", ctypes.c_int), ("in_graph", ctypes.c_int)]
 
 
 class MeshParams(ctypes.Structure):

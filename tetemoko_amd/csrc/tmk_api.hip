@@ -582,9 +582,30 @@ static int launch_tiles(hipStream_t st, int ns, int pk, int bx, int by, const KA
 
 // per-kernel timing of one step (profiling only): events around the interior launch on the
 // main stream and around the boundary launch on the side stream
+// Timing marks around the update launches of a step: HIP events (eager steps), or, inside a
+// captured graph, a one-lane kernel that writes the device's constant-rate wall clock into
+// stamps[0..3] (interior start, interior end, boundary start, boundary end).
 struct StepTimers {
-    hipEvent_t ia, ib, ba, bb;
+    hipEvent_t ia = nullptr, ib = nullptr, ba = nullptr, bb = nullptr;
+    long long* stamps = nullptr;
 };
+
+static __global__ void k_stamp(long long* at)
+{
+    if (threadIdx.x == 0) *at = wall_clock64();
+}
+
+// mark k (0 interior start, 1 interior end, 2 boundary start, 3 boundary end) on stream s
+static int timer_mark(const StepTimers* tm, hipStream_t s, int k)
+{
+    if (!tm) return TMK_OK;
+    if (tm->stamps) {
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, tm->stamps + k);
+        return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
+    }
+    const hipEvent_t e = k == 0 ? tm->ia : k == 1 ? tm->ib : k == 2 ? tm->ba : tm->bb;
+    return hipEventRecord(e, s) == hipSuccess ? TMK_OK : TMK_E_HIP;
+}
 
 // the fused update of one step over a tile list [interior | boundary]: the boundary launch runs
 // on `side` (forked from and joined back into st by events; captured graphs keep the fork), the
@@ -606,16 +627,16 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         KArgs<T> b = a;
         b.tiles = tiles + nint;
         b.ntiles = nbnd;
-        if (tm) HIPCHK(hipEventRecord(tm->ba, sb));
+        if ((e = timer_mark(tm, sb, 2))) return e;
         if ((e = launch_tiles<T>(sb, ns, pk, BND_BX, BND_BY, b, false))) return e;
-        if (tm) HIPCHK(hipEventRecord(tm->bb, sb));
+        if ((e = timer_mark(tm, sb, 3))) return e;
     }
     if (nint > 0) {
         KArgs<T> b = a;
         b.ntiles = nint;
-        if (tm) HIPCHK(hipEventRecord(tm->ia, st));
+        if ((e = timer_mark(tm, st, 0))) return e;
         if ((e = launch_tiles<T>(st, ns, pk, bx, by, b, true))) return e;
-        if (tm) HIPCHK(hipEventRecord(tm->ib, st));
+        if ((e = timer_mark(tm, st, 1))) return e;
     }
     if (split) {
         HIPCHK(hipEventRecord(join, side));
@@ -692,10 +713,12 @@ static int group_run(Group& G, hipStream_t st, int nsteps)
     }
     if (nsteps >= 2 && !G.gexec) {
         hipGraph_t gr;
+        const int g0 = G.g;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         e = group_enqueue_step(G, st);
         if (!e) e = group_enqueue_step(G, st);
         hipError_t ce = hipStreamEndCapture(st, &gr);
+        G.g = g0;  // nothing ran; two captured steps return to this parity, a failed capture must too
         if (e) return e;
         if (ce != hipSuccess) return TMK_E_HIP;
         hipError_t ie = hipGraphInstantiate(&G.gexec, gr, nullptr, nullptr, 0);
@@ -1781,45 +1804,117 @@ extern "C" int tmk_level_profile(tmk_level* L, int nsteps, double time, double d
     return e;
 }
 
+// nsteps steps of a level with timing events around each step and each update launch; ev holds
+// 6 events per step.  Returns the error of the enqueue.
+static int profile_enqueue(tmk_level* L, hipStream_t st, int nsteps, std::vector<hipEvent_t>& ev)
+{
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t* x = &ev[(size_t)6 * k];
+        StepTimers tm;
+        tm.ia = x[2]; tm.ib = x[3]; tm.ba = x[4]; tm.bb = x[5];
+        if (hipEventRecord(x[0], st) != hipSuccess) return TMK_E_HIP;
+        int e = group_enqueue_step(L->own, st, &tm);
+        if (e) return e;
+        if (hipEventRecord(x[1], st) != hipSuccess) return TMK_E_HIP;
+    }
+    return TMK_OK;
+}
+
+// the same steps with wall-clock stamps instead of events (6 per step: the four launch marks,
+// then the step's start and end on the level's stream)
+static int profile_enqueue_stamps(tmk_level* L, hipStream_t st, int nsteps, long long* stamps)
+{
+    for (int k = 0; k < nsteps; ++k) {
+        StepTimers tm;
+        tm.stamps = stamps + (size_t)6 * k;
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, tm.stamps + 4);
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+        int e = group_enqueue_step(L->own, st, &tm);
+        if (e) return e;
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, tm.stamps + 5);
+        if (hipGetLastError() != hipSuccess) return TMK_E_HIP;
+    }
+    return TMK_OK;
+}
+
 extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, double dt, tmk_profile* out)
 {
     if (!L || !out || nsteps <= 0 || L->batch || L->amr_stream) return TMK_E_ARG;
     int e = level_prepare(L, time, dt);
     if (e) return e;
     hipStream_t st = L->ctx->stream;
-    std::vector<hipEvent_t> ev((size_t)6 * nsteps, nullptr);
-    for (auto& x : ev)
-        if (hipEventCreate(&x) != hipSuccess) e = TMK_E_HIP;
-    const Group& G = L->own;
+    Group& G = L->own;
     const bool has_bnd = G.ntiles > G.nint, has_int = G.nint > 0;
-    for (int k = 0; k < nsteps && !e; ++k) {
-        hipEvent_t* x = &ev[(size_t)6 * k];
-        StepTimers tm{x[2], x[3], x[4], x[5]};
-        if (hipEventRecord(x[0], st) != hipSuccess) { e = TMK_E_HIP; break; }
-        e = group_enqueue_step(L->own, st, &tm);
-        if (!e && hipEventRecord(x[1], st) != hipSuccess) e = TMK_E_HIP;
-        L->cur ^= 1;
-        L->time = L->time + dt;
-    }
-    if (!e && hipStreamSynchronize(st) != hipSuccess) e = TMK_E_HIP;
     double sum[3] = {0, 0, 0};
-    for (int k = 0; k < nsteps && !e; ++k) {
-        const hipEvent_t* x = &ev[(size_t)6 * k];
-        float f = 0.f;
-        if (hipEventElapsedTime(&f, x[0], x[1]) != hipSuccess) { e = TMK_E_HIP; break; }
-        sum[0] += f;
-        if (has_int) {
-            if (hipEventElapsedTime(&f, x[2], x[3]) != hipSuccess) { e = TMK_E_HIP; break; }
-            sum[1] += f;
+    // The steps as one captured graph, replayed once, so that the launches run as in
+    // tmk_level_run; each mark is a one-lane kernel reading the constant-rate wall clock.
+    // Where the clock rate is unknown or capture fails, the same steps run eagerly with events.
+    int in_graph = 0;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+        khz > 0) {
+        DevBuf sb;
+        const size_t ns = (size_t)6 * nsteps;
+        if ((e = sb.ensure(ns * sizeof(long long)))) return e;
+        const int g0 = G.g;
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t gx = nullptr;
+        bool ran = false;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int ce = profile_enqueue_stamps(L, st, nsteps, (long long*)sb.p);
+        const hipError_t ee = hipStreamEndCapture(st, &gr);
+        G.g = g0;  // nothing has run yet
+        if (!ce && ee == hipSuccess && hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0) == hipSuccess &&
+            hipGraphLaunch(gx, st) == hipSuccess)
+            ran = true;
+        if (gx) (void)hipGraphExecDestroy(gx);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (ce) return ce;
+        if (ran) {
+            std::vector<long long> h(ns);
+            HIPCHK(hipMemcpyAsync(h.data(), sb.p, ns * sizeof(long long), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < nsteps; ++k) {
+                G.g ^= 1;
+                L->cur ^= 1;
+                L->time = L->time + dt;
+                const long long* x = &h[(size_t)6 * k];
+                sum[0] += (double)(x[5] - x[4]) / khz;  // kHz: ticks per ms
+                if (has_int) sum[1] += (double)(x[1] - x[0]) / khz;
+                if (has_bnd) sum[2] += (double)(x[3] - x[2]) / khz;
+            }
+            in_graph = 1;
         }
-        if (has_bnd) {
-            if (hipEventElapsedTime(&f, x[4], x[5]) != hipSuccess) { e = TMK_E_HIP; break; }
-            sum[2] += f;
-        }
+        sb.release();
     }
-    for (auto& x : ev)
-        if (x) (void)hipEventDestroy(x);
-    if (e) return e;
+    if (!in_graph) {
+        std::vector<hipEvent_t> ev((size_t)6 * nsteps, nullptr);
+        for (auto& x : ev)
+            if (hipEventCreate(&x) != hipSuccess) e = TMK_E_HIP;
+        if (!e) e = profile_enqueue(L, st, nsteps, ev);
+        for (int k = 0; k < nsteps; ++k) {
+            L->cur ^= 1;
+            L->time = L->time + dt;
+        }
+        if (!e && hipStreamSynchronize(st) != hipSuccess) e = TMK_E_HIP;
+        for (int k = 0; k < nsteps && !e; ++k) {
+            const hipEvent_t* x = &ev[(size_t)6 * k];
+            float f = 0.f;
+            if (hipEventElapsedTime(&f, x[0], x[1]) != hipSuccess) { e = TMK_E_HIP; break; }
+            sum[0] += f;
+            if (has_int) {
+                if (hipEventElapsedTime(&f, x[2], x[3]) != hipSuccess) { e = TMK_E_HIP; break; }
+                sum[1] += f;
+            }
+            if (has_bnd) {
+                if (hipEventElapsedTime(&f, x[4], x[5]) != hipSuccess) { e = TMK_E_HIP; break; }
+                sum[2] += f;
+            }
+        }
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+        if (e) return e;
+    }
     out->ms_step = sum[0] / nsteps;
     out->ms_interior = sum[1] / nsteps;
     out->ms_boundary = sum[2] / nsteps;
@@ -1827,6 +1922,7 @@ extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, 
     out->cells_boundary = G.cells_bnd;
     out->tiles_interior = G.nint;
     out->tiles_boundary = G.ntiles - G.nint;
+    out->in_graph = in_graph;
     return TMK_OK;
 }
 
@@ -2918,6 +3014,12 @@ extern "C" int tmk_amr_advance(tmk_amr* A, int nsteps)
                 t0.emplace_back();
                 for (tmk_level* L : M->L) t0.back().push_back(L->time);
             }
+            // buffer parities of every level group and member level: nothing runs during the
+            // capture, so they are restored after it, also when a captured step fails
+            std::vector<int> gpar0, cur0;
+            for (AmrLevel& lv : A->lv) gpar0.push_back(lv.grp.g);
+            for (AmrMember* M : A->mem)
+                for (tmk_level* L : M->L) cur0.push_back(L->cur);
             hipGraph_t gr;
             HIPCHK(hipStreamBeginCapture(A->stream, hipStreamCaptureModeThreadLocal));
             A->gpar = G0.g;
@@ -2926,6 +3028,12 @@ extern "C" int tmk_amr_advance(tmk_amr* A, int nsteps)
             hipError_t ce = hipStreamEndCapture(A->stream, &gr);
             for (size_t m = 0; m < A->mem.size(); ++m)  // nothing ran yet
                 for (size_t l = 0; l < A->mem[m]->L.size(); ++l) A->mem[m]->L[l]->time = t0[m][l];
+            for (size_t l = 0; l < A->lv.size(); ++l) A->lv[l].grp.g = gpar0[l];
+            {
+                size_t q = 0;
+                for (AmrMember* M : A->mem)
+                    for (tmk_level* L : M->L) L->cur = cur0[q++];
+            }
             if (e) return e;
             if (ce != hipSuccess) return TMK_E_HIP;
             hipError_t ie = hipGraphInstantiate(&A->gexec, gr, nullptr, nullptr, 0);

@@ -205,12 +205,14 @@ def split_boxes(nx, ny, maxbox):
 
 
 def box_view(U, box, ghost=GHOST):
-    """Ghosted sub-array of a level array for a box (x0, y0, x1, y1) (a copy, C-contiguous)."""
+    """Ghosted sub-array of a level array for a box (x0, y0, x1, y1): always a C-contiguous copy
+    (also for a box that covers the whole level, whose slice is already contiguous)."""
     x0, y0, x1, y1 = box
-    return np.ascontiguousarray(U[:, y0 - ghost + GHOST:y1 + ghost + GHOST + 1,
-                                  x0 - ghost + GHOST:x1 + ghost + GHOST + 1])
+    return np.array(U[:, y0 - ghost + GHOST:y1 + ghost + GHOST + 1, x0 - ghost + GHOST:x1 + ghost + GHOST + 1],
+                    order="C", copy=True)
 
 
 def box_bview(B, box, ghost=GHOST):
+    """Fault-data row of a box with its x ghosts: always a C-contiguous copy."""
     x0, _, x1, _ = box
-    return np.ascontiguousarray(B[:, x0 - ghost + GHOST:x1 + ghost + GHOST + 1])
+    return np.array(B[:, x0 - ghost + GHOST:x1 + ghost + GHOST + 1], order="C", copy=True)

@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--max-grid-size", type=int, default=32,
                     help="boxes of at most this many cells per side (the reference's default max_grid_size "
                          "is 32); 0 = the level as one box")
+    ap.add_argument("--march", type=int, default=-1,
+                    help="strip rows of the row-marching interior update (0: tile kernels; default: the library's)")
     ap.add_argument("--acc-steps", type=int, default=4000, help="steps of the accuracy leg (0: skip it)")
     ap.add_argument("--no-c1", action="store_true", help="skip the secondary 4096x2048 FP64 drop-in leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -88,12 +90,17 @@ def level_boxes(nx, ny, max_grid_size):
     return scenarios.split_boxes(nx, ny, max_grid_size)
 
 
+MARCH = -1  # strip rows of the row-marching interior update; < 0: the library's default
+
+
 def make_level(cfg, precision, boxes, tile):
     """A device-resident level holding the configuration's initial state (uniform background,
     seeded fault data), set on the device box by box (no host copy of the state)."""
     ctx = capi.Context(cfg, precision)
     if tile:
         ctx.set_tile(*tile)
+    if MARCH >= 0:
+        ctx.set_march(MARCH)
     lev = capi.Level(ctx, boxes)
     B = scenarios.fault_arrays(cfg)
     bg = scenarios.background_vector(cfg)
@@ -258,6 +265,8 @@ def main():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(3)
     tile = tuple(int(x) for x in a.tile.split("x")) if a.tile else None
+    global MARCH
+    MARCH = a.march
     cfg = scenarios.config_c3(seed=0, nx=a.nx, ny=a.ny, dx=a.dx)
     boxes = level_boxes(cfg["nx"], cfg["ny"], a.max_grid_size)
     wl = "rs-aging+drucker-prager %dx%d dx=%g" % (cfg["nx"], cfg["ny"], cfg["dx"])
@@ -287,7 +296,7 @@ def main():
                    "dx_m": cfg["dx"], "dt_s": scenarios.dt_of(cfg), "normal_predictor": "PPM",
                    "plasticity": {"mu": cfg["p_mu"], "beta": cfg["p_beta"], "eta_MPa_s": cfg["p_eta"],
                                   "cohesion_MPa": cfg["p_b"]},
-                   "tile": list(tile) if tile else "default"},
+                   "tile": list(tile) if tile else "default", "march_rows": a.march if a.march >= 0 else "default"},
         "roofline": roofline(r32["profile"], True, 4, wl, "fp32_bulk", tile),
         "fp64": {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                  "roofline": roofline(r64["profile"], True, 8, wl, "fp64", tile)},

@@ -331,6 +331,15 @@ int tmk_level_regrid(tmk_level *fresh, tmk_level *old, tmk_level *coarse, int re
  * 32x32, 64x4, 64x8, 64x16.  0, 0 = the library default for the context's precision. */
 int tmk_set_tile(tmk_ctx *ctx, int bx, int by);
 
+/* Row-marching interior update (autotuner hook; no reference counterpart: the reference updates
+ * box by box, LinElastLevelGodunov.cpp:284-358).  rows > 0: levels created from this context
+ * update the cells whose +-3 stencil lies inside the domain in strips of 58 columns x rows,
+ * one wavefront per strip and side marching up the rows with no LDS (mode II, PPM with primitive
+ * limiting; other numerics keep the tile kernels); the cells along the domain boundary run the
+ * general kernel.  0 = the tile kernels of tmk_set_tile.  Results are those of the tile kernels
+ * (bit-identical in FP64).  New contexts start at 32 rows. */
+int tmk_set_march(tmk_ctx *ctx, int rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,7 +160,7 @@ def test_patch_fluxes():
     assert np.all(_relerr(F1g, F1r) <= TOL)
 
 
-def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=False, tile=None):
+def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=False, tile=None, march=None):
     U, B = _state(cfg, seed)
     Ur, Br = U.copy(), B.copy()
     dt = scenarios.dt_of(cfg)
@@ -170,6 +170,10 @@ def _level_case(cfg, boxes, nsteps, seed, precision=capi.PREC_FP64, use_run=Fals
     ctx = capi.Context(cfg, precision)
     if tile:
         ctx.set_tile(*tile)
+    if march is not None:
+        ctx.set_march(march)
+    elif tile:
+        ctx.set_march(0)  # a tile shape under test: the interior runs the tile kernels
     lev = capi.Level(ctx, boxes)
     for k, b in enumerate(boxes):
         lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
@@ -251,6 +255,7 @@ def test_profile_kernels():
     U, B = _state(cfg, 5)
     ctx = capi.Context(cfg, capi.PREC_FP64)
     ctx.set_tile(32, 16)
+    ctx.set_march(0)
     lev = capi.Level(ctx, [(0, 0, 199, 119)])
     lev.upload(0, U, B)
     p = lev.profile_kernels(3, 0.0, scenarios.dt_of(cfg))
@@ -662,8 +667,10 @@ def test_patch_plastic_update_matches_oracle(precision):
     Ug = Ub.copy()
     ctx.plastic_update(valid, Ug, ubox, dt)
     ctx.close()
-    # ghost cells come back as they went in (the FP32-bulk drop-in holds the patch in FP32)
-    Ubr = Ub if precision == capi.PREC_FP64 else Ub.astype(np.float32).astype(np.float64)
+    # ghost cells come back as they went in; the FP32-bulk drop-in holds the patch as its
+    # difference from the background in FP32
+    bg = scenarios.background_vector(cfg)[:, None, None]
+    Ubr = Ub if precision == capi.PREC_FP64 else (Ub - bg).astype(np.float32).astype(np.float64) + bg
     assert np.array_equal(Ug[:, ~inside], Ubr[:, ~inside])
     if precision == capi.PREC_FP64:
         assert np.array_equal(Ug, Ur), np.max(np.abs(Ug - Ur))
@@ -671,3 +678,67 @@ def test_patch_plastic_update_matches_oracle(precision):
         assert np.array_equal((Ug[9] > 0) & inside, yielded)
         e = _relerr(Ug[:, inside], Ur[:, inside])
         assert np.all(e <= 2e-6), e
+
+
+# ---------------------------------------------------------------------------
+# the row-marching interior update (k_godunov_march, tmk_set_march)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("rows", [1, 5, 16, 64])
+@pytest.mark.parametrize("maxbox", [None, 24, 40])
+def test_march_level_matches_oracle(rows, maxbox):
+    """Strips of 58 x rows over the cells whose stencil lies in the domain (one box, or boxes of
+    24 / 40 cells merged into island rectangles, with partial strips at their edges), the band
+    along the domain boundary in the general kernel: bit-identical to the oracle away from the
+    fault's reach, within 1e-12 on it."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    boxes = [(0, 0, 199, 119)] if maxbox is None else scenarios.split_boxes(200, 120, maxbox)
+    nsteps = 4
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, nsteps, seed=41, march=rows, use_run=rows == 16)
+    assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+    assert np.all(fault_errors(Bg, Br) <= TOL), fault_errors(Bg, Br)
+    far = 3 * nsteps + 3
+    assert np.array_equal(Ug[:, far:, :], Ur[:, far:, :])
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+def test_march_level_plasticity(precision):
+    """The Drucker-Prager return fused into the marching update."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    cfg["plasticity"] = 1
+    cfg["p_mu"], cfg["p_beta"], cfg["p_eta"], cfg["p_b"] = 0.6, 0.3, 0.03 * scenarios.G_MPA, 1.0
+    cfg["rs"]["state_law"] = 1
+    cfg["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, scenarios.split_boxes(200, 120, 32), 4, seed=16, precision=precision,
+                                     march=24)
+    if precision == capi.PREC_FP64:
+        assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
+        assert np.array_equal(Ug[:, 15:, :], Ur[:, 15:, :])
+    else:
+        assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+    assert np.max(Ur[9]) > 0.0, "plastic flow expected somewhere"
+
+
+def test_march_fp32_bulk_level():
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 199, 119)], 6, seed=42, precision=capi.PREC_FP32_BULK, march=32)
+    assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= 1e-4), fault_errors(Bg, Br)
+
+
+def test_march_equals_tile_kernels_fp64():
+    """The marching update and the tile kernels give the same FP64 state, bit for bit, on a
+    slip-weakening level (no transcendental function anywhere)."""
+    cfg = _cfg("sw", "PPM", nx=160, ny=96)
+    boxes = scenarios.split_boxes(160, 96, 32)
+    a = _level_case(cfg, boxes, 5, seed=43)
+    b = _level_case(cfg, boxes, 5, seed=43, march=8)
+    assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[1][0], b[1][0])
+
+
+def test_march_falls_back_for_other_numerics():
+    """PLM (and characteristic limiting, anti-plane fields) keep the tile kernels under
+    tmk_set_march: the results match the oracle as before."""
+    cfg = _cfg("rs", "PLM", nx=120, ny=80, prim=1, char=0, fourth=1)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 119, 79)], 3, seed=44, march=16)
+    assert np.all(_relerr(Ug, Ur) <= TOL)
+    assert np.all(fault_errors(Bg, Br) <= TOL)

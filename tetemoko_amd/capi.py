@@ -44,6 +44,7 @@ EXPORTED = (
     "tmk_stations_destroy", "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed",
     "tmk_amr_sync", "tmk_make_boxes", "tmk_level_regrid", "tmk_amr_batch_create", "tmk_amr_add",
     "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error", "tmk_patch_update_state_noalloc",
+    "tmk_set_march",
 )
 
 
@@ -165,6 +166,7 @@ def lib():
         L.tmk_level_profile_kernels.argtypes = [vp, c_int, _D, _D, ctypes.POINTER(Profile)]
         L.tmk_level_sync.argtypes = [vp]
         L.tmk_set_tile.argtypes = [vp, c_int, c_int]
+        L.tmk_set_march.argtypes = [vp, c_int]
         sz = ctypes.c_size_t
         L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
@@ -301,6 +303,10 @@ class Context:
 
     def set_tile(self, bx, by):
         _check(lib().tmk_set_tile(self.h, bx, by), "tmk_set_tile")
+
+    def set_march(self, rows):
+        """Row-marching interior update in strips of 58 x rows (0: the tile kernels)."""
+        _check(lib().tmk_set_march(self.h, rows), "tmk_set_march")
 
     def update_state(self, valid, U, ubox, B=None, bbox=None, F0=None, F1=None, time=0.0, dt=1.0, noalloc=False):
         """LinElastPatchGodunov::updateState on host FArrayBox arrays (in place). Returns max wave speed.

@@ -21,10 +21,17 @@
 
 #include "tmk_kernels.hpp"
 #include "tmk_interior.hpp"
+#include "tmk_march.hpp"
 #include "tmk_amr.hpp"
 
 #ifndef TMK_DEFAULT_BY_F64
 #define TMK_DEFAULT_BY_F64 8
+#endif
+// strip rows of the row-marching interior update for new contexts (measured on the 8192 x 4096
+// and 4096 x 2048 rate-and-state levels: 32 rows is within 3 % of the best of 16 ... 256 for FP32
+// bulk and FP64 on both)
+#ifndef TMK_DEFAULT_MARCH_ROWS
+#define TMK_DEFAULT_MARCH_ROWS 32
 #endif
 
 static void default_tile(int precision, int& bx, int& by);
@@ -36,6 +43,7 @@ namespace tmk {
     extern template __global__ void k_godunov<T, NS, BX, BY, PK>(const KArgs<T>);     \
     extern template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
 TMK_ALL_UPDATE_KERNELS(TMK_DECLARE_EXTERN, float)
+extern template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
 }  // namespace tmk
 
 using namespace tmk;
@@ -164,6 +172,50 @@ int partition_tiles(std::vector<Tile>& t, F is_interior, G on_fault_row)
     return nint;
 }
 
+// Row-marching partition of one rectangle of an island (r.patch: its descriptor): the core,
+// the cells whose +-3 stencil lies in the domain, in strips of MARCH_COLS x rows for
+// k_godunov_march; the rest, the bands along the domain boundary, in boundary-shape pieces for
+// the general kernel, fault-row pieces apart.
+void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& strips, std::vector<Tile>& bf,
+                std::vector<Tile>& bo)
+{
+    Tile c = r;
+    c.x0 = std::max(r.x0, P.dlo[0] + 3);
+    c.x1 = std::min(r.x1, P.dhi[0] - 3);
+    c.y0 = std::max(r.y0, P.dlo[1] + 3);
+    c.y1 = std::min(r.y1, P.dhi[1] - 3);
+    auto pieces = [&](int x0, int y0, int x1, int y1) {
+        for (int py = y0; py <= y1; py += BND_BY - 2)
+            for (int px = x0; px <= x1; px += BND_BX - 2) {
+                Tile t;
+                t.patch = r.patch;
+                t.x0 = px;
+                t.y0 = py;
+                t.x1 = std::min(px + BND_BX - 3, x1);
+                t.y1 = std::min(py + BND_BY - 3, y1);
+                (t.y0 == P.dlo[1] ? bf : bo).push_back(t);
+            }
+    };
+    if (c.x0 > c.x1 || c.y0 > c.y1) {
+        pieces(r.x0, r.y0, r.x1, r.y1);
+        return;
+    }
+    for (int y = c.y0; y <= c.y1; y += rows)
+        for (int x = c.x0; x <= c.x1; x += MARCH_COLS) {
+            Tile t;
+            t.patch = r.patch;
+            t.x0 = x;
+            t.y0 = y;
+            t.x1 = std::min(x + MARCH_COLS - 1, c.x1);
+            t.y1 = std::min(y + rows - 1, c.y1);
+            strips.push_back(t);
+        }
+    pieces(r.x0, r.y0, r.x1, c.y0 - 1);  // band below the core
+    pieces(r.x0, c.y1 + 1, r.x1, r.y1);  // above
+    pieces(r.x0, c.y0, c.x0 - 1, c.y1);  // left of it
+    pieces(c.x1 + 1, c.y0, r.x1, c.y1);  // right of it
+}
+
 // state planes that sweeps read with a stencil (double-buffered); the others
 // (szz, lam, gam and, in mode II, the anti-plane planes) are updated in place
 bool is_stencil(int c, int ns)
@@ -187,6 +239,7 @@ int stencil_mask(int ns)
 struct Group {
     size_t tsize = 8;
     int ns = 2, bx = 32, by = 16;
+    int march = 0;                 // > 0: interior strips of this many rows run k_godunov_march
     int pk = PK_RUNTIME;  // predictor kind the kernel is specialised for (all members agree)
     Scen* scen_dev = nullptr;
     int nscen = 0;
@@ -245,6 +298,7 @@ struct tmk_ctx {
     DevBuf bufs[8];  // grow-only device buffers of the drop-in path, uploads and tags
     int bx = 32, by = 16;
     bool antiplane = true;
+    int march = TMK_DEFAULT_MARCH_ROWS;  // strip rows of the row-marching interior update (0: tile kernels)
     void* noalloc_scratch = nullptr;  // this device's scratch array (tmk_patch_update_state_noalloc)
 };
 
@@ -283,6 +337,7 @@ struct tmk_level {
     double time = 0.0, dt = 0.0;  // clock of this scenario
     std::vector<char> hpatch[2];  // Patch<T> descriptors for parity 0 / 1 (scen index 0)
     std::vector<Tile> tiles;
+    std::vector<Tile> urects;     // the islands' merged rectangles (patch: the island descriptor)
     std::vector<CopyRect> rects, brects;
     Group own;
     bool own_built = false;
@@ -309,9 +364,11 @@ struct tmk_batch {
     bool stale = false;           // membership changed since the group was built
 };
 
-// largest state plane the kernels address (32-bit byte offsets from the plane's base); one FP64
-// level of planes that size (three copies of 22 planes) would exceed the device's memory anyway
-static constexpr size_t kPlaneMaxBytes = (size_t)1 << 32;
+// largest state plane the kernels address (32-bit byte offsets from the plane's base, and at
+// most kDropOffset, the offset the row-marching kernel gives lanes whose accesses must be
+// dropped); one FP64 level of planes that size (three copies of 22 planes) would exceed the
+// device's memory anyway
+static constexpr size_t kPlaneMaxBytes = kDropOffset;
 
 static const char* k_errs[] = {"ok",
                                "NaN state variable psi (dPsi is nan)",
@@ -379,6 +436,9 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
         P.bg[11 * s + 9] = 0.0;
         P.bg[11 * s + 10] = 0.0;
     }
+    const double mbg = (h.sxx0 + h.syy0 + h.szz0) / 3.0;
+    const double pbg[7] = {mbg, h.sxx0 - mbg, h.syy0 - mbg, h.szz0 - mbg, h.sxy0, h.sxz0, h.syz0};
+    for (int k = 0; k < 7; ++k) P.pbg[k] = (float)pbg[k];
     P.plast = h.use_plasticity;
     P.p_mu = h.p_mu; P.p_beta = h.p_beta; P.p_eta = h.p_eta; P.p_b = h.p_b; P.p_h = h.p_h;
     for (int k = 0; k < 4; ++k) P.bc[k] = h.bc[k];
@@ -518,6 +578,13 @@ static void default_tile(int precision, int& bx, int& by)
     by = precision == TMK_PREC_FP64 ? TMK_DEFAULT_BY_F64 : 16;
 }
 
+extern "C" int tmk_set_march(tmk_ctx* c, int rows)
+{
+    if (!c || rows < 0 || rows > 65536) return TMK_E_ARG;
+    c->march = rows;
+    return TMK_OK;
+}
+
 extern "C" int tmk_set_tile(tmk_ctx* c, int bx, int by)
 {
     if (!c) return TMK_E_ARG;
@@ -610,9 +677,17 @@ static int timer_mark(const StepTimers* tm, hipStream_t s, int k)
 // the fused update of one step over a tile list [interior | boundary]: the boundary launch runs
 // on `side` (forked from and joined back into st by events; captured graphs keep the fork), the
 // interior launch on st
+// the interior strips of the row-marching partition (grid: two one-wavefront workgroups per strip)
+template <typename T>
+static int launch_march(hipStream_t st, const KArgs<T>& a)
+{
+    hipLaunchKernelGGL((k_godunov_march<T, PK_PPM_PRIM>), dim3(2 * a.ntiles), dim3(64), 0, st, a);
+    return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
+}
+
 template <typename T>
 static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join, int ns, int pk, int bx,
-                         int by, KArgs<T> a, int nint, const StepTimers* tm)
+                         int by, KArgs<T> a, int nint, const StepTimers* tm, int march = 0)
 {
     const int nbnd = a.ntiles - nint;
     const Tile* tiles = a.tiles;
@@ -635,7 +710,11 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         KArgs<T> b = a;
         b.ntiles = nint;
         if ((e = timer_mark(tm, st, 0))) return e;
-        if ((e = launch_tiles<T>(st, ns, pk, bx, by, b, true))) return e;
+        if (march > 0) {
+            if ((e = launch_march<T>(st, b))) return e;
+        } else if ((e = launch_tiles<T>(st, ns, pk, bx, by, b, true))) {
+            return e;
+        }
         if ((e = timer_mark(tm, st, 1))) return e;
     }
     if (split) {
@@ -656,7 +735,7 @@ static int group_enqueue_step_t(Group& G, hipStream_t st, const StepTimers* tm =
     a.tiles = G.tiles_dev;
     a.ntiles = G.ntiles;
     a.plastic = G.plastic;
-    int e = launch_update<T>(st, G.side, G.ev_fork, G.ev_join, G.ns, G.pk, G.bx, G.by, a, G.nint, tm);
+    int e = launch_update<T>(st, G.side, G.ev_fork, G.ev_join, G.ns, G.pk, G.bx, G.by, a, G.nint, tm, G.march);
     if (e) return e;
     const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g];  // its out planes hold the new state
     if (G.nrects > 0) {
@@ -746,6 +825,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         if (pred_kind(L->ctx->P) != PK_PPM_PRIM) G.pk = PK_RUNTIME;
     G.bx = L0->ctx->bx;
     G.by = L0->ctx->by;
+    G.march = (L0->ctx->march > 0 && G.ns == 2 && G.pk == PK_PPM_PRIM) ? L0->ctx->march : 0;
     G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     G.plastic = plastic;
     G.g = 0;
@@ -768,7 +848,11 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
                 pts[g].push_back(p);
             }
         }
-        for (Tile t : L->tiles) { t.patch += poff; tiles.push_back(t); }
+        if (G.march) {
+            for (Tile t : L->urects) { t.patch += poff; tiles.push_back(t); }
+        } else {
+            for (Tile t : L->tiles) { t.patch += poff; tiles.push_back(t); }
+        }
         for (CopyRect r : L->rects) { r.src += poff; r.dst += poff; rects.push_back(r); }
         for (CopyRect r : L->brects) { r.src += poff; r.dst += poff; brects.push_back(r); }
         poff += np;
@@ -778,9 +862,20 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     for (size_t m = 0; m < mem.size(); ++m)
         for (int b = 0; b < mem[m]->npatch; ++b) patch_scen.push_back((int)m);
     const int bx = G.bx, by = G.by;
-    G.nint = partition_tiles(
-        tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
-        [&](const Tile& t) { return t.y0 == sc[patch_scen[t.patch]].P.dlo[1]; });
+    if (G.march) {
+        // the rectangles become interior strips, then boundary pieces (fault row first)
+        std::vector<Tile> strips, bf, bo;
+        for (const Tile& r : tiles) march_rect(r, sc[patch_scen[r.patch]].P, G.march, strips, bf, bo);
+        xcd_order(strips);
+        G.nint = (int)strips.size();
+        tiles.swap(strips);
+        tiles.insert(tiles.end(), bf.begin(), bf.end());
+        tiles.insert(tiles.end(), bo.begin(), bo.end());
+    } else {
+        G.nint = partition_tiles(
+            tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
+            [&](const Tile& t) { return t.y0 == sc[patch_scen[t.patch]].P.dlo[1]; });
+    }
     G.cells_int = G.cells_bnd = 0;
     for (size_t k = 0; k < tiles.size(); ++k)
         ((int)k < G.nint ? G.cells_int : G.cells_bnd) +=
@@ -888,16 +983,39 @@ static int patch_tiles(const tmk_box& v, const DevParams& P, int bx, int by, Til
     return n;
 }
 
-// FP64 <-> FP32 copies of the state for the FP32-bulk drop-in without host buffers
-static __global__ void k_to_f32(const double* __restrict__ s, float* __restrict__ d, size_t n)
+// FP64 state <-> the FP32 bulk's stored form U - bg (stores_delta), component blockIdx.y
+static __global__ void k_to_f32(const double* __restrict__ s, float* __restrict__ d, size_t plane, StateVec bg)
 {
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-        d[k] = (float)s[k];
+    const size_t o = blockIdx.y * plane;
+    const double b = bg.c[blockIdx.y];
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < plane; k += (size_t)gridDim.x * blockDim.x)
+        d[o + k] = (float)(s[o + k] - b);
 }
-static __global__ void k_to_f64(const float* __restrict__ s, double* __restrict__ d, size_t n)
+static __global__ void k_to_f64(const float* __restrict__ s, double* __restrict__ d, size_t plane, StateVec bg)
 {
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-        d[k] = (double)s[k];
+    const size_t o = blockIdx.y * plane;
+    const double b = bg.c[blockIdx.y];
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < plane; k += (size_t)gridDim.x * blockDim.x)
+        d[o + k] = (double)s[o + k] + b;
+}
+
+// the background per component (the FP32 bulk stores U minus it)
+static StateVec background(const DevParams& P)
+{
+    StateVec v;
+    for (int k = 0; k < TMK_NCOMP; ++k) v.c[k] = P.bg[k];
+    return v;
+}
+// host copies between FP64 state and the FP32 bulk's stored form
+static void to_stored_f32(const double* U, float* d, size_t plane, const DevParams& P)
+{
+    for (int c = 0; c < TMK_NCOMP; ++c)
+        for (size_t k = 0; k < plane; ++k) d[k + c * plane] = (float)(U[k + c * plane] - P.bg[c]);
+}
+static void from_stored_f32(const float* a, double* U, size_t plane, const DevParams& P)
+{
+    for (int c = 0; c < TMK_NCOMP; ++c)
+        for (size_t k = 0; k < plane; ++k) U[k + c * plane] = (double)a[k + c * plane] + P.bg[c];
 }
 
 // Device buffers of one drop-in call.  stage: FP64 image of the state for the FP32 bulk,
@@ -953,17 +1071,18 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
     T* Uout = w.Uout;
     double* Bd = w.Bd;
     double* Fd = w.Fd;
-    const unsigned cvt_blocks = (unsigned)std::min<size_t>((nU + 255) / 256, 4096);
+    const dim3 cvt_grid((unsigned)std::min<size_t>((plane + 255) / 256, 1024), TMK_NCOMP);
     std::vector<float> tmp;
     if (sizeof(T) == sizeof(double)) {
         HIPCHK(hipMemcpyAsync(Uin, U, nU * sizeof(double), hipMemcpyHostToDevice, st));
     } else if (w.stage) {
         HIPCHK(hipMemcpyAsync(w.stage, U, nU * sizeof(double), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_to_f32, dim3(cvt_blocks), dim3(256), 0, st, (const double*)w.stage, (float*)Uin, nU);
+        hipLaunchKernelGGL(k_to_f32, cvt_grid, dim3(256), 0, st, (const double*)w.stage, (float*)Uin, plane,
+                           background(c->P));
         HIPCHK(hipGetLastError());
     } else {
         tmp.resize(nU);
-        for (size_t k = 0; k < nU; ++k) tmp[k] = (float)U[k];
+        to_stored_f32(U, tmp.data(), plane, c->P);
         HIPCHK(hipMemcpyAsync(Uin, tmp.data(), nU * sizeof(float), hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(Uout, Uin, nU * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -982,6 +1101,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         memset(&pt, 0, sizeof pt);
         for (int k = 0; k < TMK_NCOMP; ++k) { pt.in[k] = Uin + k * plane; pt.out[k] = Uout + k * plane; }
         pt.ulo0 = ub.lo[0]; pt.ulo1 = ub.lo[1]; pt.unx = (int)unx;
+        pt.pbytes = (unsigned)(plane * sizeof(T));
         pt.bin = B ? Bd : nullptr;
         pt.bout = B ? Bd + nB : nullptr;
         pt.blo0 = bb.lo[0]; pt.bnx = (int)bnx; pt.bnb = nb;
@@ -1033,7 +1153,8 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         HIPCHK(hipMemcpyAsync(U, Uout, nU * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     } else if (w.stage) {
-        hipLaunchKernelGGL(k_to_f64, dim3(cvt_blocks), dim3(256), 0, st, (const float*)Uout, w.stage, nU);
+        hipLaunchKernelGGL(k_to_f64, cvt_grid, dim3(256), 0, st, (const float*)Uout, w.stage, plane,
+                           background(c->P));
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(U, w.stage, nU * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1041,7 +1162,7 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         tmp.resize(nU);
         HIPCHK(hipMemcpyAsync(tmp.data(), Uout, nU * sizeof(float), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        for (size_t k = 0; k < nU; ++k) U[k] = (double)tmp[k];
+        from_stored_f32(tmp.data(), U, plane, c->P);
     }
     return TMK_OK;
 }
@@ -1230,6 +1351,7 @@ static void level_fill_patches(tmk_level* L)
                 else { p.in[c] = W + c * plane; p.out[c] = W + c * plane; }
             }
             p.ulo0 = I.abox.lo[0]; p.ulo1 = I.abox.lo[1]; p.unx = I.unx();
+            p.pbytes = (unsigned)(plane * sizeof(T));
             if (onf) {
                 p.bin = I.b[cur];
                 p.bout = I.b[1 - cur];
@@ -1442,7 +1564,16 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     for (size_t k = 0; k < L->isl.size(); ++k) {
         std::vector<tmk_box> rr;
         island_rects(L->boxes, L->isl[k], rr);
-        for (const tmk_box& r : rr) make_tiles(r, nboxes + (int)k, c->bx - 2, c->by - 2, L->tiles);
+        for (const tmk_box& r : rr) {
+            make_tiles(r, nboxes + (int)k, c->bx - 2, c->by - 2, L->tiles);
+            Tile u;
+            u.patch = nboxes + (int)k;
+            u.x0 = r.lo[0];
+            u.y0 = r.lo[1];
+            u.x1 = r.hi[0];
+            u.y1 = r.hi[1];
+            L->urects.push_back(u);
+        }
     }
     // exchange plan: ghost cells (inside the domain) of each box covered by valid cells of a
     // box in another island
@@ -1601,7 +1732,7 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
     const void* src = U;
     if (L->tsize == sizeof(float)) {
         tmp.resize(nU);
-        for (size_t k = 0; k < nU; ++k) tmp[k] = (float)U[k];
+        to_stored_f32(U, tmp.data(), plane, c->P);
         src = tmp.data();
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1636,7 +1767,7 @@ extern "C" int tmk_level_fill(tmk_level* L, int b, const double* state, const do
     const int gnx = ub.hi[0] - ub.lo[0] + 1, gny = ub.hi[1] - ub.lo[1] + 1;
     const size_t plane = (size_t)gnx * gny;
     StateVec v;
-    for (int k = 0; k < TMK_NCOMP; ++k) v.c[k] = state[k];
+    for (int k = 0; k < TMK_NCOMP; ++k) v.c[k] = L->tsize == sizeof(float) ? state[k] - c->P.bg[k] : state[k];
     const Island& I = L->isl[L->box_isl[b]];
     const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
     const dim3 grid((unsigned)((plane + 255) / 256));
@@ -1692,7 +1823,7 @@ extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
     } else {
         std::vector<float> a(nU);
         HIPCHK(hipMemcpy(a.data(), c->bufs[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < nU; ++k) U[k] = (double)a[k];
+        from_stored_f32(a.data(), U, plane, c->P);
     }
     if (L->has_b[b] && B) return level_get_fault(L, b, B, st);
     return TMK_OK;
@@ -2021,7 +2152,8 @@ extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
     if (!B || !L || L->batch || L->amr_stream) return TMK_E_ARG;
     if (!B->members.empty()) {
         tmk_level* L0 = B->members[0];
-        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by)
+        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by ||
+            L0->ctx->march != L->ctx->march)
             return TMK_E_ARG;
     }
     HIPCHK(hipStreamSynchronize(L->ctx->stream));
@@ -2799,7 +2931,8 @@ extern "C" int tmk_amr_add(tmk_amr* A, tmk_level* const* levels, double time, do
     if (!A->mem.empty()) {
         const tmk_level* a = A->mem[0]->L[0];
         const tmk_level* b = levels[0];
-        if (a->tsize != b->tsize || a->ns != b->ns || a->ctx->bx != b->ctx->bx || a->ctx->by != b->ctx->by)
+        if (a->tsize != b->tsize || a->ns != b->ns || a->ctx->bx != b->ctx->bx || a->ctx->by != b->ctx->by ||
+            a->ctx->march != b->ctx->march)
             return TMK_E_ARG;
     }
     {   // level 0 covers its domain

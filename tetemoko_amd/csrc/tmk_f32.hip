@@ -7,6 +7,7 @@
 // contraction.  tmk_api.hip declares these instantiations extern and launches them.
 #include "tmk_kernels.hpp"
 #include "tmk_interior.hpp"
+#include "tmk_march.hpp"
 
 namespace tmk {
 
@@ -15,5 +16,6 @@ namespace tmk {
     template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
 
 TMK_ALL_UPDATE_KERNELS(TMK_INSTANTIATE, float)
+template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
 
 }  // namespace tmk

@@ -36,6 +36,7 @@ struct Patch {
     const T* in[22];   // state planes read this step
     T* out[22];        // state planes written this step (== in for pointwise planes)
     int ulo0, ulo1, unx; // U array box (ghosted) lo and x extent
+    unsigned pbytes;   // bytes of one state plane (the bound of buffer descriptors over a plane)
     const double* bin; // fault data read (NULL: box has none)
     double* bout;      // fault data written
     int blo0, bnx;     // fault-data x lo and extent
@@ -153,11 +154,46 @@ __device__ __forceinline__ void face_flux(const Mat<T>& m, const Slots<T, NS>& S
     face_flux_side<T, NS>(m, S.v[1], S.s[1], rv, f[1]);
 }
 
-// the 11 flux components of side sd from face state (v, s) on the lo d-face of cell (i, j) into
-// the flux register array of direction d (surroundingNodes(valid box, d))
+// the fault solve's input from the lo-y face traces W (FP64, absolute: stores_delta adds the
+// background) and the face states H from its output (the background removed in FP64)
 template <typename T, int NS>
-__device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
-                                                const T (&sv)[NS], T r, int i, int j, int sd)
+__device__ __forceinline__ void fault_input(const DevParams& P, const Slots<T, NS>& W, FaultIn& fin)
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        fin.vy[s] = (double)W.v[s][0]; fin.syy[s] = (double)W.s[s][0];
+        fin.vx[s] = (double)W.v[s][1]; fin.sxy[s] = (double)W.s[s][1];
+        fin.vz[s] = NS == 3 ? (double)W.v[s][NS - 1] : 0.0;
+        fin.syz[s] = NS == 3 ? (double)W.s[s][NS - 1] : 0.0;
+        if constexpr (stores_delta<T>()) {
+            const double* b = P.bg + 11 * s;
+            fin.vx[s] += b[0]; fin.vy[s] += b[1]; fin.syy[s] += b[4]; fin.sxy[s] += b[6];
+            if (NS == 3) { fin.vz[s] += b[2]; fin.syz[s] += b[8]; }
+        }
+    }
+}
+template <typename T, int NS>
+__device__ __forceinline__ void fault_output(const DevParams& P, const FaultOut& fo, Slots<T, NS>& H)
+{
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        double vy = fo.vy[s], syy = fo.syy[s], vx = fo.vx[s], sxy = fo.sxy[s], vz = fo.vz[s], syz = fo.syz[s];
+        if constexpr (stores_delta<T>()) {
+            const double* b = P.bg + 11 * s;
+            vx -= b[0]; vy -= b[1]; syy -= b[4]; sxy -= b[6]; vz -= b[2]; syz -= b[8];
+        }
+        H.v[s][0] = (T)vy; H.s[s][0] = (T)syy;
+        H.v[s][1] = (T)vx; H.s[s][1] = (T)sxy;
+        if (NS == 3) { H.v[s][NS - 1] = (T)vz; H.s[s][NS - 1] = (T)syz; }
+    }
+}
+
+// the 11 flux components of side sd from face state (v, s) on the lo d-face of cell (i, j) into
+// the flux register array of direction d (surroundingNodes(valid box, d)); stores_delta adds
+// the flux of the background (in FP64), so the output is the flux of U in both precisions
+template <typename T, int NS>
+__device__ __forceinline__ void store_flux_side(const DevParams& P, const Patch<T>& pt, const Mat<T>& m, int d,
+                                                const T (&v)[NS], const T (&sv)[NS], T r, int i, int j, int sd)
 {
     const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
     if (pt.fmask) {
@@ -171,8 +207,22 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
     }
     const size_t fp = (size_t)fnx * fny;
     const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
-    T f[7];
-    face_flux_side<T, NS>(m, v, sv, r, f);
+    T fT[7];
+    face_flux_side<T, NS>(m, v, sv, r, fT);
+    double f[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) f[k] = (double)fT[k];
+    if constexpr (stores_delta<T>()) {
+        double bv[NS], bs[NS], fb[7];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            bv[k] = P.bg[slot_comp(d, k, 0) + 11 * sd];
+            bs[k] = P.bg[slot_comp(d, k, 1) + 11 * sd];
+        }
+        face_flux_side<double, NS>(Mat<double>(P), bv, bs, (double)r, fb);
+        f[0] += fb[0]; f[1] += fb[1]; f[2] += fb[2]; f[3] += fb[3]; f[4] += fb[4];
+        if (NS == 3) { f[5] += fb[5]; f[6] += fb[6]; }
+    }
     __attribute__((address_space(1))) double* Fo = gmem(pt.F[d]);
     const int o = 11 * sd;
     if (d == 0) {
@@ -201,11 +251,11 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
 }
 
 template <typename T, int NS>
-__device__ __forceinline__ void store_flux(const Patch<T>& pt, const Mat<T>& m, int d, const Slots<T, NS>& Fs, T rv,
-                                           int i, int j)
+__device__ __forceinline__ void store_flux(const DevParams& P, const Patch<T>& pt, const Mat<T>& m, int d,
+                                           const Slots<T, NS>& Fs, T rv, int i, int j)
 {
-    store_flux_side<T, NS>(pt, m, d, Fs.v[0], Fs.s[0], (T)1, i, j, 0);
-    store_flux_side<T, NS>(pt, m, d, Fs.v[1], Fs.s[1], rv, i, j, 1);
+    store_flux_side<T, NS>(P, pt, m, d, Fs.v[0], Fs.s[0], (T)1, i, j, 0);
+    store_flux_side<T, NS>(P, pt, m, d, Fs.v[1], Fs.s[1], rv, i, j, 1);
 }
 
 // face states of one side on the four faces of a cell: lo/hi x face, lo/hi y face
@@ -214,27 +264,21 @@ struct Faces1 {
     T xlv[NS], xls[NS], xhv[NS], xhs[NS], ylv[NS], yls[NS], yhv[NS], yhs[NS];
 };
 
-// U += dU of side sd of valid cell (i, j) from the final states on its four faces (flux
-// difference, LinElastPatchGodunov.cpp:117-126), then the optional Drucker-Prager return
+// New values of one side's state of a valid cell from its old values u[0..10] (side-local
+// component index) and the final states on its four faces: flux difference and
+// U += dU (-dt/dx) (LinElastPatchGodunov.cpp:117-126, LinElastPhysics.cpp:135-179), then, with
+// plastic, the Drucker-Prager return (LinElastLevelGodunov::plasticUpdate after the level step,
+// AMRLevelLinElast.cpp:476).  Changes u[0 1 3 4 5 6] (+ 2 7 8 with NS 3, + 7 8 9 10 with plastic).
 template <typename T, int NS>
-__device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
-                                                 const Mat<T>& m, double dt, T sc, int i, int j, int sd,
-                                                 const Faces1<T, NS>& F)
+__device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T>& m, double dt, T sc, bool plastic,
+                                                 T ry, const Faces1<T, NS>& F, T (&u)[11])
 {
-    const T ry = sd ? (T)-1 : (T)1;
     T fxl[7], fxh[7], fyl[7], fyh[7];
     face_flux_side<T, NS>(m, F.xlv, F.xls, (T)1, fxl);
     face_flux_side<T, NS>(m, F.xhv, F.xhs, (T)1, fxh);
     face_flux_side<T, NS>(m, F.ylv, F.yls, ry, fyl);
     face_flux_side<T, NS>(m, F.yhv, F.yhs, ry, fyh);
-    const unsigned u = (unsigned)uidx(pt, i, j, 0);
-    const int o = 11 * sd;
     // dU = (0 + (Fx_hi - Fx_lo)) + (Fy_hi - Fy_lo); U += dU * (-dt/dx)
-    auto upd = [&](int c, T dx_, T dy_) {
-        const T dU = ((T)0 + dx_) + dy_;
-        gst(pt.out[c], u, gld(pt.in[c], u) + dU * sc);
-    };
-    T st[6];
     // x flux: vx<-F(vn) vy<-F(vt) sxx<-F(snn) syy,szz<-F(stt) sxy<-F(snt)
     // y flux: vy<-F(vn) vx<-F(vt) syy<-F(snn) sxx,szz<-F(stt) sxy<-F(snt)
     const T dvx_x = fxh[0] - fxl[0], dvx_y = fyh[2] - fyl[2];
@@ -243,45 +287,66 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     const T dsyy_x = fxh[4] - fxl[4], dsyy_y = fyh[1] - fyl[1];
     const T dszz_x = fxh[4] - fxl[4], dszz_y = fyh[4] - fyl[4];
     const T dsxy_x = fxh[3] - fxl[3], dsxy_y = fyh[3] - fyl[3];
-    upd(o + 0, dvx_x, dvx_y);
-    upd(o + 1, dvy_x, dvy_y);
-    st[0] = gld(pt.in[o + 3], u) + (((T)0 + dsxx_x) + dsxx_y) * sc;
-    st[1] = gld(pt.in[o + 4], u) + (((T)0 + dsyy_x) + dsyy_y) * sc;
-    st[2] = gld(pt.in[o + 5], u) + (((T)0 + dszz_x) + dszz_y) * sc;
-    st[3] = gld(pt.in[o + 6], u) + (((T)0 + dsxy_x) + dsxy_y) * sc;
+    u[0] = u[0] + (((T)0 + dvx_x) + dvx_y) * sc;
+    u[1] = u[1] + (((T)0 + dvy_x) + dvy_y) * sc;
+    u[3] = u[3] + (((T)0 + dsxx_x) + dsxx_y) * sc;
+    u[4] = u[4] + (((T)0 + dsyy_x) + dsyy_y) * sc;
+    u[5] = u[5] + (((T)0 + dszz_x) + dszz_y) * sc;
+    u[6] = u[6] + (((T)0 + dsxy_x) + dsxy_y) * sc;
     if (NS == 3) {
         const T dvz_x = fxh[5] - fxl[5], dvz_y = fyh[5] - fyl[5];
         const T dsxz_x = fxh[6] - fxl[6], dsxz_y = (T)0 - (T)0;
         const T dsyz_x = (T)0 - (T)0, dsyz_y = fyh[6] - fyl[6];
-        upd(o + 2, dvz_x, dvz_y);
-        st[4] = gld(pt.in[o + 7], u) + (((T)0 + dsxz_x) + dsxz_y) * sc;
-        st[5] = gld(pt.in[o + 8], u) + (((T)0 + dsyz_x) + dsyz_y) * sc;
-    } else {
-        st[4] = (T)0;
-        st[5] = (T)0;
+        u[2] = u[2] + (((T)0 + dvz_x) + dvz_y) * sc;
+        u[7] = u[7] + (((T)0 + dsxz_x) + dsxz_y) * sc;
+        u[8] = u[8] + (((T)0 + dsyz_x) + dsyz_y) * sc;
     }
-    if (P.plast && a.plastic) {
-        // LinElastLevelGodunov::plasticUpdate after the level step (AMRLevelLinElast.cpp:476)
-        T lam = gld(pt.in[o + 9], u), gam = gld(pt.in[o + 10], u);
-        T sxz = NS == 3 ? st[4] : gld(pt.in[o + 7], u);
-        T syz = NS == 3 ? st[5] : gld(pt.in[o + 8], u);
-        plastic_flow<T>(P, (T)dt, st[0], st[1], st[2], st[3], sxz, syz, lam, gam);
-        st[4] = sxz;
-        st[5] = syz;
-        gst(pt.out[o + 9], u, lam);
-        gst(pt.out[o + 10], u, gam);
-        if (NS == 2) {
-            gst(pt.out[o + 7], u, sxz);
-            gst(pt.out[o + 8], u, syz);
-        }
+    if (plastic) {
+        if constexpr (stores_delta<T>())
+            plastic_flow_delta<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
+        else
+            plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
     }
-    gst(pt.out[o + 3], u, st[0]);
-    gst(pt.out[o + 4], u, st[1]);
-    gst(pt.out[o + 5], u, st[2]);
-    gst(pt.out[o + 6], u, st[3]);
-    if (NS == 3) {
-        gst(pt.out[o + 7], u, st[4]);
-        gst(pt.out[o + 8], u, st[5]);
+}
+
+// U += dU of side sd of valid cell (i, j) from the final states on its four faces, then the
+// optional Drucker-Prager return (cell_update_side)
+template <typename T, int NS>
+__device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<T>& pt, const DevParams& P,
+                                                 const Mat<T>& m, double dt, T sc, int i, int j, int sd,
+                                                 const Faces1<T, NS>& F)
+{
+    const T ry = sd ? (T)-1 : (T)1;
+    const unsigned u = (unsigned)uidx(pt, i, j, 0);
+    const int o = 11 * sd;
+    const bool plastic = P.plast && a.plastic;
+    T w[11];
+    w[0] = gld(pt.in[o + 0], u);
+    w[1] = gld(pt.in[o + 1], u);
+    w[3] = gld(pt.in[o + 3], u);
+    w[4] = gld(pt.in[o + 4], u);
+    w[5] = gld(pt.in[o + 5], u);
+    w[6] = gld(pt.in[o + 6], u);
+    w[2] = NS == 3 ? gld(pt.in[o + 2], u) : (T)0;
+    w[7] = (NS == 3 || plastic) ? gld(pt.in[o + 7], u) : (T)0;
+    w[8] = (NS == 3 || plastic) ? gld(pt.in[o + 8], u) : (T)0;
+    w[9] = plastic ? gld(pt.in[o + 9], u) : (T)0;
+    w[10] = plastic ? gld(pt.in[o + 10], u) : (T)0;
+    cell_update_side<T, NS>(P, m, dt, sc, plastic, ry, F, w);
+    gst(pt.out[o + 0], u, w[0]);
+    gst(pt.out[o + 1], u, w[1]);
+    if (NS == 3) gst(pt.out[o + 2], u, w[2]);
+    gst(pt.out[o + 3], u, w[3]);
+    gst(pt.out[o + 4], u, w[4]);
+    gst(pt.out[o + 5], u, w[5]);
+    gst(pt.out[o + 6], u, w[6]);
+    if (NS == 3 || plastic) {
+        gst(pt.out[o + 7], u, w[7]);
+        gst(pt.out[o + 8], u, w[8]);
+    }
+    if (plastic) {
+        gst(pt.out[o + 9], u, w[9]);
+        gst(pt.out[o + 10], u, w[10]);
     }
 }
 
@@ -450,13 +515,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                     report(errp, TMK_E_NO_BDRY);
                 } else {
                     FaultIn fin;
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        fin.vy[s] = (double)WmY.v[s][0]; fin.syy[s] = (double)WmY.s[s][0];
-                        fin.vx[s] = (double)WmY.v[s][1]; fin.sxy[s] = (double)WmY.s[s][1];
-                        fin.vz[s] = NS == 3 ? (double)WmY.v[s][NS - 1] : 0.0;
-                        fin.syz[s] = NS == 3 ? (double)WmY.s[s][NS - 1] : 0.0;
-                    }
+                    fault_input<T, NS>(P, WmY, fin);
                     FaultOut fo;
                     const size_t bi = (size_t)(i - pt.blo0);
                     const size_t bp = (size_t)pt.bnx;
@@ -475,12 +534,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                         const double V = sqrt(Vx * Vx + Vz * Vz);
                         tmp_state = gmem(pt.bin)[bi + 7 * bp] + (0.5 * dt) * V;
                     }
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        H1.v[s][0] = (T)fo.vy[s]; H1.s[s][0] = (T)fo.syy[s];
-                        H1.v[s][1] = (T)fo.vx[s]; H1.s[s][1] = (T)fo.sxy[s];
-                        if (NS == 3) { H1.v[s][NS - 1] = (T)fo.vz[s]; H1.s[s][NS - 1] = (T)fo.syz[s]; }
-                    }
+                    fault_output<T, NS>(P, fo, H1);
                 }
             } else {
 #pragma unroll
@@ -575,13 +629,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             if (!INT && j == dlo1) {
                 if (pt.bin) {
                     FaultIn fin;
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        fin.vy[s] = (double)WmY.v[s][0]; fin.syy[s] = (double)WmY.s[s][0];
-                        fin.vx[s] = (double)WmY.v[s][1]; fin.sxy[s] = (double)WmY.s[s][1];
-                        fin.vz[s] = NS == 3 ? (double)WmY.v[s][NS - 1] : 0.0;
-                        fin.syz[s] = NS == 3 ? (double)WmY.s[s][NS - 1] : 0.0;
-                    }
+                    fault_input<T, NS>(P, WmY, fin);
                     FaultOut fo;
                     const size_t bi = (size_t)(i - pt.blo0);
                     const size_t bp = (size_t)pt.bnx;
@@ -632,12 +680,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
                         Bo[bi + 7 * bp] = b7 + dt * V;
                         Bo[bi + 8 * bp] = (b8 > tnew && V > P.sw_thr) ? tnew : b8;
                     }
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        F1.v[s][0] = (T)fo.vy[s]; F1.s[s][0] = (T)fo.syy[s];
-                        F1.v[s][1] = (T)fo.vx[s]; F1.s[s][1] = (T)fo.sxy[s];
-                        if (NS == 3) { F1.v[s][NS - 1] = (T)fo.vz[s]; F1.s[s][NS - 1] = (T)fo.syz[s]; }
-                    }
+                    fault_output<T, NS>(P, fo, F1);
                 }
             } else {
 #pragma unroll
@@ -667,8 +710,8 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             }
         __syncthreads();
         // optional flux output for flux registers (LinElastPatchGodunov.cpp:117-118, LinElastPhysics.cpp:153)
-        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
-        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(P, pt, m, 0, F0, (T)1, i, j);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(P, pt, m, 1, F1, rv1, i, j);
         if (valid) {
             Slots<T, NS> F0h, F1h;
 #pragma unroll
@@ -720,7 +763,10 @@ __global__ void k_plastic(const DevParams P, T* U, int ulo0, int ulo1, int unx, 
         T sxx = U[u + (o + 3) * plane], syy = U[u + (o + 4) * plane], szz = U[u + (o + 5) * plane];
         T sxy = U[u + (o + 6) * plane], sxz = U[u + (o + 7) * plane], syz = U[u + (o + 8) * plane];
         T lam = U[u + (o + 9) * plane], gam = U[u + (o + 10) * plane];
-        plastic_flow<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
+        if constexpr (stores_delta<T>())
+            plastic_flow_delta<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
+        else
+            plastic_flow<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
         U[u + (o + 3) * plane] = sxx; U[u + (o + 4) * plane] = syy; U[u + (o + 5) * plane] = szz;
         U[u + (o + 6) * plane] = sxy; U[u + (o + 7) * plane] = sxz; U[u + (o + 8) * plane] = syz;
         U[u + (o + 9) * plane] = lam; U[u + (o + 10) * plane] = gam;

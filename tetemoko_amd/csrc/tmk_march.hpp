@@ -1,0 +1,396 @@
+// tmk_march.hpp -- the interior update by row marching: one wavefront per strip and side.
+//
+// A strip is up to MARCH_COLS columns wide and any number of rows high, with its whole stencil
+// (+-3 cells) inside the domain.  Lane l of the wavefront owns column x0 - 3 + l; the strip's
+// columns are lanes 3 .. 3 + MARCH_COLS - 1, the other six lanes carry the x stencil.  The
+// wavefront marches up the rows: after it loads row r it has what finishes row r - 3:
+//
+//   y slopes and face values   computed once per row and carried in registers to the next rows
+//                              (the y stencil of a cell is its own column);
+//   x slopes and face values   computed once per cell, the neighbours' by DPP lane shifts;
+//   predictor face states      x faces from the left lane (DPP), y faces from the row below
+//                              (registers);
+//   transverse corrections, final face states, flux difference and update as in the tile
+//   kernels (LinElastPatchGodunov.cpp:101-129, :174-454; LinElastPhysics.cpp:135-179).
+//
+// Every value is the one the tile kernels and the oracle compute for that cell, by the same
+// expressions in the same order: a slope, face value or face state shared by two cells is
+// computed once, not twice (the face value of x + 1/2 is the same expression whether it is seen
+// as the hi face of cell x or the lo face of cell x + 1, FP additions being commutative).  The
+// FP64 kernel is therefore bit-identical to the oracle like the tile kernels.  There is no LDS
+// and no barrier; the two sides of a strip are independent wavefronts (inside the domain they
+// meet only at the fault, which no strip touches).  Mode II (NS = 2) with PPM and primitive
+// limiting only: other numerics use the tile kernels.
+#pragma once
+
+#include "tmk_interior.hpp"
+
+namespace tmk {
+
+constexpr int MARCH_COLS = 58;  // 64 lanes minus the 3-cell x stencil on each side
+constexpr unsigned kDropOffset = 0xFFFFFF00u;  // buffer offset of a lane whose access must be dropped
+
+// a strip whose stencil lies in the domain (the same rule as tile_interior)
+__device__ __forceinline__ bool march_interior(const Tile& t, const DevParams& P)
+{
+    return t.x1 - t.x0 + 1 <= MARCH_COLS && t.x0 - 3 >= P.dlo[0] && t.x1 + 3 <= P.dhi[0] && t.y0 - 3 >= P.dlo[1] &&
+           t.y1 + 3 <= P.dhi[1];
+}
+
+// the monotonized central slope of a cell from its two one-sided differences (vl_slope, side 0)
+template <typename T>
+__device__ __forceinline__ T mc_slope(T dl, T dr)
+{
+    return vl_limit((T)0.5 * (dr + dl), dl, dr);
+}
+
+// PPM face value between cells a (lo) and b (hi) with slopes da, db (ppm_incr's fl / fh)
+template <typename T>
+__device__ __forceinline__ T ppm_face(T wa, T wb, T da, T db)
+{
+    const T sixth = (T)(1.0 / 6.0);
+    return (T)0.5 * (wb + wa) + sixth * (da - db);
+}
+
+// Buffer descriptors over one state plane: the kernel addresses a plane with a 32-bit byte
+// offset per lane from a descriptor in scalar registers (no 64-bit address arithmetic per
+// access), and the hardware bound check makes a load past the plane read 0.  Dword 3 holds the
+// raw-access format of the gfx9 family (32-bit data format).
+// The plane pointer and size must be the same in every lane (they come from the workgroup's
+// patch descriptor); readfirstlane states that, so the descriptor lives in scalar registers.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t plane_rsrc(const void* p, unsigned bytes)
+{
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    void* q = (void*)(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(q, 0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(rsrc_t r, unsigned off)
+{
+    if constexpr (sizeof(T) == sizeof(double)) {
+        using u2 = __attribute__((__vector_size__(2 * sizeof(unsigned)))) unsigned;
+        return __builtin_bit_cast(double, (u2)__builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    } else {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    }
+}
+template <typename T>
+__device__ __forceinline__ void bst(rsrc_t r, unsigned off, T v)
+{
+    if constexpr (sizeof(T) == sizeof(double)) {
+        using u2 = __attribute__((__vector_size__(2 * sizeof(unsigned)))) unsigned;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+    }
+}
+
+// lane shifts inside the wavefront (lane l <- lane l -+ 1; the first / last lane reads 0), in a
+// form the compiler folds into the consuming FP32 instruction
+template <int CTRL>
+__device__ __forceinline__ float dshift(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dshift(double v)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// staged quantities of one side: 0 vx, 1 vy, 2 sxx, 3 syy, 4 sxy
+// slot form per sweep: x: N (vx, sxx), S (vy, sxy); y: N (vy, syy), S (vx, sxy)
+constexpr int MQ_XV[2] = {0, 1}, MQ_XS[2] = {2, 4};
+constexpr int MQ_Y[4] = {1, 0, 3, 4};  // y-sweep quantities: vy vx syy sxy (v of slots N S, s of N S)
+
+// predictor states of one side and sweep: W- (lo face) and W+ (hi face), slot form
+template <typename T>
+struct Pred2 {
+    T mv[2], ms[2], pv[2], ps[2];
+};
+
+// grid: two workgroups of one wavefront per strip, side = blockIdx.x & 1 (a scalar, so every
+// plane pointer and descriptor of the side stays in scalar registers)
+template <typename T, int PK>
+__global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
+{
+    static_assert(PK == PK_PPM_PRIM, "row marching carries PPM with primitive limiting");
+    const Tile tl = a.tiles[blockIdx.x >> 1];
+    const Patch<T>& pt = a.patches[tl.patch];
+    Scen& S = a.scen[pt.scen];
+    const DevParams& P = S.P;
+    if (!march_interior(tl, P)) {
+        if (threadIdx.x == 0 && (blockIdx.x & 1) == 0) report(&S.err, TMK_E_BOX);
+        return;
+    }
+    const Mat<T> m(P);
+    const int lane = threadIdx.x;
+    const int sd = blockIdx.x & 1;  // side of this wavefront
+    const int i = tl.x0 - 3 + lane;
+    const int tnx = tl.x1 - tl.x0 + 1;
+    const bool valid = lane >= 3 && lane <= tnx + 2;  // column of the strip
+    const double dt = S.dt;
+    const StepConst<T> K = step_const<T>(S);
+    const T dtbydx = K.dtbydx;
+    const T scale = K.scale;
+    const T rv1 = (T)-1;
+    const T rvy = sd ? rv1 : (T)1;  // velocity sign of this side in y
+    const T scx = sd ? rv1 * scale : scale;
+    const bool plastic = P.plast && a.plastic;
+    const int o = 11 * sd;
+    // stencil planes vx vy sxx syy sxy, read with the stencil (the in copies)
+    constexpr int QC[5] = {0, 1, 3, 4, 6};
+    rsrc_t rin[5], rout[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        rin[q] = plane_rsrc(pt.in[o + QC[q]], pt.pbytes);
+        rout[q] = plane_rsrc(pt.out[o + QC[q]], pt.pbytes);
+    }
+    // pointwise planes, updated in place: szz, then sxz syz lam gam (plasticity)
+    rsrc_t rpw[5];
+    rpw[0] = plane_rsrc(pt.out[o + 5], pt.pbytes);
+#pragma unroll
+    for (int c = 7; c <= 10; ++c) rpw[c - 6] = plane_rsrc(pt.out[o + c], pt.pbytes);
+    const unsigned col = (unsigned)(i - pt.ulo0) * (unsigned)sizeof(T);
+    const unsigned rowb = (unsigned)pt.unx * (unsigned)sizeof(T);
+    auto row_off = [&](int j) { return col + (unsigned)(j - pt.ulo1) * rowb; };
+    const int y0 = tl.y0, y1 = tl.y1;
+
+    // carried from earlier rows (r = the row loaded in this iteration)
+    T w1[5], w2[5];   // W of rows r - 1, r - 2
+    T dly[4];         // y differences W(r - 1) - W(r - 2) of the y-sweep quantities
+    T dy2[4];         // y slopes of row r - 2
+    T fy[4];          // y face values at r - 5/2
+
+    // warm-up: rows y0 - 3 .. y0 give the y slopes of rows y0 - 2, y0 - 1 and the face value at
+    // y0 - 3/2 (what the first main iteration carries in)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w1[q] = w2[q] = (T)0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dly[q] = dy2[q] = fy[q] = (T)0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        T w0[5];
+        const unsigned off = row_off(y0 - 3 + r);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) w0[q] = bld<T>(rin[q], off);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T dry = w0[MQ_Y[q]] - w1[MQ_Y[q]];
+            const T dy1 = mc_slope(dly[q], dry);
+            const T fy1 = ppm_face(w2[MQ_Y[q]], w1[MQ_Y[q]], dy2[q], dy1);
+            dly[q] = dry;
+            dy2[q] = dy1;
+            fy[q] = fy1;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            w2[q] = w1[q];
+            w1[q] = w0[q];
+        }
+    }
+
+    T w3[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};  // W of row r - 3
+    Pred2<T> xp;       // x predictor of row r - 3, not yet transversely corrected
+    T h1v[2], h1s[2];  // predictor state of the lo y face of row r - 3
+    T cpu[2];          // right characteristics of row r - 3's uncorrected W+ (y)
+    T cpc[2];          // the same of its corrected W+
+    T f1v[2], f1s[2];  // final state of the lo y face of row r - 3
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        xp.mv[k] = xp.ms[k] = xp.pv[k] = xp.ps[k] = (T)0;
+        h1v[k] = h1s[k] = cpu[k] = cpc[k] = f1v[k] = f1s[k] = (T)0;
+    }
+
+    // main march: after loading row r, the predictors of row r - 2 and the update of row r - 3.
+    // The first iteration computes row y0 - 1's face states (used) and row y0 - 2's (not used).
+    // Loads run one iteration ahead: row r + 1's stencil planes (a ghost row of the array at the
+    // last iteration) and row r - 3's pointwise planes are requested at the top of iteration r,
+    // so the wavefront computes while they arrive.
+    T wn[5];
+    {
+        const unsigned off = row_off(y0 + 1);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) wn[q] = bld<T>(rin[q], off);
+    }
+    for (int r = y0 + 1; r <= y1 + 3; ++r) {
+        T w0[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) w0[q] = wn[q];
+        {
+            const unsigned offn = row_off(r + 1);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) wn[q] = bld<T>(rin[q], offn);
+        }
+        const int j3 = r - 3;
+        // row r - 3's pointwise planes (szz; sxz syz lam gam with plasticity); a lane outside the
+        // strip gets an offset at or past every plane's bound (planes hold at most 2^32 - 256
+        // bytes), far enough below 2^32 that the second dword of an 8-byte access does not wrap:
+        // the bound check makes its loads read 0 and drops its stores
+        const unsigned off3 = valid ? row_off(j3) : kDropOffset;
+        T upw[5];
+        upw[0] = bld<T>(rpw[0], off3);
+#pragma unroll
+        for (int c = 1; c < 5; ++c) upw[c] = plastic ? bld<T>(rpw[c], off3) : (T)0;
+        // y: differences, slopes of row r - 1, face values at r - 3/2
+        T dry[4], dy1[4], fy1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dry[q] = w0[MQ_Y[q]] - w1[MQ_Y[q]];
+            dy1[q] = mc_slope(dly[q], dry[q]);
+            fy1[q] = ppm_face(w2[MQ_Y[q]], w1[MQ_Y[q]], dy2[q], dy1[q]);
+        }
+        // y predictor of row r - 2 (its y faces r - 5/2 and r - 3/2)
+        Pred2<T> yp;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const T wv = w2[MQ_Y[k]], ws = w2[MQ_Y[2 + k]];
+            const T dvm = ((T)0 - wv) + fy[k], dvp = ((T)0 - wv) + fy1[k];
+            const T dsm = ((T)0 - ws) + fy[2 + k], dsp = ((T)0 - ws) + fy1[2 + k];
+            ppm_slot<T>(m, k, rvy, dtbydx, true, false, wv, ws, dvm, dvp, dsm, dsp, yp.mv[k], yp.ms[k], yp.pv[k],
+                        yp.ps[k]);
+        }
+        // x predictor of row r - 2: slopes and face values once per lane, neighbours by DPP
+        Pred2<T> xn;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            T inc[2][2];  // [v|s][m|p]
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const T wc = w2[h == 0 ? MQ_XV[k] : MQ_XS[k]];
+                const T wr = dshift<DPP_FROM_NEXT>(wc);  // W(i + 1)
+                const T dr = wr - wc;
+                const T dl = dshift<DPP_FROM_PREV>(dr);  // W(i) - W(i - 1)
+                const T d0 = mc_slope(dl, dr);
+                const T fh = ppm_face(wc, wr, d0, dshift<DPP_FROM_NEXT>(d0));  // face value at i + 1/2
+                const T fl = dshift<DPP_FROM_PREV>(fh);                        // at i - 1/2
+                inc[h][0] = ((T)0 - wc) + fl;
+                inc[h][1] = ((T)0 - wc) + fh;
+            }
+            ppm_slot<T>(m, k, (T)1, dtbydx, true, false, w2[MQ_XV[k]], w2[MQ_XS[k]], inc[0][0], inc[0][1], inc[1][0],
+                        inc[1][1], xn.mv[k], xn.ms[k], xn.pv[k], xn.ps[k]);
+        }
+        // predictor state of the lo x face of row r - 2 (its left lane's W+ and its W-)
+        T h0v[2], h0s[2];
+        {
+            T cpx[2], wpL[2], wmR[2];
+            right_chars_side<T, 2>(m, (T)1, xn.pv, xn.ps, cpx);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) wpL[k] = dshift<DPP_FROM_PREV>(cpx[k]);
+            left_chars_side<T, 2>(m, (T)1, xn.mv, xn.ms, wmR);
+            riemann_side<T, 2>(m, (T)1, wpL, wmR, h0v, h0s);
+        }
+        // predictor state of the lo y face of row r - 2 (row r - 3's uncorrected W+, its W-)
+        T h1nv[2], h1ns[2], cpun[2];
+        {
+            T wmR[2];
+            left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
+            riemann_side<T, 2>(m, rvy, cpu, wmR, h1nv, h1ns);
+            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, cpun);
+        }
+        // y predictor corrected with the x-face states of its cell (GETADWDXF, dir2 = x)
+        {
+            T hv[2], hs[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                hv[k] = dshift<DPP_FROM_NEXT>(h0v[k]);
+                hs[k] = dshift<DPP_FROM_NEXT>(h0s[k]);
+            }
+            transverse_side<T, 2>(m, scale, h0v, h0s, hv, hs, yp.mv, yp.ms);
+            transverse_side<T, 2>(m, scale, h0v, h0s, hv, hs, yp.pv, yp.ps);
+        }
+        // row r - 3: x predictor corrected with the y-face states below and above it, then its
+        // final x-face state
+        T f0v[2], f0s[2];
+        {
+            transverse_side<T, 2>(m, scx, h1v, h1s, h1nv, h1ns, xp.mv, xp.ms);
+            transverse_side<T, 2>(m, scx, h1v, h1s, h1nv, h1ns, xp.pv, xp.ps);
+            T cpx[2], wpL[2], wmR[2];
+            right_chars_side<T, 2>(m, (T)1, xp.pv, xp.ps, cpx);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) wpL[k] = dshift<DPP_FROM_PREV>(cpx[k]);
+            left_chars_side<T, 2>(m, (T)1, xp.mv, xp.ms, wmR);
+            riemann_side<T, 2>(m, (T)1, wpL, wmR, f0v, f0s);
+        }
+        // final state of the lo y face of row r - 2 (row r - 3's corrected W+, its corrected W-)
+        T f1nv[2], f1ns[2], cpcn[2];
+        {
+            T wmR[2];
+            left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
+            riemann_side<T, 2>(m, rvy, cpc, wmR, f1nv, f1ns);
+            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, cpcn);
+        }
+        const int j2 = r - 2;
+        // flux registers (multi-level): x faces of row r - 3, the lo y face of row r - 2
+        if (pt.F[0] && j3 >= y0 && lane >= 3 && lane <= tnx + 3)
+            store_flux_side<T, 2>(P, pt, m, 0, f0v, f0s, (T)1, i, j3, sd);
+        if (pt.F[1] && j2 >= y0 && valid) store_flux_side<T, 2>(P, pt, m, 1, f1nv, f1ns, rvy, i, j2, sd);
+        // update of row r - 3
+        {
+            Faces1<T, 2> F;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                F.xlv[k] = f0v[k];
+                F.xls[k] = f0s[k];
+                F.xhv[k] = dshift<DPP_FROM_NEXT>(f0v[k]);
+                F.xhs[k] = dshift<DPP_FROM_NEXT>(f0s[k]);
+                F.ylv[k] = f1v[k];
+                F.yls[k] = f1s[k];
+                F.yhv[k] = f1nv[k];
+                F.yhs[k] = f1ns[k];
+            }
+            if (j3 >= y0) {
+                // every lane runs the update (a lane outside the strip has off3 past the planes)
+                T u[11];
+                u[0] = w3[0];
+                u[1] = w3[1];
+                u[3] = w3[2];
+                u[4] = w3[3];
+                u[6] = w3[4];
+                u[2] = (T)0;
+                u[5] = upw[0];
+#pragma unroll
+                for (int c = 7; c <= 10; ++c) u[c] = upw[c - 6];
+                cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
+                bst<T>(rpw[0], off3, u[5]);
+                if (plastic) {
+#pragma unroll
+                    for (int c = 7; c <= 10; ++c) bst<T>(rpw[c - 6], off3, u[c]);
+                }
+            }
+        }
+        // carry to the next row
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            h1v[k] = h1nv[k];
+            h1s[k] = h1ns[k];
+            cpu[k] = cpun[k];
+            cpc[k] = cpcn[k];
+            f1v[k] = f1nv[k];
+            f1s[k] = f1ns[k];
+        }
+        xp = xn;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dly[q] = dry[q];
+            dy2[q] = dy1[q];
+            fy[q] = fy1[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            w3[q] = w2[q];
+            w2[q] = w1[q];
+            w1[q] = w0[q];
+        }
+    }
+}
+
+}  // namespace tmk

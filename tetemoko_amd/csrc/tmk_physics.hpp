@@ -62,7 +62,10 @@ struct DevParams {
     float frho, flam, fG, fZp, fZs, fden, flam2G, fiZp, fiZs, firho, fi2Zp, fi2Zs, fcp, fcs;
     double den;    // 2G + lam   (LinElastPhysicsF.ChF:427 form)
     double lam2G;  // lam + 2G   (LinElastPhysicsF.ChF:318 form)
-    double bg[22];
+    double bg[22];  // the uniform background state (both sides alike)
+    // the background in the form the FP32 Drucker-Prager return uses: mean stress, deviators
+    // xx yy zz, then sxy sxz syz (rounded from FP64 on the host)
+    float pbg[7];
     int plast;
     double p_mu, p_beta, p_eta, p_b, p_h;
     int bc[4];
@@ -80,6 +83,21 @@ struct DevParams {
     double dx;
     int dlo[2], dhi[2];
 };
+
+// FP32 bulk stores the difference from the uniform background, U - bg, instead of U.  The
+// background stresses (around 1e8 Pa) are far larger than the waves the update moves (1e5 to
+// 1e7 Pa), and an FP32 copy of the absolute value would round every cell at the ulp of the
+// background, an error that the rupture's nucleation amplifies.  The elastic update is linear
+// and its limiters see only differences, so the same kernels advance U - bg; the background
+// enters where the update is not translation invariant: the absorbing and free-surface face
+// states (bc_state), the fault solve (the FP64 traces get bg added and the face states bg
+// removed), the Drucker-Prager return (plastic_flow_delta) and the flux outputs (the flux of the
+// background added in FP64).  FP64 stores U itself, so it rounds like the reference.
+template <typename T>
+__host__ __device__ constexpr bool stores_delta()
+{
+    return sizeof(T) == sizeof(float);
+}
 
 template <typename T, int NS>
 struct Slots {
@@ -212,16 +230,57 @@ __device__ __forceinline__ void ppm_incr(const T* w, int ci, int dlo, int dhi, b
     dp = ((T)0 - w[2]) + fh;
 }
 
+// branch-free: both candidates are formed and one is selected (the same values as the
+// branching form, without divergent control flow)
 template <typename T>
 __device__ __forceinline__ void ppm_limit(T& m, T& p)
 {
-    if (m * p < (T)0) {
-        const T nm = copysign(fmin((T)2 * fabs(p), fabs(m)), m);
-        const T np = copysign(fmin((T)2 * fabs(m), fabs(p)), p);
-        m = nm; p = np;
-    } else {
-        m = (T)0; p = (T)0;
+    const bool keep = m * p < (T)0;
+    const T nm = copysign(fmin((T)2 * fabs(p), fabs(m)), m);
+    const T np = copysign(fmin((T)2 * fabs(m), fabs(p)), p);
+    m = keep ? nm : (T)0;
+    p = keep ? np : (T)0;
+}
+
+// The PPM predictor of slot k of one side from its face increments (face value minus cell value
+// of v and s at the lo (m) and hi (p) faces): limiter in primitive or characteristic variables,
+// CHARANALYSISF (LinElastPhysicsF.ChF:424-447), tracing of the wave moving toward lo on the lo
+// face and toward hi on the hi face, CHARSYNTHESISF (:526-550), then W+- = increment + W.
+template <typename T>
+__device__ __forceinline__ void ppm_slot(const Mat<T>& m, int k, T rvs, T dtbydx, bool limit, bool charl, T wv, T ws,
+                                         T dvm, T dvp, T dsm, T dsp, T& Wmv, T& Wms, T& Wpv, T& Wps)
+{
+    if (limit && !charl) {
+        ppm_limit(dvm, dvp);
+        ppm_limit(dsm, dsp);
     }
+    const T Z = m.Z(k);
+    const T vm = rvs * dvm;
+    const T vp = rvs * dvp;
+    T cm_m = dsm + vm * Z, cp_m = dsm - vm * Z;
+    T cm_p = dsp + vp * Z, cp_p = dsp - vp * Z;
+    if (charl) {
+        ppm_limit(cm_m, cm_p);
+        ppm_limit(cp_m, cp_p);
+    }
+    // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
+    const T sig = (k == 0 ? m.cp : m.cs) * dtbydx;
+    {
+        const T mm = cm_m, pp = cm_p;
+        cm_m = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+    }
+    {
+        const T mm = cp_m, pp = cp_p;
+        cp_p = pp + sig * ((mm - pp) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
+    }
+    const T vmi = m.divZ(rvs * (cm_m - cp_m) / (T)2, k);
+    const T vpi = m.divZ(rvs * (cm_p - cp_p) / (T)2, k);
+    const T smi = (cm_m + cp_m) / (T)2;
+    const T spi = (cm_p + cp_p) / (T)2;
+    Wmv = vmi + wv;
+    Wpv = vpi + wv;
+    Wms = smi + ws;
+    Wps = spi + ws;
 }
 
 // ---------------------------------------------------------------------------
@@ -270,32 +329,8 @@ __device__ __forceinline__ void normal_pred_side(const DevParams& P, const Mat<T
             T dvm, dvp, dsm, dsp;
             ppm_incr(w[k][0], ci, dlo, dhi, limit, dvm, dvp);
             ppm_incr(w[k][1], ci, dlo, dhi, limit, dsm, dsp);
-            if (limit && !charl) {
-                ppm_limit(dvm, dvp);
-                ppm_limit(dsm, dsp);
-            }
-            // CHARANALYSISF (LinElastPhysicsF.ChF:424-447): characteristic increments of the
-            // wave moving toward lo (m) and toward hi (p)
-            const T Z = m.Z(k);
-            const T vm = rvs * dvm;
-            const T vp = rvs * dvp;
-            T cm_m = dsm + vm * Z, cp_m = dsm - vm * Z;
-            T cm_p = dsp + vp * Z, cp_p = dsp - vp * Z;
-            if (charl) {
-                ppm_limit(cm_m, cm_p);
-                ppm_limit(cp_m, cp_p);
-            }
-            // PPM tracing: left-moving waves (lambda<0) on the lo face, right-moving on the hi face
-            const T sig = (k == 0 ? m.cp : m.cs) * dtbydx;
-            {
-                const T mm = cm_m, pp = cm_p;
-                cm_m = mm + sig * ((pp - mm) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
-            }
-            {
-                const T mm = cp_m, pp = cp_p;
-                cp_p = pp + sig * ((mm - pp) - (pp + mm) * ((T)3 - (T)2 * sig)) / (T)2;
-            }
-            synth(k, cm_m, cp_m, cm_p, cp_p);
+            ppm_slot<T>(m, k, rvs, dtbydx, limit, charl, w[k][0][2], w[k][1][2], dvm, dvp, dsm, dsp, Wmv[k], Wms[k],
+                        Wpv[k], Wps[k]);
         }
     } else {  // PLM
         const bool os = charl || P.fourth;
@@ -427,15 +462,25 @@ __device__ __forceinline__ void bc_state(const DevParams& P, const Mat<T>& m, in
         for (int k = 0; k < NS; ++k) {
             const T Z = m.Z(k);
             T wm, wp;
+            const int cv = slot_comp(d, k, 0) + 11 * s, cs_ = slot_comp(d, k, 1) + 11 * s;
             if (type == 0) {
-                const int cv = slot_comp(d, k, 0) + 11 * s, cs_ = slot_comp(d, k, 1) + 11 * s;
-                const T bv = (T)P.bg[cv], bs = (T)P.bg[cs_];
-                const T rZ = s ? rv * Z : Z;
-                if (lohisign == -1) { wm = cin[s][k]; wp = bs - rZ * bv; }
-                else { wp = cin[s][k]; wm = bs + rZ * bv; }
+                // absorbing: the incoming characteristic is the background's (zero for U - bg)
+                if constexpr (stores_delta<T>()) {
+                    if (lohisign == -1) { wm = cin[s][k]; wp = (T)0; }
+                    else { wp = cin[s][k]; wm = (T)0; }
+                } else {
+                    const T bv = (T)P.bg[cv], bs = (T)P.bg[cs_];
+                    const T rZ = s ? rv * Z : Z;
+                    if (lohisign == -1) { wm = cin[s][k]; wp = bs - rZ * bv; }
+                    else { wp = cin[s][k]; wm = bs + rZ * bv; }
+                }
             } else {
-                if (lohisign == -1) { wm = cin[s][k]; wp = -wm; }
-                else { wp = cin[s][k]; wm = -wp; }
+                // free surface: zero traction, wm + wp = 0 for U; for U - bg the two
+                // characteristics of the background add 2 bs
+                T off = (T)0;
+                if constexpr (stores_delta<T>()) off = (T)(2.0 * P.bg[cs_]);
+                if (lohisign == -1) { wm = cin[s][k]; wp = -wm - off; }
+                else { wp = cin[s][k]; wm = -wp - off; }
             }
             F.v[s][k] = s ? m.div2Z(rv * (wm - wp), k) : m.div2Z(wm - wp, k);
             F.s[s][k] = (wm + wp) / (T)2;
@@ -652,6 +697,41 @@ __device__ __forceinline__ void plastic_flow(const DevParams& P, T dt, T& sxx, T
     sxy = factor * sxy;
     sxz = factor * sxz;
     syz = factor * syz;
+}
+
+// plastic_flow on the stored difference from the background (FP32 bulk, stores_delta).  With
+// s = bg + d, mean stress sigma = sigma_bg + mean(d) and deviator dev = dev_bg + dev(d) are
+// formed from the difference and the background's own (pbg) without ever rounding an absolute
+// stress to FP32 where it is stored.  The return scales the deviator by factor = 1 - g with
+// g = dt lam G / (tau_new + dt lam G) and moves the mean by -dt lam K beta, so the new difference
+// is d - dt lam K beta - g dev (diagonal) and d - g s (shear): the same flow rule as plastic_flow
+// written as an increment.  A cell that does not yield keeps its values.
+template <typename T>
+__device__ __forceinline__ void plastic_flow_delta(const DevParams& P, T dt, T& dxx, T& dyy, T& dzz, T& dxy, T& dxz,
+                                                   T& dyz, T& plam, T& pgam)
+{
+    const T md = (dxx + dyy + dzz) / (T)3;
+    const T exx = (dxx - md) + (T)P.pbg[1], eyy = (dyy - md) + (T)P.pbg[2], ezz = (dzz - md) + (T)P.pbg[3];
+    const T sxy = dxy + (T)P.pbg[4], sxz = dxz + (T)P.pbg[5], syz = dyz + (T)P.pbg[6];
+    const T sigma = md + (T)P.pbg[0];
+    const T a = exx - eyy, b = eyy - ezz, c = ezz - exx;
+    const T J2 = (a * a + b * b + c * c) / (T)6 + sxy * sxy + sxz * sxz + syz * syz;
+    const T tau = sqrt(J2);
+    const T bb = (T)P.p_b + pgam * (T)P.p_h;
+    const T Y = tau - fmax(-(T)P.p_mu * sigma + bb, (T)0);
+    if (Y <= (T)0) { plam = (T)0; return; }
+    const T G = (T)P.G, K = (T)P.K;
+    plam = Y / (dt * ((T)P.p_h + G + (T)P.p_mu * (T)P.p_beta * K) + (T)P.p_eta);
+    pgam = pgam + dt * plam;
+    const T dG = dt * plam * G;
+    const T g = dG / ((tau - dG) + dG);
+    const T dm = dt * plam * K * (T)P.p_beta;
+    dxx = dxx - (dm + g * exx);
+    dyy = dyy - (dm + g * eyy);
+    dzz = dzz - (dm + g * ezz);
+    dxy = dxy - g * sxy;
+    dxz = dxz - g * sxz;
+    dyz = dyz - g * syz;
 }
 
 }  // namespace tmk

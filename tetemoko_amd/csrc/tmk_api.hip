@@ -439,6 +439,8 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
     const double mbg = (h.sxx0 + h.syy0 + h.szz0) / 3.0;
     const double pbg[7] = {mbg, h.sxx0 - mbg, h.syy0 - mbg, h.szz0 - mbg, h.sxy0, h.sxz0, h.syz0};
     for (int k = 0; k < 7; ++k) P.pbg[k] = (float)pbg[k];
+    const double pf[6] = {h.p_mu, h.p_b, h.p_h, h.p_h + h.G + h.p_mu * h.p_beta * P.K, h.p_eta, P.K * h.p_beta};
+    for (int k = 0; k < 6; ++k) P.pf[k] = (float)pf[k];
     P.plast = h.use_plasticity;
     P.p_mu = h.p_mu; P.p_beta = h.p_beta; P.p_eta = h.p_eta; P.p_b = h.p_b; P.p_h = h.p_h;
     for (int k = 0; k < 4; ++k) P.bc[k] = h.bc[k];
@@ -1006,6 +1008,41 @@ static StateVec background(const DevParams& P)
     for (int k = 0; k < TMK_NCOMP; ++k) v.c[k] = P.bg[k];
     return v;
 }
+// GETFLUXF (LinElastPhysicsF.ChF:303-350) of the background in direction d, per component: the
+// FP32 bulk's flux outputs are fluxes of U - bg, and the drop-in adds this to return the flux
+// of U (the same slot forms as the kernels' face_flux_side and store_flux_side)
+static void bg_flux(const DevParams& P, int d, double out[TMK_NCOMP])
+{
+    for (int sd = 0; sd < 2; ++sd) {
+        const int o = 11 * sd;
+        const double r = (d == 1 && sd == 1) ? -1.0 : 1.0;
+        double v[3], sv[3];
+        for (int k = 0; k < 3; ++k) {
+            v[k] = P.bg[slot_comp(d, k, 0) + o];
+            sv[k] = P.bg[slot_comp(d, k, 1) + o];
+        }
+        const double f0 = -(r * sv[0]) / P.rho, f2 = -(r * sv[1]) / P.rho, f5 = -(r * sv[2]) / P.rho;
+        const double f1 = -(r * v[0] * P.lam2G), f4 = -(r * v[0] * P.lam), f3 = -(r * v[1] * P.G);
+        const double f6 = -(r * v[2] * P.G);
+        double* q = out + o;
+        if (d == 0) {
+            q[0] = f0; q[1] = f2; q[2] = f5; q[3] = f1; q[4] = f4; q[5] = f4; q[6] = f3; q[7] = f6; q[8] = 0.0;
+        } else {
+            q[1] = f0; q[0] = f2; q[2] = f5; q[4] = f1; q[5] = f4; q[3] = f4; q[6] = f3; q[8] = f6; q[7] = 0.0;
+        }
+        q[9] = q[10] = 0.0;
+    }
+}
+static void add_bg_flux(const DevParams& P, int d, double* F, size_t n)
+{
+    double fb[TMK_NCOMP];
+    bg_flux(P, d, fb);
+    const size_t plane = n / TMK_NCOMP;
+    for (int c = 0; c < TMK_NCOMP; ++c)
+        if (fb[c] != 0.0)
+            for (size_t k = 0; k < plane; ++k) F[k + c * plane] += fb[c];
+}
+
 // host copies between FP64 state and the FP32 bulk's stored form
 static void to_stored_f32(const double* U, float* d, size_t plane, const DevParams& P)
 {
@@ -1148,6 +1185,10 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         if (B) HIPCHK(hipMemcpy(B, Bd + nB, nB * sizeof(double), hipMemcpyDeviceToHost));
         if (F0) HIPCHK(hipMemcpy(F0, Fd, nF0 * sizeof(double), hipMemcpyDeviceToHost));
         if (F1) HIPCHK(hipMemcpy(F1, Fd + nF0, nF1 * sizeof(double), hipMemcpyDeviceToHost));
+        if (stores_delta<T>()) {
+            if (F0) add_bg_flux(c->P, 0, F0, nF0);
+            if (F1) add_bg_flux(c->P, 1, F1, nF1);
+        }
     }
     if (sizeof(T) == sizeof(double)) {
         HIPCHK(hipMemcpyAsync(U, Uout, nU * sizeof(double), hipMemcpyDeviceToHost, st));

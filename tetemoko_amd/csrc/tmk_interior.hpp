@@ -246,8 +246,8 @@ __device__ __forceinline__ void interior_tile(const KArgs<T>& a, const Tile& tl,
                 F0h.s[s][k] = lane_shift<DPP_FROM_NEXT>(F0.s[s][k]);
             }
         __syncthreads();
-        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(P, pt, m, 0, F0, (T)1, i, j);
-        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(P, pt, m, 1, F1, rv1, i, j);
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
         if (lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny) {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
@@ -418,9 +418,9 @@ __device__ __forceinline__ void interior_tile_split(const KArgs<T>& a, const Til
         }
         __syncthreads();
         if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny)
-            store_flux_side<T, NS>(P, pt, m, 0, F.xlv, F.xls, (T)1, i, j, sd);
+            store_flux_side<T, NS>(pt, m, 0, F.xlv, F.xls, (T)1, i, j, sd);
         if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx)
-            store_flux_side<T, NS>(P, pt, m, 1, F.ylv, F.yls, rvy, i, j, sd);
+            store_flux_side<T, NS>(pt, m, 1, F.ylv, F.yls, rvy, i, j, sd);
         if (lx >= 1 && lx <= tnx && ly >= 1 && ly <= tny) {
 #pragma unroll
             for (int k = 0; k < NS; ++k) {

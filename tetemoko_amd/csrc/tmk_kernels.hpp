@@ -189,11 +189,12 @@ __device__ __forceinline__ void fault_output(const DevParams& P, const FaultOut&
 }
 
 // the 11 flux components of side sd from face state (v, s) on the lo d-face of cell (i, j) into
-// the flux register array of direction d (surroundingNodes(valid box, d)); stores_delta adds
-// the flux of the background (in FP64), so the output is the flux of U in both precisions
+// the flux register array of direction d (surroundingNodes(valid box, d)).  With stores_delta
+// this is the flux of U - bg: flux registers only ever difference fluxes of one precision, and
+// the drop-in path adds the background's flux (bg_flux) on the host.
 template <typename T, int NS>
-__device__ __forceinline__ void store_flux_side(const DevParams& P, const Patch<T>& pt, const Mat<T>& m, int d,
-                                                const T (&v)[NS], const T (&sv)[NS], T r, int i, int j, int sd)
+__device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
+                                                const T (&sv)[NS], T r, int i, int j, int sd)
 {
     const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
     if (pt.fmask) {
@@ -207,22 +208,8 @@ __device__ __forceinline__ void store_flux_side(const DevParams& P, const Patch<
     }
     const size_t fp = (size_t)fnx * fny;
     const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
-    T fT[7];
-    face_flux_side<T, NS>(m, v, sv, r, fT);
-    double f[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) f[k] = (double)fT[k];
-    if constexpr (stores_delta<T>()) {
-        double bv[NS], bs[NS], fb[7];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            bv[k] = P.bg[slot_comp(d, k, 0) + 11 * sd];
-            bs[k] = P.bg[slot_comp(d, k, 1) + 11 * sd];
-        }
-        face_flux_side<double, NS>(Mat<double>(P), bv, bs, (double)r, fb);
-        f[0] += fb[0]; f[1] += fb[1]; f[2] += fb[2]; f[3] += fb[3]; f[4] += fb[4];
-        if (NS == 3) { f[5] += fb[5]; f[6] += fb[6]; }
-    }
+    T f[7];
+    face_flux_side<T, NS>(m, v, sv, r, f);
     __attribute__((address_space(1))) double* Fo = gmem(pt.F[d]);
     const int o = 11 * sd;
     if (d == 0) {
@@ -251,11 +238,11 @@ __device__ __forceinline__ void store_flux_side(const DevParams& P, const Patch<
 }
 
 template <typename T, int NS>
-__device__ __forceinline__ void store_flux(const DevParams& P, const Patch<T>& pt, const Mat<T>& m, int d,
-                                           const Slots<T, NS>& Fs, T rv, int i, int j)
+__device__ __forceinline__ void store_flux(const Patch<T>& pt, const Mat<T>& m, int d, const Slots<T, NS>& Fs, T rv,
+                                           int i, int j)
 {
-    store_flux_side<T, NS>(P, pt, m, d, Fs.v[0], Fs.s[0], (T)1, i, j, 0);
-    store_flux_side<T, NS>(P, pt, m, d, Fs.v[1], Fs.s[1], rv, i, j, 1);
+    store_flux_side<T, NS>(pt, m, d, Fs.v[0], Fs.s[0], (T)1, i, j, 0);
+    store_flux_side<T, NS>(pt, m, d, Fs.v[1], Fs.s[1], rv, i, j, 1);
 }
 
 // face states of one side on the four faces of a cell: lo/hi x face, lo/hi y face
@@ -273,6 +260,31 @@ template <typename T, int NS>
 __device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T>& m, double dt, T sc, bool plastic,
                                                  T ry, const Faces1<T, NS>& F, T (&u)[11])
 {
+    if constexpr (regroup<T>()) {
+        // the flux is linear in the face state: difference the face states first, then one
+        // uniform coefficient per component (-sc / rho, -sc lam2G, -sc lam, -sc G)
+        const T xs0 = F.xhs[0] - F.xls[0], xs1 = F.xhs[1] - F.xls[1];
+        const T xv0 = F.xhv[0] - F.xlv[0], xv1 = F.xhv[1] - F.xlv[1];
+        const T ys0 = F.yhs[0] - F.yls[0], ys1 = F.yhs[1] - F.yls[1];
+        const T yv0 = F.yhv[0] - F.ylv[0], yv1 = F.yhv[1] - F.ylv[1];
+        const T kv = -(sc * m.irho), kl2 = -(sc * m.lam2G), kl = -(sc * m.lam), kg = -(sc * m.G);
+        const T ryv0 = ry * yv0;
+        u[0] = fma(kv, fma(ry, ys1, xs0), u[0]);
+        u[1] = fma(kv, fma(ry, ys0, xs1), u[1]);
+        u[3] = fma(kl2, xv0, fma(kl, ryv0, u[3]));
+        u[4] = fma(kl, xv0, fma(kl2, ryv0, u[4]));
+        u[5] = fma(kl, xv0 + ryv0, u[5]);
+        u[6] = fma(kg, fma(ry, yv1, xv1), u[6]);
+        if (NS == 3) {
+            const T xs2 = F.xhs[NS - 1] - F.xls[NS - 1], ys2 = F.yhs[NS - 1] - F.yls[NS - 1];
+            const T xv2 = F.xhv[NS - 1] - F.xlv[NS - 1], yv2 = F.yhv[NS - 1] - F.ylv[NS - 1];
+            u[2] = fma(kv, fma(ry, ys2, xs2), u[2]);
+            u[7] = fma(kg, xv2, u[7]);
+            u[8] = fma(kg, ry * yv2, u[8]);
+        }
+        if (plastic) plastic_flow_delta<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
+        return;
+    }
     T fxl[7], fxh[7], fyl[7], fyh[7];
     face_flux_side<T, NS>(m, F.xlv, F.xls, (T)1, fxl);
     face_flux_side<T, NS>(m, F.xhv, F.xhs, (T)1, fxh);
@@ -301,12 +313,7 @@ __device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T
         u[7] = u[7] + (((T)0 + dsxz_x) + dsxz_y) * sc;
         u[8] = u[8] + (((T)0 + dsyz_x) + dsyz_y) * sc;
     }
-    if (plastic) {
-        if constexpr (stores_delta<T>())
-            plastic_flow_delta<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
-        else
-            plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
-    }
+    if (plastic) plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
 }
 
 // U += dU of side sd of valid cell (i, j) from the final states on its four faces, then the
@@ -710,8 +717,8 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
             }
         __syncthreads();
         // optional flux output for flux registers (LinElastPatchGodunov.cpp:117-118, LinElastPhysics.cpp:153)
-        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(P, pt, m, 0, F0, (T)1, i, j);
-        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(P, pt, m, 1, F1, rv1, i, j);
+        if (pt.F[0] && lx >= 1 && lx <= tnx + 1 && ly >= 1 && ly <= tny) store_flux<T, NS>(pt, m, 0, F0, (T)1, i, j);
+        if (pt.F[1] && ly >= 1 && ly <= tny + 1 && lx >= 1 && lx <= tnx) store_flux<T, NS>(pt, m, 1, F1, rv1, i, j);
         if (valid) {
             Slots<T, NS> F0h, F1h;
 #pragma unroll

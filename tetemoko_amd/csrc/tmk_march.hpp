@@ -41,7 +41,7 @@ __device__ __forceinline__ bool march_interior(const Tile& t, const DevParams& P
 template <typename T>
 __device__ __forceinline__ T mc_slope(T dl, T dr)
 {
-    return vl_limit((T)0.5 * (dr + dl), dl, dr);
+    return vl_limit_centred((T)0.5 * (dr + dl), dl, dr);
 }
 
 // PPM face value between cells a (lo) and b (hi) with slopes da, db (ppm_incr's fl / fh)
@@ -115,10 +115,28 @@ struct Pred2 {
     T mv[2], ms[2], pv[2], ps[2];
 };
 
+// what one row of the march hands to the next (r = the row that row loaded)
+template <typename T>
+struct MarchCarry {
+    T wn[5];               // W of row r + 1 (requested ahead)
+    T w1[5], w2[5], w3[5]; // W of rows r, r - 1, r - 2 (the next row's r - 1, r - 2, r - 3)
+    T dly[4];              // y differences W(r) - W(r - 1) of the y-sweep quantities
+    T dy2[4];              // y slopes of row r - 1
+    T fy[4];               // y face values at r - 3/2
+    Pred2<T> xp;           // x predictor of row r - 2, not yet transversely corrected
+    T h1v[2], h1s[2];      // predictor state of the lo y face of row r - 2
+    T cpu[2];              // right characteristics of row r - 2's uncorrected W+ (y)
+    T cpc[2];              // the same of its corrected W+
+    T f1v[2], f1s[2];      // final state of the lo y face of row r - 2
+};
+
 // grid: two workgroups of one wavefront per strip, side = blockIdx.x & 1 (a scalar, so every
-// plane pointer and descriptor of the side stays in scalar registers)
+// plane pointer and descriptor of the side stays in scalar registers).  FP32: at least four
+// wavefronts per SIMD (at most 128 VGPRs) keep enough loads in flight to cover their latency;
+// FP64 needs about twice the registers and is left to the compiler.
 template <typename T, int PK>
-__global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == sizeof(float) ? 4 : 1)))
+k_godunov_march(const KArgs<T> a)
 {
     static_assert(PK == PK_PPM_PRIM, "row marching carries PPM with primitive limiting");
     const Tile tl = a.tiles[blockIdx.x >> 1];
@@ -162,18 +180,13 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
     auto row_off = [&](int j) { return col + (unsigned)(j - pt.ulo1) * rowb; };
     const int y0 = tl.y0, y1 = tl.y1;
 
-    // carried from earlier rows (r = the row loaded in this iteration)
-    T w1[5], w2[5];   // W of rows r - 1, r - 2
-    T dly[4];         // y differences W(r - 1) - W(r - 2) of the y-sweep quantities
-    T dy2[4];         // y slopes of row r - 2
-    T fy[4];          // y face values at r - 5/2
-
     // warm-up: rows y0 - 3 .. y0 give the y slopes of rows y0 - 2, y0 - 1 and the face value at
     // y0 - 3/2 (what the first main iteration carries in)
+    MarchCarry<T> A, B;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) w1[q] = w2[q] = (T)0;
+    for (int q = 0; q < 5; ++q) A.w1[q] = A.w2[q] = A.w3[q] = (T)0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dly[q] = dy2[q] = fy[q] = (T)0;
+    for (int q = 0; q < 4; ++q) A.dly[q] = A.dy2[q] = A.fy[q] = (T)0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         T w0[5];
@@ -182,51 +195,44 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
         for (int q = 0; q < 5; ++q) w0[q] = bld<T>(rin[q], off);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const T dry = w0[MQ_Y[q]] - w1[MQ_Y[q]];
-            const T dy1 = mc_slope(dly[q], dry);
-            const T fy1 = ppm_face(w2[MQ_Y[q]], w1[MQ_Y[q]], dy2[q], dy1);
-            dly[q] = dry;
-            dy2[q] = dy1;
-            fy[q] = fy1;
+            const T dry = w0[MQ_Y[q]] - A.w1[MQ_Y[q]];
+            const T dy1 = mc_slope(A.dly[q], dry);
+            const T fy1 = ppm_face(A.w2[MQ_Y[q]], A.w1[MQ_Y[q]], A.dy2[q], dy1);
+            A.dly[q] = dry;
+            A.dy2[q] = dy1;
+            A.fy[q] = fy1;
         }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            w2[q] = w1[q];
-            w1[q] = w0[q];
+            A.w2[q] = A.w1[q];
+            A.w1[q] = w0[q];
         }
     }
-
-    T w3[5] = {(T)0, (T)0, (T)0, (T)0, (T)0};  // W of row r - 3
-    Pred2<T> xp;       // x predictor of row r - 3, not yet transversely corrected
-    T h1v[2], h1s[2];  // predictor state of the lo y face of row r - 3
-    T cpu[2];          // right characteristics of row r - 3's uncorrected W+ (y)
-    T cpc[2];          // the same of its corrected W+
-    T f1v[2], f1s[2];  // final state of the lo y face of row r - 3
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        xp.mv[k] = xp.ms[k] = xp.pv[k] = xp.ps[k] = (T)0;
-        h1v[k] = h1s[k] = cpu[k] = cpc[k] = f1v[k] = f1s[k] = (T)0;
+        A.xp.mv[k] = A.xp.ms[k] = A.xp.pv[k] = A.xp.ps[k] = (T)0;
+        A.h1v[k] = A.h1s[k] = A.cpu[k] = A.cpc[k] = A.f1v[k] = A.f1s[k] = (T)0;
     }
-
-    // main march: after loading row r, the predictors of row r - 2 and the update of row r - 3.
-    // The first iteration computes row y0 - 1's face states (used) and row y0 - 2's (not used).
-    // Loads run one iteration ahead: row r + 1's stencil planes (a ghost row of the array at the
-    // last iteration) and row r - 3's pointwise planes are requested at the top of iteration r,
-    // so the wavefront computes while they arrive.
-    T wn[5];
     {
         const unsigned off = row_off(y0 + 1);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) wn[q] = bld<T>(rin[q], off);
+        for (int q = 0; q < 5; ++q) A.wn[q] = bld<T>(rin[q], off);
     }
-    for (int r = y0 + 1; r <= y1 + 3; ++r) {
+
+    // one row of the march: after loading row r, the predictors of row r - 2 and the update of
+    // row r - 3, from what c carries in; n receives what the next row needs.  The first row
+    // computes row y0 - 1's face states (used) and row y0 - 2's (not used).  Loads run one row
+    // ahead: row r + 1's stencil planes (a ghost row of the array at the last row) and row r - 3's
+    // pointwise planes are requested before the row's arithmetic, so the wavefront computes while
+    // they arrive.
+    auto row = [&](const int r, const MarchCarry<T>& c, MarchCarry<T>& n) {
         T w0[5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) w0[q] = wn[q];
+        for (int q = 0; q < 5; ++q) w0[q] = c.wn[q];
         {
             const unsigned offn = row_off(r + 1);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) wn[q] = bld<T>(rin[q], offn);
+            for (int q = 0; q < 5; ++q) n.wn[q] = bld<T>(rin[q], offn);
         }
         const int j3 = r - 3;
         // row r - 3's pointwise planes (szz; sxz syz lam gam with plasticity); a lane outside the
@@ -237,22 +243,22 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
         T upw[5];
         upw[0] = bld<T>(rpw[0], off3);
 #pragma unroll
-        for (int c = 1; c < 5; ++c) upw[c] = plastic ? bld<T>(rpw[c], off3) : (T)0;
+        for (int q = 1; q < 5; ++q) upw[q] = plastic ? bld<T>(rpw[q], off3) : (T)0;
         // y: differences, slopes of row r - 1, face values at r - 3/2
         T dry[4], dy1[4], fy1[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            dry[q] = w0[MQ_Y[q]] - w1[MQ_Y[q]];
-            dy1[q] = mc_slope(dly[q], dry[q]);
-            fy1[q] = ppm_face(w2[MQ_Y[q]], w1[MQ_Y[q]], dy2[q], dy1[q]);
+            dry[q] = w0[MQ_Y[q]] - c.w1[MQ_Y[q]];
+            dy1[q] = mc_slope(c.dly[q], dry[q]);
+            fy1[q] = ppm_face(c.w2[MQ_Y[q]], c.w1[MQ_Y[q]], c.dy2[q], dy1[q]);
         }
         // y predictor of row r - 2 (its y faces r - 5/2 and r - 3/2)
         Pred2<T> yp;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            const T wv = w2[MQ_Y[k]], ws = w2[MQ_Y[2 + k]];
-            const T dvm = ((T)0 - wv) + fy[k], dvp = ((T)0 - wv) + fy1[k];
-            const T dsm = ((T)0 - ws) + fy[2 + k], dsp = ((T)0 - ws) + fy1[2 + k];
+            const T wv = c.w2[MQ_Y[k]], ws = c.w2[MQ_Y[2 + k]];
+            const T dvm = incr(c.fy[k], wv), dvp = incr(fy1[k], wv);
+            const T dsm = incr(c.fy[2 + k], ws), dsp = incr(fy1[2 + k], ws);
             ppm_slot<T>(m, k, rvy, dtbydx, true, false, wv, ws, dvm, dvp, dsm, dsp, yp.mv[k], yp.ms[k], yp.pv[k],
                         yp.ps[k]);
         }
@@ -263,18 +269,18 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
             T inc[2][2];  // [v|s][m|p]
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const T wc = w2[h == 0 ? MQ_XV[k] : MQ_XS[k]];
+                const T wc = c.w2[h == 0 ? MQ_XV[k] : MQ_XS[k]];
                 const T wr = dshift<DPP_FROM_NEXT>(wc);  // W(i + 1)
                 const T dr = wr - wc;
                 const T dl = dshift<DPP_FROM_PREV>(dr);  // W(i) - W(i - 1)
                 const T d0 = mc_slope(dl, dr);
                 const T fh = ppm_face(wc, wr, d0, dshift<DPP_FROM_NEXT>(d0));  // face value at i + 1/2
                 const T fl = dshift<DPP_FROM_PREV>(fh);                        // at i - 1/2
-                inc[h][0] = ((T)0 - wc) + fl;
-                inc[h][1] = ((T)0 - wc) + fh;
+                inc[h][0] = incr(fl, wc);
+                inc[h][1] = incr(fh, wc);
             }
-            ppm_slot<T>(m, k, (T)1, dtbydx, true, false, w2[MQ_XV[k]], w2[MQ_XS[k]], inc[0][0], inc[0][1], inc[1][0],
-                        inc[1][1], xn.mv[k], xn.ms[k], xn.pv[k], xn.ps[k]);
+            ppm_slot<T>(m, k, (T)1, dtbydx, true, false, c.w2[MQ_XV[k]], c.w2[MQ_XS[k]], inc[0][0], inc[0][1],
+                        inc[1][0], inc[1][1], xn.mv[k], xn.ms[k], xn.pv[k], xn.ps[k]);
         }
         // predictor state of the lo x face of row r - 2 (its left lane's W+ and its W-)
         T h0v[2], h0s[2];
@@ -287,12 +293,11 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
             riemann_side<T, 2>(m, (T)1, wpL, wmR, h0v, h0s);
         }
         // predictor state of the lo y face of row r - 2 (row r - 3's uncorrected W+, its W-)
-        T h1nv[2], h1ns[2], cpun[2];
         {
             T wmR[2];
             left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
-            riemann_side<T, 2>(m, rvy, cpu, wmR, h1nv, h1ns);
-            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, cpun);
+            riemann_side<T, 2>(m, rvy, c.cpu, wmR, n.h1v, n.h1s);
+            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, n.cpu);
         }
         // y predictor corrected with the x-face states of its cell (GETADWDXF, dir2 = x)
         {
@@ -309,8 +314,9 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
         // final x-face state
         T f0v[2], f0s[2];
         {
-            transverse_side<T, 2>(m, scx, h1v, h1s, h1nv, h1ns, xp.mv, xp.ms);
-            transverse_side<T, 2>(m, scx, h1v, h1s, h1nv, h1ns, xp.pv, xp.ps);
+            Pred2<T> xp = c.xp;
+            transverse_side<T, 2>(m, scx, c.h1v, c.h1s, n.h1v, n.h1s, xp.mv, xp.ms);
+            transverse_side<T, 2>(m, scx, c.h1v, c.h1s, n.h1v, n.h1s, xp.pv, xp.ps);
             T cpx[2], wpL[2], wmR[2];
             right_chars_side<T, 2>(m, (T)1, xp.pv, xp.ps, cpx);
 #pragma unroll
@@ -319,20 +325,19 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
             riemann_side<T, 2>(m, (T)1, wpL, wmR, f0v, f0s);
         }
         // final state of the lo y face of row r - 2 (row r - 3's corrected W+, its corrected W-)
-        T f1nv[2], f1ns[2], cpcn[2];
         {
             T wmR[2];
             left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
-            riemann_side<T, 2>(m, rvy, cpc, wmR, f1nv, f1ns);
-            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, cpcn);
+            riemann_side<T, 2>(m, rvy, c.cpc, wmR, n.f1v, n.f1s);
+            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, n.cpc);
         }
         const int j2 = r - 2;
         // flux registers (multi-level): x faces of row r - 3, the lo y face of row r - 2
         if (pt.F[0] && j3 >= y0 && lane >= 3 && lane <= tnx + 3)
-            store_flux_side<T, 2>(P, pt, m, 0, f0v, f0s, (T)1, i, j3, sd);
-        if (pt.F[1] && j2 >= y0 && valid) store_flux_side<T, 2>(P, pt, m, 1, f1nv, f1ns, rvy, i, j2, sd);
+            store_flux_side<T, 2>(pt, m, 0, f0v, f0s, (T)1, i, j3, sd);
+        if (pt.F[1] && j2 >= y0 && valid) store_flux_side<T, 2>(pt, m, 1, n.f1v, n.f1s, rvy, i, j2, sd);
         // update of row r - 3
-        {
+        if (j3 >= y0) {
             Faces1<T, 2> F;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -340,56 +345,55 @@ __global__ void __launch_bounds__(64) k_godunov_march(const KArgs<T> a)
                 F.xls[k] = f0s[k];
                 F.xhv[k] = dshift<DPP_FROM_NEXT>(f0v[k]);
                 F.xhs[k] = dshift<DPP_FROM_NEXT>(f0s[k]);
-                F.ylv[k] = f1v[k];
-                F.yls[k] = f1s[k];
-                F.yhv[k] = f1nv[k];
-                F.yhs[k] = f1ns[k];
+                F.ylv[k] = c.f1v[k];
+                F.yls[k] = c.f1s[k];
+                F.yhv[k] = n.f1v[k];
+                F.yhs[k] = n.f1s[k];
             }
-            if (j3 >= y0) {
-                // every lane runs the update (a lane outside the strip has off3 past the planes)
-                T u[11];
-                u[0] = w3[0];
-                u[1] = w3[1];
-                u[3] = w3[2];
-                u[4] = w3[3];
-                u[6] = w3[4];
-                u[2] = (T)0;
-                u[5] = upw[0];
+            // every lane runs the update (a lane outside the strip has off3 past the planes)
+            T u[11];
+            u[0] = c.w3[0];
+            u[1] = c.w3[1];
+            u[3] = c.w3[2];
+            u[4] = c.w3[3];
+            u[6] = c.w3[4];
+            u[2] = (T)0;
+            u[5] = upw[0];
 #pragma unroll
-                for (int c = 7; c <= 10; ++c) u[c] = upw[c - 6];
-                cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
+            for (int q = 7; q <= 10; ++q) u[q] = upw[q - 6];
+            cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
 #pragma unroll
-                for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
-                bst<T>(rpw[0], off3, u[5]);
-                if (plastic) {
+            for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
+            bst<T>(rpw[0], off3, u[5]);
+            if (plastic) {
 #pragma unroll
-                    for (int c = 7; c <= 10; ++c) bst<T>(rpw[c - 6], off3, u[c]);
-                }
+                for (int q = 7; q <= 10; ++q) bst<T>(rpw[q - 6], off3, u[q]);
             }
         }
-        // carry to the next row
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            h1v[k] = h1nv[k];
-            h1s[k] = h1ns[k];
-            cpu[k] = cpun[k];
-            cpc[k] = cpcn[k];
-            f1v[k] = f1nv[k];
-            f1s[k] = f1ns[k];
-        }
-        xp = xn;
+        // carried to the next row
+        n.xp = xn;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            dly[q] = dry[q];
-            dy2[q] = dy1[q];
-            fy[q] = fy1[q];
+            n.dly[q] = dry[q];
+            n.dy2[q] = dy1[q];
+            n.fy[q] = fy1[q];
         }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            w3[q] = w2[q];
-            w2[q] = w1[q];
-            w1[q] = w0[q];
+            n.w3[q] = c.w2[q];
+            n.w2[q] = c.w1[q];
+            n.w1[q] = w0[q];
         }
+    };
+
+    // two rows per trip, the carried state alternating between A and B (no copies between rows;
+    // the DPP lane shifts are convergent, so the compiler does not unroll the loop itself)
+    const int rlast = y1 + 3;
+    for (int r = y0 + 1;; r += 2) {
+        row(r, A, B);
+        if (r + 1 > rlast) break;
+        row(r + 1, B, A);
+        if (r + 2 > rlast) break;
     }
 }
 

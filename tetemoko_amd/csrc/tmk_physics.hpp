@@ -66,6 +66,8 @@ struct DevParams {
     // the background in the form the FP32 Drucker-Prager return uses: mean stress, deviators
     // xx yy zz, then sxy sxz syz (rounded from FP64 on the host)
     float pbg[7];
+    // FP32 plasticity constants: mu, cohesion b, hardening h, h + G + mu beta K, eta, K beta
+    float pf[6];
     int plast;
     double p_mu, p_beta, p_eta, p_b, p_h;
     int bc[4];
@@ -97,6 +99,25 @@ template <typename T>
 __host__ __device__ constexpr bool stores_delta()
 {
     return sizeof(T) == sizeof(float);
+}
+
+// FP32 bulk arithmetic regroups some expressions into fewer instructions (constants folded
+// first, differences taken before products); FP64 keeps the reference's operation order so that
+// it rounds like the reference.
+template <typename T>
+__host__ __device__ constexpr bool regroup()
+{
+    return sizeof(T) == sizeof(float);
+}
+
+// f - w: the reference's (0 - w) + f in FP64
+template <typename T>
+__device__ __forceinline__ T incr(T f, T w)
+{
+    if constexpr (regroup<T>())
+        return f - w;
+    else
+        return ((T)0 - w) + f;
 }
 
 template <typename T, int NS>
@@ -153,6 +174,7 @@ struct Mat {
         return k == 0 ? divr(x, (T)2 * Zp, i2Zp) : divr(x, (T)2 * Zs, i2Zs);
     }
     __device__ __forceinline__ T divrho(T x) const { return divr(x, rho, irho); }
+    __device__ __forceinline__ T i2Z(int k) const { return k == 0 ? i2Zp : i2Zs; }
 };
 
 // component index of (slot, velocity/stress) for direction d, side 1
@@ -175,6 +197,21 @@ __device__ __forceinline__ T vl_limit(T c, T l, T r)
     lim = fmin(lim, fabs(c));
     if (l * r < (T)0) lim = (T)0;
     return copysign(lim, c);
+}
+
+// vl_limit for c = (l + r) / 2 (every slope vl_slope and the row-marching kernel limit): then
+// sign(c) = sign(l) = sign(r) whenever l r > 0 and the limited slope is minmod(2 l, 2 r, c).  FP32
+// forms it from medians with zero: med3(a, b, 0) is whichever of a, b lies nearer zero when they
+// share a sign and 0 otherwise (doubling is exact).
+template <typename T>
+__device__ __forceinline__ T vl_limit_centred(T c, T l, T r)
+{
+    if constexpr (regroup<T>()) {
+        const T lr = (T)2 * __builtin_amdgcn_fmed3f(l, r, (T)0);
+        return __builtin_amdgcn_fmed3f(lr, c, (T)0);
+    } else {
+        return vl_limit(c, l, r);
+    }
 }
 
 // side classification of stencil cell: 0 centred, 1 domain lo, 2 domain hi
@@ -202,7 +239,7 @@ __device__ __forceinline__ T vl_slope(const T* w, int q, int side, bool limit)
 {
     T C, L, R;
     slope_diffs(w, q, side, C, L, R);
-    return limit ? vl_limit(C, L, R) : C;
+    return limit ? vl_limit_centred(C, L, R) : C;
 }
 
 // PPM face increments (face value minus cell value) at the lo and hi faces of
@@ -226,8 +263,8 @@ __device__ __forceinline__ void ppm_incr(const T* w, int ci, int dlo, int dhi, b
         const T dpp = vl_slope(w, 3, cell_side(ci + 1, dlo, dhi), limit);
         fh = (T)0.5 * (w[3] + w[2]) + sixth * (d0 - dpp);
     }
-    dm = ((T)0 - w[2]) + fl;
-    dp = ((T)0 - w[2]) + fh;
+    dm = incr(fl, w[2]);
+    dp = incr(fh, w[2]);
 }
 
 // branch-free: both candidates are formed and one is selected (the same values as the
@@ -235,6 +272,15 @@ __device__ __forceinline__ void ppm_incr(const T* w, int ci, int dlo, int dhi, b
 template <typename T>
 __device__ __forceinline__ void ppm_limit(T& m, T& p)
 {
+    if constexpr (regroup<T>()) {
+        // m, -2p share a sign exactly when m p < 0: the median with zero is then the one nearer
+        // zero, copysign(min(|m|, 2|p|), m), and 0 otherwise
+        const T nm = __builtin_amdgcn_fmed3f(m, (T)-2 * p, (T)0);
+        const T np = __builtin_amdgcn_fmed3f(p, (T)-2 * m, (T)0);
+        m = nm;
+        p = np;
+        return;
+    }
     const bool keep = m * p < (T)0;
     const T nm = copysign(fmin((T)2 * fabs(p), fabs(m)), m);
     const T np = copysign(fmin((T)2 * fabs(m), fabs(p)), p);
@@ -253,6 +299,29 @@ __device__ __forceinline__ void ppm_slot(const Mat<T>& m, int k, T rvs, T dtbydx
     if (limit && !charl) {
         ppm_limit(dvm, dvp);
         ppm_limit(dsm, dsp);
+    }
+    if constexpr (regroup<T>()) {
+        // the same tracing and synthesis with the constants folded: m + sig ((p - m) - (p + m)
+        // (3 - 2 sig)) / 2 = m (1 - c1 - c2) + p (c1 - c2), c1 = sig / 2, c2 = c1 (3 - 2 sig);
+        // the velocity of a characteristic pair is (cm - cp) rvs / (2 Z)
+        const T zr = rvs * m.Z(k);
+        T cm_m = fma(dvm, zr, dsm), cp_m = fma(-dvm, zr, dsm);
+        T cm_p = fma(dvp, zr, dsp), cp_p = fma(-dvp, zr, dsp);
+        if (charl) {
+            ppm_limit(cm_m, cm_p);
+            ppm_limit(cp_m, cp_p);
+        }
+        const T sig = (k == 0 ? m.cp : m.cs) * dtbydx;
+        const T c1 = (T)0.5 * sig, c2 = c1 * ((T)3 - (T)2 * sig);
+        const T ta = ((T)1 - c1) - c2, tb = c1 - c2;
+        const T tm = fma(cm_m, ta, cm_p * tb);  // traced left-moving wave on the lo face
+        const T tp = fma(cp_p, ta, cp_m * tb);  // traced right-moving wave on the hi face
+        const T h = rvs * m.i2Z(k);
+        Wmv = fma(tm - cp_m, h, wv);
+        Wpv = fma(cm_p - tp, h, wv);
+        Wms = fma(tm + cp_m, (T)0.5, ws);
+        Wps = fma(cm_p + tp, (T)0.5, ws);
+        return;
     }
     const T Z = m.Z(k);
     const T vm = rvs * dvm;
@@ -425,8 +494,13 @@ __device__ __forceinline__ void riemann_side(const Mat<T>& m, T rvs, const T (&w
 {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        v[k] = m.divZ(rvs * (wm[k] - wp[k]) / (T)2, k);
-        s[k] = (wm[k] + wp[k]) / (T)2;
+        if constexpr (regroup<T>()) {
+            v[k] = (wm[k] - wp[k]) * (rvs * m.i2Z(k));
+            s[k] = (wm[k] + wp[k]) * (T)0.5;
+        } else {
+            v[k] = m.divZ(rvs * (wm[k] - wp[k]) / (T)2, k);
+            s[k] = (wm[k] + wp[k]) / (T)2;
+        }
     }
 }
 
@@ -495,6 +569,16 @@ __device__ __forceinline__ void transverse_side(const Mat<T>& m, T sc, const T (
 {
     const T dNv = Hhiv[0] - Hlov[0], dNs = Hhis[0] - Hlos[0];
     const T dSv = Hhiv[1] - Hlov[1], dSs = Hhis[1] - Hlos[1];
+    if constexpr (regroup<T>()) {
+        // sc / rho, sc lam, sc G are uniform: one fma per component
+        const T kr = -(sc * m.irho);
+        Wv[0] = fma(kr, dSs, Wv[0]);
+        Ws[0] = fma(-(sc * m.lam), dNv, Ws[0]);
+        Wv[1] = fma(kr, dNs, Wv[1]);
+        Ws[1] = fma(-(sc * m.G), dSv, Ws[1]);
+        if (NS == 3) Wv[NS - 1] = fma(kr, Hhis[NS - 1] - Hlos[NS - 1], Wv[NS - 1]);
+        return;
+    }
     Wv[0] = Wv[0] + (-m.divrho(sc * dSs));
     Ws[0] = Ws[0] + (-(sc * dNv * m.lam));
     Wv[1] = Wv[1] + (-m.divrho(sc * dNs));
@@ -710,22 +794,23 @@ template <typename T>
 __device__ __forceinline__ void plastic_flow_delta(const DevParams& P, T dt, T& dxx, T& dyy, T& dzz, T& dxy, T& dxz,
                                                    T& dyz, T& plam, T& pgam)
 {
-    const T md = (dxx + dyy + dzz) / (T)3;
+    const T md = (dxx + dyy + dzz) * (T)(1.0 / 3.0);
     const T exx = (dxx - md) + (T)P.pbg[1], eyy = (dyy - md) + (T)P.pbg[2], ezz = (dzz - md) + (T)P.pbg[3];
     const T sxy = dxy + (T)P.pbg[4], sxz = dxz + (T)P.pbg[5], syz = dyz + (T)P.pbg[6];
     const T sigma = md + (T)P.pbg[0];
     const T a = exx - eyy, b = eyy - ezz, c = ezz - exx;
-    const T J2 = (a * a + b * b + c * c) / (T)6 + sxy * sxy + sxz * sxz + syz * syz;
+    const T J2 = (a * a + b * b + c * c) * (T)(1.0 / 6.0) + sxy * sxy + sxz * sxz + syz * syz;
+    const T bb = (T)P.pf[1] + pgam * (T)P.pf[2];
+    const T ys = fmax(-(T)P.pf[0] * sigma + bb, (T)0);
+    // yield iff sqrt(J2) > ys >= 0, i.e. J2 > ys^2: the root only where the cell yields
+    if (!(J2 > ys * ys)) { plam = (T)0; return; }
     const T tau = sqrt(J2);
-    const T bb = (T)P.p_b + pgam * (T)P.p_h;
-    const T Y = tau - fmax(-(T)P.p_mu * sigma + bb, (T)0);
-    if (Y <= (T)0) { plam = (T)0; return; }
-    const T G = (T)P.G, K = (T)P.K;
-    plam = Y / (dt * ((T)P.p_h + G + (T)P.p_mu * (T)P.p_beta * K) + (T)P.p_eta);
+    const T Y = tau - ys;
+    plam = Y / (dt * (T)P.pf[3] + (T)P.pf[4]);
     pgam = pgam + dt * plam;
-    const T dG = dt * plam * G;
-    const T g = dG / ((tau - dG) + dG);
-    const T dm = dt * plam * K * (T)P.p_beta;
+    const T dG = dt * plam * (T)P.fG;
+    const T g = dG / tau;  // 1 - tau_new / (tau_new + dt lam G), tau_new = tau - dt lam G
+    const T dm = dt * plam * (T)P.pf[5];
     dxx = dxx - (dm + g * exx);
     dyy = dyy - (dm + g * eyy);
     dzz = dzz - (dm + g * ezz);

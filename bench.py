@@ -60,7 +60,10 @@ def parse():
                     help="boxes of at most this many cells per side (the reference's default max_grid_size "
                          "is 32); 0 = the level as one box")
     ap.add_argument("--march", type=int, default=-1,
-                    help="strip rows of the row-marching interior update (0: tile kernels; default: the library's)")
+                    help="strip rows of the row-marching interior update (0: tile kernels; default: the library's); "
+                         "implies --no-tune")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default update shape instead of autotuning it per level")
     ap.add_argument("--acc-steps", type=int, default=4000, help="steps of the accuracy leg (0: skip it)")
     ap.add_argument("--no-c1", action="store_true", help="skip the secondary 4096x2048 FP64 drop-in leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -68,18 +71,18 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(workload, precision, tile):
+def measured_traffic(workload, precision, shape):
     """HBM bytes per launch of the interior update kernel from the committed counter runs
     (profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
-    copy-kernel calibration), if one was taken on this workload, precision and tile."""
+    copy-kernel calibration), if one was taken on this workload, precision and update shape
+    ("march32", "tile32x16", ...)."""
     d = os.path.join(ROOT, "profiles")
     for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
         if not (name.startswith("traffic_") and name.endswith(".json")):
             continue
         with open(os.path.join(d, name)) as f:
             t = json.load(f)
-        if t.get("workload") == workload and t.get("precision") == precision and \
-                t.get("tile") == (list(tile) if tile else "default"):
+        if t.get("workload") == workload and t.get("precision") == precision and t.get("shape") == shape:
             return t.get("bytes_per_launch"), "profiles/" + name
     return None, None
 
@@ -109,12 +112,19 @@ def make_level(cfg, precision, boxes, tile):
     return ctx, lev
 
 
+TUNE = True  # autotune each level's update shape before it is timed
+
+
 def timed_leg(cfg, precision, boxes, warmup, steps, tile):
-    """W untimed graph-replayed steps, then exactly K graph-replayed steps bracketed by HIP
-    events on the level's stream (after a device synchronisation), then per-launch durations of
-    the update kernels inside a replayed graph (HIP events on each kernel's stream)."""
+    """The level's update shape autotuned from its initial state (tmk_level_autotune: every
+    strip height and tile shape timed, the level returned to that state), then W untimed
+    graph-replayed steps, then exactly K graph-replayed steps bracketed by HIP events on the
+    level's stream (after a device synchronisation), then per-launch durations of the update
+    kernels inside a replayed graph (stamps on each kernel's stream)."""
     ctx, lev = make_level(cfg, precision, boxes, tile)
     dt = scenarios.dt_of(cfg)
+    tuning = lev.autotune(0.0, dt, nsteps=10) if TUNE else None
+    shape = lev.update_shape()
     t = 0.0
     lev.run(warmup, t, dt)
     t += warmup * dt
@@ -127,16 +137,32 @@ def timed_leg(cfg, precision, boxes, warmup, steps, tile):
     lev.close()
     ctx.close()
     ncell = cfg["nx"] * cfg["ny"]
-    return {"ms_per_step": ms / steps, "value": ncell * steps / (ms * 1e-3), "profile": prof}
+    prof["march_rows"], prof["bx"], prof["by"] = shape
+    return {"ms_per_step": ms / steps, "value": ncell * steps / (ms * 1e-3), "profile": prof, "shape": shape,
+            "tuning": tuning}
 
 
-def roofline(prof, plastic, tsize, workload, precision, tile):
+def shape_record(r):
+    """The update shape a timed leg ran, and what the tuner measured for each candidate."""
+    rows, bx, by = r["shape"]
+    out = {"march_rows": rows, "tile": [bx, by], "autotuned": r["tuning"] is not None}
+    if r["tuning"] is not None:
+        out["tried_ms_per_step"] = {("march%d" % t[0]) if t[0] else ("tile%dx%d" % (t[1], t[2])): round(t[3], 4)
+                                    for t in r["tuning"]["tried"]}
+    return out
+
+
+def roofline(prof, plastic, tsize, workload, precision):
     bpc = stored_bytes(plastic, tsize)
     achieved = prof["cells_interior"] * bpc / (prof["ms_interior"] * 1e-3) / 1e9
-    traffic, src = measured_traffic(workload, precision, tile)
+    rows, bx, by = prof["march_rows"], prof["bx"], prof["by"]
+    shape = ("march%d" % rows) if rows else ("tile%dx%d" % (bx, by))
+    traffic, src = measured_traffic(workload, precision, shape)
+    kernel = ("k_godunov_march (interior strips of 58 x %d)" % rows) if rows else \
+        ("k_godunov_interior (interior tiles %dx%d)" % (bx, by))
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
-            "kernel": "k_godunov_interior (interior tiles)", "kernel_ms": prof["ms_interior"],
+            "kernel": kernel, "kernel_ms": prof["ms_interior"],
             "kernel_ms_timed_in_graph": bool(prof["in_graph"]),
             "cells_per_launch": prof["cells_interior"], "bytes_per_cell_update": bpc,
             "bytes_per_cell_update_22comp": 22 * 2 * tsize,
@@ -157,10 +183,12 @@ def station_faces(cfg, n=64, half_width_m=4500.0, seed=1234):
     return faces, (faces + 0.5) * dx
 
 
-def accuracy_leg(cfg, precision, boxes, nsteps, xs, tile, chunk=500):
-    """nsteps steps from t = 0 with fault stations recorded on the device after every step:
-    final slip, peak slip rate (over every step) and rupture time at each station."""
+def accuracy_leg(cfg, precision, boxes, nsteps, xs, tile, shape, chunk=500):
+    """nsteps steps from t = 0, in the update shape the timed leg of this precision ran, with
+    fault stations recorded on the device after every step: final slip, peak slip rate (over
+    every step) and rupture time at each station."""
     ctx, lev = make_level(cfg, precision, boxes, tile)
+    lev.set_update_shape(*shape)
     st = capi.Stations(lev, xs, capacity=chunk)
     dt = scenarios.dt_of(cfg)
     peak = np.zeros(len(xs))
@@ -265,8 +293,9 @@ def main():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(3)
     tile = tuple(int(x) for x in a.tile.split("x")) if a.tile else None
-    global MARCH
+    global MARCH, TUNE
     MARCH = a.march
+    TUNE = not a.no_tune and a.march < 0 and not a.tile
     cfg = scenarios.config_c3(seed=0, nx=a.nx, ny=a.ny, dx=a.dx)
     boxes = level_boxes(cfg["nx"], cfg["ny"], a.max_grid_size)
     wl = "rs-aging+drucker-prager %dx%d dx=%g" % (cfg["nx"], cfg["ny"], cfg["dx"])
@@ -296,17 +325,17 @@ def main():
                    "dx_m": cfg["dx"], "dt_s": scenarios.dt_of(cfg), "normal_predictor": "PPM",
                    "plasticity": {"mu": cfg["p_mu"], "beta": cfg["p_beta"], "eta_MPa_s": cfg["p_eta"],
                                   "cohesion_MPa": cfg["p_b"]},
-                   "tile": list(tile) if tile else "default", "march_rows": a.march if a.march >= 0 else "default"},
-        "roofline": roofline(r32["profile"], True, 4, wl, "fp32_bulk", tile),
+                   "update_shape_fp32_bulk": shape_record(r32), "update_shape_fp64": shape_record(r64)},
+        "roofline": roofline(r32["profile"], True, 4, wl, "fp32_bulk"),
         "fp64": {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
-                 "roofline": roofline(r64["profile"], True, 8, wl, "fp64", tile)},
+                 "roofline": roofline(r64["profile"], True, 8, wl, "fp64")},
         "speedup_fp32_bulk_vs_fp64": r64["ms_per_step"] / r32["ms_per_step"],
     }
     if a.acc_steps > 0:
         faces, xs = station_faces(cfg)
         print("bench: accuracy leg, %d steps per precision, 64 stations" % a.acc_steps, file=sys.stderr, flush=True)
-        s64 = accuracy_leg(cfg, capi.PREC_FP64, boxes, a.acc_steps, xs, tile)
-        s32 = accuracy_leg(cfg, capi.PREC_FP32_BULK, boxes, a.acc_steps, xs, tile)
+        s64 = accuracy_leg(cfg, capi.PREC_FP64, boxes, a.acc_steps, xs, tile, r64["shape"])
+        s32 = accuracy_leg(cfg, capi.PREC_FP32_BULK, boxes, a.acc_steps, xs, tile, r32["shape"])
         r64t, r32t = s64["rupture"], s32["rupture"]
         both = (r64t < 1e9) & (r32t < 1e9)
         cells = np.abs(r64t[both] - r32t[both]) * cfg["cs"] / cfg["dx"]
@@ -336,10 +365,11 @@ def main():
         out["secondary_4096x2048_rs"] = {
             "workload": "rate-and-state (aging law), single level 4096x2048 in %d boxes, dx=25 m, PPM, no "
                         "plasticity" % len(b1),
-            "fp64": {"value": q64["value"], "ms_per_step": q64["ms_per_step"],
-                     "roofline": roofline(q64["profile"], False, 8, wl1, "fp64", tile)},
+            "fp64": {"value": q64["value"], "ms_per_step": q64["ms_per_step"], "update_shape": shape_record(q64),
+                     "roofline": roofline(q64["profile"], False, 8, wl1, "fp64")},
             "fp32_bulk": {"value": q32["value"], "ms_per_step": q32["ms_per_step"],
-                          "roofline": roofline(q32["profile"], False, 4, wl1, "fp32_bulk", tile)},
+                          "update_shape": shape_record(q32),
+                          "roofline": roofline(q32["profile"], False, 4, wl1, "fp32_bulk")},
         }
     if not a.no_dropin:
         print("bench: per-patch drop-in leg", file=sys.stderr, flush=True)

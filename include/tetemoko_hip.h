@@ -340,6 +340,29 @@ int tmk_set_tile(tmk_ctx *ctx, int bx, int by);
  * (bit-identical in FP64).  New contexts start at 32 rows. */
 int tmk_set_march(tmk_ctx *ctx, int rows);
 
+/* A level's own update shape (no reference counterpart; levels start with their context's):
+ * march_rows as tmk_set_march, (bx, by) as tmk_set_tile.  Not while the level is in a batch or
+ * hierarchy; members of one batch or hierarchy level need equal shapes.  get reports the
+ * shape the update runs (march_rows 0 where the numerics keep the tile kernels). */
+int tmk_level_set_update_shape(tmk_level *lev, int march_rows, int bx, int by);
+int tmk_level_get_update_shape(const tmk_level *lev, int *march_rows, int *bx, int *by);
+
+/* Autotuning of a level's update shape (no reference counterpart).  Times nsteps
+ * graph-replayed steps from (time, dt), after two untimed ones, for every strip height of the
+ * row-marching update (8 .. 64 rows, where the numerics allow it) and every tile shape, each
+ * from the level's current state, then sets the fastest shape and returns the level to the
+ * state it started from (a device copy of the state is held meanwhile).  Not while the level
+ * is in a batch or hierarchy or has fault stations. */
+#define TMK_TUNE_MAX 32
+typedef struct {
+    int march_rows, bx, by;  /* the shape chosen and set */
+    double ms_per_step;      /* its mean step time */
+    int ntried;
+    int march_rows_tried[TMK_TUNE_MAX], bx_tried[TMK_TUNE_MAX], by_tried[TMK_TUNE_MAX];
+    double ms_tried[TMK_TUNE_MAX];
+} tmk_tuning;
+int tmk_level_autotune(tmk_level *lev, double time, double dt, int nsteps, tmk_tuning *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -742,3 +742,82 @@ def test_march_falls_back_for_other_numerics():
     (Ug, Ur), (Bg, Br) = _level_case(cfg, [(0, 0, 119, 79)], 3, seed=44, march=16)
     assert np.all(_relerr(Ug, Ur) <= TOL)
     assert np.all(fault_errors(Bg, Br) <= TOL)
+
+
+# ---------------------------------------------------------------------------
+# update shapes of a level and their autotuning (tmk_level_set_update_shape, tmk_level_autotune)
+# ---------------------------------------------------------------------------
+def _uploaded_level(cfg, boxes, precision, seed):
+    U, B = _state(cfg, seed)
+    ctx = capi.Context(cfg, precision)
+    lev = capi.Level(ctx, boxes)
+    for k, b in enumerate(boxes):
+        lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+    return ctx, lev, U, B
+
+
+def _downloaded(lev, boxes, U, B):
+    Ug, Bg = np.full_like(U, np.nan), np.full_like(B, np.nan)
+    for k, b in enumerate(boxes):
+        ub = scenarios.box_view(U, b)
+        bb = scenarios.box_bview(B, b) if b[1] == 0 else None
+        lev.download(k, ub, bb)
+        x0, y0, x1, y1 = b
+        Ug[:, y0 + G:y1 + G + 1, x0 + G:x1 + G + 1] = ub[:, G:-G, G:-G]
+        if bb is not None:
+            Bg[:, x0 + G:x1 + G + 1] = bb[:, G:-G]
+    return Ug[:, G:-G, G:-G], Bg[:, G:-G]
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+def test_autotune_restores_state_and_sets_fastest(precision):
+    """Every strip height and tile shape is timed from the level's state; the level then holds
+    that state again, bit for bit, and runs the fastest shape.  FP64: the tuned level steps to
+    the same state as an untuned one, bit for bit (every shape evaluates the same expressions;
+    slip weakening has no transcendental function whose library code could differ between
+    kernels); FP32 bulk: within 1e-5 of each component's largest magnitude."""
+    cfg = _cfg("sw", "PPM", nx=200, ny=120)
+    boxes = scenarios.split_boxes(200, 120, 40)
+    dt = scenarios.dt_of(cfg)
+    ctx, lev, U, B = _uploaded_level(cfg, boxes, precision, seed=51)
+    before = _downloaded(lev, boxes, U, B)
+    r = lev.autotune(0.0, dt, nsteps=3)
+    after = _downloaded(lev, boxes, U, B)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    tried = r["tried"]
+    assert len(tried) == 18  # 10 strip heights, 8 tile shapes
+    assert r["ms_per_step"] == min(t[3] for t in tried) > 0.0
+    assert (r["march_rows"], r["bx"], r["by"]) == lev.update_shape()
+    assert (r["march_rows"], r["bx"], r["by"], r["ms_per_step"]) in [tuple(t) for t in tried]
+    lev.run(4, 0.0, dt)
+    lev.sync()
+    tuned = _downloaded(lev, boxes, U, B)
+    lev.close()
+    ctx.close()
+    ctx, lev, U, B = _uploaded_level(cfg, boxes, precision, seed=51)
+    lev.set_update_shape(0, 32, 8)
+    lev.run(4, 0.0, dt)
+    lev.sync()
+    plain = _downloaded(lev, boxes, U, B)
+    lev.close()
+    ctx.close()
+    if precision == capi.PREC_FP64:
+        assert np.array_equal(tuned[0], plain[0]) and np.array_equal(tuned[1], plain[1])
+    else:
+        assert np.all(_relerr(tuned[0], plain[0]) <= 1e-5), _relerr(tuned[0], plain[0])
+
+
+def test_update_shape_arguments():
+    """Shapes outside the kernels' set are refused; numerics without the marching update
+    report the tile kernels."""
+    cfg = _cfg("rs", "PLM", nx=96, ny=64)
+    ctx = capi.Context(cfg, capi.PREC_FP64)
+    lev = capi.Level(ctx, [(0, 0, 95, 63)])
+    with pytest.raises(capi.TmkError):
+        lev.set_update_shape(16, 24, 24)
+    with pytest.raises(capi.TmkError):
+        lev.set_update_shape(-1, 32, 8)
+    lev.set_update_shape(16, 64, 8)
+    assert lev.update_shape() == (0, 64, 8)  # PLM keeps the tile kernels
+    lev.close()
+    ctx.close()

@@ -44,14 +44,23 @@ EXPORTED = (
     "tmk_stations_destroy", "tmk_amr_create", "tmk_amr_destroy", "tmk_amr_run", "tmk_amr_run_timed",
     "tmk_amr_sync", "tmk_make_boxes", "tmk_level_regrid", "tmk_amr_batch_create", "tmk_amr_add",
     "tmk_amr_remove", "tmk_amr_advance", "tmk_amr_member_error", "tmk_patch_update_state_noalloc",
-    "tmk_set_march",
+    "tmk_set_march", "tmk_level_set_update_shape", "tmk_level_get_update_shape", "tmk_level_autotune",
 )
+
+TUNE_MAX = 32
 
 
 class Profile(ctypes.Structure):
     _fields_ = [("ms_step", ctypes.c_double), ("ms_interior", ctypes.c_double), ("ms_boundary", ctypes.c_double),
                 ("cells_interior", ctypes.c_longlong), ("cells_boundary", ctypes.c_longlong),
                 ("tiles_interior", ctypes.c_int), ("tiles_boundary", ctypes.c_int), ("in_graph", ctypes.c_int)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("march_rows", ctypes.c_int), ("bx", ctypes.c_int), ("by", ctypes.c_int),
+                ("ms_per_step", ctypes.c_double), ("ntried", ctypes.c_int),
+                ("march_rows_tried", ctypes.c_int * TUNE_MAX), ("bx_tried", ctypes.c_int * TUNE_MAX),
+                ("by_tried", ctypes.c_int * TUNE_MAX), ("ms_tried", ctypes.c_double * TUNE_MAX)]
 
 
 class MeshParams(ctypes.Structure):
@@ -167,6 +176,10 @@ def lib():
         L.tmk_level_sync.argtypes = [vp]
         L.tmk_set_tile.argtypes = [vp, c_int, c_int]
         L.tmk_set_march.argtypes = [vp, c_int]
+        L.tmk_level_set_update_shape.argtypes = [vp, c_int, c_int, c_int]
+        L.tmk_level_get_update_shape.argtypes = [vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                                 ctypes.POINTER(c_int)]
+        L.tmk_level_autotune.argtypes = [vp, _D, _D, c_int, ctypes.POINTER(Tuning)]
         sz = ctypes.c_size_t
         L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
@@ -399,6 +412,27 @@ class Level:
 
     def sync(self):
         _check(lib().tmk_level_sync(self.h), "tmk_level_sync")
+
+    def set_update_shape(self, march_rows, bx=0, by=0):
+        """The level's own update shape: row-marching strips of march_rows rows (0: tiles of
+        bx x by; 0, 0 = the default tile)."""
+        _check(lib().tmk_level_set_update_shape(self.h, march_rows, bx, by), "tmk_level_set_update_shape")
+
+    def update_shape(self):
+        """(march_rows, bx, by) the update runs (march_rows 0: the tile kernels)."""
+        r, x, y = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().tmk_level_get_update_shape(self.h, ctypes.byref(r), ctypes.byref(x), ctypes.byref(y)),
+               "tmk_level_get_update_shape")
+        return r.value, x.value, y.value
+
+    def autotune(self, time, dt, nsteps=10):
+        """Times every update shape from the level's current state, sets the fastest and returns
+        the level to that state: {"march_rows", "bx", "by", "ms_per_step", "tried": [(rows, bx,
+        by, ms), ...]}."""
+        t = Tuning()
+        _check(lib().tmk_level_autotune(self.h, time, dt, nsteps, ctypes.byref(t)), "tmk_level_autotune")
+        tried = [(t.march_rows_tried[k], t.bx_tried[k], t.by_tried[k], t.ms_tried[k]) for k in range(t.ntried)]
+        return {"march_rows": t.march_rows, "bx": t.bx, "by": t.by, "ms_per_step": t.ms_per_step, "tried": tried}
 
 
 class Amr:

@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <vector>
+#include <array>
 
 #include "tetemoko_hip.h"
 
@@ -330,6 +331,8 @@ struct tmk_level {
     std::vector<int> has_b;
     int ns = 2;
     size_t tsize = 8;
+    int bx = 32, by = 16;         // update tile (the context's at creation; tmk_level_set_update_shape)
+    int march = 0;                // strip rows of the row-marching update (0: tile kernels)
     std::vector<Island> isl;
     std::vector<int> box_isl;     // island of each box
     int npatch = 0;               // descriptors: one per box, then one per island (the tiles')
@@ -825,9 +828,9 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     G.pk = PK_PPM_PRIM;
     for (tmk_level* L : mem)
         if (pred_kind(L->ctx->P) != PK_PPM_PRIM) G.pk = PK_RUNTIME;
-    G.bx = L0->ctx->bx;
-    G.by = L0->ctx->by;
-    G.march = (L0->ctx->march > 0 && G.ns == 2 && G.pk == PK_PPM_PRIM) ? L0->ctx->march : 0;
+    G.bx = L0->bx;
+    G.by = L0->by;
+    G.march = (L0->march > 0 && G.ns == 2 && G.pk == PK_PPM_PRIM) ? L0->march : 0;
     G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     G.plastic = plastic;
     G.g = 0;
@@ -1543,6 +1546,29 @@ static void level_free_arrays(tmk_level* L)
     }
 }
 
+// tiles of the level's tile shape over each island's merged rectangles, and the rectangles
+// themselves (the row-marching partition's input); both use the island's descriptor
+static void level_build_tiles(tmk_level* L)
+{
+    const int nboxes = (int)L->boxes.size();
+    L->tiles.clear();
+    L->urects.clear();
+    for (size_t k = 0; k < L->isl.size(); ++k) {
+        std::vector<tmk_box> rr;
+        island_rects(L->boxes, L->isl[k], rr);
+        for (const tmk_box& r : rr) {
+            make_tiles(r, nboxes + (int)k, L->bx - 2, L->by - 2, L->tiles);
+            Tile u;
+            u.patch = nboxes + (int)k;
+            u.x0 = r.lo[0];
+            u.y0 = r.lo[1];
+            u.x1 = r.hi[0];
+            u.y1 = r.hi[1];
+            L->urects.push_back(u);
+        }
+    }
+}
+
 extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tmk_level** out)
 {
     if (!c || !out || nboxes <= 0 || !boxes) return TMK_E_ARG;
@@ -1554,6 +1580,9 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     if (!L) return TMK_E_NOMEM;
     L->ctx = c;
     L->boxes.assign(boxes, boxes + nboxes);
+    L->bx = c->bx;
+    L->by = c->by;
+    L->march = c->march;
     L->ns = c->antiplane ? 3 : 2;
     L->tsize = c->precision == TMK_PREC_FP64 ? sizeof(double) : sizeof(float);
     L->bytes = level_bytes(c->precision, c->hp.fault_kind, c->hp.domain, nboxes, boxes);
@@ -1601,21 +1630,7 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
     }
     if (L->tsize == sizeof(double)) level_fill_patches<double>(L);
     else level_fill_patches<float>(L);
-    // tiles over each island's merged rectangles; they use the island's descriptor
-    for (size_t k = 0; k < L->isl.size(); ++k) {
-        std::vector<tmk_box> rr;
-        island_rects(L->boxes, L->isl[k], rr);
-        for (const tmk_box& r : rr) {
-            make_tiles(r, nboxes + (int)k, c->bx - 2, c->by - 2, L->tiles);
-            Tile u;
-            u.patch = nboxes + (int)k;
-            u.x0 = r.lo[0];
-            u.y0 = r.lo[1];
-            u.x1 = r.hi[0];
-            u.y1 = r.hi[1];
-            L->urects.push_back(u);
-        }
-    }
+    level_build_tiles(L);
     // exchange plan: ghost cells (inside the domain) of each box covered by valid cells of a
     // box in another island
     for (int d = 0; d < nboxes; ++d) {
@@ -2114,6 +2129,139 @@ extern "C" int tmk_level_sync(tmk_level* L)
 }
 
 // ---------------------------------------------------------------------------
+// update shape of a level (row-marching strips or tiles) and its tuning
+// ---------------------------------------------------------------------------
+extern "C" int tmk_level_set_update_shape(tmk_level* L, int march_rows, int bx, int by)
+{
+    if (!L || L->batch || L->amr_stream || march_rows < 0 || march_rows > 65536) return TMK_E_ARG;
+    if (bx == 0 && by == 0) default_tile(L->ctx->precision, bx, by);
+    if (!tile_ok(bx, by)) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    L->march = march_rows;
+    if (bx != L->bx || by != L->by) {
+        L->bx = bx;
+        L->by = by;
+        level_build_tiles(L);
+    }
+    L->own_built = false;  // tables and graph are rebuilt at the next step
+    return TMK_OK;
+}
+
+extern "C" int tmk_level_get_update_shape(const tmk_level* L, int* march_rows, int* bx, int* by)
+{
+    if (!L || !march_rows || !bx || !by) return TMK_E_ARG;
+    const bool m = L->march > 0 && L->ns == 2 && pred_kind(L->ctx->P) == PK_PPM_PRIM;
+    *march_rows = m ? L->march : 0;
+    *bx = L->bx;
+    *by = L->by;
+    return TMK_OK;
+}
+
+// the strip heights and tile shapes the tuner times
+static const int kTuneMarch[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64};
+static const int kTuneTile[][2] = {{32, 16}, {32, 8}, {32, 12}, {32, 32}, {16, 16}, {64, 4}, {64, 8}, {64, 16}};
+
+// a device copy of a level's state (every island's current stencil planes, in-place planes and
+// fault rows), to run tuning steps from and return to
+struct LevelSnapshot {
+    std::vector<void*> bufs;
+    int cur = 0;
+    ~LevelSnapshot()
+    {
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
+    }
+    static size_t fault_bytes(const tmk_level* L, const Island& I)
+    {
+        const int nb = L->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
+        return I.has_b ? (size_t)I.bnx * nb * sizeof(double) : 0;
+    }
+    int take(const tmk_level* L, hipStream_t st)
+    {
+        cur = L->cur;
+        for (const Island& I : L->isl) {
+            const size_t sb = I.plane() * TMK_NCOMP * L->tsize, fb = fault_bytes(L, I);
+            void *u = nullptr, *w = nullptr, *b = nullptr;
+            if (hipMalloc(&u, sb) != hipSuccess || hipMalloc(&w, sb) != hipSuccess ||
+                (fb && hipMalloc(&b, fb) != hipSuccess)) {
+                if (u) (void)hipFree(u);
+                if (w) (void)hipFree(w);
+                return TMK_E_NOMEM;
+            }
+            bufs.push_back(u);
+            bufs.push_back(w);
+            bufs.push_back(b);
+            HIPCHK(hipMemcpyAsync(u, I.u[cur], sb, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(w, I.w, sb, hipMemcpyDeviceToDevice, st));
+            if (fb) HIPCHK(hipMemcpyAsync(b, I.b[cur], fb, hipMemcpyDeviceToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        return TMK_OK;
+    }
+    int restore(tmk_level* L, hipStream_t st) const
+    {
+        for (size_t k = 0; k < L->isl.size(); ++k) {
+            const Island& I = L->isl[k];
+            const size_t sb = I.plane() * TMK_NCOMP * L->tsize, fb = fault_bytes(L, I);
+            HIPCHK(hipMemcpyAsync(I.u[cur], bufs[3 * k], sb, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(I.w, bufs[3 * k + 1], sb, hipMemcpyDeviceToDevice, st));
+            if (fb) HIPCHK(hipMemcpyAsync(I.b[cur], bufs[3 * k + 2], fb, hipMemcpyDeviceToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        L->cur = cur;
+        L->own_built = false;  // the group's parity follows L->cur from its next build
+        return TMK_OK;
+    }
+};
+
+extern "C" int tmk_level_autotune(tmk_level* L, double time, double dt, int nsteps, tmk_tuning* out)
+{
+    if (!L || !out || nsteps < 2 || L->batch || L->amr_stream || L->stations) return TMK_E_ARG;
+    tmk_ctx* c = L->ctx;
+    hipStream_t st = c->stream;
+    memset(out, 0, sizeof *out);
+    int e = level_prepare(L, time, dt);  // exchanges uploaded data first: the snapshot is a valid state
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(st));
+    LevelSnapshot snap;
+    if ((e = snap.take(L, st))) return e;
+    const int march0 = L->march, bx0 = L->bx, by0 = L->by;
+    std::vector<std::array<int, 3>> cand;
+    if (L->ns == 2 && pred_kind(c->P) == PK_PPM_PRIM)
+        for (int rows : kTuneMarch) cand.push_back({rows, bx0, by0});
+    for (const auto& t : kTuneTile) cand.push_back({0, t[0], t[1]});
+    if ((int)cand.size() > TMK_TUNE_MAX) cand.resize(TMK_TUNE_MAX);
+    int best = -1;
+    for (size_t k = 0; k < cand.size() && !e; ++k) {
+        if ((e = snap.restore(L, st))) break;
+        if ((e = tmk_level_set_update_shape(L, cand[k][0], cand[k][1], cand[k][2]))) break;
+        double ms = 0.0;
+        // two untimed steps (group build, graph capture), then nsteps graph-replayed steps
+        if ((e = tmk_level_run(L, 2, time, dt))) break;
+        if ((e = tmk_level_run_timed(L, nsteps, time + dt + dt, dt, &ms))) break;
+        if ((e = tmk_level_sync(L))) break;
+        out->march_rows_tried[k] = cand[k][0];
+        out->bx_tried[k] = cand[k][1];
+        out->by_tried[k] = cand[k][2];
+        out->ms_tried[k] = ms / nsteps;
+        out->ntried = (int)k + 1;
+        if (best < 0 || out->ms_tried[k] < out->ms_tried[best]) best = (int)k;
+    }
+    int e2 = snap.restore(L, st);
+    L->time = time;
+    if (e || best < 0) {
+        (void)tmk_level_set_update_shape(L, march0, bx0, by0);
+        return e ? e : e2;
+    }
+    if ((e = tmk_level_set_update_shape(L, cand[best][0], cand[best][1], cand[best][2]))) return e;
+    out->march_rows = cand[best][0];
+    out->bx = cand[best][1];
+    out->by = cand[best][2];
+    out->ms_per_step = out->ms_tried[best];
+    return e2;
+}
+
+// ---------------------------------------------------------------------------
 // batches: boxes of many scenarios in shared launches (continuous batching)
 // ---------------------------------------------------------------------------
 extern "C" int tmk_batch_create(tmk_batch** out)
@@ -2193,8 +2341,7 @@ extern "C" int tmk_batch_add(tmk_batch* B, tmk_level* L, double time, double dt)
     if (!B || !L || L->batch || L->amr_stream) return TMK_E_ARG;
     if (!B->members.empty()) {
         tmk_level* L0 = B->members[0];
-        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->ctx->bx != L->ctx->bx || L0->ctx->by != L->ctx->by ||
-            L0->ctx->march != L->ctx->march)
+        if (L0->tsize != L->tsize || L0->ns != L->ns || L0->bx != L->bx || L0->by != L->by || L0->march != L->march)
             return TMK_E_ARG;
     }
     HIPCHK(hipStreamSynchronize(L->ctx->stream));
@@ -2970,11 +3117,13 @@ extern "C" int tmk_amr_add(tmk_amr* A, tmk_level* const* levels, double time, do
             if (levels[k] == L) return TMK_E_ARG;
     }
     if (!A->mem.empty()) {
-        const tmk_level* a = A->mem[0]->L[0];
-        const tmk_level* b = levels[0];
-        if (a->tsize != b->tsize || a->ns != b->ns || a->ctx->bx != b->ctx->bx || a->ctx->by != b->ctx->by ||
-            a->ctx->march != b->ctx->march)
-            return TMK_E_ARG;
+        // level l of every member runs in one group: the same arithmetic and update shape
+        for (int l = 0; l < A->nlev; ++l) {
+            const tmk_level* a = A->mem[0]->L[l];
+            const tmk_level* b = levels[l];
+            if (a->tsize != b->tsize || a->ns != b->ns || a->bx != b->bx || a->by != b->by || a->march != b->march)
+                return TMK_E_ARG;
+        }
     }
     {   // level 0 covers its domain
         const tmk_level* L0 = levels[0];

@@ -334,10 +334,11 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     w[4] = gld(pt.in[o + 4], u);
     w[5] = gld(pt.in[o + 5], u);
     w[6] = gld(pt.in[o + 6], u);
+    // mode II keeps sxz, syz zero (the plastic return scales them); lam is only written
     w[2] = NS == 3 ? gld(pt.in[o + 2], u) : (T)0;
-    w[7] = (NS == 3 || plastic) ? gld(pt.in[o + 7], u) : (T)0;
-    w[8] = (NS == 3 || plastic) ? gld(pt.in[o + 8], u) : (T)0;
-    w[9] = plastic ? gld(pt.in[o + 9], u) : (T)0;
+    w[7] = NS == 3 ? gld(pt.in[o + 7], u) : (T)0;
+    w[8] = NS == 3 ? gld(pt.in[o + 8], u) : (T)0;
+    w[9] = (T)0;
     w[10] = plastic ? gld(pt.in[o + 10], u) : (T)0;
     cell_update_side<T, NS>(P, m, dt, sc, plastic, ry, F, w);
     gst(pt.out[o + 0], u, w[0]);
@@ -347,7 +348,7 @@ __device__ __forceinline__ void update_cell_side(const KArgs<T>& a, const Patch<
     gst(pt.out[o + 4], u, w[4]);
     gst(pt.out[o + 5], u, w[5]);
     gst(pt.out[o + 6], u, w[6]);
-    if (NS == 3 || plastic) {
+    if (NS == 3) {
         gst(pt.out[o + 7], u, w[7]);
         gst(pt.out[o + 8], u, w[8]);
     }

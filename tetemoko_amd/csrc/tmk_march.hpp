@@ -170,11 +170,12 @@ k_godunov_march(const KArgs<T> a)
         rin[q] = plane_rsrc(pt.in[o + QC[q]], pt.pbytes);
         rout[q] = plane_rsrc(pt.out[o + QC[q]], pt.pbytes);
     }
-    // pointwise planes, updated in place: szz, then sxz syz lam gam (plasticity)
+    // pointwise planes, updated in place: szz, then lam gam (plasticity); index c - 6 for c = 9, 10
     rsrc_t rpw[5];
     rpw[0] = plane_rsrc(pt.out[o + 5], pt.pbytes);
+    rpw[1] = rpw[2] = rpw[0];  // sxz, syz: not accessed in mode II
 #pragma unroll
-    for (int c = 7; c <= 10; ++c) rpw[c - 6] = plane_rsrc(pt.out[o + c], pt.pbytes);
+    for (int c = 9; c <= 10; ++c) rpw[c - 6] = plane_rsrc(pt.out[o + c], pt.pbytes);
     const unsigned col = (unsigned)(i - pt.ulo0) * (unsigned)sizeof(T);
     const unsigned rowb = (unsigned)pt.unx * (unsigned)sizeof(T);
     auto row_off = [&](int j) { return col + (unsigned)(j - pt.ulo1) * rowb; };
@@ -235,15 +236,14 @@ k_godunov_march(const KArgs<T> a)
             for (int q = 0; q < 5; ++q) n.wn[q] = bld<T>(rin[q], offn);
         }
         const int j3 = r - 3;
-        // row r - 3's pointwise planes (szz; sxz syz lam gam with plasticity); a lane outside the
-        // strip gets an offset at or past every plane's bound (planes hold at most 2^32 - 256
-        // bytes), far enough below 2^32 that the second dword of an 8-byte access does not wrap:
-        // the bound check makes its loads read 0 and drops its stores
+        // row r - 3's pointwise planes: szz, and gam with plasticity (in mode II sxz and syz
+        // stay zero and the plastic multiplier lam is only written); a lane outside the strip
+        // gets an offset at or past every plane's bound (planes hold at most 2^32 - 256 bytes),
+        // far enough below 2^32 that the second dword of an 8-byte access does not wrap: the
+        // bound check makes its loads read 0 and drops its stores
         const unsigned off3 = valid ? row_off(j3) : kDropOffset;
-        T upw[5];
-        upw[0] = bld<T>(rpw[0], off3);
-#pragma unroll
-        for (int q = 1; q < 5; ++q) upw[q] = plastic ? bld<T>(rpw[q], off3) : (T)0;
+        const T szz = bld<T>(rpw[0], off3);
+        const T gam = plastic ? bld<T>(rpw[4], off3) : (T)0;
         // y: differences, slopes of row r - 1, face values at r - 3/2
         T dry[4], dy1[4], fy1[4];
 #pragma unroll
@@ -357,17 +357,16 @@ k_godunov_march(const KArgs<T> a)
             u[3] = c.w3[2];
             u[4] = c.w3[3];
             u[6] = c.w3[4];
-            u[2] = (T)0;
-            u[5] = upw[0];
-#pragma unroll
-            for (int q = 7; q <= 10; ++q) u[q] = upw[q - 6];
+            u[2] = u[7] = u[8] = u[9] = (T)0;
+            u[5] = szz;
+            u[10] = gam;
             cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
 #pragma unroll
             for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
             bst<T>(rpw[0], off3, u[5]);
             if (plastic) {
-#pragma unroll
-                for (int q = 7; q <= 10; ++q) bst<T>(rpw[q - 6], off3, u[q]);
+                bst<T>(rpw[3], off3, u[9]);
+                bst<T>(rpw[4], off3, u[10]);
             }
         }
         // carried to the next row

@@ -55,13 +55,13 @@ def parse():
     ap.add_argument("--nx", type=int, default=8192)
     ap.add_argument("--ny", type=int, default=4096)
     ap.add_argument("--dx", type=float, default=12.5)
-    ap.add_argument("--tile", type=str, default="", help="update tile, e.g. 32x16 (default: the library's)")
+    ap.add_argument("--shape-fp32", type=str, default="",
+                    help="update shape of the FP32-bulk levels, march<rows> or tile<bx>x<by> (default: autotuned)")
+    ap.add_argument("--shape-fp64", type=str, default="",
+                    help="update shape of the FP64 levels, march<rows> or tile<bx>x<by> (default: autotuned)")
     ap.add_argument("--max-grid-size", type=int, default=32,
                     help="boxes of at most this many cells per side (the reference's default max_grid_size "
                          "is 32); 0 = the level as one box")
-    ap.add_argument("--march", type=int, default=-1,
-                    help="strip rows of the row-marching interior update (0: tile kernels; default: the library's); "
-                         "implies --no-tune")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default update shape instead of autotuning it per level")
     ap.add_argument("--acc-steps", type=int, default=4000, help="steps of the accuracy leg (0: skip it)")
@@ -93,18 +93,28 @@ def level_boxes(nx, ny, max_grid_size):
     return scenarios.split_boxes(nx, ny, max_grid_size)
 
 
-MARCH = -1  # strip rows of the row-marching interior update; < 0: the library's default
+def parse_shape(text):
+    """"march32" -> (32, 0, 0); "tile64x16" -> (0, 64, 16); "" -> None"""
+    if not text:
+        return None
+    if text.startswith("march"):
+        return int(text[5:]), 0, 0
+    if text.startswith("tile"):
+        bx, by = text[4:].split("x")
+        return 0, int(bx), int(by)
+    raise SystemExit("bench.py: update shape %r is neither march<rows> nor tile<bx>x<by>" % text)
 
 
-def make_level(cfg, precision, boxes, tile):
+SHAPES = {}  # precision -> fixed update shape (march_rows, bx, by); absent: autotuned
+
+
+def make_level(cfg, precision, boxes):
     """A device-resident level holding the configuration's initial state (uniform background,
     seeded fault data), set on the device box by box (no host copy of the state)."""
     ctx = capi.Context(cfg, precision)
-    if tile:
-        ctx.set_tile(*tile)
-    if MARCH >= 0:
-        ctx.set_march(MARCH)
     lev = capi.Level(ctx, boxes)
+    if precision in SHAPES:
+        lev.set_update_shape(*SHAPES[precision])
     B = scenarios.fault_arrays(cfg)
     bg = scenarios.background_vector(cfg)
     for k, b in enumerate(boxes):
@@ -115,15 +125,15 @@ def make_level(cfg, precision, boxes, tile):
 TUNE = True  # autotune each level's update shape before it is timed
 
 
-def timed_leg(cfg, precision, boxes, warmup, steps, tile):
+def timed_leg(cfg, precision, boxes, warmup, steps):
     """The level's update shape autotuned from its initial state (tmk_level_autotune: every
     strip height and tile shape timed, the level returned to that state), then W untimed
     graph-replayed steps, then exactly K graph-replayed steps bracketed by HIP events on the
     level's stream (after a device synchronisation), then per-launch durations of the update
     kernels inside a replayed graph (stamps on each kernel's stream)."""
-    ctx, lev = make_level(cfg, precision, boxes, tile)
+    ctx, lev = make_level(cfg, precision, boxes)
     dt = scenarios.dt_of(cfg)
-    tuning = lev.autotune(0.0, dt, nsteps=10) if TUNE else None
+    tuning = lev.autotune(0.0, dt, nsteps=10) if TUNE and precision not in SHAPES else None
     shape = lev.update_shape()
     t = 0.0
     lev.run(warmup, t, dt)
@@ -183,11 +193,11 @@ def station_faces(cfg, n=64, half_width_m=4500.0, seed=1234):
     return faces, (faces + 0.5) * dx
 
 
-def accuracy_leg(cfg, precision, boxes, nsteps, xs, tile, shape, chunk=500):
+def accuracy_leg(cfg, precision, boxes, nsteps, xs, shape, chunk=500):
     """nsteps steps from t = 0, in the update shape the timed leg of this precision ran, with
     fault stations recorded on the device after every step: final slip, peak slip rate (over
     every step) and rupture time at each station."""
-    ctx, lev = make_level(cfg, precision, boxes, tile)
+    ctx, lev = make_level(cfg, precision, boxes)
     lev.set_update_shape(*shape)
     st = capi.Stations(lev, xs, capacity=chunk)
     dt = scenarios.dt_of(cfg)
@@ -292,18 +302,19 @@ def main():
     if not capi.lib().tmk_device_ok():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(3)
-    tile = tuple(int(x) for x in a.tile.split("x")) if a.tile else None
-    global MARCH, TUNE
-    MARCH = a.march
-    TUNE = not a.no_tune and a.march < 0 and not a.tile
+    global TUNE
+    TUNE = not a.no_tune
+    for prec, text in ((capi.PREC_FP32_BULK, a.shape_fp32), (capi.PREC_FP64, a.shape_fp64)):
+        if parse_shape(text):
+            SHAPES[prec] = parse_shape(text)
     cfg = scenarios.config_c3(seed=0, nx=a.nx, ny=a.ny, dx=a.dx)
     boxes = level_boxes(cfg["nx"], cfg["ny"], a.max_grid_size)
     wl = "rs-aging+drucker-prager %dx%d dx=%g" % (cfg["nx"], cfg["ny"], cfg["dx"])
     print("bench: %s, %d boxes, FP32 bulk, warmup %d, steps %d" % (wl, len(boxes), a.warmup, a.steps),
           file=sys.stderr, flush=True)
-    r32 = timed_leg(cfg, capi.PREC_FP32_BULK, boxes, a.warmup, a.steps, tile)
+    r32 = timed_leg(cfg, capi.PREC_FP32_BULK, boxes, a.warmup, a.steps)
     print("bench: FP64, warmup %d, steps %d" % (a.warmup, a.steps), file=sys.stderr, flush=True)
-    r64 = timed_leg(cfg, capi.PREC_FP64, boxes, a.warmup, a.steps, tile)
+    r64 = timed_leg(cfg, capi.PREC_FP64, boxes, a.warmup, a.steps)
     out = {
         "metric": METRIC,
         "value": r32["value"],
@@ -334,8 +345,8 @@ def main():
     if a.acc_steps > 0:
         faces, xs = station_faces(cfg)
         print("bench: accuracy leg, %d steps per precision, 64 stations" % a.acc_steps, file=sys.stderr, flush=True)
-        s64 = accuracy_leg(cfg, capi.PREC_FP64, boxes, a.acc_steps, xs, tile, r64["shape"])
-        s32 = accuracy_leg(cfg, capi.PREC_FP32_BULK, boxes, a.acc_steps, xs, tile, r32["shape"])
+        s64 = accuracy_leg(cfg, capi.PREC_FP64, boxes, a.acc_steps, xs, r64["shape"])
+        s32 = accuracy_leg(cfg, capi.PREC_FP32_BULK, boxes, a.acc_steps, xs, r32["shape"])
         r64t, r32t = s64["rupture"], s32["rupture"]
         both = (r64t < 1e9) & (r32t < 1e9)
         cells = np.abs(r64t[both] - r32t[both]) * cfg["cs"] / cfg["dx"]
@@ -360,8 +371,8 @@ def main():
         print("bench: secondary leg, 4096x2048 FP64 drop-in configuration", file=sys.stderr, flush=True)
         b1 = level_boxes(c1["nx"], c1["ny"], a.max_grid_size)
         wl1 = "rs-aging 4096x2048 dx=25"
-        q64 = timed_leg(c1, capi.PREC_FP64, b1, a.warmup, a.steps, tile)
-        q32 = timed_leg(c1, capi.PREC_FP32_BULK, b1, a.warmup, a.steps, tile)
+        q64 = timed_leg(c1, capi.PREC_FP64, b1, a.warmup, a.steps)
+        q32 = timed_leg(c1, capi.PREC_FP32_BULK, b1, a.warmup, a.steps)
         out["secondary_4096x2048_rs"] = {
             "workload": "rate-and-state (aging law), single level 4096x2048 in %d boxes, dx=25 m, PPM, no "
                         "plasticity" % len(b1),

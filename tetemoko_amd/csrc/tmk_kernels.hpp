@@ -256,8 +256,9 @@ struct Faces1 {
 // U += dU (-dt/dx) (LinElastPatchGodunov.cpp:117-126, LinElastPhysics.cpp:135-179), then, with
 // plastic, the Drucker-Prager return (LinElastLevelGodunov::plasticUpdate after the level step,
 // AMRLevelLinElast.cpp:476).  Changes u[0 1 3 4 5 6] (+ 2 7 8 with NS 3, + 7 8 9 10 with plastic).
-template <typename T, int NS>
-__device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T>& m, double dt, T sc, bool plastic,
+// PP: DevParams, or for FP32 PlasticConstL (the constants the FP32 plastic return reads)
+template <typename T, int NS, typename PP>
+__device__ __forceinline__ void cell_update_side(const PP& P, const Mat<T>& m, double dt, T sc, bool plastic,
                                                  T ry, const Faces1<T, NS>& F, T (&u)[11])
 {
     if constexpr (regroup<T>()) {
@@ -282,7 +283,7 @@ __device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T
             u[7] = fma(kg, xv2, u[7]);
             u[8] = fma(kg, ry * yv2, u[8]);
         }
-        if (plastic) plastic_flow_delta<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
+        if (plastic) plastic_flow_delta<T, PP>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
         return;
     }
     T fxl[7], fxh[7], fyl[7], fyh[7];
@@ -313,7 +314,8 @@ __device__ __forceinline__ void cell_update_side(const DevParams& P, const Mat<T
         u[7] = u[7] + (((T)0 + dsxz_x) + dsxz_y) * sc;
         u[8] = u[8] + (((T)0 + dsyz_x) + dsyz_y) * sc;
     }
-    if (plastic) plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
+    if constexpr (!regroup<T>())
+        if (plastic) plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
 }
 
 // U += dU of side sd of valid cell (i, j) from the final states on its four faces, then the
@@ -772,7 +774,7 @@ __global__ void k_plastic(const DevParams P, T* U, int ulo0, int ulo1, int unx, 
         T sxy = U[u + (o + 6) * plane], sxz = U[u + (o + 7) * plane], syz = U[u + (o + 8) * plane];
         T lam = U[u + (o + 9) * plane], gam = U[u + (o + 10) * plane];
         if constexpr (stores_delta<T>())
-            plastic_flow_delta<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
+            plastic_flow_delta<T, DevParams>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
         else
             plastic_flow<T>(P, (T)dt, sxx, syy, szz, sxy, sxz, syz, lam, gam);
         U[u + (o + 3) * plane] = sxx; U[u + (o + 4) * plane] = syy; U[u + (o + 5) * plane] = szz;

@@ -30,6 +30,14 @@ namespace tmk {
 constexpr int MARCH_COLS = 58;  // 64 lanes minus the 3-cell x stencil on each side
 constexpr unsigned kDropOffset = 0xFFFFFF00u;  // buffer offset of a lane whose access must be dropped
 
+// FP32: rows requested ahead of their use into an LDS ring (one slot per row: the 5 stencil
+// planes of row q and the pointwise planes szz, gam of row q - 3), filled by buffer loads that
+// write LDS directly, so the requests in flight hold no VGPRs.  4 slots x 7 planes x 64 lanes x
+// 4 B = 7 KiB per wavefront; 16 wavefronts per CU use 112 of its 160 KiB.
+constexpr int MARCH_RING = 4;
+constexpr int MARCH_RING_LOADS = 7;   // loads per slot
+constexpr int MARCH_ROW_STORES = 8;   // stores per row: 5 stencil planes, szz, lam, gam
+
 // a strip whose stencil lies in the domain (the same rule as tile_interior)
 __device__ __forceinline__ bool march_interior(const Tile& t, const DevParams& P)
 {
@@ -86,6 +94,28 @@ __device__ __forceinline__ void bst(rsrc_t r, unsigned off, T v)
     } else {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
     }
+}
+
+// one plane row into LDS: lane l's dword at dst[l] (the LDS address is the wave-uniform dst plus
+// 4 l); an offset past the plane reads 0
+__device__ __forceinline__ void lds_row(rsrc_t r, float* dst, unsigned off)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, off, 0, 0, 0);
+}
+
+// wait until at most N vector memory operations of this wavefront are outstanding, and return
+// addr (an LDS address) through the wait: reads at the returned address depend on it and so
+// stay after it, while nothing else is tied to it (no memory clobber, which would make the
+// compiler reload kernel arguments from memory inside the loop and wait for them).  Loads into
+// LDS are not tracked by the compiler's own waits, so the kernel states them.  The counter
+// retires loads and stores in issue order, so N no larger than the number of operations issued
+// after a slot's loads makes the slot complete.
+template <int N>
+__device__ __forceinline__ unsigned wait_vmem(unsigned addr)
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt is a 6-bit count");
+    asm volatile("s_waitcnt vmcnt(%1)" : "+v"(addr) : "n"(N));
+    return addr;
 }
 
 // lane shifts inside the wavefront (lane l <- lane l -+ 1; the first / last lane reads 0), in a
@@ -178,8 +208,39 @@ k_godunov_march(const KArgs<T> a)
     for (int c = 9; c <= 10; ++c) rpw[c - 6] = plane_rsrc(pt.out[o + c], pt.pbytes);
     const unsigned col = (unsigned)(i - pt.ulo0) * (unsigned)sizeof(T);
     const unsigned rowb = (unsigned)pt.unx * (unsigned)sizeof(T);
-    auto row_off = [&](int j) { return col + (unsigned)(j - pt.ulo1) * rowb; };
+    // values the loop needs from the patch descriptor and the parameters, read once here: after
+    // the loop's first stores the compiler could only read them again with vector loads, each
+    // waiting for every request in flight
+    const int ulo1 = __builtin_amdgcn_readfirstlane(pt.ulo1);
+    const bool flux0 = pt.F[0] != nullptr, flux1 = pt.F[1] != nullptr;
+    auto row_off = [=](int j) { return col + (unsigned)(j - ulo1) * rowb; };
     const int y0 = tl.y0, y1 = tl.y1;
+
+    const int rlast = y1 + 3;  // the last row loaded (row y1 is updated after it)
+    constexpr bool ring_fetch = sizeof(T) == sizeof(float);
+    static_assert(MARCH_RING == 4, "the waits below are written out for four slots");
+    // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL)
+    __shared__ float ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64 + 16];
+    float* const pconst = &ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64];
+    if constexpr (ring_fetch) {
+        if (lane < 14) pconst[lane] = lane < 7 ? P.pbg[lane] : lane < 13 ? P.pf[lane - 7] : P.fG;
+    }
+    const PlasticConstL pc{(const volatile __attribute__((address_space(3))) float*)&pconst[0]};
+    // this lane's dword in slot 0, plane 0 (an LDS byte address)
+    const unsigned ring_lane =
+        (unsigned)(uintptr_t)(__attribute__((address_space(3))) float*)&ring[0] + 4u * (unsigned)lane;
+    // the rows row(q) reads into slot (q - y0) mod MARCH_RING: every row issues exactly
+    // MARCH_RING_LOADS loads (rows past the strip and lanes outside it read past the planes)
+    auto request = [&](const int q) {
+        const unsigned sl = (unsigned)(q - y0) & (MARCH_RING - 1);
+        const bool live = q <= rlast;
+        const unsigned offs = live ? row_off(q) : kDropOffset;
+        const unsigned offp = (live && valid && q - 3 >= y0) ? row_off(q - 3) : kDropOffset;
+#pragma unroll
+        for (int p = 0; p < 5; ++p) lds_row(rin[p], &ring[(sl * MARCH_RING_LOADS + p) * 64], offs);
+        lds_row(rpw[0], &ring[(sl * MARCH_RING_LOADS + 5) * 64], offp);
+        lds_row(rpw[4], &ring[(sl * MARCH_RING_LOADS + 6) * 64], plastic ? offp : kDropOffset);
+    };
 
     // warm-up: rows y0 - 3 .. y0 give the y slopes of rows y0 - 2, y0 - 1 and the face value at
     // y0 - 3/2 (what the first main iteration carries in)
@@ -214,7 +275,10 @@ k_godunov_march(const KArgs<T> a)
         A.xp.mv[k] = A.xp.ms[k] = A.xp.pv[k] = A.xp.ps[k] = (T)0;
         A.h1v[k] = A.h1s[k] = A.cpu[k] = A.cpc[k] = A.f1v[k] = A.f1s[k] = (T)0;
     }
-    {
+    if constexpr (ring_fetch) {
+#pragma unroll
+        for (int q = 1; q < MARCH_RING; ++q) request(y0 + q);
+    } else {
         const unsigned off = row_off(y0 + 1);
 #pragma unroll
         for (int q = 0; q < 5; ++q) A.wn[q] = bld<T>(rin[q], off);
@@ -227,23 +291,41 @@ k_godunov_march(const KArgs<T> a)
     // pointwise planes are requested before the row's arithmetic, so the wavefront computes while
     // they arrive.
     auto row = [&](const int r, const MarchCarry<T>& c, MarchCarry<T>& n) {
-        T w0[5];
+        const int j3 = r - 3;
+        // row r - 3's pointwise planes: szz, and gam with plasticity (in mode II sxz and syz
+        // stay zero and the plastic multiplier lam is only written); a lane outside the strip,
+        // or a row before it, gets an offset at or past every plane's bound (planes hold at most
+        // 2^32 - 256 bytes), far enough below 2^32 that the second dword of an 8-byte access
+        // does not wrap: the bound check makes its loads read 0 and drops its stores
+        const unsigned off3 = (valid && j3 >= y0) ? row_off(j3) : kDropOffset;
+        T w0[5], szz, gam;
+        if constexpr (ring_fetch) {
+            // request the row MARCH_RING - 1 ahead, then wait for this row's slot.  Issued after
+            // that slot's loads: the MARCH_RING - 1 later slots (7 loads each) and the stores of
+            // the rows since (8 each; the slots of the first rows were requested before any)
+            request(r + MARCH_RING - 1);
+            const int k = r - (y0 + 1);
+            constexpr int L = (MARCH_RING - 1) * MARCH_RING_LOADS, S = MARCH_ROW_STORES;
+            const unsigned sl = (unsigned)(r - y0) & (MARCH_RING - 1);
+            unsigned at = ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float));
+            if (k >= 3) at = wait_vmem<L + 3 * S>(at);
+            else if (k == 2) at = wait_vmem<L + 2 * S>(at);
+            else if (k == 1) at = wait_vmem<L + S>(at);
+            else at = wait_vmem<L>(at);
+            const __attribute__((address_space(3))) float* slot = (const __attribute__((address_space(3))) float*)at;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) w0[q] = c.wn[q];
-        {
+            for (int q = 0; q < 5; ++q) w0[q] = slot[64 * q];
+            szz = slot[64 * 5];
+            gam = plastic ? slot[64 * 6] : (T)0;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) w0[q] = c.wn[q];
             const unsigned offn = row_off(r + 1);
 #pragma unroll
             for (int q = 0; q < 5; ++q) n.wn[q] = bld<T>(rin[q], offn);
+            szz = bld<T>(rpw[0], off3);
+            gam = plastic ? bld<T>(rpw[4], off3) : (T)0;
         }
-        const int j3 = r - 3;
-        // row r - 3's pointwise planes: szz, and gam with plasticity (in mode II sxz and syz
-        // stay zero and the plastic multiplier lam is only written); a lane outside the strip
-        // gets an offset at or past every plane's bound (planes hold at most 2^32 - 256 bytes),
-        // far enough below 2^32 that the second dword of an 8-byte access does not wrap: the
-        // bound check makes its loads read 0 and drops its stores
-        const unsigned off3 = valid ? row_off(j3) : kDropOffset;
-        const T szz = bld<T>(rpw[0], off3);
-        const T gam = plastic ? bld<T>(rpw[4], off3) : (T)0;
         // y: differences, slopes of row r - 1, face values at r - 3/2
         T dry[4], dy1[4], fy1[4];
 #pragma unroll
@@ -333,11 +415,13 @@ k_godunov_march(const KArgs<T> a)
         }
         const int j2 = r - 2;
         // flux registers (multi-level): x faces of row r - 3, the lo y face of row r - 2
-        if (pt.F[0] && j3 >= y0 && lane >= 3 && lane <= tnx + 3)
+        if (flux0 && j3 >= y0 && lane >= 3 && lane <= tnx + 3)
             store_flux_side<T, 2>(pt, m, 0, f0v, f0s, (T)1, i, j3, sd);
-        if (pt.F[1] && j2 >= y0 && valid) store_flux_side<T, 2>(pt, m, 1, n.f1v, n.f1s, rvy, i, j2, sd);
+        if (flux1 && j2 >= y0 && valid) store_flux_side<T, 2>(pt, m, 1, n.f1v, n.f1s, rvy, i, j2, sd);
         // update of row r - 3
-        if (j3 >= y0) {
+        // FP32 runs the update on every row (rows before the strip store nothing), so that every
+        // row issues exactly MARCH_ROW_STORES stores for the ring's waits
+        if (ring_fetch || j3 >= y0) {
             Faces1<T, 2> F;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -360,11 +444,18 @@ k_godunov_march(const KArgs<T> a)
             u[2] = u[7] = u[8] = u[9] = (T)0;
             u[5] = szz;
             u[10] = gam;
-            cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
+            if constexpr (ring_fetch)
+                cell_update_side<T, 2>(pc, m, dt, K.sc, plastic, rvy, F, u);
+            else
+                cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
 #pragma unroll
             for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
             bst<T>(rpw[0], off3, u[5]);
-            if (plastic) {
+            if (ring_fetch) {
+                const unsigned offl = plastic ? off3 : kDropOffset;
+                bst<T>(rpw[3], offl, u[9]);
+                bst<T>(rpw[4], offl, u[10]);
+            } else if (plastic) {
                 bst<T>(rpw[3], off3, u[9]);
                 bst<T>(rpw[4], off3, u[10]);
             }
@@ -387,7 +478,6 @@ k_godunov_march(const KArgs<T> a)
 
     // two rows per trip, the carried state alternating between A and B (no copies between rows;
     // the DPP lane shifts are convergent, so the compiler does not unroll the loop itself)
-    const int rlast = y1 + 3;
     for (int r = y0 + 1;; r += 2) {
         row(r, A, B);
         if (r + 1 > rlast) break;

@@ -790,27 +790,40 @@ __device__ __forceinline__ void plastic_flow(const DevParams& P, T dt, T& sxx, T
 // g = dt lam G / (tau_new + dt lam G) and moves the mean by -dt lam K beta, so the new difference
 // is d - dt lam K beta - g dev (diagonal) and d - g s (shear): the same flow rule as plastic_flow
 // written as an increment.  A cell that does not yield keeps its values.
-template <typename T>
-__device__ __forceinline__ void plastic_flow_delta(const DevParams& P, T dt, T& dxx, T& dyy, T& dzz, T& dxy, T& dxz,
+// the FP32 Drucker-Prager constants held in LDS (pbg[7], pf[6], then G), for a kernel that keeps
+// no copy of them in VGPRs across its loop; volatile, so that every use reads them again
+struct PlasticConstL {
+    const volatile __attribute__((address_space(3))) float* c;
+};
+__device__ __forceinline__ float plastic_pbg(const DevParams& P, int k) { return P.pbg[k]; }
+__device__ __forceinline__ float plastic_pf(const DevParams& P, int k) { return P.pf[k]; }
+__device__ __forceinline__ float plastic_G(const DevParams& P) { return P.fG; }
+__device__ __forceinline__ float plastic_pbg(const PlasticConstL& P, int k) { return P.c[k]; }
+__device__ __forceinline__ float plastic_pf(const PlasticConstL& P, int k) { return P.c[7 + k]; }
+__device__ __forceinline__ float plastic_G(const PlasticConstL& P) { return P.c[13]; }
+
+// PP: DevParams or PlasticConstL
+template <typename T, typename PP>
+__device__ __forceinline__ void plastic_flow_delta(const PP& P, T dt, T& dxx, T& dyy, T& dzz, T& dxy, T& dxz,
                                                    T& dyz, T& plam, T& pgam)
 {
     const T md = (dxx + dyy + dzz) * (T)(1.0 / 3.0);
-    const T exx = (dxx - md) + (T)P.pbg[1], eyy = (dyy - md) + (T)P.pbg[2], ezz = (dzz - md) + (T)P.pbg[3];
-    const T sxy = dxy + (T)P.pbg[4], sxz = dxz + (T)P.pbg[5], syz = dyz + (T)P.pbg[6];
-    const T sigma = md + (T)P.pbg[0];
+    const T exx = (dxx - md) + (T)plastic_pbg(P, 1), eyy = (dyy - md) + (T)plastic_pbg(P, 2), ezz = (dzz - md) + (T)plastic_pbg(P, 3);
+    const T sxy = dxy + (T)plastic_pbg(P, 4), sxz = dxz + (T)plastic_pbg(P, 5), syz = dyz + (T)plastic_pbg(P, 6);
+    const T sigma = md + (T)plastic_pbg(P, 0);
     const T a = exx - eyy, b = eyy - ezz, c = ezz - exx;
     const T J2 = (a * a + b * b + c * c) * (T)(1.0 / 6.0) + sxy * sxy + sxz * sxz + syz * syz;
-    const T bb = (T)P.pf[1] + pgam * (T)P.pf[2];
-    const T ys = fmax(-(T)P.pf[0] * sigma + bb, (T)0);
+    const T bb = (T)plastic_pf(P, 1) + pgam * (T)plastic_pf(P, 2);
+    const T ys = fmax(-(T)plastic_pf(P, 0) * sigma + bb, (T)0);
     // yield iff sqrt(J2) > ys >= 0, i.e. J2 > ys^2: the root only where the cell yields
     if (!(J2 > ys * ys)) { plam = (T)0; return; }
     const T tau = sqrt(J2);
     const T Y = tau - ys;
-    plam = Y / (dt * (T)P.pf[3] + (T)P.pf[4]);
+    plam = Y / (dt * (T)plastic_pf(P, 3) + (T)plastic_pf(P, 4));
     pgam = pgam + dt * plam;
-    const T dG = dt * plam * (T)P.fG;
+    const T dG = dt * plam * (T)plastic_G(P);
     const T g = dG / tau;  // 1 - tau_new / (tau_new + dt lam G), tau_new = tau - dt lam G
-    const T dm = dt * plam * (T)P.pf[5];
+    const T dm = dt * plam * (T)plastic_pf(P, 5);
     dxx = dxx - (dm + g * exx);
     dyy = dyy - (dm + g * eyy);
     dzz = dzz - (dm + g * ezz);

@@ -349,7 +349,7 @@ int tmk_level_get_update_shape(const tmk_level *lev, int *march_rows, int *bx, i
 
 /* Autotuning of a level's update shape (no reference counterpart).  Times nsteps
  * graph-replayed steps from (time, dt), after two untimed ones, for every strip height of the
- * row-marching update (8 .. 64 rows, where the numerics allow it) and every tile shape, each
+ * row-marching update (4 .. 64 rows, where the numerics allow it) and every tile shape, each
  * from the level's current state, then sets the fastest shape and returns the level to the
  * state it started from (a device copy of the state is held meanwhile).  Not while the level
  * is in a batch or hierarchy or has fault stations. */

@@ -785,7 +785,7 @@ def test_autotune_restores_state_and_sets_fastest(precision):
     after = _downloaded(lev, boxes, U, B)
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     tried = r["tried"]
-    assert len(tried) == 18  # 10 strip heights, 8 tile shapes
+    assert len(tried) == 20  # 12 strip heights, 8 tile shapes
     assert r["ms_per_step"] == min(t[3] for t in tried) > 0.0
     assert (r["march_rows"], r["bx"], r["by"]) == lev.update_shape()
     assert (r["march_rows"], r["bx"], r["by"], r["ms_per_step"]) in [tuple(t) for t in tried]

@@ -2158,7 +2158,7 @@ extern "C" int tmk_level_get_update_shape(const tmk_level* L, int* march_rows, i
 }
 
 // the strip heights and tile shapes the tuner times
-static const int kTuneMarch[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64};
+static const int kTuneMarch[] = {4, 6, 8, 12, 16, 20, 24, 28, 32, 40, 48, 64};
 static const int kTuneTile[][2] = {{32, 16}, {32, 8}, {32, 12}, {32, 32}, {16, 16}, {64, 4}, {64, 8}, {64, 16}};
 
 // a device copy of a level's state (every island's current stencil planes, in-place planes and

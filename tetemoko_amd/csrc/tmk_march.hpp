@@ -118,6 +118,21 @@ __device__ __forceinline__ unsigned wait_vmem(unsigned addr)
     return addr;
 }
 
+// the wait before row k of a strip reads its ring slot (k counted from the strip's first row):
+// after the slot's loads come the loads of the MARCH_RING - 1 later slots and the stores of the
+// min(k, MARCH_RING - 1) rows since (the first slots are requested before any store)
+template <int K = 0>
+__device__ __forceinline__ unsigned wait_slot(int k, unsigned at)
+{
+    constexpr int N = (MARCH_RING - 1) * MARCH_RING_LOADS + K * MARCH_ROW_STORES;
+    if constexpr (K == MARCH_RING - 1) {
+        return wait_vmem<N>(at);
+    } else {
+        if (k <= K) return wait_vmem<N>(at);
+        return wait_slot<K + 1>(k, at);
+    }
+}
+
 // lane shifts inside the wavefront (lane l <- lane l -+ 1; the first / last lane reads 0), in a
 // form the compiler folds into the consuming FP32 instruction
 template <int CTRL>
@@ -218,7 +233,7 @@ k_godunov_march(const KArgs<T> a)
 
     const int rlast = y1 + 3;  // the last row loaded (row y1 is updated after it)
     constexpr bool ring_fetch = sizeof(T) == sizeof(float);
-    static_assert(MARCH_RING == 4, "the waits below are written out for four slots");
+    static_assert((MARCH_RING - 1) * (MARCH_RING_LOADS + MARCH_ROW_STORES) <= 63, "vmcnt counts to 63");
     // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL)
     __shared__ float ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64 + 16];
     float* const pconst = &ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64];
@@ -232,7 +247,7 @@ k_godunov_march(const KArgs<T> a)
     // the rows row(q) reads into slot (q - y0) mod MARCH_RING: every row issues exactly
     // MARCH_RING_LOADS loads (rows past the strip and lanes outside it read past the planes)
     auto request = [&](const int q) {
-        const unsigned sl = (unsigned)(q - y0) & (MARCH_RING - 1);
+        const unsigned sl = (unsigned)(q - y0) % (unsigned)MARCH_RING;
         const bool live = q <= rlast;
         const unsigned offs = live ? row_off(q) : kDropOffset;
         const unsigned offp = (live && valid && q - 3 >= y0) ? row_off(q - 3) : kDropOffset;
@@ -304,14 +319,9 @@ k_godunov_march(const KArgs<T> a)
             // that slot's loads: the MARCH_RING - 1 later slots (7 loads each) and the stores of
             // the rows since (8 each; the slots of the first rows were requested before any)
             request(r + MARCH_RING - 1);
-            const int k = r - (y0 + 1);
-            constexpr int L = (MARCH_RING - 1) * MARCH_RING_LOADS, S = MARCH_ROW_STORES;
-            const unsigned sl = (unsigned)(r - y0) & (MARCH_RING - 1);
-            unsigned at = ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float));
-            if (k >= 3) at = wait_vmem<L + 3 * S>(at);
-            else if (k == 2) at = wait_vmem<L + 2 * S>(at);
-            else if (k == 1) at = wait_vmem<L + S>(at);
-            else at = wait_vmem<L>(at);
+            const unsigned sl = (unsigned)(r - y0) % (unsigned)MARCH_RING;
+            const unsigned at =
+                wait_slot(r - (y0 + 1), ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float)));
             const __attribute__((address_space(3))) float* slot = (const __attribute__((address_space(3))) float*)at;
 #pragma unroll
             for (int q = 0; q < 5; ++q) w0[q] = slot[64 * q];

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import amr as oamr
+import tmo
 from faultcheck import SENTINEL, fault_errors
 from tetemoko_amd import amr, capi, scenarios
 
@@ -190,7 +191,10 @@ def test_levels_of_a_hierarchy_refuse_direct_steps():
     H.close()
 
 
-def _levels_equal(h, H, exact=True):
+def _levels_equal(h, H, exact=True, tol=1e-12):
+    """Every level of the device hierarchy H against the oracle's h; returns the largest relative
+    state error (per component, against its largest magnitude) over the levels."""
+    worst = 0.0
     for l, lev in enumerate(h.levels):
         Ug, Bg = H.download(l)
         mask = lev.valid
@@ -201,9 +205,12 @@ def _levels_equal(h, H, exact=True):
             assert np.array_equal(Bgv, Br), l
         else:
             for c in range(scenarios.NCOMP):
-                assert np.max(np.abs(Ugv[c] - Ur[c])) <= 1e-12 * max(np.max(np.abs(Ur[c])), 1e-300), (l, c)
+                rel = np.max(np.abs(Ugv[c] - Ur[c])) / max(np.max(np.abs(Ur[c])), 1e-300)
+                worst = max(worst, rel)
+                assert rel <= tol, (l, c, rel)
             e = fault_errors(Bgv, Br)
-            assert np.all(e <= 1e-12), (l, e)
+            assert np.all(e <= tol), (l, e)
+    return worst
 
 
 def _pair(cfg, boxes, r, arrays):
@@ -482,3 +489,53 @@ def test_separate_fine_islands_build_every_time_and_match_oracle():
         H = amr.Hierarchy(cfg, BOXES_ISL, 2, arrays=arrays)
         H.close()
     _compare(cfg, BOXES_ISL, 2, 3, arrays, exact=True)
+
+
+def test_c2_three_levels_ratio_four_rate_and_state_plastic_tagged_regrid():
+    """The multi-level workload on a small base: three levels at ratio 4 (dx 100, 25, 6.25 m),
+    rate-and-state with the aging law and Drucker-Prager viscoplasticity (cohesion 1 MPa), tags
+    on the velocity gradient, the stress gradient and the fault slip rate, and a regrid between
+    steps (AMRLevelLinElast.cpp:577-717, 784-949).  The bulk background carries a deviatoric
+    stress just below the yield stress (sxx, szz = -120 -+ 72.95 MPa, leaving the fault's normal
+    and shear traction as they are), so that the first waves make cells yield on every level
+    inside the short window.  Level 0's device tags equal the oracle's bit for bit.  Every level
+    stays within 1e-10 of the oracle before the regrid, right after it and two coarse steps later:
+    the last-bit differences of exp/log/asinh on fault faces (1e-16 relative) grow through the
+    plastic return, whose Y = tau - Y_s cancels about three digits with the bulk this close to
+    yield (measured: 3.2e-12, 3.7e-12 and 5.0e-12 at the three comparisons)."""
+    cfg = scenarios.config_c3(seed=6, nx=64, ny=32, dx=100.0)
+    cfg["rs"]["rupture_thr"] = 1e-6  # faces of every level rupture inside the window
+    cfg["background"] = (-120.0 + 72.95, -120.0, -120.0 - 72.95, 0.0, 0.0, 0.0)
+    r = 4
+    cfgs = _cfgs(cfg, 3, r)
+    boxes = [[(0, 0, 31, 31), (32, 0, 63, 31)], [(96, 0, 159, 31)], [(448, 0, 575, 31)]]
+    h, H = _pair(cfg, boxes, r, _background_arrays(cfgs))
+    assert h.advance(2) == 0
+    H.run(2)
+    H.sync()
+    errs = [_levels_equal(h, H, exact=False, tol=1e-10)]
+    crit = dict(refine_thresh=0.05, stress_thresh=0.02, sliprate_thresh=1e-4, tag_buffer=2)
+    full = dict(refine_thresh=0.01, plastic_thresh=1e-5, tag_buffer=2, x_coarsen=(-1e40, 1e40),
+                y_coarsen=(-1e40, 1e40), slope_coarsen=0.0, width_coarsen=1e40, x_coarsen2=(-1e40, 1e40),
+                y_coarsen2=(-1e40, 1e40), level=0, coarsen2_max_level=3, stress_thresh=0.0, sliprate_thresh=0.0)
+    full.update(crit)
+    want = tmo.tag_level(h.levels[0].op, h.levels[0].U, h.levels[0].B, full)
+    H.amr.close()  # a level tags on its own, outside the hierarchy (as Hierarchy.regrid does)
+    got = H.levels[0].tag(capi.TagParams.make(**full), cfg["nx"], cfg["ny"])
+    assert np.array_equal(got, want) and want.any()
+    counts = H.regrid(lambda l: capi.TagParams.make(level=l, **crit), fill_ratio=0.7, block_factor=4,
+                      max_grid_size=32)
+    assert len(counts) == 3 and counts[1] >= 2 and counts[2] >= 4, counts
+    for l in (1, 2):
+        h.set_layout(l, H.boxes[l], *scenarios.level_arrays(cfgs[l]))
+    errs.append(_levels_equal(h, H, exact=False, tol=1e-10))
+    assert h.advance(2) == 0
+    H.run(2)
+    H.sync()
+    errs.append(_levels_equal(h, H, exact=False, tol=1e-10))
+    print("largest relative state error: before the regrid, after it, two steps later:", errs, counts)
+    for lev in h.levels:
+        U = lev.U[:, G:-G, G:-G]
+        assert np.count_nonzero((U[9] > 0) & lev.valid) + np.count_nonzero((U[20] > 0) & lev.valid) > 0
+        assert np.count_nonzero(lev.B[9, G:-G][lev.fvalid] != SENTINEL) >= 8
+    H.close()

@@ -60,9 +60,30 @@ using namespace tmk;
 
 namespace {
 
+// grow-only device buffer owned by one object: freed when that object goes away
 struct DevBuf {
     void* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int ensure(size_t bytes)
     {
         if (bytes <= n) return TMK_OK;
@@ -466,9 +487,10 @@ static void fill_dev_params(const tmk_params& h, DevParams& P)
 
 // Streams of destroyed contexts, idle and synchronized, for the next contexts: creating and
 // destroying a stream per scenario took milliseconds of host time in a serving loop.  Kept
-// until the process ends; every stream is non-blocking at the default priority.
+// until the process ends (the list itself is never destroyed, so no static destructor runs
+// after the HIP runtime has gone); every stream is non-blocking at the default priority.
 static std::mutex g_stream_mu;
-static std::vector<hipStream_t> g_stream_pool;
+static std::vector<hipStream_t>& g_stream_pool = *new std::vector<hipStream_t>();
 
 static hipError_t stream_take(hipStream_t* s)
 {

@@ -322,7 +322,8 @@ k_godunov_march(const KArgs<T> a)
             const unsigned sl = (unsigned)(r - y0) % (unsigned)MARCH_RING;
             const unsigned at =
                 wait_slot(r - (y0 + 1), ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float)));
-            const __attribute__((address_space(3))) float* slot = (const __attribute__((address_space(3))) float*)at;
+            const __attribute__((address_space(3))) float* slot =
+                (const __attribute__((address_space(3))) float*)(uintptr_t)at;
 #pragma unroll
             for (int q = 0; q < 5; ++q) w0[q] = slot[64 * q];
             szz = slot[64 * 5];

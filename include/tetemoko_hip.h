@@ -105,8 +105,21 @@ int tmk_ctx_destroy(tmk_ctx *ctx);
 const char *tmk_strerror(int code);
 /* 1 when the library was built for gfx950 and a device is visible */
 int tmk_device_ok(void);
-/* free and total device memory (admission control of the serving loop) */
+/* free and total device memory (admission control of the serving loop); free_bytes includes
+ * the blocks the level-array cache holds on the current device */
 int tmk_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* Level-array cache (no counterpart in the reference, whose levels live for a whole run; for
+ * serving loops that create and destroy levels continuously).  Island, fault-row and owner
+ * arrays of levels, hierarchy members' flux arrays and face masks, and contexts' per-face
+ * parameter arrays are taken from a per-device cache of blocks in size classes (at most 1/8
+ * above the request) and go back to it when their owner is destroyed.  tmk_cache_info reports
+ * the current device's idle cached bytes, the bytes in use by live objects, and the limit on
+ * cached bytes per device (default 32 GiB); tmk_cache_set_limit sets that limit and frees idle
+ * blocks above it on every device (0 turns caching off); tmk_cache_trim frees every idle block
+ * of the current device. */
+int tmk_cache_info(size_t *cached_bytes, size_t *live_bytes, size_t *limit_bytes);
+int tmk_cache_set_limit(size_t bytes);
+int tmk_cache_trim(void);
 
 /* LinElastPatchGodunov::updateState (LinElastPatchGodunov.cpp:101-129), with
  * LinElastPhysics::computeUpdate (LinElastPhysics.cpp:135-179) and the IBC

@@ -44,15 +44,33 @@ def test_strerror_without_device():
     assert capi.strerror(-99) == "unknown error"
 
 
+def _cls(n):
+    """Bytes the level-array cache allocates for a request of n: n rounded up to a multiple of
+    1/8 of the largest power of two not above it (at least 256 bytes)."""
+    if n < 256:
+        return 256
+    g = max(256, (1 << (n.bit_length() - 1)) >> 3)
+    return -(-n // g) * g
+
+
+def test_size_classes_waste_under_an_eighth():
+    for n in [1, 255, 256, 257, 4095, 4096, 4097, 12345, 1 << 20, (1 << 20) + 1, 3 * (1 << 30) + 7]:
+        c = _cls(n)
+        assert c >= n and (n < 2048 or c - n < n / 8)
+
+
 def test_footprint_without_a_context():
-    """tmk_level_footprint_params is host code: the device bytes of a level before any context."""
+    """tmk_level_footprint_params is host code: the device bytes of a level before any context
+    (each array as the level-array cache allocates it)."""
     from tetemoko_amd import scenarios
     cfg = scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0)
     g = (96 + 8) * (48 + 8)
-    fault = 2 * (96 + 8) * 11 * 8 + (96 + 8) * 4
-    assert capi.level_footprint_params(cfg, [(0, 0, 95, 47)]) == 3 * 22 * g * 8 + g * 4 + fault
-    assert capi.level_footprint_params(cfg, [(0, 0, 95, 47)], capi.PREC_FP32_BULK) == 3 * 22 * g * 4 + g * 4 + fault
+    fault = 2 * _cls((96 + 8) * 11 * 8) + _cls((96 + 8) * 4)
+    assert capi.level_footprint_params(cfg, [(0, 0, 95, 47)]) == 3 * _cls(22 * g * 8) + _cls(g * 4) + fault
+    assert (capi.level_footprint_params(cfg, [(0, 0, 95, 47)], capi.PREC_FP32_BULK) ==
+            3 * _cls(22 * g * 4) + _cls(g * 4) + fault)
     # two boxes far apart are two islands, each over its own ghosted box; only one has fault data
     a, b = (16 + 8) * (16 + 8), (32 + 8) * (24 + 8)
-    want = 3 * 22 * (a + b) * 8 + (a + b) * 4 + 2 * (16 + 8) * 11 * 8 + (16 + 8) * 4
+    want = (3 * (_cls(22 * a * 8) + _cls(22 * b * 8)) + _cls(a * 4) + _cls(b * 4) +
+            2 * _cls((16 + 8) * 11 * 8) + _cls((16 + 8) * 4))
     assert capi.level_footprint_params(cfg, [(0, 0, 15, 15), (64, 24, 95, 47)]) == want

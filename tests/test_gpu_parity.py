@@ -590,9 +590,21 @@ def test_mem_info_and_footprint():
     cfg = scenarios.config_c1(seed=1, nx=96, ny=48, dx=25.0)
     ctx = capi.Context(cfg)
     n = capi.level_footprint(ctx, [(0, 0, 95, 47)])
+
+    def cls(b):  # bytes the level-array cache allocates for a request of b
+        g = max(256, (1 << (b.bit_length() - 1)) >> 3)
+        return max(256, -(-b // g) * g)
+
     # three state arrays over the ghosted box, its cell-owner map, two fault-data rows and their
     # face-owner map
-    assert n == 3 * 22 * (96 + 8) * (48 + 8) * 8 + (96 + 8) * (48 + 8) * 4 + 2 * (96 + 8) * 11 * 8 + (96 + 8) * 4
+    g = (96 + 8) * (48 + 8)
+    assert n == 3 * cls(22 * g * 8) + cls(g * 4) + 2 * cls((96 + 8) * 11 * 8) + cls((96 + 8) * 4)
+    # and that is what a level of that layout holds
+    live0 = capi.cache_info()[1]
+    lev = capi.Level(ctx, [(0, 0, 95, 47)])
+    assert capi.cache_info()[1] - live0 == n
+    lev.close()
+    assert capi.cache_info()[1] == live0
     # the context-free form agrees on every layout (one island, split islands, boxes off the fault)
     for boxes in ([(0, 0, 95, 47)], [(0, 0, 47, 47), (48, 0, 95, 47)], [(0, 0, 15, 15), (64, 24, 95, 47)]):
         assert capi.level_footprint_params(cfg, boxes) == capi.level_footprint(ctx, boxes), boxes

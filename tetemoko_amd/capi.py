@@ -35,6 +35,7 @@ PREC_FP32_BULK = 1
 
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
+    "tmk_cache_info", "tmk_cache_set_limit", "tmk_cache_trim",
     "tmk_level_footprint", "tmk_level_footprint_params", "tmk_batch_create", "tmk_batch_destroy",
     "tmk_batch_add", "tmk_batch_remove", "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill",
     "tmk_level_download_fault", "tmk_level_tag", "tmk_patch_update_state", "tmk_patch_plastic_update",
@@ -182,6 +183,9 @@ def lib():
         L.tmk_level_autotune.argtypes = [vp, _D, _D, c_int, ctypes.POINTER(Tuning)]
         sz = ctypes.c_size_t
         L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        L.tmk_cache_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        L.tmk_cache_set_limit.argtypes = [sz]
+        L.tmk_cache_trim.argtypes = []
         L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
         L.tmk_level_footprint_params.argtypes = [ctypes.POINTER(Params), c_int, c_int, PBox, ctypes.POINTER(sz)]
         L.tmk_batch_create.argtypes = [ctypes.POINTER(vp)]
@@ -541,6 +545,23 @@ def mem_info():
     f, t = ctypes.c_size_t(), ctypes.c_size_t()
     _check(lib().tmk_mem_info(ctypes.byref(f), ctypes.byref(t)), "tmk_mem_info")
     return f.value, t.value
+
+
+def cache_info():
+    """(idle cached, in use, limit) bytes of the level-array cache on the current device."""
+    c, u, m = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().tmk_cache_info(ctypes.byref(c), ctypes.byref(u), ctypes.byref(m)), "tmk_cache_info")
+    return c.value, u.value, m.value
+
+
+def cache_set_limit(nbytes):
+    """Limit on the level-array cache's idle bytes per device (0: no caching)."""
+    _check(lib().tmk_cache_set_limit(int(nbytes)), "tmk_cache_set_limit")
+
+
+def cache_trim():
+    """Free every idle block of the level-array cache on the current device."""
+    _check(lib().tmk_cache_trim(), "tmk_cache_trim")
 
 
 def level_footprint(ctx, boxes):

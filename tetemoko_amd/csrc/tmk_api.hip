@@ -13,8 +13,10 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 #include <array>
 
@@ -84,23 +86,154 @@ struct DevBuf {
         return *this;
     }
     ~DevBuf() { release(); }
-    int ensure(size_t bytes)
-    {
-        if (bytes <= n) return TMK_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc(&p, bytes) != hipSuccess) return TMK_E_NOMEM;
-        n = bytes;
-        return TMK_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
+    int ensure(size_t bytes);
+    void release();
 };
+
+// ---------------------------------------------------------------------------
+// Block cache for level arrays.  A serving loop creates and destroys levels without pause, and
+// hipMalloc / hipFree of their arrays cost milliseconds each while the device is busy (hipFree
+// waits for it).  The island, fault-row and owner arrays of levels and the flux arrays and face
+// masks of hierarchy members come from a per-device cache of blocks in size classes (at most 1/8
+// above the request) and go back to it.  A block goes back only when no queued work can touch it:
+// tmk_level_destroy refuses batch members and synchronises the level's stream, and a hierarchy
+// member is released after its batch's stream has been synchronised.  Cached bytes count as free
+// in tmk_mem_info; a failed hipMalloc empties the device's cache and tries once more.
+struct BlockCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void*>> idle;  // (device, class bytes) -> blocks
+    std::unordered_map<void*, std::pair<int, size_t>> live;      // block -> (device, class bytes)
+    std::map<int, size_t> cached, in_use;                        // bytes per device
+    size_t limit = (size_t)32 << 30;                             // cached bytes per device, at most
+};
+
+static BlockCache& block_cache()
+{
+    static BlockCache* c = new BlockCache;  // never destroyed: blocks stay valid until exit
+    return *c;
+}
+
+static size_t size_class(size_t n)
+{
+    if (n < 256) return 256;
+    const size_t g = std::max<size_t>(256, ((size_t)1 << (63 - __builtin_clzll(n))) >> 3);
+    return (n + g - 1) / g * g;
+}
+
+// frees every idle block of device dev (the caller holds no lock)
+static void cache_trim_device(int dev, size_t keep)
+{
+    BlockCache& C = block_cache();
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        for (auto it = C.idle.begin(); it != C.idle.end() && C.cached[dev] > keep;) {
+            if (it->first.first != dev) { ++it; continue; }
+            while (!it->second.empty() && C.cached[dev] > keep) {
+                drop.push_back(it->second.back());
+                it->second.pop_back();
+                C.cached[dev] -= it->first.second;
+            }
+            it = it->second.empty() ? C.idle.erase(it) : std::next(it);
+        }
+    }
+    for (void* p : drop) (void)hipFree(p);
+}
+
+static hipError_t dev_alloc(void** p, size_t n)
+{
+    BlockCache& C = block_cache();
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t cls = size_class(n);
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        auto it = C.idle.find({dev, cls});
+        if (it != C.idle.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            C.cached[dev] -= cls;
+            C.in_use[dev] += cls;
+            C.live[*p] = {dev, cls};
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, cls);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_trim_device(dev, 0);
+        e = hipMalloc(p, cls);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    std::lock_guard<std::mutex> lk(C.mu);
+    C.live[*p] = {dev, cls};
+    C.in_use[dev] += cls;
+    return hipSuccess;
+}
+
+template <typename X>
+static hipError_t dev_alloc(X** p, size_t n)
+{
+    void* q = nullptr;
+    const hipError_t e = dev_alloc(&q, n);
+    *p = (X*)q;
+    return e;
+}
+
+static void dev_free(void* p)
+{
+    if (!p) return;
+    BlockCache& C = block_cache();
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        auto it = C.live.find(p);
+        if (it != C.live.end()) {
+            const int dev = it->second.first;
+            const size_t cls = it->second.second;
+            C.live.erase(it);
+            C.in_use[dev] -= cls;
+            if (C.cached[dev] + cls <= C.limit) {
+                C.idle[{dev, cls}].push_back(p);
+                C.cached[dev] += cls;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
+// A DevBuf's block comes from the block cache; it grows or goes back only when no queued work
+// reads it (every owner synchronises the streams that use it first).
+int DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= n) return TMK_OK;
+    release();
+    if (dev_alloc(&p, bytes) != hipSuccess) return TMK_E_NOMEM;
+    n = bytes;
+    return TMK_OK;
+}
+
+void DevBuf::release()
+{
+    dev_free(p);
+    p = nullptr;
+    n = 0;
+}
+
+// A copy ordered on stream st, complete on return.  hipMemcpy runs on the null stream, which
+// waits for every stream of the device: a serving loop's host work would stall behind other
+// scenarios' running launches.
+static hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t n, hipMemcpyKind kind)
+{
+    if (!n) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
 
 inline bool box_contains(const tmk_box& o, const tmk_box& b)
 {
@@ -430,6 +563,44 @@ extern "C" int tmk_mem_info(size_t* free_bytes, size_t* total_bytes)
 {
     if (!free_bytes || !total_bytes) return TMK_E_ARG;
     HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    BlockCache& C = block_cache();
+    std::lock_guard<std::mutex> lk(C.mu);
+    *free_bytes += C.cached[dev];
+    return TMK_OK;
+}
+
+extern "C" int tmk_cache_info(size_t* cached_bytes, size_t* live_bytes, size_t* limit_bytes)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    BlockCache& C = block_cache();
+    std::lock_guard<std::mutex> lk(C.mu);
+    if (cached_bytes) *cached_bytes = C.cached[dev];
+    if (live_bytes) *live_bytes = C.in_use[dev];
+    if (limit_bytes) *limit_bytes = C.limit;
+    return TMK_OK;
+}
+
+extern "C" int tmk_cache_set_limit(size_t bytes)
+{
+    BlockCache& C = block_cache();
+    std::vector<int> devs;
+    {
+        std::lock_guard<std::mutex> lk(C.mu);
+        C.limit = bytes;
+        for (const auto& kv : C.cached) devs.push_back(kv.first);
+    }
+    for (int d : devs) cache_trim_device(d, bytes);
+    return TMK_OK;
+}
+
+extern "C" int tmk_cache_trim(void)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    cache_trim_device(dev, 0);
     return TMK_OK;
 }
 
@@ -558,7 +729,7 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
         const size_t n = (size_t)p->nface;
         std::vector<double> host(n * nfa);
         double* d = nullptr;
-        if (hipMalloc(&d, sizeof(double) * host.size()) != hipSuccess) { tmk_ctx_destroy(c); return TMK_E_NOMEM; }
+        if (dev_alloc(&d, sizeof(double) * host.size()) != hipSuccess) { tmk_ctx_destroy(c); return TMK_E_NOMEM; }
         c->face_dev.push_back(d);
         int q = 0;
         for (int k = 0; k < 8; ++k) {
@@ -567,7 +738,7 @@ extern "C" int tmk_ctx_create(tmk_ctx** out, const tmk_params* p, int precision)
             *fd[k] = d + q * n;
             ++q;
         }
-        if (hipMemcpy(d, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (copy_on(c->stream, d, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
             tmk_ctx_destroy(c);
             return TMK_E_HIP;
         }
@@ -581,7 +752,7 @@ extern "C" int tmk_ctx_destroy(tmk_ctx* c)
 {
     if (!c) return TMK_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (double* d : c->face_dev) (void)hipFree(d);
+    for (double* d : c->face_dev) dev_free(d);  // the stream is idle and the levels are gone
     for (auto& b : c->bufs) b.release();
     if (c->stream) stream_give(c->stream);
     delete c;
@@ -914,11 +1085,11 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     }
     // the tables are rewritten in place: every launch that read them has completed
     // (the callers synchronise the group's stream first; the side stream joins it)
-    auto put = [](DevBuf& buf, const void* src, size_t bytes, void** dst) -> int {
+    auto put = [&G](DevBuf& buf, const void* src, size_t bytes, void** dst) -> int {
         int e = buf.ensure(std::max<size_t>(bytes, 1));
         if (e) return e;
         *dst = buf.p;
-        if (bytes && hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return TMK_E_HIP;
+        if (copy_on(G.side, buf.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return TMK_E_HIP;
         return TMK_OK;
     };
     int e;
@@ -957,8 +1128,7 @@ static int group_set_clock(Group& G, hipStream_t st, int m, double time, double 
 static int group_error(Group& G, hipStream_t st, int m, int* code)
 {
     int e = 0;
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(&e, (char*)(G.scen_dev + m) + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(copy_on(st, &e, (char*)(G.scen_dev + m) + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost));
     *code = e;
     return TMK_OK;
 }
@@ -1207,9 +1377,9 @@ static int patch_update_t(tmk_ctx* c, tmk_box valid, double* U, tmk_box ub, doub
         HIPCHK(hipMemcpyAsync(&herr, meta + off_s + offsetof(Scen, err), sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (herr) return herr;
-        if (B) HIPCHK(hipMemcpy(B, Bd + nB, nB * sizeof(double), hipMemcpyDeviceToHost));
-        if (F0) HIPCHK(hipMemcpy(F0, Fd, nF0 * sizeof(double), hipMemcpyDeviceToHost));
-        if (F1) HIPCHK(hipMemcpy(F1, Fd + nF0, nF1 * sizeof(double), hipMemcpyDeviceToHost));
+        if (B) HIPCHK(copy_on(st, B, Bd + nB, nB * sizeof(double), hipMemcpyDeviceToHost));
+        if (F0) HIPCHK(copy_on(st, F0, Fd, nF0 * sizeof(double), hipMemcpyDeviceToHost));
+        if (F1) HIPCHK(copy_on(st, F1, Fd + nF0, nF1 * sizeof(double), hipMemcpyDeviceToHost));
         if (stores_delta<T>()) {
             if (F0) add_bg_flux(c->P, 0, F0, nF0);
             if (F1) add_bg_flux(c->P, 1, F1, nF1);
@@ -1529,8 +1699,8 @@ static size_t level_bytes(int precision, int fault_kind, tmk_box domain, int nbo
     build_islands(std::vector<tmk_box>(boxes, boxes + nboxes), domain, isl, bi);
     size_t tot = 0;
     for (const Island& I : isl) {
-        tot += 3 * I.plane() * TMK_NCOMP * ts + I.plane() * sizeof(int);
-        if (I.has_b) tot += 2 * (size_t)I.bnx * nb * sizeof(double) + I.bnx * sizeof(int);
+        tot += 3 * size_class(I.plane() * TMK_NCOMP * ts) + size_class(I.plane() * sizeof(int));
+        if (I.has_b) tot += 2 * size_class((size_t)I.bnx * nb * sizeof(double)) + size_class(I.bnx * sizeof(int));
     }
     return tot;
 }
@@ -1555,14 +1725,14 @@ static void level_free_arrays(tmk_level* L)
 {
     for (Island& I : L->isl) {
         for (int k = 0; k < 2; ++k) {
-            if (I.u[k]) (void)hipFree(I.u[k]);
-            if (I.b[k]) (void)hipFree(I.b[k]);
+            dev_free(I.u[k]);
+            dev_free(I.b[k]);
             I.u[k] = nullptr;
             I.b[k] = nullptr;
         }
-        if (I.w) (void)hipFree(I.w);
-        if (I.owner) (void)hipFree(I.owner);
-        if (I.bowner) (void)hipFree(I.bowner);
+        dev_free(I.w);
+        dev_free(I.owner);
+        dev_free(I.bowner);
         I.w = nullptr;
         I.owner = I.bowner = nullptr;
     }
@@ -1591,6 +1761,49 @@ static void level_build_tiles(tmk_level* L)
     }
 }
 
+// Owner maps built on the device: after a fill with -1 (all bytes 0xff), workgroup row y of a
+// launch writes box id[y] over its rectangle, cell (i, j) at (i - ax) + (j - ay) * unx.  The
+// boxes travel in the kernel arguments, so creating a level copies nothing from the host.
+constexpr int kOwnerBoxes = 32;
+struct OwnerBoxes {
+    int lo0[kOwnerBoxes], lo1[kOwnerBoxes], w[kOwnerBoxes], h[kOwnerBoxes], id[kOwnerBoxes];
+};
+
+static __global__ void k_owner_boxes(int* owner, int unx, int ax, int ay, OwnerBoxes ob)
+{
+    const int y = blockIdx.y;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = ob.w[y];
+    if (t >= (long long)w * ob.h[y]) return;
+    const int i = ob.lo0[y] + (int)(t % w), j = ob.lo1[y] + (int)(t / w);
+    owner[(size_t)(i - ax) + (size_t)(j - ay) * unx] = ob.id[y];
+}
+
+// the rectangles (lo0, lo1, w, h) of boxes id into owner, kOwnerBoxes per launch
+static int owner_launch(hipStream_t st, int* owner, int unx, int ax, int ay,
+                        const std::vector<std::array<int, 5>>& rects)
+{
+    for (size_t k0 = 0; k0 < rects.size(); k0 += kOwnerBoxes) {
+        OwnerBoxes ob;
+        memset(&ob, 0, sizeof ob);
+        const int n = (int)std::min<size_t>(kOwnerBoxes, rects.size() - k0);
+        long long most = 1;
+        for (int y = 0; y < n; ++y) {
+            const std::array<int, 5>& r = rects[k0 + y];
+            ob.lo0[y] = r[0];
+            ob.lo1[y] = r[1];
+            ob.w[y] = r[2];
+            ob.h[y] = r[3];
+            ob.id[y] = r[4];
+            most = std::max(most, (long long)r[2] * r[3]);
+        }
+        hipLaunchKernelGGL(k_owner_boxes, dim3((unsigned)((most + 255) / 256), (unsigned)n), dim3(256), 0, st, owner,
+                           unx, ax, ay, ob);
+        HIPCHK(hipGetLastError());
+    }
+    return TMK_OK;
+}
+
 extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tmk_level** out)
 {
     if (!c || !out || nboxes <= 0 || !boxes) return TMK_E_ARG;
@@ -1617,30 +1830,30 @@ extern "C" int tmk_level_create(tmk_ctx* c, int nboxes, const tmk_box* boxes, tm
         const size_t bytes = plane * TMK_NCOMP * L->tsize;
         // the kernels address a plane with 32-bit byte offsets (gld/gst)
         ok = ok && plane * L->tsize <= kPlaneMaxBytes;
-        ok = ok && hipMalloc(&I.u[0], bytes) == hipSuccess && hipMalloc(&I.u[1], bytes) == hipSuccess &&
-             hipMalloc(&I.w, bytes) == hipSuccess && hipMalloc((void**)&I.owner, plane * sizeof(int)) == hipSuccess;
+        ok = ok && dev_alloc(&I.u[0], bytes) == hipSuccess && dev_alloc(&I.u[1], bytes) == hipSuccess &&
+             dev_alloc(&I.w, bytes) == hipSuccess && dev_alloc(&I.owner, plane * sizeof(int)) == hipSuccess;
+        // owner maps, enqueued on the context's stream: every use of the level is on that stream
+        // or synchronises it first (batch and hierarchy membership, regrid, destroy)
         if (ok) {
-            std::vector<int> own(plane, -1);
-            const int unx = I.unx();
+            std::vector<std::array<int, 5>> rects;
             for (int k : I.boxes) {
                 const tmk_box v = L->boxes[k];
-                for (int j = v.lo[1]; j <= v.hi[1]; ++j)
-                    for (int i = v.lo[0]; i <= v.hi[0]; ++i)
-                        own[(size_t)(i - I.abox.lo[0]) + (size_t)(j - I.abox.lo[1]) * unx] = k;
+                rects.push_back({v.lo[0], v.lo[1], v.hi[0] - v.lo[0] + 1, v.hi[1] - v.lo[1] + 1, k});
             }
-            ok = hipMemcpy(I.owner, own.data(), plane * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+            ok = hipMemsetAsync(I.owner, 0xff, plane * sizeof(int), c->stream) == hipSuccess &&
+                 owner_launch(c->stream, I.owner, I.unx(), I.abox.lo[0], I.abox.lo[1], rects) == TMK_OK;
         }
         if (ok && I.has_b) {
             const size_t nB = (size_t)I.bnx * nb;
-            ok = hipMalloc((void**)&I.b[0], nB * sizeof(double)) == hipSuccess &&
-                 hipMalloc((void**)&I.b[1], nB * sizeof(double)) == hipSuccess &&
-                 hipMalloc((void**)&I.bowner, I.bnx * sizeof(int)) == hipSuccess;
+            ok = dev_alloc(&I.b[0], nB * sizeof(double)) == hipSuccess &&
+                 dev_alloc(&I.b[1], nB * sizeof(double)) == hipSuccess &&
+                 dev_alloc(&I.bowner, I.bnx * sizeof(int)) == hipSuccess;
             if (ok) {
-                std::vector<int> own(I.bnx, -1);
+                std::vector<std::array<int, 5>> rects;
                 for (int k : I.boxes)
-                    if (L->has_b[k])
-                        for (int i = L->boxes[k].lo[0]; i <= L->boxes[k].hi[0]; ++i) own[i - I.blo0] = k;
-                ok = hipMemcpy(I.bowner, own.data(), I.bnx * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+                    if (L->has_b[k]) rects.push_back({L->boxes[k].lo[0], 0, L->boxes[k].hi[0] - L->boxes[k].lo[0] + 1, 1, k});
+                ok = hipMemsetAsync(I.bowner, 0xff, I.bnx * sizeof(int), c->stream) == hipSuccess &&
+                     owner_launch(c->stream, I.bowner, I.bnx, I.blo0, 0, rects) == TMK_OK;
             }
         }
         if (!ok) break;
@@ -1767,7 +1980,7 @@ static int level_put_fault(tmk_level* L, int b, const double* B)
     const int n = L->boxes[b].hi[0] - L->boxes[b].lo[0] + 1 + 2 * TMK_GHOST;
     int e;
     if ((e = c->bufs[0].ensure((size_t)n * nb * sizeof(double)))) return e;
-    HIPCHK(hipMemcpy(c->bufs[0].p, B, (size_t)n * nb * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_on(c->stream, c->bufs[0].p, B, (size_t)n * nb * sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_row_put, dim3((n + 255) / 256), dim3(256), 0, c->stream, I.b[0], I.b[1], I.bowner,
                        L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (const double*)c->bufs[0].p, n, nb, b);
     HIPCHK(hipGetLastError());
@@ -1786,8 +1999,7 @@ static int level_get_fault(tmk_level* L, int b, double* B, hipStream_t st)
     hipLaunchKernelGGL(k_row_get, dim3((n + 255) / 256), dim3(256), 0, st, I.b[L->cur],
                        L->boxes[b].lo[0] - TMK_GHOST - I.blo0, I.bnx, (double*)c->bufs[0].p, n, nb);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipMemcpy(B, c->bufs[0].p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(copy_on(st, B, c->bufs[0].p, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToHost));
     return TMK_OK;
 }
 
@@ -1816,7 +2028,7 @@ extern "C" int tmk_level_upload(tmk_level* L, int b, const double* U, const doub
     HIPCHK(hipStreamSynchronize(c->stream));
     int e;
     if ((e = c->bufs[7].ensure(nU * L->tsize))) return e;
-    HIPCHK(hipMemcpy(c->bufs[7].p, src, nU * L->tsize, hipMemcpyHostToDevice));
+    HIPCHK(copy_on(c->stream, c->bufs[7].p, src, nU * L->tsize, hipMemcpyHostToDevice));
     const Island& I = L->isl[L->box_isl[b]];
     const int ox = ub.lo[0] - I.abox.lo[0], oy = ub.lo[1] - I.abox.lo[1];
     const dim3 grid((unsigned)((plane + 255) / 256));
@@ -1897,10 +2109,10 @@ extern "C" int tmk_level_download(tmk_level* L, int b, double* U, double* B)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     if (L->tsize == sizeof(double)) {
-        HIPCHK(hipMemcpy(U, c->bufs[7].p, nU * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(copy_on(st, U, c->bufs[7].p, nU * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         std::vector<float> a(nU);
-        HIPCHK(hipMemcpy(a.data(), c->bufs[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(copy_on(st, a.data(), c->bufs[7].p, nU * sizeof(float), hipMemcpyDeviceToHost));
         from_stored_f32(a.data(), U, plane, c->P);
     }
     if (L->has_b[b] && B) return level_get_fault(L, b, B, st);
@@ -2088,9 +2300,13 @@ extern "C" int tmk_level_profile_kernels(tmk_level* L, int nsteps, double time, 
                 L->cur ^= 1;
                 L->time = L->time + dt;
                 const long long* x = &h[(size_t)6 * k];
-                sum[0] += (double)(x[5] - x[4]) / khz;  // kHz: ticks per ms
-                if (has_int) sum[1] += (double)(x[1] - x[0]) / khz;
-                if (has_bnd) sum[2] += (double)(x[3] - x[2]) / khz;
+                // the wall clock counts up and wraps as an unsigned 64-bit counter
+                auto ticks = [](long long a, long long b) {
+                    return (double)(long long)((unsigned long long)b - (unsigned long long)a);
+                };
+                sum[0] += ticks(x[4], x[5]) / khz;  // kHz: ticks per ms
+                if (has_int) sum[1] += ticks(x[0], x[1]) / khz;
+                if (has_bnd) sum[2] += ticks(x[2], x[3]) / khz;
             }
             in_graph = 1;
         }
@@ -2145,7 +2361,8 @@ extern "C" int tmk_level_sync(tmk_level* L)
     if (e) return e;
     if (code) {
         const int z = 0;
-        HIPCHK(hipMemcpy((char*)(L->own.scen_dev) + offsetof(Scen, err), &z, sizeof z, hipMemcpyHostToDevice));
+        HIPCHK(copy_on(L->ctx->stream, (char*)(L->own.scen_dev) + offsetof(Scen, err), &z, sizeof z,
+                       hipMemcpyHostToDevice));
     }
     return code;
 }
@@ -2318,7 +2535,7 @@ static int batch_detach_all(tmk_batch* B)
     if (!B->built) return TMK_OK;
     std::vector<Scen> sc(B->members.size());
     if (!sc.empty())
-        HIPCHK(hipMemcpy(sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+        HIPCHK(copy_on(B->stream, sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
     for (size_t m = 0; m < B->members.size(); ++m) {
         tmk_level* L = B->members[m];
         L->cur = B->phase[m] ^ B->grp.g;
@@ -2439,11 +2656,12 @@ extern "C" int tmk_stations_create(tmk_level* L, int n, const double* x, int cap
     if (!e) {
         char* m = (char*)S->meta.p;
         const int zero[2] = {0, 0};
-        if (hipMemcpy(m, d.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m + n * sizeof(double), patch.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(m + n * (sizeof(double) + sizeof(int)), face.data(), n * sizeof(int), hipMemcpyHostToDevice) !=
-                hipSuccess ||
-            hipMemcpy(S->count.p, zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
+        hipStream_t st = c->stream;
+        if (copy_on(st, m, d.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            copy_on(st, m + n * sizeof(double), patch.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            copy_on(st, m + n * (sizeof(double) + sizeof(int)), face.data(), n * sizeof(int),
+                    hipMemcpyHostToDevice) != hipSuccess ||
+            copy_on(st, S->count.p, zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess)
             e = TMK_E_HIP;
         S->dev.n = n;
         S->dev.nb = nb;
@@ -2469,15 +2687,15 @@ extern "C" int tmk_stations_create(tmk_level* L, int n, const double* x, int cap
 extern "C" int tmk_stations_read(tmk_stations* S, double* records, int max_records, int* nrecords)
 {
     if (!S || !S->L || !records || !nrecords || max_records < S->dev.capacity) return TMK_E_ARG;
-    HIPCHK(hipStreamSynchronize(S->L->ctx->stream));
+    hipStream_t st = S->L->ctx->stream;
     int cnt[2] = {0, 0};
-    HIPCHK(hipMemcpy(cnt, S->count.p, sizeof cnt, hipMemcpyDeviceToHost));
+    HIPCHK(copy_on(st, cnt, S->count.p, sizeof cnt, hipMemcpyDeviceToHost));
     const int n = std::min(cnt[0], S->dev.capacity);
     if (n > 0)
-        HIPCHK(hipMemcpy(records, S->rec.p, (size_t)n * S->dev.n * (1 + S->dev.nb) * sizeof(double),
-                         hipMemcpyDeviceToHost));
+        HIPCHK(copy_on(st, records, S->rec.p, (size_t)n * S->dev.n * (1 + S->dev.nb) * sizeof(double),
+                       hipMemcpyDeviceToHost));
     const int zero[2] = {0, 0};
-    HIPCHK(hipMemcpy(S->count.p, zero, sizeof zero, hipMemcpyHostToDevice));
+    HIPCHK(copy_on(st, S->count.p, zero, sizeof zero, hipMemcpyHostToDevice));
     *nrecords = n;
     return cnt[1] ? TMK_E_OVERFLOW : TMK_OK;
 }
@@ -2514,7 +2732,7 @@ extern "C" int tmk_batch_sync(tmk_batch* B)
     HIPCHK(hipStreamSynchronize(B->stream));
     if (!B->built) return TMK_OK;
     std::vector<Scen> sc(B->members.size());
-    HIPCHK(hipMemcpy(sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_on(B->stream, sc.data(), B->grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
     for (const Scen& s : sc)
         if (s.err) return s.err;
     return TMK_OK;
@@ -2634,21 +2852,40 @@ static inline int flux_off(const tmk_box& v, int d, int fi, int fj)
 }
 
 template <typename X>
-static int upload_list(DevBuf& buf, const std::vector<X>& v)
+static int upload_list(DevBuf& buf, const std::vector<X>& v, hipStream_t st)
 {
     int e = buf.ensure(std::max<size_t>(sizeof(X) * v.size(), 1));
     if (e) return e;
-    if (!v.empty()) HIPCHK(hipMemcpy(buf.p, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+    HIPCHK(copy_on(st, buf.p, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
     return TMK_OK;
 }
 
-// coverage maps over the coarse domain of the pair (coarse c, fine f)
+// Coverage maps of the pair (coarse c, fine f).  The 2-D maps cover a window of the coarse
+// domain: the coarsened bounding box of the fine boxes grown by kPairMargin cells, which holds
+// every coarse cell the pair's lists read (fine ghost cells reach TMK_GHOST / r coarse cells out,
+// their interpolation stencils one more, the register faces one).  c_at answers outside the
+// window too, from the coarse boxes, so the window's size decides speed and never results.
+constexpr int kPairMargin = 4;
 struct PairMaps {
-    int ncx = 0, ncy = 0;
-    std::vector<int> own_c;   // coarse box holding the cell
-    std::vector<int> own_f;   // fine box covering the cell (coarsened)
-    std::vector<int> own_cb;  // coarse box with fault data holding fault face i
-    std::vector<int> own_fb;  // fine box with fault data covering coarse fault face i
+    int ncx = 0, ncy = 0;                  // coarse domain
+    int wx0 = 0, wy0 = 0, wnx = 0, wny = 0;  // window of own_c / own_f
+    const tmk_level* C = nullptr;
+    std::vector<int> own_c;   // coarse box holding the cell (window)
+    std::vector<int> own_f;   // fine box covering the cell, coarsened (window)
+    std::vector<int> own_cb;  // coarse box with fault data holding fault face i (domain)
+    std::vector<int> own_fb;  // fine box with fault data covering coarse fault face i (domain)
+    bool in_w(int i, int j) const { return i >= wx0 && i < wx0 + wnx && j >= wy0 && j < wy0 + wny; }
+    size_t wq(int i, int j) const { return (size_t)(i - wx0) + (size_t)(j - wy0) * wnx; }
+    int c_at(int i, int j) const
+    {
+        if (in_w(i, j)) return own_c[wq(i, j)];
+        for (size_t b = 0; b < C->boxes.size(); ++b) {
+            const tmk_box& v = C->boxes[b];
+            if (i >= v.lo[0] && i <= v.hi[0] && j >= v.lo[1] && j <= v.hi[1]) return (int)b;
+        }
+        return -1;
+    }
+    int f_at(int i, int j) const { return in_w(i, j) ? own_f[wq(i, j)] : -1; }  // the window holds every fine box
 };
 
 static int pair_maps(const tmk_level* C, const tmk_level* Fl, int r, PairMaps& m)
@@ -2657,35 +2894,49 @@ static int pair_maps(const tmk_level* C, const tmk_level* Fl, int r, PairMaps& m
     if (cd.lo[0] != 0 || cd.lo[1] != 0 || fd.lo[0] != 0 || fd.lo[1] != 0) return TMK_E_NESTING;
     if (fd.hi[0] + 1 != (cd.hi[0] + 1) * r || fd.hi[1] + 1 != (cd.hi[1] + 1) * r) return TMK_E_NESTING;
     if (Fl->ctx->hp.dx * r != C->ctx->hp.dx) return TMK_E_NESTING;
+    m.C = C;
     m.ncx = cd.hi[0] + 1;
     m.ncy = cd.hi[1] + 1;
-    const size_t n = (size_t)m.ncx * m.ncy;
+    int x0 = m.ncx, y0 = m.ncy, x1 = -1, y1 = -1;
+    for (const tmk_box& v : Fl->boxes) {
+        for (int d = 0; d < 2; ++d)
+            if (v.lo[d] % r != 0 || (v.hi[d] + 1) % r != 0) return TMK_E_NESTING;
+        x0 = std::min(x0, v.lo[0] / r);
+        y0 = std::min(y0, v.lo[1] / r);
+        x1 = std::max(x1, v.hi[0] / r);
+        y1 = std::max(y1, v.hi[1] / r);
+    }
+    if (x1 < 0) return TMK_E_NESTING;
+    m.wx0 = std::max(0, x0 - kPairMargin);
+    m.wy0 = std::max(0, y0 - kPairMargin);
+    m.wnx = std::min(m.ncx - 1, x1 + kPairMargin) - m.wx0 + 1;
+    m.wny = std::min(m.ncy - 1, y1 + kPairMargin) - m.wy0 + 1;
+    const size_t n = (size_t)m.wnx * m.wny;
     m.own_c.assign(n, -1);
     m.own_f.assign(n, -1);
     m.own_cb.assign(m.ncx, -1);
     m.own_fb.assign(m.ncx, -1);
+    const tmk_box win{{m.wx0, m.wy0}, {m.wx0 + m.wnx - 1, m.wy0 + m.wny - 1}};
     for (size_t b = 0; b < C->boxes.size(); ++b) {
         const tmk_box v = C->boxes[b];
-        for (int j = v.lo[1]; j <= v.hi[1]; ++j)
-            for (int i = v.lo[0]; i <= v.hi[0]; ++i) m.own_c[(size_t)j * m.ncx + i] = (int)b;
+        const tmk_box x = box_and(v, win);
+        for (int j = x.lo[1]; j <= x.hi[1]; ++j)
+            for (int i = x.lo[0]; i <= x.hi[0]; ++i) m.own_c[m.wq(i, j)] = (int)b;
         if (v.lo[1] == 0)
             for (int i = v.lo[0]; i <= v.hi[0]; ++i) m.own_cb[i] = (int)b;
     }
     for (size_t b = 0; b < Fl->boxes.size(); ++b) {
         const tmk_box v = Fl->boxes[b];
-        for (int d = 0; d < 2; ++d)
-            if (v.lo[d] % r != 0 || (v.hi[d] + 1) % r != 0) return TMK_E_NESTING;
         for (int j = v.lo[1] / r; j <= v.hi[1] / r; ++j)
             for (int i = v.lo[0] / r; i <= v.hi[0] / r; ++i) {
-                if (m.own_c[(size_t)j * m.ncx + i] < 0) return TMK_E_NESTING;
-                m.own_f[(size_t)j * m.ncx + i] = (int)b;
+                if (m.own_c[m.wq(i, j)] < 0) return TMK_E_NESTING;
+                m.own_f[m.wq(i, j)] = (int)b;
             }
         if (v.lo[1] == 0)
             for (int i = v.lo[0] / r; i <= v.hi[0] / r; ++i) m.own_fb[i] = (int)b;
     }
     return TMK_OK;
 }
-
 
 // register faces of the pair in the order (direction, side, j, i): an uncovered coarse
 // cell with the refined region on that side.  fn(d, side, i, j, coarse box, fine box)
@@ -2695,14 +2946,14 @@ static void for_each_reg_face(const PairMaps& m, Fn fn)
     const int ncx = m.ncx, ncy = m.ncy;
     for (int d = 0; d < 2; ++d)
         for (int side = 0; side < 2; ++side)
-            for (int j = 0; j < ncy; ++j)
-                for (int i = 0; i < ncx; ++i) {
-                    const size_t q = (size_t)j * ncx + i;
+            for (int j = m.wy0; j < m.wy0 + m.wny; ++j)
+                for (int i = m.wx0; i < m.wx0 + m.wnx; ++i) {
+                    const size_t q = m.wq(i, j);
                     if (m.own_c[q] < 0 || m.own_f[q] >= 0) continue;
                     const int sg = side ? 1 : -1;
                     const int ni = i + (d == 0 ? sg : 0), nj = j + (d == 1 ? sg : 0);
                     if (ni < 0 || ni >= ncx || nj < 0 || nj >= ncy) continue;
-                    const int fb = m.own_f[(size_t)nj * ncx + ni];
+                    const int fb = m.f_at(ni, nj);
                     if (fb >= 0) fn(d, side, i, j, m.own_c[q], fb);
                 }
 }
@@ -2732,7 +2983,7 @@ static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
     if (e) return e;
     const int ncx = m.ncx, ncy = m.ncy;
     const tmk_box fd = Fl->ctx->hp.domain;
-    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.own_c[(size_t)j * ncx + i] >= 0; };
+    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.c_at(i, j) >= 0; };
     // coarse-fine fill of the state: one entry per (fine box, ghost cell inside the domain
     // and outside every fine box)
     std::vector<CfCell> cells;
@@ -2743,7 +2994,7 @@ static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
         for (int j = j0; j <= j1; ++j)
             for (int i = i0; i <= i1; ++i) {
                 const int ic = i / r, jc = j / r;
-                if (m.own_f[(size_t)jc * ncx + ic] >= 0) continue;  // a cell of the fine level
+                if (m.f_at(ic, jc) >= 0) continue;  // a cell of the fine level
                 CfCell cc;
                 memset(&cc, 0, sizeof cc);
                 cc.dpatch = (int)b;
@@ -2753,7 +3004,7 @@ static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
                         const int ci = ic + ii, cj = jc + jj;
                         cc.cpatch[k] = -1;
                         if (in_c(ci, cj)) {
-                            const int cb = m.own_c[(size_t)cj * ncx + ci];
+                            const int cb = m.c_at(ci, cj);
                             cc.cpatch[k] = cb;
                             cc.coff[k] = cell_off(C, cb, ci, cj);
                         }
@@ -2869,7 +3120,7 @@ static int member_pair_lists(const AmrMember& M, int l, int r, PairLists& P)
         for (int jc = v.lo[1] / r; jc <= v.hi[1] / r; ++jc)
             for (int ic = v.lo[0] / r; ic <= v.hi[0] / r; ++ic) {
                 AvgCell a;
-                a.cpatch = m.own_c[(size_t)jc * ncx + ic];
+                a.cpatch = m.c_at(ic, jc);
                 a.coff = cell_off(C, a.cpatch, ic, jc);
                 a.fpatch = (int)b;
                 a.foff = cell_off(Fl, (int)b, ic * r, jc * r);
@@ -2921,11 +3172,9 @@ static void member_release(AmrMember& M)
 {
     for (int d = 0; d < 2; ++d)
         for (auto& v : M.F[d])
-            for (double* p : v)
-                if (p) (void)hipFree(p);
+            for (double* p : v) dev_free(p);
     for (auto& v : M.fmask)
-        for (unsigned char* p : v)
-            if (p) (void)hipFree(p);
+        for (unsigned char* p : v) dev_free(p);
     for (int d = 0; d < 2; ++d) M.F[d].assign(M.L.size(), {});
     M.fmask.assign(M.L.size(), {});
     if (!M.L.empty()) {
@@ -2958,7 +3207,7 @@ static int member_layouts(AmrMember& M, int r)
             for (int d = 0; d < 2; ++d) {
                 void* p = nullptr;
                 const size_t nf = d == 0 ? (w + 1) * h : w * (h + 1);
-                if (hipMalloc(&p, sizeof(double) * TMK_NCOMP * nf) != hipSuccess) return TMK_E_NOMEM;
+                if (dev_alloc(&p, sizeof(double) * TMK_NCOMP * nf) != hipSuccess) return TMK_E_NOMEM;
                 M.F[d][l][k] = (double*)p;
             }
             const size_t mw = w + 2, mh = h + 2;
@@ -2973,9 +3222,9 @@ static int member_layouts(AmrMember& M, int r)
             for (const tmk_box& b : L->boxes) mark(b, 1);
             for (const tmk_box& b : finer) mark(b, 2);
             void* p = nullptr;
-            if (hipMalloc(&p, mk.size()) != hipSuccess) return TMK_E_NOMEM;
+            if (dev_alloc(&p, mk.size()) != hipSuccess) return TMK_E_NOMEM;
             M.fmask[l][k] = (unsigned char*)p;
-            HIPCHK(hipMemcpy(p, mk.data(), mk.size(), hipMemcpyHostToDevice));
+            HIPCHK(copy_on(L->ctx->stream, p, mk.data(), mk.size(), hipMemcpyHostToDevice));
         }
     }
     return TMK_OK;
@@ -2992,7 +3241,7 @@ static int amr_detach(tmk_amr* A)
         AmrLevel& a = A->lv[l];
         std::vector<Scen> sc(A->mem.size());
         if (!sc.empty())
-            HIPCHK(hipMemcpy(sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+            HIPCHK(copy_on(A->stream, sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
         for (size_t m = 0; m < A->mem.size(); ++m) {
             tmk_level* L = A->mem[m]->L[l];
             L->cur = a.phase[m] ^ a.grp.g;
@@ -3093,10 +3342,11 @@ static int amr_build_t(tmk_amr* A)
         c.nrcells = (int)all.rcells.size();
         c.navgc = (int)all.avgc.size();
         c.navgf = (int)all.avgf.size();
-        if ((e = upload_list(f.cells, all.cells)) || (e = upload_list(f.faces, all.faces)) ||
-            (e = upload_list(f.comps, comps)) || (e = upload_list(c.regf, all.regf)) ||
-            (e = upload_list(c.rcells, all.rcells)) || (e = upload_list(c.rscale, all.rscale)) ||
-            (e = upload_list(c.avgc, all.avgc)) || (e = upload_list(c.avgf, all.avgf)))
+        hipStream_t st = A->stream;
+        if ((e = upload_list(f.cells, all.cells, st)) || (e = upload_list(f.faces, all.faces, st)) ||
+            (e = upload_list(f.comps, comps, st)) || (e = upload_list(c.regf, all.regf, st)) ||
+            (e = upload_list(c.rcells, all.rcells, st)) || (e = upload_list(c.rscale, all.rscale, st)) ||
+            (e = upload_list(c.avgc, all.avgc, st)) || (e = upload_list(c.avgf, all.avgf, st)))
             return e;
         if ((e = c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * all.regf.size(), 1)))) return e;
     }
@@ -3486,7 +3736,7 @@ extern "C" int tmk_amr_sync(tmk_amr* A)
     if (!A->built || first) return first;
     for (AmrLevel& a : A->lv) {
         std::vector<Scen> sc(A->mem.size());
-        HIPCHK(hipMemcpy(sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
+        HIPCHK(copy_on(A->stream, sc.data(), a.grp.scen_dev, sizeof(Scen) * sc.size(), hipMemcpyDeviceToHost));
         for (size_t m = 0; m < sc.size() && !first; ++m)
             if (sc[m].err) first = sc[m].err;
     }
@@ -3690,7 +3940,7 @@ static int level_regrid_t(tmk_level* N, tmk_level* O, tmk_level* C, int r)
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipStreamSynchronize(C->ctx->stream));
     if (O) HIPCHK(hipStreamSynchronize(O->ctx->stream));
-    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.own_c[(size_t)j * ncx + i] >= 0; };
+    auto in_c = [&](int i, int j) { return i >= 0 && i < ncx && j >= 0 && j < ncy && m.c_at(i, j) >= 0; };
     const int nN = (int)N->boxes.size();
     std::vector<CfCell> cells;
     std::vector<CfFace> faces;
@@ -3727,7 +3977,7 @@ static int level_regrid_t(tmk_level* N, tmk_level* O, tmk_level* C, int r)
                         const int ci = ic + ii, cj = jc + jj;
                         cc.cpatch[k] = -1;
                         if (in_c(ci, cj)) {
-                            const int cb = m.own_c[(size_t)cj * ncx + ci];
+                            const int cb = m.c_at(ci, cj);
                             cc.cpatch[k] = cb;
                             cc.coff[k] = cell_off(C, cb, ci, cj);
                         }
@@ -3775,9 +4025,10 @@ static int level_regrid_t(tmk_level* N, tmk_level* O, tmk_level* C, int r)
     std::vector<char> copytab(N->hpatch[N->cur ^ 1]);
     if (O) copytab.insert(copytab.end(), O->hpatch[O->cur ^ 1].begin(), O->hpatch[O->cur ^ 1].end());
     DevBuf dN, dC, dX, dcells, dfaces, dcomps, drects, dbrects;
-    if ((e = upload_list(dN, N->hpatch[N->cur])) || (e = upload_list(dC, C->hpatch[C->cur])) ||
-        (e = upload_list(dX, copytab)) || (e = upload_list(dcells, cells)) || (e = upload_list(dfaces, faces)) ||
-        (e = upload_list(dcomps, comps)) || (e = upload_list(drects, rects)) || (e = upload_list(dbrects, brects)))
+    if ((e = upload_list(dN, N->hpatch[N->cur], st)) || (e = upload_list(dC, C->hpatch[C->cur], st)) ||
+        (e = upload_list(dX, copytab, st)) || (e = upload_list(dcells, cells, st)) ||
+        (e = upload_list(dfaces, faces, st)) || (e = upload_list(dcomps, comps, st)) ||
+        (e = upload_list(drects, rects, st)) || (e = upload_list(dbrects, brects, st)))
         return e;
     (void)ps;
     const Patch<T>* tN = (const Patch<T>*)dN.p;

@@ -42,11 +42,14 @@ static int scratch_device_ptr(void** out);
 
 // the FP32-bulk update kernels are instantiated in tmk_f32.hip (built with FMA contraction)
 namespace tmk {
-#define TMK_DECLARE_EXTERN(T, NS, BX, BY, PK)                                         \
-    extern template __global__ void k_godunov<T, NS, BX, BY, PK>(const KArgs<T>);     \
+#define TMK_DECLARE_EXTERN(T, NS, BX, BY, PK) \
     extern template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
 TMK_ALL_UPDATE_KERNELS(TMK_DECLARE_EXTERN, float)
 extern template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
+extern template __global__ void k_godunov_bnd<float, 2, PK_RUNTIME>(const KArgs<float>);
+extern template __global__ void k_godunov_bnd<float, 3, PK_RUNTIME>(const KArgs<float>);
+extern template __global__ void k_godunov_bnd<float, 2, PK_PPM_PRIM>(const KArgs<float>);
+extern template __global__ void k_godunov_bnd<float, 3, PK_PPM_PRIM>(const KArgs<float>);
 }  // namespace tmk
 
 using namespace tmk;
@@ -292,10 +295,9 @@ bool tile_interior_host(const Tile& t, const DevParams& P, int bx, int by)
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 
-// Workgroup shape of the boundary-tile launch, whatever the interior shape: boundary tiles are
-// cut into pieces of at most (BND_BX - 2) x (BND_BY - 2) cells, so that the general kernel's
-// short workgroups start early and fit between the interior ones.
-constexpr int BND_BX = 32, BND_BY = 8;
+// Boundary tiles are cut into pieces of at most (BND_BX - 2) x (BND_BY - 2) cells (or the
+// transpose, along a left or right band), so that the boundary kernel's short workgroups start
+// early and fit between the interior ones.
 
 // interior tiles first (XCD order), then the boundary tiles cut into boundary-shape pieces,
 // fault-row pieces first (they carry the fault solve, the longest workgroups); returns the
@@ -339,15 +341,17 @@ void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& 
     c.x1 = std::min(r.x1, P.dhi[0] - 3);
     c.y0 = std::max(r.y0, P.dlo[1] + 3);
     c.y1 = std::min(r.y1, P.dhi[1] - 3);
-    auto pieces = [&](int x0, int y0, int x1, int y1) {
-        for (int py = y0; py <= y1; py += BND_BY - 2)
-            for (int px = x0; px <= x1; px += BND_BX - 2) {
+    // pieces of a band: (BND_BX - 2) x (BND_BY - 2) along x, the transpose along y (col)
+    auto pieces = [&](int x0, int y0, int x1, int y1, bool col = false) {
+        const int pw = col ? BND_BY - 2 : BND_BX - 2, ph = col ? BND_BX - 2 : BND_BY - 2;
+        for (int py = y0; py <= y1; py += ph)
+            for (int px = x0; px <= x1; px += pw) {
                 Tile t;
                 t.patch = r.patch;
                 t.x0 = px;
                 t.y0 = py;
-                t.x1 = std::min(px + BND_BX - 3, x1);
-                t.y1 = std::min(py + BND_BY - 3, y1);
+                t.x1 = std::min(px + pw - 1, x1);
+                t.y1 = std::min(py + ph - 1, y1);
                 (t.y0 == P.dlo[1] ? bf : bo).push_back(t);
             }
     };
@@ -367,8 +371,8 @@ void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& 
         }
     pieces(r.x0, r.y0, r.x1, c.y0 - 1);  // band below the core
     pieces(r.x0, c.y1 + 1, r.x1, r.y1);  // above
-    pieces(r.x0, c.y0, c.x0 - 1, c.y1);  // left of it
-    pieces(c.x1 + 1, c.y0, r.x1, c.y1);  // right of it
+    pieces(r.x0, c.y0, c.x0 - 1, c.y1, true);  // left of it
+    pieces(c.x1 + 1, c.y0, r.x1, c.y1, true);  // right of it
 }
 
 // state planes that sweeps read with a stencil (double-buffered); the others
@@ -803,46 +807,57 @@ static int pred_kind(const DevParams& P)
 }
 
 template <typename T, int NS, int BX, int BY, int PK>
-static int launch_tiles_t(hipStream_t st, const KArgs<T>& a, bool interior)
+static int launch_tiles_t(hipStream_t st, const KArgs<T>& a)
 {
-    constexpr int NT = BX * BY;
-    constexpr int NC = 4 * NS;
-    const size_t s1 = (size_t)NC * BY * (BX + 4), s2 = (size_t)NC * (BY + 4) * BX, s3 = (size_t)8 * NS * NT;
-    const size_t shm = (interior ? interior_lds_elems<NS, BX, BY>() : std::max(s1, std::max(s2, s3))) * sizeof(T);
+    const size_t shm = interior_lds_elems<NS, BX, BY>() * sizeof(T);
     if (a.ntiles == 0) return TMK_OK;
-    const void* k = interior ? (const void*)k_godunov_interior<T, NS, BX, BY, PK> : (const void*)k_godunov<T, NS, BX, BY, PK>;
+    const void* k = (const void*)k_godunov_interior<T, NS, BX, BY, PK>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (interior)
-        hipLaunchKernelGGL((k_godunov_interior<T, NS, BX, BY, PK>), dim3(a.ntiles),
-                           dim3(interior_threads<T, NS, BX, BY>()), shm, st, a);
-    else
-        hipLaunchKernelGGL((k_godunov<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(NT), shm, st, a);
+    hipLaunchKernelGGL((k_godunov_interior<T, NS, BX, BY, PK>), dim3(a.ntiles), dim3(interior_threads<T, NS, BX, BY>()),
+                       shm, st, a);
     return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
 }
 
 template <typename T, int NS, int PK>
-static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a, bool in)
+static int launch_tiles_ns(hipStream_t st, int bx, int by, const KArgs<T>& a)
 {
-    if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16, PK>(st, a, in);
-    if (bx == 16 && by == 16) return launch_tiles_t<T, NS, 16, 16, PK>(st, a, in);
-    if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8, PK>(st, a, in);
-    if (bx == 32 && by == 12) return launch_tiles_t<T, NS, 32, 12, PK>(st, a, in);
-    if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32, PK>(st, a, in);
-    if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8, PK>(st, a, in);
-    if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4, PK>(st, a, in);
-    if (bx == 64 && by == 16) return launch_tiles_t<T, NS, 64, 16, PK>(st, a, in);
+    if (bx == 32 && by == 16) return launch_tiles_t<T, NS, 32, 16, PK>(st, a);
+    if (bx == 16 && by == 16) return launch_tiles_t<T, NS, 16, 16, PK>(st, a);
+    if (bx == 32 && by == 8) return launch_tiles_t<T, NS, 32, 8, PK>(st, a);
+    if (bx == 32 && by == 12) return launch_tiles_t<T, NS, 32, 12, PK>(st, a);
+    if (bx == 32 && by == 32) return launch_tiles_t<T, NS, 32, 32, PK>(st, a);
+    if (bx == 64 && by == 8) return launch_tiles_t<T, NS, 64, 8, PK>(st, a);
+    if (bx == 64 && by == 4) return launch_tiles_t<T, NS, 64, 4, PK>(st, a);
+    if (bx == 64 && by == 16) return launch_tiles_t<T, NS, 64, 16, PK>(st, a);
     return TMK_E_ARG;
 }
 
-// one launch over a.ntiles tiles; interior: every tile is interior (k_godunov_interior)
+// the boundary pieces a.tiles[0 .. a.ntiles) in one launch of k_godunov_bnd
+template <typename T, int NS, int PK>
+static int launch_bnd_t(hipStream_t st, const KArgs<T>& a)
+{
+    const size_t shm = bnd_lds_elems<T, NS>() * sizeof(T);
+    if (a.ntiles == 0) return TMK_OK;
+    const void* k = (const void*)k_godunov_bnd<T, NS, PK>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL((k_godunov_bnd<T, NS, PK>), dim3(a.ntiles), dim3(BND_BX * BND_BY), shm, st, a);
+    return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
+}
+
 template <typename T>
-static int launch_tiles(hipStream_t st, int ns, int pk, int bx, int by, const KArgs<T>& a, bool interior)
+static int launch_bnd(hipStream_t st, int ns, int pk, const KArgs<T>& a)
+{
+    if (pk == PK_PPM_PRIM) return ns == 3 ? launch_bnd_t<T, 3, PK_PPM_PRIM>(st, a) : launch_bnd_t<T, 2, PK_PPM_PRIM>(st, a);
+    return ns == 3 ? launch_bnd_t<T, 3, PK_RUNTIME>(st, a) : launch_bnd_t<T, 2, PK_RUNTIME>(st, a);
+}
+
+// one launch of k_godunov_interior over a.ntiles interior tiles of bx x by
+template <typename T>
+static int launch_tiles(hipStream_t st, int ns, int pk, int bx, int by, const KArgs<T>& a)
 {
     if (pk == PK_PPM_PRIM)
-        return ns == 3 ? launch_tiles_ns<T, 3, PK_PPM_PRIM>(st, bx, by, a, interior)
-                       : launch_tiles_ns<T, 2, PK_PPM_PRIM>(st, bx, by, a, interior);
-    return ns == 3 ? launch_tiles_ns<T, 3, PK_RUNTIME>(st, bx, by, a, interior)
-                   : launch_tiles_ns<T, 2, PK_RUNTIME>(st, bx, by, a, interior);
+        return ns == 3 ? launch_tiles_ns<T, 3, PK_PPM_PRIM>(st, bx, by, a) : launch_tiles_ns<T, 2, PK_PPM_PRIM>(st, bx, by, a);
+    return ns == 3 ? launch_tiles_ns<T, 3, PK_RUNTIME>(st, bx, by, a) : launch_tiles_ns<T, 2, PK_RUNTIME>(st, bx, by, a);
 }
 
 // per-kernel timing of one step (profiling only): events around the interior launch on the
@@ -901,7 +916,7 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         b.tiles = tiles + nint;
         b.ntiles = nbnd;
         if ((e = timer_mark(tm, sb, 2))) return e;
-        if ((e = launch_tiles<T>(sb, ns, pk, BND_BX, BND_BY, b, false))) return e;
+        if ((e = launch_bnd<T>(sb, ns, pk, b))) return e;
         if ((e = timer_mark(tm, sb, 3))) return e;
     }
     if (nint > 0) {
@@ -910,7 +925,7 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         if ((e = timer_mark(tm, st, 0))) return e;
         if (march > 0) {
             if ((e = launch_march<T>(st, b))) return e;
-        } else if ((e = launch_tiles<T>(st, ns, pk, bx, by, b, true))) {
+        } else if ((e = launch_tiles<T>(st, ns, pk, bx, by, b))) {
             return e;
         }
         if ((e = timer_mark(tm, st, 1))) return e;

@@ -11,11 +11,14 @@
 
 namespace tmk {
 
-#define TMK_INSTANTIATE(T, NS, BX, BY, PK)                                     \
-    template __global__ void k_godunov<T, NS, BX, BY, PK>(const KArgs<T>);     \
+#define TMK_INSTANTIATE(T, NS, BX, BY, PK) \
     template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
 
 TMK_ALL_UPDATE_KERNELS(TMK_INSTANTIATE, float)
+template __global__ void k_godunov_bnd<float, 2, PK_RUNTIME>(const KArgs<float>);
+template __global__ void k_godunov_bnd<float, 3, PK_RUNTIME>(const KArgs<float>);
+template __global__ void k_godunov_bnd<float, 2, PK_PPM_PRIM>(const KArgs<float>);
+template __global__ void k_godunov_bnd<float, 3, PK_PPM_PRIM>(const KArgs<float>);
 template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
 
 }  // namespace tmk

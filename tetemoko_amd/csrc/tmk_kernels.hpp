@@ -747,15 +747,33 @@ __device__ __forceinline__ bool tile_interior(const Tile& t, const DevParams& P)
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 
-// any tile: domain boundaries and the fault are handled where the tile meets them
-template <typename T, int NS, int BX, int BY, int PK>
-__global__ void __launch_bounds__(BX * BY)
-k_godunov(const KArgs<T> a)
+// Boundary pieces: the bands of cells within 3 of the domain's edges, whose stencils need the
+// boundary conditions.  A piece along a top or bottom band (at most BND_BX - 2 wide, BND_BY - 2
+// high) runs on a BND_BX x BND_BY workgroup; a piece along a left or right band, 3 cells wide,
+// is taller than BND_BY - 2 and runs on the transposed BND_BY x BND_BX workgroup, so that its
+// 256 lanes cover 3 x 30 cells rather than 3 x 6.  Both shapes use the same lanes and LDS.
+constexpr int BND_BX = 32, BND_BY = 8;
+
+template <typename T, int NS>
+constexpr size_t bnd_lds_elems()
+{
+    constexpr size_t NC = 4 * NS;
+    constexpr size_t s1 = NC * BND_BY * (BND_BX + 4), s2 = NC * (BND_BY + 4) * BND_BX;
+    constexpr size_t s1t = NC * BND_BX * (BND_BY + 4), s2t = NC * (BND_BX + 4) * BND_BY;
+    constexpr size_t s3 = (size_t)8 * NS * BND_BX * BND_BY;
+    constexpr size_t a = s1 > s2 ? s1 : s2, b = s1t > s2t ? s1t : s2t;
+    return (a > b ? a : b) > s3 ? (a > b ? a : b) : s3;
+}
+
+template <typename T, int NS, int PK>
+__global__ void __launch_bounds__(BND_BX * BND_BY)
+k_godunov_bnd(const KArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* sm = reinterpret_cast<T*>(smem_raw);
     const Tile tl = a.tiles[blockIdx.x];
-    godunov_tile<T, NS, BX, BY, PK, false>(a, tl, sm);
+    if (tl.y1 - tl.y0 + 1 > BND_BY - 2) godunov_tile<T, NS, BND_BY, BND_BX, PK, false>(a, tl, sm);
+    else godunov_tile<T, NS, BND_BX, BND_BY, PK, false>(a, tl, sm);
 }
 
 // LinElastPatchGodunov::plasticUpdate on its own (PLASTICCORRECTION over the valid box)

@@ -263,9 +263,12 @@ def test_profile_kernels():
     lev.close()
     ctx.close()
     assert p["cells_interior"] + p["cells_boundary"] == 200 * 120
-    # 30 x 14 tiles; interior: full size with 3 cells of domain around -> x0 = 30..150, y0 = 14..98
-    assert p["tiles_interior"] == 5 * 7
-    assert p["cells_interior"] == 30 * 14 * p["tiles_interior"]
+    # 7 x 9 tiles of at most 30 x 14; each tile within 3 cells of the domain's edges gives its core
+    # to the interior kernel and its bands (3 cells thick) to the boundary pieces
+    assert p["tiles_interior"] == 7 * 9
+    assert p["cells_interior"] == (200 - 6) * (120 - 6)
+    # one band piece per edge a tile touches: 7 tiles along each row band, 9 along each column band
+    assert p["tiles_boundary"] == 2 * 7 + 2 * 9
     assert p["ms_interior"] > 0 and p["ms_boundary"] > 0 and p["ms_step"] > 0
 
 

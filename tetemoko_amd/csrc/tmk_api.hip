@@ -295,54 +295,20 @@ bool tile_interior_host(const Tile& t, const DevParams& P, int bx, int by)
            t.y0 - 3 >= P.dlo[1] && t.y1 + 3 <= P.dhi[1];
 }
 
-// Boundary tiles are cut into pieces of at most (BND_BX - 2) x (BND_BY - 2) cells (or the
-// transpose, along a left or right band), so that the boundary kernel's short workgroups start
-// early and fit between the interior ones.
-
-// interior tiles first (XCD order), then the boundary tiles cut into boundary-shape pieces,
-// fault-row pieces first (they carry the fault solve, the longest workgroups); returns the
-// interior count
-template <typename F, typename G>
-int partition_tiles(std::vector<Tile>& t, F is_interior, G on_fault_row)
-{
-    std::vector<Tile> in, bf, bo;
-    for (const Tile& x : t) {
-        if (is_interior(x)) {
-            in.push_back(x);
-            continue;
-        }
-        for (int y0 = x.y0; y0 <= x.y1; y0 += BND_BY - 2)
-            for (int x0 = x.x0; x0 <= x.x1; x0 += BND_BX - 2) {
-                Tile p = x;
-                p.x0 = x0;
-                p.y0 = y0;
-                p.x1 = std::min(x0 + BND_BX - 3, x.x1);
-                p.y1 = std::min(y0 + BND_BY - 3, x.y1);
-                (on_fault_row(p) ? bf : bo).push_back(p);
-            }
-    }
-    xcd_order(in);
-    const int nint = (int)in.size();
-    t.swap(in);
-    t.insert(t.end(), bf.begin(), bf.end());
-    t.insert(t.end(), bo.begin(), bo.end());
-    return nint;
-}
-
-// Row-marching partition of one rectangle of an island (r.patch: its descriptor): the core,
-// the cells whose +-3 stencil lies in the domain, in strips of MARCH_COLS x rows for
-// k_godunov_march; the rest, the bands along the domain boundary, in boundary-shape pieces for
-// the general kernel, fault-row pieces apart.
-void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& strips, std::vector<Tile>& bf,
-                std::vector<Tile>& bo)
+// The cells of rectangle r within 3 of the domain's edges, whose stencils need the boundary
+// conditions, form bands at most 3 cells thick: the rows below and above r's core (full width)
+// and the columns left and right of it.  band_split passes them to put as boundary pieces --
+// (BND_BX - 2) x (BND_BY - 2) along a row band, the transpose along a column band -- and returns
+// the core, the cells whose +-3 stencil lies inside the domain (empty when there are none).
+template <typename Put>
+Tile band_split(const Tile& r, const DevParams& P, Put put)
 {
     Tile c = r;
     c.x0 = std::max(r.x0, P.dlo[0] + 3);
     c.x1 = std::min(r.x1, P.dhi[0] - 3);
     c.y0 = std::max(r.y0, P.dlo[1] + 3);
     c.y1 = std::min(r.y1, P.dhi[1] - 3);
-    // pieces of a band: (BND_BX - 2) x (BND_BY - 2) along x, the transpose along y (col)
-    auto pieces = [&](int x0, int y0, int x1, int y1, bool col = false) {
+    auto pieces = [&](int x0, int y0, int x1, int y1, bool col) {
         const int pw = col ? BND_BY - 2 : BND_BX - 2, ph = col ? BND_BX - 2 : BND_BY - 2;
         for (int py = y0; py <= y1; py += ph)
             for (int px = x0; px <= x1; px += pw) {
@@ -352,13 +318,57 @@ void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& 
                 t.y0 = py;
                 t.x1 = std::min(px + pw - 1, x1);
                 t.y1 = std::min(py + ph - 1, y1);
-                (t.y0 == P.dlo[1] ? bf : bo).push_back(t);
+                put(t);
             }
     };
     if (c.x0 > c.x1 || c.y0 > c.y1) {
-        pieces(r.x0, r.y0, r.x1, r.y1);
-        return;
+        // no core: r lies in a band (or the domain is under 7 cells across)
+        pieces(r.x0, r.y0, r.x1, r.y1, r.x1 - r.x0 + 1 <= BND_BY - 2 && r.y1 - r.y0 + 1 > BND_BY - 2);
+        c.x1 = c.x0 - 1;
+        return c;
     }
+    pieces(r.x0, r.y0, r.x1, c.y0 - 1, false);  // band below the core
+    pieces(r.x0, c.y1 + 1, r.x1, r.y1, false);  // above
+    pieces(r.x0, c.y0, c.x0 - 1, c.y1, true);   // left of it
+    pieces(c.x1 + 1, c.y0, r.x1, c.y1, true);   // right of it
+    return c;
+}
+
+inline bool tile_empty(const Tile& t) { return t.x0 > t.x1 || t.y0 > t.y1; }
+
+// interior tiles first (XCD order), then the boundary pieces, fault-row pieces first (they
+// carry the fault solve, the longest workgroups); a tile that is not interior contributes its
+// core as an interior tile and its bands as pieces.  P(t): the parameters of tile t's scenario.
+// Returns the interior count.
+template <typename F, typename G>
+int partition_tiles(std::vector<Tile>& t, F is_interior, G params)
+{
+    std::vector<Tile> in, bf, bo;
+    for (const Tile& x : t) {
+        if (is_interior(x)) {
+            in.push_back(x);
+            continue;
+        }
+        const DevParams& P = params(x);
+        const Tile c = band_split(x, P, [&](const Tile& p) { (p.y0 == P.dlo[1] ? bf : bo).push_back(p); });
+        if (!tile_empty(c)) in.push_back(c);
+    }
+    xcd_order(in);
+    const int nint = (int)in.size();
+    t.swap(in);
+    t.insert(t.end(), bf.begin(), bf.end());
+    t.insert(t.end(), bo.begin(), bo.end());
+    return nint;
+}
+
+// Row-marching partition of one rectangle of an island (r.patch: its descriptor): the core in
+// strips of MARCH_COLS x rows for k_godunov_march, the bands in boundary pieces (fault-row
+// pieces apart).
+void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& strips, std::vector<Tile>& bf,
+                std::vector<Tile>& bo)
+{
+    const Tile c = band_split(r, P, [&](const Tile& p) { (p.y0 == P.dlo[1] ? bf : bo).push_back(p); });
+    if (tile_empty(c)) return;
     for (int y = c.y0; y <= c.y1; y += rows)
         for (int x = c.x0; x <= c.x1; x += MARCH_COLS) {
             Tile t;
@@ -369,10 +379,6 @@ void march_rect(const Tile& r, const DevParams& P, int rows, std::vector<Tile>& 
             t.y1 = std::min(y + rows - 1, c.y1);
             strips.push_back(t);
         }
-    pieces(r.x0, r.y0, r.x1, c.y0 - 1);  // band below the core
-    pieces(r.x0, c.y1 + 1, r.x1, r.y1);  // above
-    pieces(r.x0, c.y0, c.x0 - 1, c.y1, true);  // left of it
-    pieces(c.x1 + 1, c.y0, r.x1, c.y1, true);  // right of it
 }
 
 // state planes that sweeps read with a stencil (double-buffered); the others
@@ -1087,7 +1093,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     } else {
         G.nint = partition_tiles(
             tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
-            [&](const Tile& t) { return t.y0 == sc[patch_scen[t.patch]].P.dlo[1]; });
+            [&](const Tile& t) -> const DevParams& { return sc[patch_scen[t.patch]].P; });
     }
     G.cells_int = G.cells_bnd = 0;
     for (size_t k = 0; k < tiles.size(); ++k)
@@ -1152,10 +1158,10 @@ static int group_error(Group& G, hipStream_t st, int m, int* code)
 // per-patch drop-in (host FArrayBox data)
 // ---------------------------------------------------------------------------
 
-// The tiles of one box for the drop-in path, with no heap allocation: interior tiles, then the
-// boundary tiles cut into boundary-shape pieces, fault-row pieces first -- the partition of
-// partition_tiles (without its XCD ordering, which a single box does not need; the order of the
-// tiles does not change any result).  Writes at most cap tiles to out (out may be NULL to
+// The tiles of one box for the drop-in path, with no heap allocation: interior tiles and the
+// cores of the others, then those tiles' bands in boundary pieces, fault-row pieces first -- the
+// partition of partition_tiles (without its XCD ordering, which a single box does not need; the
+// order of the tiles does not change any result).  Writes at most cap tiles to out (out may be NULL to
 // count); returns the count, and the interior count in *nint.
 static int patch_tiles(const tmk_box& v, const DevParams& P, int bx, int by, Tile* out, int cap, int* nint)
 {
@@ -1176,19 +1182,18 @@ static int patch_tiles(const tmk_box& v, const DevParams& P, int bx, int by, Til
                 t.y1 = std::min(y0 + ty - 1, v.hi[1]);
                 const bool in = tile_interior_host(t, P, bx, by);
                 if (pass == 0) {
-                    if (in) put(t);
+                    if (in) {
+                        put(t);
+                    } else {
+                        const Tile c = band_split(t, P, [](const Tile&) {});
+                        if (!tile_empty(c)) put(c);
+                    }
                     continue;
                 }
                 if (in) continue;
-                for (int py = t.y0; py <= t.y1; py += BND_BY - 2)
-                    for (int px = t.x0; px <= t.x1; px += BND_BX - 2) {
-                        Tile p = t;
-                        p.x0 = px;
-                        p.y0 = py;
-                        p.x1 = std::min(px + BND_BX - 3, t.x1);
-                        p.y1 = std::min(py + BND_BY - 3, t.y1);
-                        if ((p.y0 == P.dlo[1]) == (pass == 1)) put(p);
-                    }
+                (void)band_split(t, P, [&](const Tile& p) {
+                    if ((p.y0 == P.dlo[1]) == (pass == 1)) put(p);
+                });
             }
         if (pass == 0) *nint = n;
     }

@@ -748,11 +748,11 @@ __device__ __forceinline__ bool tile_interior(const Tile& t, const DevParams& P)
 }
 
 // Boundary pieces: the bands of cells within 3 of the domain's edges, whose stencils need the
-// boundary conditions.  A piece along a top or bottom band (at most BND_BX - 2 wide, BND_BY - 2
-// high) runs on a BND_BX x BND_BY workgroup; a piece along a left or right band, 3 cells wide,
-// is taller than BND_BY - 2 and runs on the transposed BND_BY x BND_BX workgroup, so that its
-// 256 lanes cover 3 x 30 cells rather than 3 x 6.  Both shapes use the same lanes and LDS.
-constexpr int BND_BX = 32, BND_BY = 8;
+// boundary conditions; every band is at most 3 cells thick.  A piece along a row band (at most
+// BND_BX - 2 = 49 wide, 3 high) runs on a BND_BX x BND_BY workgroup, a piece along a column band
+// (taller than 3) on the transposed BND_BY x BND_BX one: either way the 255 lanes cover 3 rows or
+// columns of cells and the ring around them, with no idle row.
+constexpr int BND_BX = 51, BND_BY = 5;
 
 template <typename T, int NS>
 constexpr size_t bnd_lds_elems()
@@ -760,7 +760,7 @@ constexpr size_t bnd_lds_elems()
     constexpr size_t NC = 4 * NS;
     constexpr size_t s1 = NC * BND_BY * (BND_BX + 4), s2 = NC * (BND_BY + 4) * BND_BX;
     constexpr size_t s1t = NC * BND_BX * (BND_BY + 4), s2t = NC * (BND_BX + 4) * BND_BY;
-    constexpr size_t s3 = (size_t)8 * NS * BND_BX * BND_BY;
+    constexpr size_t s3 = (size_t)8 * NS * BND_BX * BND_BY;  // the exchange slots (either shape)
     constexpr size_t a = s1 > s2 ? s1 : s2, b = s1t > s2t ? s1t : s2t;
     return (a > b ? a : b) > s3 ? (a > b ? a : b) : s3;
 }

@@ -94,3 +94,23 @@ def test_served_results_match_oracle():
             e = fault_errors(Bg, Br)
             assert np.all(e <= 1e-12), (sc["id"], e)
         assert np.count_nonzero(Br[rupture_comp(Br)] != SENTINEL) > 0, sc["id"]
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+def test_tuned_shapes_serve_the_same_results(precision):
+    """The loop's shape tuner (batches of the stream's own scenarios, each strip height timed)
+    picks heights it offers, and the scenarios served with them give the same fault data as
+    with the contexts' default shape: the strip height changes which lanes compute a cell,
+    never its arithmetic."""
+    sl = serving.draw_scenarios(8, 11, nx_range=(256, 512), steps_range=(40, 120), steps_scale=1.0)
+    shapes = serving.tune_shapes(sl, precision, members=3, steps=4)
+    assert shapes["single"] in serving.TUNE_ROWS and shapes["refined"] in serving.TUNE_ROWS
+    assert set(shapes["timings"]["single"]) == set(serving.TUNE_ROWS)
+    assert set(shapes["timings"]["refined"]) == set(serving.TUNE_ROWS)
+    # force heights other than the default so that the comparison means something
+    shapes = {"single": 8, "refined": 16}
+    results, stats = serving.serve(sl, 0.0, precision=precision, chunk=16, keep_fault=True, shapes=shapes)
+    assert stats["failed"] == 0 and stats["scenarios"] == len(sl)
+    by_id = {r["id"]: r for r in results}
+    for sc in sl:
+        assert np.array_equal(by_id[sc["id"]]["fault_data"], _alone(sc, precision)), sc["id"]

@@ -126,9 +126,81 @@ class Running:
             ctx.close()
 
 
-def _start(sc, precision, tile):
-    """Contexts and levels of a scenario: one level, or a hierarchy around its nucleation."""
-    return amr.make_levels(sc["cfg"], amr_layout(sc), AMR_RATIO, precision, tile)
+def _start(sc, precision, tile, shapes=None):
+    """Contexts and levels of a scenario: one level, or a hierarchy around its nucleation.
+    shapes: {"single": rows, "refined": rows} strip heights of the row-marching update for the
+    coarsest level and for the refined ones (every member of a launch must agree)."""
+    ctxs, levels = amr.make_levels(sc["cfg"], amr_layout(sc), AMR_RATIO, precision, tile)
+    if shapes:
+        for l, lev in enumerate(levels):
+            lev.set_update_shape(shapes["single"] if l == 0 else shapes["refined"])
+    return ctxs, levels
+
+
+TUNE_ROWS = (8, 16, 32)
+
+
+def _time_batch(scs, precision, rows0, rowsf, steps):
+    """ms per (coarse) step of one batch of scenarios scs with strip heights rows0 (coarsest
+    level) and rowsf (refined levels), best of two timed runs after a warm-up."""
+    shapes = {"single": rows0, "refined": rowsf}
+    started = [_start(sc, precision, None, shapes) for sc in scs]
+    nlev = len(started[0][1])
+    B = capi.Batch() if nlev == 1 else capi.AmrBatch(nlev, AMR_RATIO)
+    for sc, (_, levels) in zip(scs, started):
+        if nlev == 1:
+            B.add(levels[0], 0.0, scenarios.dt_of(sc["cfg"]))
+        else:
+            B.add(levels, 0.0, scenarios.dt_of(sc["cfg"]))
+    go = B.run if nlev == 1 else B.advance
+    go(2)
+    B.sync()
+    best = float("inf")
+    for _ in range(2):
+        t = time.perf_counter()
+        go(steps)
+        B.sync()
+        best = min(best, (time.perf_counter() - t) / steps * 1e3)
+    for _, levels in started:
+        B.remove(levels[0])
+    B.close()
+    for ctxs, levels in started:
+        for lev in levels:
+            lev.close()
+        for c in ctxs:
+            c.close()
+    return best
+
+
+def tune_shapes(scen_list, precision=capi.PREC_FP32_BULK, members=12, steps=40, log=None):
+    """Update shapes for the loop's launches, timed on batches of the stream's own scenarios.
+
+    Every member of a launch must run the same update shape, so the loop takes one row-marching
+    strip height for the coarsest level of every scenario and one for the refined levels.  Each
+    candidate (TUNE_ROWS) runs the first `members` single-level scenarios of the stream as one
+    batch; the fastest height becomes the coarsest levels'.  With it, the first scenarios with
+    the most refined levels run as one hierarchy batch for each candidate height of their refined
+    levels, and the fastest of those becomes the refined levels'.  A level tuned alone
+    (tmk_level_autotune) favours the shortest strips, its latency; in a batch the launch is
+    full and the choice differs.  Returns {"single", "refined", "timings"}."""
+    single = [sc for sc in scen_list if sc.get("levels", 0) == 0][:members]
+    most = max(sc.get("levels", 0) for sc in scen_list)
+    refined = [sc for sc in scen_list if sc.get("levels", 0) == most][:members] if most > 0 else []
+    timings = {"single": {}, "refined": {}}
+    rows0 = 32
+    if single:
+        for r in TUNE_ROWS:
+            timings["single"][r] = _time_batch(single, precision, r, r, steps)
+        rows0 = min(timings["single"], key=timings["single"].get)
+    rowsf = rows0
+    if refined:
+        for r in TUNE_ROWS:
+            timings["refined"][r] = _time_batch(refined, precision, rows0, r, max(steps // 4, 4))
+        rowsf = min(timings["refined"], key=timings["refined"].get)
+    if log:
+        log("update shapes: coarsest levels %d-row strips, refined levels %d-row strips (ms per step %s)" %
+            (rows0, rowsf, json.dumps(timings)))
+    return {"single": rows0, "refined": rowsf, "timings": timings}
 
 
 def _footprint(sc, precision):
@@ -170,7 +242,7 @@ def _result(r, B):
 
 
 def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reserve=8 << 30, tile=None,
-          log=None, keep_fault=False):
+          log=None, keep_fault=False, shapes=None):
     """Run the arrival stream; returns (results, stats).
 
     The loop keeps the device busy while the host works: chunk k+1 of the shared
@@ -223,7 +295,7 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
             if footprint[key] + mem_reserve > free:
                 break
             queue.pop(0)
-            ctxs, levels = _start(sc, precision, tile)
+            ctxs, levels = _start(sc, precision, tile, shapes)
             ready.append(Running(sc, ctxs, levels, arr, now(), scenarios.dt_of(cfg)))
 
     def retire(r, B, failed_code=None):
@@ -341,11 +413,15 @@ def main():
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--max-levels", type=int, default=2, help="refined levels per scenario drawn from 0..this")
+    ap.add_argument("--no-tune", action="store_true", help="the contexts' default update shape, untuned")
     a = ap.parse_args()
     sl = draw_scenarios(a.n, a.seed, nx_range=(256, a.nx_max), steps_scale=a.steps_scale, max_levels=a.max_levels)
     prec = capi.PREC_FP64 if a.fp64 else capi.PREC_FP32_BULK
-    res, stats = serve(sl, a.rate, precision=prec, chunk=a.chunk,
-                       log=lambda m: print(m, flush=True))
+    log = lambda m: print(m, flush=True)
+    shapes = None if a.no_tune else tune_shapes(sl, prec, log=log)
+    res, stats = serve(sl, a.rate, precision=prec, chunk=a.chunk, log=log, shapes=shapes)
+    if shapes:
+        stats["update_shapes"] = {"coarsest_rows": shapes["single"], "refined_rows": shapes["refined"]}
     stats.update({"n": a.n, "steps_scale": a.steps_scale, "precision": "fp64" if a.fp64 else "fp32-bulk",
                   "scenarios_by_levels": {str(k): sum(1 for sc in sl if sc.get("levels", 0) == k)
                                           for k in range(a.max_levels + 1)},

@@ -242,6 +242,10 @@ int tmk_batch_remove(tmk_batch *b, tmk_level *lev);
 int tmk_batch_run(tmk_batch *b, int nsteps);
 /* synchronize; first error reported by any member (0 = none) */
 int tmk_batch_sync(tmk_batch *b);
+/* without waiting: *done = 1 when every step enqueued on the batch has completed, else 0.
+ * No counterpart in the reference: lets a serving loop tend whichever batch has finished
+ * its chunk while the others run. */
+int tmk_batch_done(tmk_batch *b, int *done);
 /* synchronize; *code = the first error member lev reported (0 = none).  No
  * counterpart in the reference, which runs one scenario per process: a serving loop
  * uses it to retire only the scenario that failed (a member's error does not touch
@@ -293,6 +297,9 @@ int tmk_amr_run_timed(tmk_amr *amr, int nsteps, double time, double dt0, double 
 /* waits; the first error any member reported (0 = none); a member's error stays with it
  * until it leaves (tmk_amr_member_error names the member) */
 int tmk_amr_sync(tmk_amr *amr);
+/* without waiting: *done = 1 when every step enqueued on the hierarchy (or hierarchy batch)
+ * has completed, else 0 (no counterpart in the reference, as tmk_batch_done) */
+int tmk_amr_done(tmk_amr *amr, int *done);
 
 /* Hierarchy batches: scenarios with the same number of levels and ratio whose level-l boxes
  * share the level-l update launches (and whose fills, registers and averages share theirs),

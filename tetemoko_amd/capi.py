@@ -35,7 +35,7 @@ PREC_FP32_BULK = 1
 
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
-    "tmk_cache_info", "tmk_cache_set_limit", "tmk_cache_trim",
+    "tmk_cache_info", "tmk_cache_set_limit", "tmk_cache_trim", "tmk_batch_done", "tmk_amr_done",
     "tmk_level_footprint", "tmk_level_footprint_params", "tmk_batch_create", "tmk_batch_destroy",
     "tmk_batch_add", "tmk_batch_remove", "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill",
     "tmk_level_download_fault", "tmk_level_tag", "tmk_patch_update_state", "tmk_patch_plastic_update",
@@ -186,6 +186,8 @@ def lib():
         L.tmk_cache_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.tmk_cache_set_limit.argtypes = [sz]
         L.tmk_cache_trim.argtypes = []
+        L.tmk_batch_done.argtypes = [vp, ctypes.POINTER(c_int)]
+        L.tmk_amr_done.argtypes = [vp, ctypes.POINTER(c_int)]
         L.tmk_level_footprint.argtypes = [vp, c_int, PBox, ctypes.POINTER(sz)]
         L.tmk_level_footprint_params.argtypes = [ctypes.POINTER(Params), c_int, c_int, PBox, ctypes.POINTER(sz)]
         L.tmk_batch_create.argtypes = [ctypes.POINTER(vp)]
@@ -506,6 +508,12 @@ class AmrBatch:
     def sync(self):
         _check(lib().tmk_amr_sync(self.h), "tmk_amr_sync")
 
+    def done(self):
+        """True when every step enqueued so far has completed (does not wait)."""
+        d = ctypes.c_int()
+        _check(lib().tmk_amr_done(self.h, ctypes.byref(d)), "tmk_amr_done")
+        return bool(d.value)
+
     def member_error(self, level0):
         code = ctypes.c_int()
         _check(lib().tmk_amr_member_error(self.h, level0.h, ctypes.byref(code)), "tmk_amr_member_error")
@@ -612,6 +620,12 @@ class Batch:
 
     def sync(self):
         _check(lib().tmk_batch_sync(self.h), "tmk_batch_sync")
+
+    def done(self):
+        """True when every step enqueued so far has completed (does not wait)."""
+        d = ctypes.c_int()
+        _check(lib().tmk_batch_done(self.h, ctypes.byref(d)), "tmk_batch_done")
+        return bool(d.value)
 
     def member_error(self, level):
         """Error code the member reported (0 = none); other members are unaffected by it."""

@@ -2746,6 +2746,20 @@ extern "C" int tmk_batch_member_error(tmk_batch* B, tmk_level* L, int* code)
     return group_error(B->grp, B->stream, (int)m, code);
 }
 
+extern "C" int tmk_batch_done(tmk_batch* B, int* done)
+{
+    if (!B || !done) return TMK_E_ARG;
+    const hipError_t e = hipStreamQuery(B->stream);  // the side stream joins the batch stream
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        *done = 0;
+        return TMK_OK;
+    }
+    HIPCHK(e);
+    *done = 1;
+    return TMK_OK;
+}
+
 extern "C" int tmk_batch_sync(tmk_batch* B)
 {
     if (!B) return TMK_E_ARG;
@@ -3746,6 +3760,20 @@ extern "C" int tmk_amr_member_error(tmk_amr* A, tmk_level* level0, int* code)
 
 // waits for the hierarchy's work; returns the first error any member reported.  As in a
 // batch, a member's error stays with it (tmk_amr_member_error) until it leaves.
+extern "C" int tmk_amr_done(tmk_amr* A, int* done)
+{
+    if (!A || !done) return TMK_E_ARG;
+    const hipError_t e = hipStreamQuery(A->stream);
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        *done = 0;
+        return TMK_OK;
+    }
+    HIPCHK(e);
+    *done = 1;
+    return TMK_OK;
+}
+
 extern "C" int tmk_amr_sync(tmk_amr* A)
 {
     if (!A) return TMK_E_ARG;

@@ -241,33 +241,52 @@ def _result(r, B):
             "first_rupture_time": float(np.min(rt[ruptured])) if ruptured.any() else None}
 
 
+class _Lane:
+    """One batch of the loop -- the single-level batch, or the hierarchy batch of one level count
+    -- with its members and its own chunk cadence."""
+
+    def __init__(self, nlev):
+        self.nlev = nlev
+        self.b = capi.Batch() if nlev == 1 else capi.AmrBatch(nlev, AMR_RATIO)
+        self.running = []   # members
+        self.ready = []     # set up, joining at this lane's next chunk boundary
+        self.inflight = 0   # steps of the chunk in flight (0: none)
+
+    def add(self, r):
+        if self.nlev == 1:
+            self.b.add(r.lev, 0.0, r.dt)
+        else:
+            self.b.add(r.levels, 0.0, r.dt)
+
+    def launch(self, n):
+        (self.b.run if self.nlev == 1 else self.b.advance)(n)
+        self.inflight = n
+
+
 def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reserve=8 << 30, tile=None,
           log=None, keep_fault=False, shapes=None):
     """Run the arrival stream; returns (results, stats).
 
-    The loop keeps the device busy while the host works: chunk k+1 of the shared
-    launches is enqueued before the results of chunk k are handled, and new
-    scenarios are set up (context, level allocation, initial upload) while a chunk
-    runs.  Membership changes take effect at chunk boundaries.
+    Each batch (the single-level one, and one hierarchy batch per level count) is a lane with its
+    own chunk cadence: the loop polls the lanes without waiting (tmk_batch_done, tmk_amr_done),
+    and a lane whose chunk has completed retires its finished members, takes in the scenarios
+    that joined it and starts its next chunk while the other lanes' chunks run.  New scenarios
+    are set up (context, levels, initial state) while chunks run.  Membership changes take effect
+    at the lane's chunk boundaries.
     """
     rng = np.random.default_rng(12345)
     gaps = rng.exponential(1.0 / rate_hz, size=len(scen_list)) if rate_hz > 0 else np.zeros(len(scen_list))
     arrivals = np.cumsum(gaps) - gaps[0]
     pending = list(zip(arrivals, range(len(scen_list))))
     queue = []
-    batch = capi.Batch()                  # single-level scenarios
-    hbatches = {}                         # level count -> hierarchy batch
-    running = []      # members of the batches
-    ready = []        # set up, joining the batch at the next chunk boundary
+    lanes = {}        # level count -> lane
     results, latencies = [], []
     n_failed = 0
     cell_updates = 0
     footprint = {}    # (fault, nx, ny) -> device bytes of one scenario's level
     t0 = time.perf_counter()
     ip = 0
-    phase = {"admit": 0.0, "join": 0.0, "launch_levels": 0.0, "launch": 0.0, "sync": 0.0, "retire": 0.0,
-             "idle": 0.0}
-    inflight = 0      # steps of the chunk in flight (0: none)
+    phase = {"admit": 0.0, "join": 0.0, "launch": 0.0, "poll": 0.0, "sync": 0.0, "retire": 0.0, "idle": 0.0}
 
     def now():
         return time.perf_counter() - t0
@@ -281,6 +300,11 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
 
         def __exit__(self, *exc):
             phase[self.name] += time.perf_counter() - self.t
+
+    def lane_of(nlev):
+        if nlev not in lanes:
+            lanes[nlev] = _Lane(nlev)
+        return lanes[nlev]
 
     def admit():
         # admission by free device memory, in arrival order
@@ -296,11 +320,12 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
                 break
             queue.pop(0)
             ctxs, levels = _start(sc, precision, tile, shapes)
-            ready.append(Running(sc, ctxs, levels, arr, now(), scenarios.dt_of(cfg)))
+            r = Running(sc, ctxs, levels, arr, now(), scenarios.dt_of(cfg))
+            lane_of(r.nlev).ready.append(r)
 
-    def retire(r, B, failed_code=None):
+    def retire(lane, r, B, failed_code=None):
+        lane.running.remove(r)
         r.close()
-        running.remove(r)
         if failed_code is not None:
             return
         tdone = now()
@@ -316,82 +341,85 @@ def serve(scen_list, rate_hz, precision=capi.PREC_FP32_BULK, chunk=50, mem_reser
             cfg = r.sc["cfg"]
             log("done %d (%dx%d %s, %d refined levels, %d steps) latency %.2fs running %d queued %d" %
                 (res["id"], cfg["nx"], cfg["ny"], cfg["fault"], r.sc.get("levels", 0), r.sc["steps"],
-                 res["latency_s"], len(running), len(queue)))
+                 res["latency_s"], sum(len(l.running) for l in lanes.values()), len(queue)))
 
-    while ip < len(pending) or queue or running or ready or inflight:
+    def tend(lane):
+        """A lane whose chunk has completed: account it, retire, join, start the next chunk."""
+        nonlocal n_failed, cell_updates
+        if lane.inflight:
+            failed = {}
+            with _clock("sync"):
+                # a scenario's own error (e.g. a NaN state variable) retires only that scenario
+                try:
+                    lane.b.sync()
+                except capi.TmkError:
+                    for r in lane.running:
+                        code = lane.b.member_error(r.lev)
+                        if code != 0:
+                            failed[id(r)] = code
+            n = lane.inflight
+            lane.inflight = 0
+            for r in lane.running:
+                r.done_steps += n
+                cell_updates += _cell_updates_per_step(r.sc) * n
+            with _clock("retire"):
+                for r in [r for r in lane.running if id(r) in failed]:
+                    code = failed[id(r)]
+                    lane.b.remove(r.lev)
+                    res = {"id": r.sc["id"], "failed": capi.strerror(code), "arrival_s": float(r.arrival),
+                           "latency_s": float(now() - r.arrival)}
+                    results.append(res)
+                    n_failed += 1
+                    retire(lane, r, None, code)
+                    if log:
+                        log("failed %d: %s" % (res["id"], res["failed"]))
+                for r in [r for r in lane.running if r.done_steps >= r.sc["steps"]]:
+                    lane.b.remove(r.lev)
+                    cfg = r.sc["cfg"]
+                    B = np.empty((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * GHOST))
+                    r.lev.download_fault(0, B)
+                    retire(lane, r, B)
+        with _clock("join"):
+            for r in lane.ready:
+                lane.add(r)
+                lane.running.append(r)
+            lane.ready.clear()
+        if lane.running:
+            n = min(chunk, min(r.sc["steps"] - r.done_steps for r in lane.running))
+            with _clock("launch"):
+                lane.launch(n)
+
+    def busy():
+        return any(l.running or l.ready or l.inflight for l in lanes.values())
+
+    while ip < len(pending) or queue or busy():
         t = now()
         while ip < len(pending) and pending[ip][0] <= t:
             queue.append(pending[ip])
             ip += 1
         with _clock("admit"):
             admit()
-        if inflight:
-            failed = {}
-            with _clock("sync"):
-                # a scenario's own error (e.g. a NaN state variable) retires only that scenario
-                try:
-                    batch.sync()
-                except capi.TmkError:
-                    for r in running:
-                        if r.nlev == 1 and batch.member_error(r.lev) != 0:
-                            failed[id(r)] = batch.member_error(r.lev)
-                for nl, ab in hbatches.items():
-                    try:
-                        ab.sync()
-                    except capi.TmkError:
-                        for r in running:
-                            if r.nlev == nl and ab.member_error(r.lev) != 0:
-                                failed[id(r)] = ab.member_error(r.lev)
-            n = inflight
-            inflight = 0
-            for r in running:
-                r.done_steps += n
-                cell_updates += _cell_updates_per_step(r.sc) * n
-            with _clock("retire"):
-                for r in [r for r in running if id(r) in failed]:
-                    code = failed[id(r)]
-                    (batch if r.nlev == 1 else hbatches[r.nlev]).remove(r.lev)
-                    res = {"id": r.sc["id"], "failed": capi.strerror(code), "arrival_s": float(r.arrival),
-                           "latency_s": float(now() - r.arrival)}
-                    results.append(res)
-                    n_failed += 1
-                    retire(r, None, code)
-                    if log:
-                        log("failed %d: %s" % (res["id"], res["failed"]))
-                for r in [r for r in running if r.done_steps >= r.sc["steps"]]:
-                    (batch if r.nlev == 1 else hbatches[r.nlev]).remove(r.lev)
-                    cfg = r.sc["cfg"]
-                    B = np.empty((scenarios.NB[cfg["fault"]], cfg["nx"] + 2 * GHOST))
-                    r.lev.download_fault(0, B)
-                    retire(r, B)
-        with _clock("join"):
-            for r in ready:
-                if r.nlev == 1:
-                    batch.add(r.lev, 0.0, r.dt)
-                else:
-                    if r.nlev not in hbatches:
-                        hbatches[r.nlev] = capi.AmrBatch(r.nlev, AMR_RATIO)
-                    hbatches[r.nlev].add(r.levels, 0.0, r.dt)
-                running.append(r)
-            ready.clear()
-        if running:
-            n = min(chunk, min(r.sc["steps"] - r.done_steps for r in running))
-            # the single-level batch first, so the device works while the host prepares the
-            # hierarchy batches (each shares launches on its own stream; the first call after a
-            # membership change rebuilds its groups and captures its graph)
-            with _clock("launch"):
-                batch.run(n)
-            with _clock("launch_levels"):
-                for ab in hbatches.values():
-                    ab.advance(n)
-            inflight = n
-        if not running and not ready and not queue and ip < len(pending):
-            with _clock("idle"):
-                time.sleep(max(0.0, pending[ip][0] - now()))
+        tended = False
+        # the single-level lane first: its chunks are the shortest
+        for nlev in sorted(lanes):
+            lane = lanes[nlev]
+            if lane.inflight:
+                with _clock("poll"):
+                    if not lane.b.done():
+                        continue
+            if lane.inflight or lane.ready:
+                tend(lane)
+                tended = True
+        if not tended:
+            if not busy() and not queue and ip < len(pending):
+                with _clock("idle"):
+                    time.sleep(max(0.0, pending[ip][0] - now()))
+            else:
+                with _clock("poll"):
+                    time.sleep(2e-4)
     wall = now()
-    batch.close()
-    for ab in hbatches.values():
-        ab.close()
+    for lane in lanes.values():
+        lane.b.close()
     lat = np.array(latencies)
     stats = {"scenarios": len(results) - n_failed, "failed": n_failed, "wall_s": wall,
              "scenarios_per_s": (len(results) - n_failed) / wall,

@@ -32,7 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from tetemoko_amd import capi, scenarios  # noqa: E402
+from tetemoko_amd import capi, scenarios, serving  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 G = scenarios.GHOST
@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--no-c1", action="store_true", help="skip the secondary 4096x2048 FP64 drop-in leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dropin", action="store_true", help="skip the per-patch drop-in leg")
+    ap.add_argument("--serving-n", type=int, default=120,
+                    help="scenarios of the serving leg (the C4 stream at 8/s; 0: skip it)")
     return ap.parse_args()
 
 
@@ -293,6 +295,24 @@ def cpu_baseline(cfg_full, nthreads, seconds_hint):
                                                                           model)}
 
 
+def serving_leg(n, rate=8.0):
+    """The first n scenarios of the serving stream (grids 256x128..2048x1024, slip weakening or
+    rate-and-state, 0-2 refined levels, 1000-4000 coarse steps, FP32 bulk), arriving as a Poisson
+    process at `rate`, served with the loop's tuned update shapes: time to solution p50 / p99
+    (completion - arrival) and throughput."""
+    print("bench: serving leg, %d scenarios at %g/s" % (n, rate), file=sys.stderr, flush=True)
+    sl = serving.draw_scenarios(n, 7, max_levels=2)
+    shapes = serving.tune_shapes(sl, capi.PREC_FP32_BULK)
+    _, st = serving.serve(sl, rate, precision=capi.PREC_FP32_BULK, shapes=shapes)
+    return {"scenarios": st["scenarios"], "failed": st["failed"], "arrival_rate_hz": rate,
+            "p50_time_to_solution_s": st["p50_latency_s"], "p99_time_to_solution_s": st["p99_latency_s"],
+            "scenarios_per_s": st["scenarios_per_s"], "cell_updates_per_s": st["cell_updates_per_s"],
+            "wall_s": st["wall_s"], "refined_levels": "0-2 per scenario",
+            "update_shapes": {"coarsest_rows": shapes["single"], "refined_rows": shapes["refined"]},
+            "stream": "python -m tetemoko_amd.serving --n %d --rate %g --steps-scale 1.0 (its first scenarios)"
+                      % (n, rate)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -385,6 +405,8 @@ def main():
     if not a.no_dropin:
         print("bench: per-patch drop-in leg", file=sys.stderr, flush=True)
         out["drop_in_patch_fp64"] = drop_in_patch_rate(capi.PREC_FP64)
+    if a.serving_n > 0:
+        out["serving"] = serving_leg(a.serving_n)
     if not a.no_cpu:
         print("bench: CPU baseline leg", file=sys.stderr, flush=True)
         cb = cpu_baseline(cfg, 1, 12.0)

@@ -250,21 +250,16 @@ template <typename T>
 __global__ void k_amr_avg_state(const AvgCell* __restrict__ cells, int ncells, const Patch<T>* __restrict__ crs,
                                 const Patch<T>* __restrict__ fine, int r, double scale)
 {
-    // one coarse cell per lane, every component in turn: the cell's entry and descriptors are
-    // read once
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncells) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells * TMK_NCOMP) return;
+    const int c = t / ncells, k = t % ncells;
     const AvgCell a = cells[k];
     const Patch<T>& pf = fine[a.fpatch];
-    const Patch<T>& pc = crs[a.cpatch];
-    const size_t fnx = (size_t)pf.unx;
-    for (int c = 0; c < TMK_NCOMP; ++c) {
-        const __attribute__((address_space(1))) T* f = gmem(pf.in[c]) + a.foff;
-        double acc = 0.0;
-        for (int jj = 0; jj < r; ++jj)
-            for (int ii = 0; ii < r; ++ii) acc = acc + (double)f[ii + jj * fnx];
-        gmem((T*)pc.in[c])[a.coff] = (T)(acc * scale);
-    }
+    const __attribute__((address_space(1))) T* f = gmem(pf.in[c]) + a.foff;
+    double acc = 0.0;
+    for (int jj = 0; jj < r; ++jj)
+        for (int ii = 0; ii < r; ++ii) acc = acc + (double)f[ii + (size_t)jj * pf.unx];
+    gmem((T*)crs[a.cpatch].in[c])[a.coff] = (T)(acc * scale);
 }
 
 // coarse fault face = (sum of the r fine faces) / r (D1AVERAGE)

@@ -3599,7 +3599,7 @@ static int amr_post_t(tmk_amr* A, int l)
         HIPCHK(hipGetLastError());
     }
     if (a.navgc > 0) {
-        hipLaunchKernelGGL(k_amr_avg_state<T>, amr_grid((long long)a.navgc), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_amr_avg_state<T>, amr_grid((long long)a.navgc * TMK_NCOMP), dim3(256), 0, st,
                            (const AvgCell*)a.avgc.p, a.navgc, cur, fcur, r, 1.0 / (r * r));
         HIPCHK(hipGetLastError());
     }

@@ -129,7 +129,8 @@ TUNE = True  # autotune each level's update shape before it is timed
 
 def timed_leg(cfg, precision, boxes, warmup, steps):
     """The level's update shape autotuned from its initial state (tmk_level_autotune: every
-    strip height and tile shape timed, the level returned to that state), then W untimed
+    strip height and tile shape timed, then the fastest with the other workgroup order, the level
+    returned to that state), then W untimed
     graph-replayed steps, then exactly K graph-replayed steps bracketed by HIP events on the
     level's stream (after a device synchronisation), then per-launch durations of the update
     kernels inside a replayed graph (stamps on each kernel's stream)."""
@@ -159,8 +160,9 @@ def shape_record(r):
     rows, bx, by = r["shape"]
     out = {"march_rows": rows, "tile": [bx, by], "autotuned": r["tuning"] is not None}
     if r["tuning"] is not None:
-        out["tried_ms_per_step"] = {("march%d" % t[0]) if t[0] else ("tile%dx%d" % (t[1], t[2])): round(t[3], 4)
-                                    for t in r["tuning"]["tried"]}
+        out["workgroup_order"] = "xcd bands" if r["tuning"]["xcd"] else "row-major"
+        out["tried_ms_per_step"] = {(("march%d" % t[0]) if t[0] else ("tile%dx%d" % (t[1], t[2]))) +
+                                    ("" if t[3] else "-rowmajor"): round(t[4], 4) for t in r["tuning"]["tried"]}
     return out
 
 

@@ -366,13 +366,21 @@ int tmk_set_march(tmk_ctx *ctx, int rows);
  * shape the update runs (march_rows 0 where the numerics keep the tile kernels). */
 int tmk_level_set_update_shape(tmk_level *lev, int march_rows, int bx, int by);
 int tmk_level_get_update_shape(const tmk_level *lev, int *march_rows, int *bx, int *by);
+/* The order of a level's interior workgroups (no reference counterpart, whose box loop is
+ * serial, LinElastLevelGodunov.cpp:284-358): xcd = 1 (the default) gives each of the device's
+ * 8 XCDs a contiguous band of the interior tiles or strips, so that the halos neighbouring
+ * workgroups share stay in that XCD's L2; 0 keeps row-major order.  Results do not depend on
+ * it.  A batch or hierarchy level takes its first member's order.  Not while the level is in a
+ * batch or hierarchy. */
+int tmk_level_set_tile_order(tmk_level *lev, int xcd);
 
 /* Autotuning of a level's update shape (no reference counterpart).  Times nsteps
  * graph-replayed steps from (time, dt), after two untimed ones, for every strip height of the
  * row-marching update (4 .. 64 rows, where the numerics allow it) and every tile shape, each
- * from the level's current state, then sets the fastest shape and returns the level to the
- * state it started from (a device copy of the state is held meanwhile).  Not while the level
- * is in a batch or hierarchy or has fault stations. */
+ * from the level's current state, then the fastest of them with the other order of the
+ * interior workgroups (tmk_level_set_tile_order); sets the fastest shape and order and returns
+ * the level to the state it started from (a device copy of the state is held meanwhile).  Not
+ * while the level is in a batch or hierarchy or has fault stations. */
 #define TMK_TUNE_MAX 32
 typedef struct {
     int march_rows, bx, by;  /* the shape chosen and set */
@@ -380,6 +388,8 @@ typedef struct {
     int ntried;
     int march_rows_tried[TMK_TUNE_MAX], bx_tried[TMK_TUNE_MAX], by_tried[TMK_TUNE_MAX];
     double ms_tried[TMK_TUNE_MAX];
+    int xcd;                 /* the order chosen and set (tmk_level_set_tile_order) */
+    int xcd_tried[TMK_TUNE_MAX];
 } tmk_tuning;
 int tmk_level_autotune(tmk_level *lev, double time, double dt, int nsteps, tmk_tuning *out);
 

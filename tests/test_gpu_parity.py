@@ -785,6 +785,28 @@ def _downloaded(lev, boxes, U, B):
 
 
 @pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+@pytest.mark.parametrize("shape", [(8, 0, 0), (0, 32, 16)])
+def test_workgroup_order_does_not_change_results(precision, shape):
+    """Interior workgroups in XCD bands or in row-major order (tmk_level_set_tile_order) run the
+    same arithmetic on the same cells: bit-identical states and fault data after several steps.
+    The level is large enough (hundreds of interior strips or tiles) for the orders to differ."""
+    cfg = _cfg("sw", "PPM", nx=512, ny=256)
+    boxes = [(0, 0, 511, 255)]
+    dt = scenarios.dt_of(cfg)
+    out = []
+    for xcd in (1, 0):
+        ctx, lev, U, B = _uploaded_level(cfg, boxes, precision, seed=52)
+        lev.set_update_shape(*shape)
+        lev.set_tile_order(xcd)
+        lev.run(5, 0.0, dt)
+        lev.sync()
+        out.append(_downloaded(lev, boxes, U, B))
+        lev.close()
+        ctx.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
 def test_autotune_restores_state_and_sets_fastest(precision):
     """Every strip height and tile shape is timed from the level's state; the level then holds
     that state again, bit for bit, and runs the fastest shape.  FP64: the tuned level steps to
@@ -800,10 +822,12 @@ def test_autotune_restores_state_and_sets_fastest(precision):
     after = _downloaded(lev, boxes, U, B)
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
     tried = r["tried"]
-    assert len(tried) == 20  # 12 strip heights, 8 tile shapes
-    assert r["ms_per_step"] == min(t[3] for t in tried) > 0.0
+    assert len(tried) == 21  # 12 strip heights, 8 tile shapes, the fastest in the other order
+    assert [t[3] for t in tried] == [1] * 20 + [0]
+    assert tried[20][:3] == min(tried[:20], key=lambda t: t[4])[:3]
+    assert r["ms_per_step"] == min(t[4] for t in tried) > 0.0
     assert (r["march_rows"], r["bx"], r["by"]) == lev.update_shape()
-    assert (r["march_rows"], r["bx"], r["by"], r["ms_per_step"]) in [tuple(t) for t in tried]
+    assert (r["march_rows"], r["bx"], r["by"], r["xcd"], r["ms_per_step"]) in [tuple(t) for t in tried]
     lev.run(4, 0.0, dt)
     lev.sync()
     tuned = _downloaded(lev, boxes, U, B)

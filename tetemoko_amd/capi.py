@@ -36,6 +36,7 @@ PREC_FP32_BULK = 1
 EXPORTED = (
     "tmk_ctx_create", "tmk_ctx_destroy", "tmk_strerror", "tmk_device_ok", "tmk_mem_info",
     "tmk_cache_info", "tmk_cache_set_limit", "tmk_cache_trim", "tmk_batch_done", "tmk_amr_done",
+    "tmk_level_set_tile_order",
     "tmk_level_footprint", "tmk_level_footprint_params", "tmk_batch_create", "tmk_batch_destroy",
     "tmk_batch_add", "tmk_batch_remove", "tmk_batch_run", "tmk_batch_sync", "tmk_level_fill",
     "tmk_level_download_fault", "tmk_level_tag", "tmk_patch_update_state", "tmk_patch_plastic_update",
@@ -61,7 +62,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("march_rows", ctypes.c_int), ("bx", ctypes.c_int), ("by", ctypes.c_int),
                 ("ms_per_step", ctypes.c_double), ("ntried", ctypes.c_int),
                 ("march_rows_tried", ctypes.c_int * TUNE_MAX), ("bx_tried", ctypes.c_int * TUNE_MAX),
-                ("by_tried", ctypes.c_int * TUNE_MAX), ("ms_tried", ctypes.c_double * TUNE_MAX)]
+                ("by_tried", ctypes.c_int * TUNE_MAX), ("ms_tried", ctypes.c_double * TUNE_MAX),
+                ("xcd", ctypes.c_int), ("xcd_tried", ctypes.c_int * TUNE_MAX)]
 
 
 class MeshParams(ctypes.Structure):
@@ -181,6 +183,7 @@ def lib():
         L.tmk_level_get_update_shape.argtypes = [vp, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                                  ctypes.POINTER(c_int)]
         L.tmk_level_autotune.argtypes = [vp, _D, _D, c_int, ctypes.POINTER(Tuning)]
+        L.tmk_level_set_tile_order.argtypes = [vp, c_int]
         sz = ctypes.c_size_t
         L.tmk_mem_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
         L.tmk_cache_info.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)]
@@ -431,14 +434,20 @@ class Level:
                "tmk_level_get_update_shape")
         return r.value, x.value, y.value
 
+    def set_tile_order(self, xcd):
+        """Interior workgroups in XCD bands (1, the default) or row-major order (0)."""
+        _check(lib().tmk_level_set_tile_order(self.h, int(xcd)), "tmk_level_set_tile_order")
+
     def autotune(self, time, dt, nsteps=10):
-        """Times every update shape from the level's current state, sets the fastest and returns
-        the level to that state: {"march_rows", "bx", "by", "ms_per_step", "tried": [(rows, bx,
-        by, ms), ...]}."""
+        """Times every update shape from the level's current state, then the fastest with the
+        other workgroup order; sets the fastest and returns the level to that state:
+        {"march_rows", "bx", "by", "xcd", "ms_per_step", "tried": [(rows, bx, by, xcd, ms), ...]}."""
         t = Tuning()
         _check(lib().tmk_level_autotune(self.h, time, dt, nsteps, ctypes.byref(t)), "tmk_level_autotune")
-        tried = [(t.march_rows_tried[k], t.bx_tried[k], t.by_tried[k], t.ms_tried[k]) for k in range(t.ntried)]
-        return {"march_rows": t.march_rows, "bx": t.bx, "by": t.by, "ms_per_step": t.ms_per_step, "tried": tried}
+        tried = [(t.march_rows_tried[k], t.bx_tried[k], t.by_tried[k], t.xcd_tried[k], t.ms_tried[k])
+                 for k in range(t.ntried)]
+        return {"march_rows": t.march_rows, "bx": t.bx, "by": t.by, "xcd": t.xcd, "ms_per_step": t.ms_per_step,
+                "tried": tried}
 
 
 class Amr:

@@ -341,7 +341,7 @@ inline bool tile_empty(const Tile& t) { return t.x0 > t.x1 || t.y0 > t.y1; }
 // core as an interior tile and its bands as pieces.  P(t): the parameters of tile t's scenario.
 // Returns the interior count.
 template <typename F, typename G>
-int partition_tiles(std::vector<Tile>& t, F is_interior, G params)
+int partition_tiles(std::vector<Tile>& t, F is_interior, G params, bool xcd = true)
 {
     std::vector<Tile> in, bf, bo;
     for (const Tile& x : t) {
@@ -353,7 +353,7 @@ int partition_tiles(std::vector<Tile>& t, F is_interior, G params)
         const Tile c = band_split(x, P, [&](const Tile& p) { (p.y0 == P.dlo[1] ? bf : bo).push_back(p); });
         if (!tile_empty(c)) in.push_back(c);
     }
-    xcd_order(in);
+    if (xcd) xcd_order(in);
     const int nint = (int)in.size();
     t.swap(in);
     t.insert(t.end(), bf.begin(), bf.end());
@@ -405,6 +405,7 @@ struct Group {
     size_t tsize = 8;
     int ns = 2, bx = 32, by = 16;
     int march = 0;                 // > 0: interior strips of this many rows run k_godunov_march
+    int xcd = 1;                   // interior tiles / strips in XCD bands (the first member's choice)
     int pk = PK_RUNTIME;  // predictor kind the kernel is specialised for (all members agree)
     Scen* scen_dev = nullptr;
     int nscen = 0;
@@ -497,6 +498,7 @@ struct tmk_level {
     size_t tsize = 8;
     int bx = 32, by = 16;         // update tile (the context's at creation; tmk_level_set_update_shape)
     int march = 0;                // strip rows of the row-marching update (0: tile kernels)
+    int xcd = 1;                  // interior tiles / strips in XCD bands (0: row-major order)
     std::vector<Island> isl;
     std::vector<int> box_isl;     // island of each box
     int npatch = 0;               // descriptors: one per box, then one per island (the tiles')
@@ -1045,6 +1047,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     G.bx = L0->bx;
     G.by = L0->by;
     G.march = (L0->march > 0 && G.ns == 2 && G.pk == PK_PPM_PRIM) ? L0->march : 0;
+    G.xcd = L0->xcd;
     G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     G.plastic = plastic;
     G.g = 0;
@@ -1085,7 +1088,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
         // the rectangles become interior strips, then boundary pieces (fault row first)
         std::vector<Tile> strips, bf, bo;
         for (const Tile& r : tiles) march_rect(r, sc[patch_scen[r.patch]].P, G.march, strips, bf, bo);
-        xcd_order(strips);
+        if (G.xcd) xcd_order(strips);
         G.nint = (int)strips.size();
         tiles.swap(strips);
         tiles.insert(tiles.end(), bf.begin(), bf.end());
@@ -1093,7 +1096,7 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     } else {
         G.nint = partition_tiles(
             tiles, [&](const Tile& t) { return tile_interior_host(t, sc[patch_scen[t.patch]].P, bx, by); },
-            [&](const Tile& t) -> const DevParams& { return sc[patch_scen[t.patch]].P; });
+            [&](const Tile& t) -> const DevParams& { return sc[patch_scen[t.patch]].P; }, G.xcd != 0);
     }
     G.cells_int = G.cells_bnd = 0;
     for (size_t k = 0; k < tiles.size(); ++k)
@@ -2406,6 +2409,15 @@ extern "C" int tmk_level_set_update_shape(tmk_level* L, int march_rows, int bx, 
     return TMK_OK;
 }
 
+extern "C" int tmk_level_set_tile_order(tmk_level* L, int xcd)
+{
+    if (!L || L->batch || L->amr_stream || (xcd != 0 && xcd != 1)) return TMK_E_ARG;
+    HIPCHK(hipStreamSynchronize(L->ctx->stream));
+    L->xcd = xcd;
+    L->own_built = false;
+    return TMK_OK;
+}
+
 extern "C" int tmk_level_get_update_shape(const tmk_level* L, int* march_rows, int* bx, int* by)
 {
     if (!L || !march_rows || !bx || !by) return TMK_E_ARG;
@@ -2484,7 +2496,7 @@ extern "C" int tmk_level_autotune(tmk_level* L, double time, double dt, int nste
     HIPCHK(hipStreamSynchronize(st));
     LevelSnapshot snap;
     if ((e = snap.take(L, st))) return e;
-    const int march0 = L->march, bx0 = L->bx, by0 = L->by;
+    const int march0 = L->march, bx0 = L->bx, by0 = L->by, xcd0 = L->xcd;
     std::vector<std::array<int, 3>> cand;
     if (L->ns == 2 && pred_kind(c->P) == PK_PPM_PRIM)
         for (int rows : kTuneMarch) cand.push_back({rows, bx0, by0});
@@ -2506,16 +2518,41 @@ extern "C" int tmk_level_autotune(tmk_level* L, double time, double dt, int nste
         out->ntried = (int)k + 1;
         if (best < 0 || out->ms_tried[k] < out->ms_tried[best]) best = (int)k;
     }
+    for (int k = 0; k < out->ntried; ++k) out->xcd_tried[k] = xcd0;
+    // the order of the fastest shape's interior workgroups: the other of XCD bands and row-major
+    int best_xcd = xcd0;
+    if (!e && best >= 0 && out->ntried < TMK_TUNE_MAX) {
+        const int k = out->ntried;
+        double ms = 0.0;
+        if (!(e = snap.restore(L, st)) &&
+            !(e = tmk_level_set_update_shape(L, cand[best][0], cand[best][1], cand[best][2])) &&
+            !(e = tmk_level_set_tile_order(L, 1 - xcd0)) && !(e = tmk_level_run(L, 2, time, dt)) &&
+            !(e = tmk_level_run_timed(L, nsteps, time + dt + dt, dt, &ms)) && !(e = tmk_level_sync(L))) {
+            out->march_rows_tried[k] = cand[best][0];
+            out->bx_tried[k] = cand[best][1];
+            out->by_tried[k] = cand[best][2];
+            out->xcd_tried[k] = 1 - xcd0;
+            out->ms_tried[k] = ms / nsteps;
+            out->ntried = k + 1;
+            if (out->ms_tried[k] < out->ms_tried[best]) {
+                best = k;
+                best_xcd = 1 - xcd0;
+            }
+        }
+    }
     int e2 = snap.restore(L, st);
     L->time = time;
     if (e || best < 0) {
         (void)tmk_level_set_update_shape(L, march0, bx0, by0);
+        (void)tmk_level_set_tile_order(L, xcd0);
         return e ? e : e2;
     }
-    if ((e = tmk_level_set_update_shape(L, cand[best][0], cand[best][1], cand[best][2]))) return e;
-    out->march_rows = cand[best][0];
-    out->bx = cand[best][1];
-    out->by = cand[best][2];
+    const int rows = out->march_rows_tried[best], bx = out->bx_tried[best], by = out->by_tried[best];
+    if ((e = tmk_level_set_update_shape(L, rows, bx, by)) || (e = tmk_level_set_tile_order(L, best_xcd))) return e;
+    out->march_rows = rows;
+    out->bx = bx;
+    out->by = by;
+    out->xcd = best_xcd;
     out->ms_per_step = out->ms_tried[best];
     return e2;
 }

@@ -1087,7 +1087,11 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     if (G.march) {
         // the rectangles become interior strips, then boundary pieces (fault row first)
         std::vector<Tile> strips, bf, bo;
-        for (const Tile& r : tiles) march_rect(r, sc[patch_scen[r.patch]].P, G.march, strips, bf, bo);
+        // strips of levels with flux registers hold at most 63 rows (the kernel's face masks)
+        bool fluxes = false;
+        for (const Patch<T>& p : pts[0]) fluxes = fluxes || p.F[0] || p.F[1];
+        const int rows = fluxes ? std::min(G.march, 63) : G.march;
+        for (const Tile& r : tiles) march_rect(r, sc[patch_scen[r.patch]].P, rows, strips, bf, bo);
         if (G.xcd) xcd_order(strips);
         G.nint = (int)strips.size();
         tiles.swap(strips);

@@ -192,25 +192,25 @@ __device__ __forceinline__ void fault_output(const DevParams& P, const FaultOut&
 // the flux register array of direction d (surroundingNodes(valid box, d)).  With stores_delta
 // this is the flux of U - bg: flux registers only ever difference fluxes of one precision, and
 // the drop-in path adds the background's flux (bg_flux) on the host.
-template <typename T, int NS>
-__device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
-                                                const T (&sv)[NS], T r, int i, int j, int sd)
+// whether a face between cells with face-mask bits a and b carries a register flux: a face of
+// the level's boundary (exactly one side a cell of the level), or, between two cells of the
+// level, a face around the finer level (exactly one side covered by it)
+__device__ __forceinline__ bool register_face(unsigned a, unsigned b)
 {
-    const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
-    if (pt.fmask) {
-        const int mw = pt.vhi0 - pt.vlo0 + 3;
-        const __attribute__((address_space(1))) unsigned char* mk =
-            gmem(pt.fmask) + (size_t)(i - pt.vlo0 + 1) + (size_t)(j - pt.vlo1 + 1) * mw;
-        const unsigned a = d == 0 ? mk[-1] : mk[-mw], b = *mk;
-        const bool edge = ((a ^ b) & 1) != 0;                 // level boundary (fine side)
-        const bool cf = (a & b & 1) && ((a ^ b) & 2);        // around the finer level (coarse side)
-        if (!edge && !cf) return;
-    }
-    const size_t fp = (size_t)fnx * fny;
-    const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+    const bool edge = ((a ^ b) & 1) != 0;           // level boundary (fine side)
+    const bool cf = (a & b & 1) && ((a ^ b) & 2);  // around the finer level (coarse side)
+    return edge || cf;
+}
+
+// the 11 components of side sd of the face flux in direction d at index fi of the flux array
+// Fo (fp faces per component plane)
+template <typename T, int NS>
+__device__ __forceinline__ void put_face_flux(__attribute__((address_space(1))) double* Fo, size_t fi, size_t fp,
+                                              const Mat<T>& m, int d, const T (&v)[NS], const T (&sv)[NS], T r,
+                                              int sd)
+{
     T f[7];
     face_flux_side<T, NS>(m, v, sv, r, f);
-    __attribute__((address_space(1))) double* Fo = gmem(pt.F[d]);
     const int o = 11 * sd;
     if (d == 0) {
         Fo[fi + (o + 0) * fp] = (double)f[0];
@@ -235,6 +235,22 @@ __device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>
     }
     Fo[fi + (o + 9) * fp] = 0.0;
     Fo[fi + (o + 10) * fp] = 0.0;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ void store_flux_side(const Patch<T>& pt, const Mat<T>& m, int d, const T (&v)[NS],
+                                                const T (&sv)[NS], T r, int i, int j, int sd)
+{
+    const int fnx = pt.vhi0 - pt.vlo0 + 1 + (d == 0), fny = pt.vhi1 - pt.vlo1 + 1 + (d == 1);
+    if (pt.fmask) {
+        const int mw = pt.vhi0 - pt.vlo0 + 3;
+        const __attribute__((address_space(1))) unsigned char* mk =
+            gmem(pt.fmask) + (size_t)(i - pt.vlo0 + 1) + (size_t)(j - pt.vlo1 + 1) * mw;
+        if (!register_face(d == 0 ? mk[-1] : mk[-mw], *mk)) return;
+    }
+    const size_t fp = (size_t)fnx * fny;
+    const size_t fi = (size_t)(i - pt.vlo0) + (size_t)(j - pt.vlo1) * fnx;
+    put_face_flux<T, NS>(gmem(pt.F[d]), fi, fp, m, d, v, sv, r, sd);
 }
 
 template <typename T, int NS>

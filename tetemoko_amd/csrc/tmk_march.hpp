@@ -234,13 +234,64 @@ k_godunov_march(const KArgs<T> a)
     const int rlast = y1 + 3;  // the last row loaded (row y1 is updated after it)
     constexpr bool ring_fetch = sizeof(T) == sizeof(float);
     static_assert((MARCH_RING - 1) * (MARCH_RING_LOADS + MARCH_ROW_STORES) <= 63, "vmcnt counts to 63");
-    // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL)
-    __shared__ float ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64 + 16];
-    float* const pconst = &ring[(ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64];
+    // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL), then
+    // the lanes' flux-register face masks (below)
+    constexpr int RING_FLOATS = (ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64;
+    __shared__ float ring[RING_FLOATS + 16 + 4 * 64];
+    float* const pconst = &ring[RING_FLOATS];
     if constexpr (ring_fetch) {
         if (lane < 14) pconst[lane] = lane < 7 ? P.pbg[lane] : lane < 13 ? P.pf[lane - 7] : P.fG;
     }
     const PlasticConstL pc{(const volatile __attribute__((address_space(3))) float*)&pconst[0]};
+    // Flux registers (levels of a hierarchy): which faces this lane stores on each row of the
+    // strip -- bit j - y0 of a 64-bit mask for its lo x face and one for its lo y face, words
+    // [0, 64) / [64, 128) and [128, 192) / [192, 256) -- decided here from the level's face masks,
+    // with the flux arrays' extents read once: the row loop then reads no descriptor and no mask
+    // (a vector load there would wait for every row request in flight).  Strips of such levels
+    // have at most 63 rows (the group build caps them).
+    volatile __attribute__((address_space(3))) unsigned* const fsel =
+        (volatile __attribute__((address_space(3))) unsigned*)(__attribute__((address_space(3))) float*)&ring[
+            RING_FLOATS + 16];
+    const int fvlo0 = __builtin_amdgcn_readfirstlane(pt.vlo0), fvlo1 = __builtin_amdgcn_readfirstlane(pt.vlo1);
+    const int fvnx = __builtin_amdgcn_readfirstlane(pt.vhi0) - fvlo0 + 1;
+    const int fvny = __builtin_amdgcn_readfirstlane(pt.vhi1) - fvlo1 + 1;
+    // the flux arrays' addresses in scalar registers (uniform): the stores then take a scalar
+    // base and a vector offset, and hold no 64-bit vector address through the loop
+    auto uniform_ptr = [](const double* q) {
+        const unsigned long long u = (unsigned long long)(uintptr_t)q;
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+        return (__attribute__((address_space(1))) double*)(uintptr_t)(((unsigned long long)hi << 32) | lo);
+    };
+    __attribute__((address_space(1))) double* const Fo0 = uniform_ptr(pt.F[0]);
+    __attribute__((address_space(1))) double* const Fo1 = uniform_ptr(pt.F[1]);
+    if ((flux0 || flux1) && y1 - y0 + 1 > 63) {  // the masks hold 64 rows of faces
+        if (lane == 0 && sd == 0) report(&S.err, TMK_E_BOX);
+        return;
+    }
+    if (flux0 || flux1) {
+        unsigned long long m0 = 0, m1 = 0;
+        const bool lx = flux0 && lane >= 3 && lane <= tnx + 3, ly = flux1 && valid;
+        if (!pt.fmask) {
+            m0 = lx ? ~0ull : 0ull;
+            m1 = ly ? ~0ull : 0ull;
+        } else if (lx || ly) {
+            const __attribute__((address_space(1))) unsigned char* mk = gmem(pt.fmask);
+            const int mw = fvnx + 2;  // the masks cover the descriptor's valid box grown by one
+            for (int j = y0; j <= y1 + 1; ++j) {
+                const size_t q = (size_t)(i - fvlo0 + 1) + (size_t)(j - fvlo1 + 1) * mw;
+                const unsigned b = mk[q];
+                if (lx && j <= y1 && register_face(mk[q - 1], b)) m0 |= 1ull << (j - y0);
+                if (ly && register_face(mk[q - mw], b)) m1 |= 1ull << (j - y0);
+            }
+        }
+        fsel[lane] = (unsigned)m0;
+        fsel[64 + lane] = (unsigned)(m0 >> 32);
+        fsel[128 + lane] = (unsigned)m1;
+        fsel[192 + lane] = (unsigned)(m1 >> 32);
+    }
+    // bit k (0 .. 63) of this lane's mask w (0: lo x face, 1: lo y face)
+    auto fbit = [&](int w, int k) { return ((fsel[128 * w + 64 * (k >> 5) + lane] >> (k & 31)) & 1u) != 0; };
     // this lane's dword in slot 0, plane 0 (an LDS byte address)
     const unsigned ring_lane =
         (unsigned)(uintptr_t)(__attribute__((address_space(3))) float*)&ring[0] + 4u * (unsigned)lane;
@@ -426,9 +477,20 @@ k_godunov_march(const KArgs<T> a)
         }
         const int j2 = r - 2;
         // flux registers (multi-level): x faces of row r - 3, the lo y face of row r - 2
-        if (flux0 && j3 >= y0 && lane >= 3 && lane <= tnx + 3)
-            store_flux_side<T, 2>(pt, m, 0, f0v, f0s, (T)1, i, j3, sd);
-        if (flux1 && j2 >= y0 && valid) store_flux_side<T, 2>(pt, m, 1, n.f1v, n.f1s, rvy, i, j2, sd);
+        // the lane's column, hidden from the compiler here, so that it builds each face's address
+        // in place rather than keeping a 64-bit per-lane address alive through the loop
+        if (flux0 && j3 >= y0 && fbit(0, j3 - y0)) {
+            int ix = i - fvlo0;
+            asm volatile("" : "+v"(ix));
+            put_face_flux<T, 2>(Fo0, (size_t)ix + (size_t)(j3 - fvlo1) * (fvnx + 1), (size_t)(fvnx + 1) * fvny, m, 0,
+                                f0v, f0s, (T)1, sd);
+        }
+        if (flux1 && j2 >= y0 && fbit(1, j2 - y0)) {
+            int ix = i - fvlo0;
+            asm volatile("" : "+v"(ix));
+            put_face_flux<T, 2>(Fo1, (size_t)ix + (size_t)(j2 - fvlo1) * fvnx, (size_t)fvnx * (fvny + 1), m, 1, n.f1v,
+                                n.f1s, rvy, sd);
+        }
         // update of row r - 3
         // FP32 runs the update on every row (rows before the strip store nothing), so that every
         // row issues exactly MARCH_ROW_STORES stores for the ring's waits

@@ -427,20 +427,44 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
 
     Slots<T, NS> WmX, WpX, WmY, WpY;
 
+    // Staging of a sweep's stencil planes into LDS: the planes in an unrolled outer loop (each
+    // plane's address uniform), every load of this thread issued before the first LDS store, so
+    // that the loads' latencies overlap (one plane address per element, read through a varying
+    // index, had put two dependent global loads and waits on every element).
+    const int ulo0 = pt.ulo0, ulo1 = pt.ulo1, unx = pt.unx;
+    auto stage = [&](int d, int RW, int RH, int gx0, int gy0, int hx, int hy) {
+        // plane c of sweep d over the RW x RH region from (gx0, gy0); only the tile's stencil
+        // region (cells up to tl.x1 + hx, tl.y1 + hy), inside the domain, is read
+        const int n = RW * RH;
+        constexpr int IT = (((BY + 4) * (BX + 4) - 16) + NT - 1) / NT;  // bound on per-plane trips
+        T v[NC][IT];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const T* base = pt.in[stage_comp<NS>(d, c)];
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int r = tid + k * NT;
+                const int yy = r / RW, xx = r - yy * RW;
+                const int gi = gx0 + xx, gj = gy0 + yy;
+                v[c][k] = (T)0;
+                if (r < n && (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + hx &&
+                                      gj <= tl.y1 + hy)))
+                    v[c][k] = gld(base, (unsigned)((gi - ulo0) + (gj - ulo1) * unx));
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int r = tid + k * NT;
+                if (r < n) sm[c * n + r] = v[c][k];
+            }
+    };
+
     // ---------------- stage 1: x predictor ----------------
     {
         constexpr int SW = BX + 4;
-        for (int e = tid; e < NC * BY * SW; e += NT) {
-            const int c = e / (BY * SW);
-            const int r = e - c * (BY * SW);
-            const int yy = r / SW, xx = r - yy * SW;
-            const int gi = tl.x0 - 3 + xx, gj = tl.y0 - 1 + yy;
-            T v = (T)0;
-            // only the tile's stencil region (x: tile +-3, y: tile +-1), inside the domain
-            if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 3 && gj <= tl.y1 + 1))
-                v = gld(pt.in[stage_comp<NS>(0, c)], (unsigned)uidx(pt, gi, gj, 0));
-            sm[e] = v;
-        }
+        stage(0, SW, BY, tl.x0 - 3, tl.y0 - 1, 3, 1);
         __syncthreads();
         if (r1) {
             T w[2][NS][2][5];
@@ -462,17 +486,7 @@ __device__ __forceinline__ void godunov_tile(const KArgs<T>& a, const Tile& tl, 
     // ---------------- stage 2: y predictor ----------------
     {
         constexpr int SH = BY + 4;
-        for (int e = tid; e < NC * SH * BX; e += NT) {
-            const int c = e / (SH * BX);
-            const int r = e - c * (SH * BX);
-            const int yy = r / BX, xx = r - yy * BX;
-            const int gi = tl.x0 - 1 + xx, gj = tl.y0 - 3 + yy;
-            T v = (T)0;
-            // only the tile's stencil region (x: tile +-1, y: tile +-3), inside the domain
-            if (INT || (gi >= dlo0 && gi <= dhi0 && gj >= dlo1 && gj <= dhi1 && gi <= tl.x1 + 1 && gj <= tl.y1 + 3))
-                v = gld(pt.in[stage_comp<NS>(1, c)], (unsigned)uidx(pt, gi, gj, 0));
-            sm[e] = v;
-        }
+        stage(1, BX, SH, tl.x0 - 1, tl.y0 - 3, 1, 3);
         __syncthreads();
         if (r1) {
             T w[2][NS][2][5];

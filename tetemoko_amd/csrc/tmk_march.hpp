@@ -118,9 +118,9 @@ __device__ __forceinline__ unsigned wait_vmem(unsigned addr)
     return addr;
 }
 
-// the wait before row k of a strip reads its ring slot (k counted from the strip's first row):
-// after the slot's loads come the loads of the MARCH_RING - 1 later slots and the stores of the
-// min(k, MARCH_RING - 1) rows since (the first slots are requested before any store)
+// the wait before a row of the march reads its ring slot, k = the number of rows that stored
+// since the slot was requested, up to MARCH_RING - 1 (k <= 0: none): after the slot's loads come
+// the loads of the MARCH_RING - 1 later slots and the stores of those k rows
 template <int K = 0>
 __device__ __forceinline__ unsigned wait_slot(int k, unsigned at)
 {
@@ -356,23 +356,31 @@ k_godunov_march(const KArgs<T> a)
     // ahead: row r + 1's stencil planes (a ghost row of the array at the last row) and row r - 3's
     // pointwise planes are requested before the row's arithmetic, so the wavefront computes while
     // they arrive.
-    auto row = [&](const int r, const MarchCarry<T>& c, MarchCarry<T>& n) {
+    // The first two rows of the march only feed the faces of the rows after them: they skip row
+    // r - 3's final x faces, update and stores.  FP64 runs them as separate code (head = true),
+    // so that the loop's rows have no branch around the update; FP32 runs every row in the loop
+    // and branches (this measured faster for FP32: its loop body stays one copy).
+    auto row = [&](auto head, const int r, const MarchCarry<T>& c, MarchCarry<T>& n) {
+        constexpr bool H = decltype(head)::value;
         const int j3 = r - 3;
+        const bool strip_row = !H && (!ring_fetch || j3 >= y0);  // the same in every lane
         // row r - 3's pointwise planes: szz, and gam with plasticity (in mode II sxz and syz
         // stay zero and the plastic multiplier lam is only written); a lane outside the strip,
         // or a row before it, gets an offset at or past every plane's bound (planes hold at most
         // 2^32 - 256 bytes), far enough below 2^32 that the second dword of an 8-byte access
         // does not wrap: the bound check makes its loads read 0 and drops its stores
-        const unsigned off3 = (valid && j3 >= y0) ? row_off(j3) : kDropOffset;
+        const unsigned off3 = (valid && strip_row) ? row_off(j3) : kDropOffset;
         T w0[5], szz, gam;
         if constexpr (ring_fetch) {
             // request the row MARCH_RING - 1 ahead, then wait for this row's slot.  Issued after
             // that slot's loads: the MARCH_RING - 1 later slots (7 loads each) and the stores of
-            // the rows since (8 each; the slots of the first rows were requested before any)
+            // the rows since (8 each).  The slots of the first rows were requested before the
+            // march, and its first two rows (rows y0 + 1, y0 + 2) store nothing, so the rows that
+            // stored since this slot's request number r - (y0 + 3), at most MARCH_RING - 1.
             request(r + MARCH_RING - 1);
             const unsigned sl = (unsigned)(r - y0) % (unsigned)MARCH_RING;
             const unsigned at =
-                wait_slot(r - (y0 + 1), ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float)));
+                wait_slot(r - (y0 + 3), ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float)));
             const __attribute__((address_space(3))) float* slot =
                 (const __attribute__((address_space(3))) float*)(uintptr_t)at;
 #pragma unroll
@@ -454,10 +462,27 @@ k_godunov_march(const KArgs<T> a)
             transverse_side<T, 2>(m, scale, h0v, h0s, hv, hs, yp.mv, yp.ms);
             transverse_side<T, 2>(m, scale, h0v, h0s, hv, hs, yp.pv, yp.ps);
         }
-        // row r - 3: x predictor corrected with the y-face states below and above it, then its
-        // final x-face state
-        T f0v[2], f0s[2];
+        // final state of the lo y face of row r - 2 (row r - 3's corrected W+, its corrected W-)
         {
+            T wmR[2];
+            left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
+            riemann_side<T, 2>(m, rvy, c.cpc, wmR, n.f1v, n.f1s);
+            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, n.cpc);
+        }
+        const int j2 = r - 2;
+        // flux registers (multi-level): the lo y face of row r - 2, the x faces of row r - 3
+        // (below); the lane's column, hidden from the compiler here, so that it builds each face's
+        // address in place rather than keeping a 64-bit per-lane address alive through the loop
+        if (flux1 && j2 >= y0 && fbit(1, j2 - y0)) {
+            int ix = i - fvlo0;
+            asm volatile("" : "+v"(ix));
+            put_face_flux<T, 2>(Fo1, (size_t)ix + (size_t)(j2 - fvlo1) * fvnx, (size_t)fvnx * (fvny + 1), m, 1, n.f1v,
+                                n.f1s, rvy, sd);
+        }
+        // row r - 3, a row of the strip: x predictor corrected with the y-face states below and
+        // above it, its final x-face state and its update
+        if (strip_row) {
+            T f0v[2], f0s[2];
             Pred2<T> xp = c.xp;
             transverse_side<T, 2>(m, scx, c.h1v, c.h1s, n.h1v, n.h1s, xp.mv, xp.ms);
             transverse_side<T, 2>(m, scx, c.h1v, c.h1s, n.h1v, n.h1s, xp.pv, xp.ps);
@@ -467,34 +492,14 @@ k_godunov_march(const KArgs<T> a)
             for (int k = 0; k < 2; ++k) wpL[k] = dshift<DPP_FROM_PREV>(cpx[k]);
             left_chars_side<T, 2>(m, (T)1, xp.mv, xp.ms, wmR);
             riemann_side<T, 2>(m, (T)1, wpL, wmR, f0v, f0s);
-        }
-        // final state of the lo y face of row r - 2 (row r - 3's corrected W+, its corrected W-)
-        {
-            T wmR[2];
-            left_chars_side<T, 2>(m, rvy, yp.mv, yp.ms, wmR);
-            riemann_side<T, 2>(m, rvy, c.cpc, wmR, n.f1v, n.f1s);
-            right_chars_side<T, 2>(m, rvy, yp.pv, yp.ps, n.cpc);
-        }
-        const int j2 = r - 2;
-        // flux registers (multi-level): x faces of row r - 3, the lo y face of row r - 2
-        // the lane's column, hidden from the compiler here, so that it builds each face's address
-        // in place rather than keeping a 64-bit per-lane address alive through the loop
-        if (flux0 && j3 >= y0 && fbit(0, j3 - y0)) {
-            int ix = i - fvlo0;
-            asm volatile("" : "+v"(ix));
-            put_face_flux<T, 2>(Fo0, (size_t)ix + (size_t)(j3 - fvlo1) * (fvnx + 1), (size_t)(fvnx + 1) * fvny, m, 0,
-                                f0v, f0s, (T)1, sd);
-        }
-        if (flux1 && j2 >= y0 && fbit(1, j2 - y0)) {
-            int ix = i - fvlo0;
-            asm volatile("" : "+v"(ix));
-            put_face_flux<T, 2>(Fo1, (size_t)ix + (size_t)(j2 - fvlo1) * fvnx, (size_t)fvnx * (fvny + 1), m, 1, n.f1v,
-                                n.f1s, rvy, sd);
-        }
-        // update of row r - 3
-        // FP32 runs the update on every row (rows before the strip store nothing), so that every
-        // row issues exactly MARCH_ROW_STORES stores for the ring's waits
-        if (ring_fetch || j3 >= y0) {
+            if (flux0 && fbit(0, j3 - y0)) {
+                int ix = i - fvlo0;
+                asm volatile("" : "+v"(ix));
+                put_face_flux<T, 2>(Fo0, (size_t)ix + (size_t)(j3 - fvlo1) * (fvnx + 1), (size_t)(fvnx + 1) * fvny, m,
+                                    0, f0v, f0s, (T)1, sd);
+            }
+            // update of row r - 3: every row of the strip issues exactly MARCH_ROW_STORES stores
+            // (the ring's waits count them)
             Faces1<T, 2> F;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -551,10 +556,17 @@ k_godunov_march(const KArgs<T> a)
 
     // two rows per trip, the carried state alternating between A and B (no copies between rows;
     // the DPP lane shifts are convergent, so the compiler does not unroll the loop itself)
-    for (int r = y0 + 1;; r += 2) {
-        row(r, A, B);
+    // (rlast >= y0 + 3: a strip has at least one row)
+    int r0 = y0 + 1;
+    if constexpr (!ring_fetch) {
+        row(std::true_type{}, y0 + 1, A, B);
+        row(std::true_type{}, y0 + 2, B, A);
+        r0 = y0 + 3;
+    }
+    for (int r = r0;; r += 2) {
+        row(std::false_type{}, r, A, B);
         if (r + 1 > rlast) break;
-        row(r + 1, B, A);
+        row(std::false_type{}, r + 1, B, A);
         if (r + 2 > rlast) break;
     }
 }

@@ -740,6 +740,22 @@ def test_march_fp32_bulk_level():
     assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= 1e-4), fault_errors(Bg, Br)
 
 
+@pytest.mark.parametrize("rows", [1, 2, 3])
+def test_march_fp32_short_strips(rows):
+    """FP32 strips of 1-3 rows over boxes of 40 cells: the LDS ring's waits at the start of a
+    strip, where the first two rows of the march store nothing, and strips that end while rows are
+    still in flight.  Within the FP32 tolerance of the oracle, and bit-identical to 8-row strips (a
+    cell's arithmetic does not depend on the strip height)."""
+    cfg = _cfg("rs", "PPM", nx=200, ny=120)
+    boxes = scenarios.split_boxes(200, 120, 40)
+    kw = dict(precision=capi.PREC_FP32_BULK)
+    (Ug, Ur), (Bg, Br) = _level_case(cfg, boxes, 6, seed=45, march=rows, **kw)
+    assert np.all(_relerr(Ug, Ur) <= 1e-5), _relerr(Ug, Ur)
+    assert np.all(_relerr(Bg[[6, 8, 10]], Br[[6, 8, 10]]) <= 1e-4), fault_errors(Bg, Br)
+    (U8, _), (B8, _) = _level_case(cfg, boxes, 6, seed=45, march=8, **kw)
+    assert np.array_equal(Ug, U8, equal_nan=True) and np.array_equal(Bg, B8, equal_nan=True)
+
+
 def test_march_equals_tile_kernels_fp64():
     """The marching update and the tile kernels give the same FP64 state, bit for bit, on a
     slip-weakening level (no transcendental function anywhere)."""

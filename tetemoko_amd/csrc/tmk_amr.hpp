@@ -245,14 +245,15 @@ __global__ void k_amr_reflux(const RefluxCell* __restrict__ cells, int ncells, c
     *u = (T)v;
 }
 
-// coarse = (sum of the r x r fine cells, i fastest) / r^2
+// coarse = (sum of the r x r fine cells, i fastest) / r^2, for the components in comps
 template <typename T>
-__global__ void k_amr_avg_state(const AvgCell* __restrict__ cells, int ncells, const Patch<T>* __restrict__ crs,
-                                const Patch<T>* __restrict__ fine, int r, double scale)
+__global__ void k_amr_avg_state(const AvgCell* __restrict__ cells, int ncells, const int* __restrict__ comps,
+                                int ncomp, const Patch<T>* __restrict__ crs, const Patch<T>* __restrict__ fine, int r,
+                                double scale)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncells * TMK_NCOMP) return;
-    const int c = t / ncells, k = t % ncells;
+    if (t >= ncells * ncomp) return;
+    const int c = comps[t / ncells], k = t % ncells;
     const AvgCell a = cells[k];
     const Patch<T>& pf = fine[a.fpatch];
     const __attribute__((address_space(1))) T* f = gmem(pf.in[c]) + a.foff;

@@ -2874,8 +2874,8 @@ struct AmrLevel {
     DevBuf cells, faces, comps;
     int ncells = 0, nfaces = 0, ncomps = 0;
     // flux register with the next finer level, and the average-down from it (levels .. n-2)
-    DevBuf regf, reg, rcells, rscale, avgc, avgf;
-    int nreg = 0, nrcells = 0, navgc = 0, navgf = 0;
+    DevBuf regf, reg, rcells, rscale, avgc, avgf, acomps;
+    int nreg = 0, nrcells = 0, navgc = 0, navgf = 0, nacomps = 0;
 };
 
 // one scenario of a hierarchy batch: its levels, and the face fluxes and face masks of their
@@ -3407,9 +3407,15 @@ static int amr_build_t(tmk_amr* A)
                 all.avgf.push_back(x);
             }
         }
-        std::vector<int> comps;
+        std::vector<int> comps, acomps;
         for (int k = 0; k < TMK_NCOMP; ++k)
             if (is_stencil(k, A->mem[0]->L[l]->ns)) comps.push_back(k);
+        // the average-down skips the anti-plane planes of mode II levels: zero on every level
+        // (uploads and fills refuse other values there, the update never writes them, and the
+        // members of a batch agree on the mode), so their average is the zero already there
+        for (int k = 0; k < TMK_NCOMP; ++k)
+            if (A->mem[0]->L[l]->ns == 3 || !(k % 11 == 2 || k % 11 == 7 || k % 11 == 8)) acomps.push_back(k);
+        c.nacomps = (int)acomps.size();
         f.ncells = (int)all.cells.size();
         f.nfaces = (int)all.faces.size();
         f.ncomps = (int)comps.size();
@@ -3421,7 +3427,8 @@ static int amr_build_t(tmk_amr* A)
         if ((e = upload_list(f.cells, all.cells, st)) || (e = upload_list(f.faces, all.faces, st)) ||
             (e = upload_list(f.comps, comps, st)) || (e = upload_list(c.regf, all.regf, st)) ||
             (e = upload_list(c.rcells, all.rcells, st)) || (e = upload_list(c.rscale, all.rscale, st)) ||
-            (e = upload_list(c.avgc, all.avgc, st)) || (e = upload_list(c.avgf, all.avgf, st)))
+            (e = upload_list(c.avgc, all.avgc, st)) || (e = upload_list(c.avgf, all.avgf, st)) ||
+            (e = upload_list(c.acomps, acomps, st)))
             return e;
         if ((e = c.reg.ensure(std::max<size_t>(sizeof(double) * TMK_NCOMP * all.regf.size(), 1)))) return e;
     }
@@ -3640,8 +3647,9 @@ static int amr_post_t(tmk_amr* A, int l)
         HIPCHK(hipGetLastError());
     }
     if (a.navgc > 0) {
-        hipLaunchKernelGGL(k_amr_avg_state<T>, amr_grid((long long)a.navgc * TMK_NCOMP), dim3(256), 0, st,
-                           (const AvgCell*)a.avgc.p, a.navgc, cur, fcur, r, 1.0 / (r * r));
+        hipLaunchKernelGGL(k_amr_avg_state<T>, amr_grid((long long)a.navgc * a.nacomps), dim3(256), 0, st,
+                           (const AvgCell*)a.avgc.p, a.navgc, (const int*)a.acomps.p, a.nacomps, cur, fcur, r,
+                           1.0 / (r * r));
         HIPCHK(hipGetLastError());
     }
     if (a.navgf > 0) {

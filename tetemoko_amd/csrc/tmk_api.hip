@@ -45,7 +45,8 @@ namespace tmk {
 #define TMK_DECLARE_EXTERN(T, NS, BX, BY, PK) \
     extern template __global__ void k_godunov_interior<T, NS, BX, BY, PK>(const KArgs<T>);
 TMK_ALL_UPDATE_KERNELS(TMK_DECLARE_EXTERN, float)
-extern template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
+extern template __global__ void k_godunov_march<float, PK_PPM_PRIM, true>(const KArgs<float>);
+extern template __global__ void k_godunov_march<float, PK_PPM_PRIM, false>(const KArgs<float>);
 extern template __global__ void k_godunov_bnd<float, 2, PK_RUNTIME>(const KArgs<float>);
 extern template __global__ void k_godunov_bnd<float, 3, PK_RUNTIME>(const KArgs<float>);
 extern template __global__ void k_godunov_bnd<float, 2, PK_PPM_PRIM>(const KArgs<float>);
@@ -405,6 +406,7 @@ struct Group {
     size_t tsize = 8;
     int ns = 2, bx = 32, by = 16;
     int march = 0;                 // > 0: interior strips of this many rows run k_godunov_march
+    bool any_plast = true;         // some member has plasticity (the march variant with lam, gam)
     int xcd = 1;                   // interior tiles / strips in XCD bands (the first member's choice)
     int pk = PK_RUNTIME;  // predictor kind the kernel is specialised for (all members agree)
     Scen* scen_dev = nullptr;
@@ -898,17 +900,21 @@ static int timer_mark(const StepTimers* tm, hipStream_t s, int k)
 // the fused update of one step over a tile list [interior | boundary]: the boundary launch runs
 // on `side` (forked from and joined back into st by events; captured graphs keep the fork), the
 // interior launch on st
-// the interior strips of the row-marching partition (grid: two one-wavefront workgroups per strip)
+// the interior strips of the row-marching partition (grid: two one-wavefront workgroups per strip);
+// plast: some scenario of the launch has plasticity (the variant that reads and writes lam, gam)
 template <typename T>
-static int launch_march(hipStream_t st, const KArgs<T>& a)
+static int launch_march(hipStream_t st, const KArgs<T>& a, bool plast)
 {
-    hipLaunchKernelGGL((k_godunov_march<T, PK_PPM_PRIM>), dim3(2 * a.ntiles), dim3(64), 0, st, a);
+    if (plast && a.plastic)
+        hipLaunchKernelGGL((k_godunov_march<T, PK_PPM_PRIM, true>), dim3(2 * a.ntiles), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_godunov_march<T, PK_PPM_PRIM, false>), dim3(2 * a.ntiles), dim3(64), 0, st, a);
     return hipGetLastError() == hipSuccess ? TMK_OK : TMK_E_HIP;
 }
 
 template <typename T>
 static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join, int ns, int pk, int bx,
-                         int by, KArgs<T> a, int nint, const StepTimers* tm, int march = 0)
+                         int by, KArgs<T> a, int nint, const StepTimers* tm, int march = 0, bool plast = true)
 {
     const int nbnd = a.ntiles - nint;
     const Tile* tiles = a.tiles;
@@ -932,7 +938,7 @@ static int launch_update(hipStream_t st, hipStream_t side, hipEvent_t fork, hipE
         b.ntiles = nint;
         if ((e = timer_mark(tm, st, 0))) return e;
         if (march > 0) {
-            if ((e = launch_march<T>(st, b))) return e;
+            if ((e = launch_march<T>(st, b, plast))) return e;
         } else if ((e = launch_tiles<T>(st, ns, pk, bx, by, b))) {
             return e;
         }
@@ -956,7 +962,8 @@ static int group_enqueue_step_t(Group& G, hipStream_t st, const StepTimers* tm =
     a.tiles = G.tiles_dev;
     a.ntiles = G.ntiles;
     a.plastic = G.plastic;
-    int e = launch_update<T>(st, G.side, G.ev_fork, G.ev_join, G.ns, G.pk, G.bx, G.by, a, G.nint, tm, G.march);
+    int e = launch_update<T>(st, G.side, G.ev_fork, G.ev_join, G.ns, G.pk, G.bx, G.by, a, G.nint, tm, G.march,
+                             G.any_plast);
     if (e) return e;
     const Patch<T>* pn = (const Patch<T>*)G.patches_dev[G.g];  // its out planes hold the new state
     if (G.nrects > 0) {
@@ -1050,6 +1057,8 @@ static int group_build_t(Group& G, const std::vector<tmk_level*>& mem, const std
     G.xcd = L0->xcd;
     G.nb = L0->ctx->hp.fault_kind == TMK_FAULT_RS ? TMK_NBDRY_RS : TMK_NBDRY_SW;
     G.plastic = plastic;
+    G.any_plast = false;
+    for (tmk_level* L : mem) G.any_plast = G.any_plast || L->ctx->P.plast;
     G.g = 0;
     std::vector<Scen> sc(mem.size());
     std::vector<Patch<T>> pts[2];

@@ -19,6 +19,7 @@ template __global__ void k_godunov_bnd<float, 2, PK_RUNTIME>(const KArgs<float>)
 template __global__ void k_godunov_bnd<float, 3, PK_RUNTIME>(const KArgs<float>);
 template __global__ void k_godunov_bnd<float, 2, PK_PPM_PRIM>(const KArgs<float>);
 template __global__ void k_godunov_bnd<float, 3, PK_PPM_PRIM>(const KArgs<float>);
-template __global__ void k_godunov_march<float, PK_PPM_PRIM>(const KArgs<float>);
+template __global__ void k_godunov_march<float, PK_PPM_PRIM, true>(const KArgs<float>);
+template __global__ void k_godunov_march<float, PK_PPM_PRIM, false>(const KArgs<float>);
 
 }  // namespace tmk

@@ -272,7 +272,7 @@ struct Faces1 {
 // U += dU (-dt/dx) (LinElastPatchGodunov.cpp:117-126, LinElastPhysics.cpp:135-179), then, with
 // plastic, the Drucker-Prager return (LinElastLevelGodunov::plasticUpdate after the level step,
 // AMRLevelLinElast.cpp:476).  Changes u[0 1 3 4 5 6] (+ 2 7 8 with NS 3, + 7 8 9 10 with plastic).
-// PP: DevParams, or for FP32 PlasticConstL (the constants the FP32 plastic return reads)
+// PP: DevParams, or PlasticConstL (FP32) / PlasticConstL64 (FP64): the plastic return's constants in LDS
 template <typename T, int NS, typename PP>
 __device__ __forceinline__ void cell_update_side(const PP& P, const Mat<T>& m, double dt, T sc, bool plastic,
                                                  T ry, const Faces1<T, NS>& F, T (&u)[11])
@@ -331,7 +331,7 @@ __device__ __forceinline__ void cell_update_side(const PP& P, const Mat<T>& m, d
         u[8] = u[8] + (((T)0 + dsyz_x) + dsyz_y) * sc;
     }
     if constexpr (!regroup<T>())
-        if (plastic) plastic_flow<T>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
+        if (plastic) plastic_flow<T, PP>(P, (T)dt, u[3], u[4], u[5], u[6], u[7], u[8], u[9], u[10]);
 }
 
 // U += dU of side sd of valid cell (i, j) from the final states on its four faces, then the

@@ -17,9 +17,9 @@
 // expressions in the same order: a slope, face value or face state shared by two cells is
 // computed once, not twice (the face value of x + 1/2 is the same expression whether it is seen
 // as the hi face of cell x or the lo face of cell x + 1, FP additions being commutative).  The
-// FP64 kernel is therefore bit-identical to the oracle like the tile kernels.  There is no LDS
-// and no barrier; the two sides of a strip are independent wavefronts (inside the domain they
-// meet only at the fault, which no strip touches).  Mode II (NS = 2) with PPM and primitive
+// FP64 kernel is therefore bit-identical to the oracle like the tile kernels.  There is no
+// barrier (each wavefront uses LDS of its own); the two sides of a strip are independent
+// wavefronts (inside the domain they meet only at the fault, which no strip touches).  Mode II (NS = 2) with PPM and primitive
 // limiting only: other numerics use the tile kernels.
 #pragma once
 
@@ -30,11 +30,13 @@ namespace tmk {
 constexpr int MARCH_COLS = 58;  // 64 lanes minus the 3-cell x stencil on each side
 constexpr unsigned kDropOffset = 0xFFFFFF00u;  // buffer offset of a lane whose access must be dropped
 
-// FP32: rows requested ahead of their use into an LDS ring (one slot per row: the 5 stencil
-// planes of row q and the pointwise planes szz, gam of row q - 3), filled by buffer loads that
-// write LDS directly, so the requests in flight hold no VGPRs.  4 slots x 7 planes x 64 lanes x
-// 4 B = 7 KiB per wavefront; 16 wavefronts per CU use 112 of its 160 KiB.
-constexpr int MARCH_RING = 4;
+// Rows requested ahead of their use into an LDS ring (one slot per row: the 5 stencil planes of
+// row q and the pointwise planes szz, gam of row q - 3), filled by buffer loads that write LDS
+// directly, so the requests in flight hold no VGPRs.  FP32: 4 slots x 7 planes x 64 lanes x 4 B =
+// 7 KiB per wavefront, 16 wavefronts per CU use 112 of its 160 KiB.  FP64: each plane row is two
+// loads (low and high words), 3 slots (10.5 KiB with plasticity), 8 wavefronts per CU.  Every
+// vector memory operation that can be in flight at a ring wait fits the 6-bit counter.
+constexpr int MARCH_RING = 4;  // slots at most
 constexpr int MARCH_RING_LOADS = 7;   // loads per slot
 constexpr int MARCH_ROW_STORES = 8;   // stores per row: 5 stencil planes, szz, lam, gam
 
@@ -119,17 +121,17 @@ __device__ __forceinline__ unsigned wait_vmem(unsigned addr)
 }
 
 // the wait before a row of the march reads its ring slot, k = the number of rows that stored
-// since the slot was requested, up to MARCH_RING - 1 (k <= 0: none): after the slot's loads come
-// the loads of the MARCH_RING - 1 later slots and the stores of those k rows
-template <int K = 0>
+// since the slot was requested, up to RING - 1 (k <= 0: none): after the slot's loads come the
+// LOADS loads of each of the RING - 1 later slots and the STORES stores of each of those k rows
+template <int RING, int LOADS, int STORES, int K = 0>
 __device__ __forceinline__ unsigned wait_slot(int k, unsigned at)
 {
-    constexpr int N = (MARCH_RING - 1) * MARCH_RING_LOADS + K * MARCH_ROW_STORES;
-    if constexpr (K == MARCH_RING - 1) {
+    constexpr int N = (RING - 1) * LOADS + K * STORES;
+    if constexpr (K == RING - 1) {
         return wait_vmem<N>(at);
     } else {
         if (k <= K) return wait_vmem<N>(at);
-        return wait_slot<K + 1>(k, at);
+        return wait_slot<RING, LOADS, STORES, K + 1>(k, at);
     }
 }
 
@@ -163,7 +165,7 @@ struct Pred2 {
 // what one row of the march hands to the next (r = the row that row loaded)
 template <typename T>
 struct MarchCarry {
-    T wn[5];               // W of row r + 1 (requested ahead)
+    T wn[5];               // W of row r + 1 (requested ahead, without the ring)
     T w1[5], w2[5], w3[5]; // W of rows r, r - 1, r - 2 (the next row's r - 1, r - 2, r - 3)
     T dly[4];              // y differences W(r) - W(r - 1) of the y-sweep quantities
     T dy2[4];              // y slopes of row r - 1
@@ -178,8 +180,10 @@ struct MarchCarry {
 // grid: two workgroups of one wavefront per strip, side = blockIdx.x & 1 (a scalar, so every
 // plane pointer and descriptor of the side stays in scalar registers).  FP32: at least four
 // wavefronts per SIMD (at most 128 VGPRs) keep enough loads in flight to cover their latency;
-// FP64 needs about twice the registers and is left to the compiler.
-template <typename T, int PK>
+// FP64 needs about twice the registers (two wavefronts per SIMD) and is left to the compiler.
+// PL: the launch may hold scenarios with plasticity.  Without (PL false), a row neither reads gam
+// nor writes lam and gam, and a scenario with plasticity is an error.
+template <typename T, int PK, bool PL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == sizeof(float) ? 4 : 1)))
 k_godunov_march(const KArgs<T> a)
 {
@@ -205,7 +209,11 @@ k_godunov_march(const KArgs<T> a)
     const T rv1 = (T)-1;
     const T rvy = sd ? rv1 : (T)1;  // velocity sign of this side in y
     const T scx = sd ? rv1 * scale : scale;
-    const bool plastic = P.plast && a.plastic;
+    if (!PL && P.plast && a.plastic) {
+        if (threadIdx.x == 0 && (blockIdx.x & 1) == 0) report(&S.err, TMK_E_ARG);
+        return;
+    }
+    const bool plastic = PL && P.plast && a.plastic;
     const int o = 11 * sd;
     // stencil planes vx vy sxx syy sxy, read with the stencil (the in copies)
     constexpr int QC[5] = {0, 1, 3, 4, 6};
@@ -232,17 +240,36 @@ k_godunov_march(const KArgs<T> a)
     const int y0 = tl.y0, y1 = tl.y1;
 
     const int rlast = y1 + 3;  // the last row loaded (row y1 is updated after it)
-    constexpr bool ring_fetch = sizeof(T) == sizeof(float);
-    static_assert((MARCH_RING - 1) * (MARCH_RING_LOADS + MARCH_ROW_STORES) <= 63, "vmcnt counts to 63");
+    constexpr bool f32 = sizeof(T) == sizeof(float);
+    constexpr bool peel = !f32;  // the first two rows of the march as code of their own
+    // FP64 without plasticity requests one row ahead into VGPRs instead (its loop reads nothing
+    // else from memory, and the ring's extra loads and LDS reads measured slower there)
+    constexpr bool ring_fetch = f32 || PL;
+    // the ring: an element is WORDS dwords, each plane row of a slot WORDS loads of one dword per
+    // lane (FP64: low words, then high words); a slot holds NPL planes (gam only with plasticity)
+    // and a row of the strip stores ROW_STORES planes (lam and gam only with plasticity).  As many
+    // slots, at most MARCH_RING, as keep every operation that can be in flight at a wait -- the
+    // slot's own loads, the later slots' and the stores of RING - 1 rows -- within what the 6-bit
+    // counter holds
+    constexpr int WORDS = (int)(sizeof(T) / sizeof(float));
+    constexpr int NPL = PL ? MARCH_RING_LOADS : MARCH_RING_LOADS - 1;
+    constexpr int SLOT_LOADS = NPL * WORDS;
+    constexpr int ROW_STORES = PL ? MARCH_ROW_STORES : MARCH_ROW_STORES - 2;
+    constexpr int RING_FIT = (63 + ROW_STORES) / (SLOT_LOADS + ROW_STORES);
+    constexpr int RING = RING_FIT < MARCH_RING ? RING_FIT : MARCH_RING;
+    static_assert(RING >= 2 && RING * SLOT_LOADS + (RING - 1) * ROW_STORES <= 63, "vmcnt counts to 63");
     // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL), then
     // the lanes' flux-register face masks (below)
-    constexpr int RING_FLOATS = (ring_fetch ? MARCH_RING : 1) * MARCH_RING_LOADS * 64;
+    constexpr int RING_FLOATS = (ring_fetch ? RING * SLOT_LOADS : 0) * 64;
     __shared__ float ring[RING_FLOATS + 16 + 4 * 64];
     float* const pconst = &ring[RING_FLOATS];
-    if constexpr (ring_fetch) {
+    if constexpr (f32) {
         if (lane < 14) pconst[lane] = lane < 7 ? P.pbg[lane] : lane < 13 ? P.pf[lane - 7] : P.fG;
+    } else {
+        if (lane < PD_N) ((double*)pconst)[lane] = plastic_d(P, lane);
     }
     const PlasticConstL pc{(const volatile __attribute__((address_space(3))) float*)&pconst[0]};
+    const PlasticConstL64 pc64{(const volatile __attribute__((address_space(3))) double*)(double*)&pconst[0]};
     // Flux registers (levels of a hierarchy): which faces this lane stores on each row of the
     // strip -- bit j - y0 of a 64-bit mask for its lo x face and one for its lo y face, words
     // [0, 64) / [64, 128) and [128, 192) / [192, 256) -- decided here from the level's face masks,
@@ -295,17 +322,22 @@ k_godunov_march(const KArgs<T> a)
     // this lane's dword in slot 0, plane 0 (an LDS byte address)
     const unsigned ring_lane =
         (unsigned)(uintptr_t)(__attribute__((address_space(3))) float*)&ring[0] + 4u * (unsigned)lane;
-    // the rows row(q) reads into slot (q - y0) mod MARCH_RING: every row issues exactly
-    // MARCH_RING_LOADS loads (rows past the strip and lanes outside it read past the planes)
+    // the rows row(q) reads into slot (q - y0) mod RING: every row issues exactly SLOT_LOADS loads
+    // (rows past the strip and lanes outside it read past the planes)
     auto request = [&](const int q) {
-        const unsigned sl = (unsigned)(q - y0) % (unsigned)MARCH_RING;
+        const unsigned sl = (unsigned)(q - y0) % (unsigned)RING;
         const bool live = q <= rlast;
         const unsigned offs = live ? row_off(q) : kDropOffset;
         const unsigned offp = (live && valid && q - 3 >= y0) ? row_off(q - 3) : kDropOffset;
+        auto put = [&](rsrc_t rs, int p, unsigned off) {
 #pragma unroll
-        for (int p = 0; p < 5; ++p) lds_row(rin[p], &ring[(sl * MARCH_RING_LOADS + p) * 64], offs);
-        lds_row(rpw[0], &ring[(sl * MARCH_RING_LOADS + 5) * 64], offp);
-        lds_row(rpw[4], &ring[(sl * MARCH_RING_LOADS + 6) * 64], plastic ? offp : kDropOffset);
+            for (int w = 0; w < WORDS; ++w)
+                lds_row(rs, &ring[((sl * NPL + p) * WORDS + w) * 64], off + 4u * (unsigned)w);
+        };
+#pragma unroll
+        for (int p = 0; p < 5; ++p) put(rin[p], p, offs);
+        put(rpw[0], 5, offp);
+        if constexpr (PL) put(rpw[4], 6, plastic ? offp : kDropOffset);
     };
 
     // warm-up: rows y0 - 3 .. y0 give the y slopes of rows y0 - 2, y0 - 1 and the face value at
@@ -343,7 +375,7 @@ k_godunov_march(const KArgs<T> a)
     }
     if constexpr (ring_fetch) {
 #pragma unroll
-        for (int q = 1; q < MARCH_RING; ++q) request(y0 + q);
+        for (int q = 1; q < RING; ++q) request(y0 + q);
     } else {
         const unsigned off = row_off(y0 + 1);
 #pragma unroll
@@ -352,10 +384,10 @@ k_godunov_march(const KArgs<T> a)
 
     // one row of the march: after loading row r, the predictors of row r - 2 and the update of
     // row r - 3, from what c carries in; n receives what the next row needs.  The first row
-    // computes row y0 - 1's face states (used) and row y0 - 2's (not used).  Loads run one row
-    // ahead: row r + 1's stencil planes (a ghost row of the array at the last row) and row r - 3's
-    // pointwise planes are requested before the row's arithmetic, so the wavefront computes while
-    // they arrive.
+    // computes row y0 - 1's face states (used) and row y0 - 2's (not used).  Loads run RING - 1
+    // rows ahead: row r + RING - 1's stencil planes (ghost rows of the array at the last rows)
+    // and row r + RING - 4's pointwise planes are requested before the row's arithmetic, so the
+    // wavefront computes while they arrive.
     // The first two rows of the march only feed the faces of the rows after them: they skip row
     // r - 3's final x faces, update and stores.  FP64 runs them as separate code (head = true),
     // so that the loop's rows have no branch around the update; FP32 runs every row in the loop
@@ -363,7 +395,7 @@ k_godunov_march(const KArgs<T> a)
     auto row = [&](auto head, const int r, const MarchCarry<T>& c, MarchCarry<T>& n) {
         constexpr bool H = decltype(head)::value;
         const int j3 = r - 3;
-        const bool strip_row = !H && (!ring_fetch || j3 >= y0);  // the same in every lane
+        const bool strip_row = !H && (peel || j3 >= y0);  // the same in every lane
         // row r - 3's pointwise planes: szz, and gam with plasticity (in mode II sxz and syz
         // stay zero and the plastic multiplier lam is only written); a lane outside the strip,
         // or a row before it, gets an offset at or past every plane's bound (planes hold at most
@@ -372,21 +404,32 @@ k_godunov_march(const KArgs<T> a)
         const unsigned off3 = (valid && strip_row) ? row_off(j3) : kDropOffset;
         T w0[5], szz, gam;
         if constexpr (ring_fetch) {
-            // request the row MARCH_RING - 1 ahead, then wait for this row's slot.  Issued after
-            // that slot's loads: the MARCH_RING - 1 later slots (7 loads each) and the stores of
-            // the rows since (8 each).  The slots of the first rows were requested before the
-            // march, and its first two rows (rows y0 + 1, y0 + 2) store nothing, so the rows that
-            // stored since this slot's request number r - (y0 + 3), at most MARCH_RING - 1.
-            request(r + MARCH_RING - 1);
-            const unsigned sl = (unsigned)(r - y0) % (unsigned)MARCH_RING;
-            const unsigned at =
-                wait_slot(r - (y0 + 3), ring_lane + sl * (unsigned)(MARCH_RING_LOADS * 64 * sizeof(float)));
+            // request the row RING - 1 ahead, then wait for this row's slot.  Issued after that
+            // slot's loads: the RING - 1 later slots (SLOT_LOADS loads each) and the stores of the
+            // rows since (ROW_STORES each).  The slots of the first rows were requested before the march,
+            // and its first two rows (rows y0 + 1, y0 + 2) store nothing, so the rows that stored
+            // since this slot's request number r - (y0 + 3), at most RING - 1.
+            request(r + RING - 1);
+            const unsigned sl = (unsigned)(r - y0) % (unsigned)RING;
+            const unsigned at = wait_slot<RING, SLOT_LOADS, ROW_STORES>(
+                r - (y0 + 3), ring_lane + sl * (unsigned)(SLOT_LOADS * 64 * sizeof(float)));
             const __attribute__((address_space(3))) float* slot =
                 (const __attribute__((address_space(3))) float*)(uintptr_t)at;
+            // plane p of the slot (FP64: its low and high words, 64 lanes apart)
+            auto sval = [&](int p) -> T {
+                if constexpr (WORDS == 1) {
+                    return slot[64 * p];
+                } else {
+                    const unsigned long long lo = __float_as_uint(slot[64 * (2 * p)]);
+                    const unsigned long long hi = __float_as_uint(slot[64 * (2 * p + 1)]);
+                    return __longlong_as_double((long long)((hi << 32) | lo));
+                }
+            };
 #pragma unroll
-            for (int q = 0; q < 5; ++q) w0[q] = slot[64 * q];
-            szz = slot[64 * 5];
-            gam = plastic ? slot[64 * 6] : (T)0;
+            for (int q = 0; q < 5; ++q) w0[q] = sval(q);
+            szz = sval(5);
+            if constexpr (PL) gam = plastic ? sval(6) : (T)0;
+            else gam = (T)0;
         } else {
 #pragma unroll
             for (int q = 0; q < 5; ++q) w0[q] = c.wn[q];
@@ -394,7 +437,7 @@ k_godunov_march(const KArgs<T> a)
 #pragma unroll
             for (int q = 0; q < 5; ++q) n.wn[q] = bld<T>(rin[q], offn);
             szz = bld<T>(rpw[0], off3);
-            gam = plastic ? bld<T>(rpw[4], off3) : (T)0;
+            gam = (T)0;
         }
         // y: differences, slopes of row r - 1, face values at r - 3/2
         T dry[4], dy1[4], fy1[4];
@@ -498,8 +541,8 @@ k_godunov_march(const KArgs<T> a)
                 put_face_flux<T, 2>(Fo0, (size_t)ix + (size_t)(j3 - fvlo1) * (fvnx + 1), (size_t)(fvnx + 1) * fvny, m,
                                     0, f0v, f0s, (T)1, sd);
             }
-            // update of row r - 3: every row of the strip issues exactly MARCH_ROW_STORES stores
-            // (the ring's waits count them)
+            // update of row r - 3: every row of the strip issues exactly ROW_STORES stores (the
+            // ring's waits count them)
             Faces1<T, 2> F;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -522,20 +565,17 @@ k_godunov_march(const KArgs<T> a)
             u[2] = u[7] = u[8] = u[9] = (T)0;
             u[5] = szz;
             u[10] = gam;
-            if constexpr (ring_fetch)
+            if constexpr (f32)
                 cell_update_side<T, 2>(pc, m, dt, K.sc, plastic, rvy, F, u);
             else
-                cell_update_side<T, 2>(P, m, dt, K.sc, plastic, rvy, F, u);
+                cell_update_side<T, 2>(pc64, m, dt, K.sc, plastic, rvy, F, u);
 #pragma unroll
             for (int q = 0; q < 5; ++q) bst<T>(rout[q], off3, u[QC[q]]);
             bst<T>(rpw[0], off3, u[5]);
-            if (ring_fetch) {
+            if constexpr (PL) {
                 const unsigned offl = plastic ? off3 : kDropOffset;
                 bst<T>(rpw[3], offl, u[9]);
                 bst<T>(rpw[4], offl, u[10]);
-            } else if (plastic) {
-                bst<T>(rpw[3], off3, u[9]);
-                bst<T>(rpw[4], off3, u[10]);
             }
         }
         // carried to the next row
@@ -558,7 +598,7 @@ k_godunov_march(const KArgs<T> a)
     // the DPP lane shifts are convergent, so the compiler does not unroll the loop itself)
     // (rlast >= y0 + 3: a strip has at least one row)
     int r0 = y0 + 1;
-    if constexpr (!ring_fetch) {
+    if constexpr (peel) {
         row(std::true_type{}, y0 + 1, A, B);
         row(std::true_type{}, y0 + 2, B, A);
         r0 = y0 + 3;

@@ -757,23 +757,37 @@ __device__ __forceinline__ void sw_fault(const DevParams& P, int i, double slip,
 }
 
 // PLASTICFLOW + INVARIANTS + YIELD (PlasticPhysicsF.ChF:74-181)
-template <typename T>
-__device__ __forceinline__ void plastic_flow(const DevParams& P, T dt, T& sxx, T& syy, T& szz, T& sxy, T& sxz,
-                                             T& syz, T& plam, T& pgam)
+// the FP64 Drucker-Prager constants, from the parameters or (plastic_flow's PP = PlasticConstL64)
+// held in LDS by a kernel that keeps no copy of them in registers across its loop
+enum { PD_B, PD_H, PD_MU, PD_G, PD_K, PD_BETA, PD_ETA, PD_N };
+struct PlasticConstL64 {
+    const volatile __attribute__((address_space(3))) double* c;
+};
+__device__ __forceinline__ double plastic_d(const DevParams& P, int k)
+{
+    return k == PD_B ? P.p_b : k == PD_H ? P.p_h : k == PD_MU ? P.p_mu : k == PD_G ? P.G : k == PD_K ? P.K
+         : k == PD_BETA ? P.p_beta : P.p_eta;
+}
+__device__ __forceinline__ double plastic_d(const PlasticConstL64& P, int k) { return P.c[k]; }
+
+template <typename T, typename PP>
+__device__ __forceinline__ void plastic_flow(const PP& P, T dt, T& sxx, T& syy, T& szz, T& sxy, T& sxz, T& syz,
+                                             T& plam, T& pgam)
 {
     T sigma = (sxx + syy + szz) / (T)3;
     const T a = sxx - syy, b = syy - szz, c = szz - sxx;
     const T J2 = (a * a + b * b + c * c) / (T)6 + sxy * sxy + sxz * sxz + syz * syz;
     T tau = sqrt(J2);
-    const T bb = (T)P.p_b + pgam * (T)P.p_h;
-    const T Y = tau - fmax(-(T)P.p_mu * sigma + bb, (T)0);
+    const T bb = (T)plastic_d(P, PD_B) + pgam * (T)plastic_d(P, PD_H);
+    const T Y = tau - fmax(-(T)plastic_d(P, PD_MU) * sigma + bb, (T)0);
     if (Y <= (T)0) { plam = (T)0; return; }
-    const T G = (T)P.G, K = (T)P.K;
-    plam = Y / (dt * ((T)P.p_h + G + (T)P.p_mu * (T)P.p_beta * K) + (T)P.p_eta);
+    const T G = (T)plastic_d(P, PD_G), K = (T)plastic_d(P, PD_K);
+    plam = Y / (dt * ((T)plastic_d(P, PD_H) + G + (T)plastic_d(P, PD_MU) * (T)plastic_d(P, PD_BETA) * K) +
+                (T)plastic_d(P, PD_ETA));
     pgam = pgam + dt * plam;
     const T sig_e = sigma;
     tau = tau - dt * plam * G;
-    sigma = sigma - dt * plam * K * (T)P.p_beta;
+    sigma = sigma - dt * plam * K * (T)plastic_d(P, PD_BETA);
     const T factor = tau / (tau + dt * plam * G);
     sxx = sigma + factor * (sxx - sig_e);
     syy = sigma + factor * (syy - sig_e);

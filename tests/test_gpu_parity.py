@@ -465,6 +465,62 @@ def test_batch_member_equals_alone(precision):
         o.close()
 
 
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
+def test_batch_mixed_plasticity(precision):
+    """A batch of a scenario with plasticity and one without runs the row march's variant with
+    plasticity for both (each member's own flag decides); alone, the scenario without plasticity
+    runs the variant that never reads gam or writes lam and gam.  Each member matches its run
+    alone: FP64 bit for bit, FP32 bulk within 1e-5 (its two variants may contract differently)."""
+    cp = _cfg("rs", "PPM", nx=200, ny=120)
+    cp["plasticity"] = 1
+    cp["p_mu"], cp["p_beta"], cp["p_eta"], cp["p_b"] = 0.6, 0.3, 0.03 * scenarios.G_MPA, 1.0
+    cp["rs"]["state_law"] = 1
+    cp["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
+    cn = _cfg("rs", "PPM", nx=160, ny=96, seed=8)
+    cases = [(cp, 16), (cn, 17)]
+
+    def make(cfg, seed):
+        U, B = _state(cfg, seed)
+        ctx = capi.Context(cfg, precision)
+        ctx.set_march(8)
+        boxes = scenarios.split_boxes(cfg["nx"], cfg["ny"], 32)
+        lev = capi.Level(ctx, boxes)
+        for k, b in enumerate(boxes):
+            lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
+        return ctx, lev, U, B, boxes
+
+    alone = []
+    for cfg, seed in cases:
+        ctx, lev, U, B, boxes = make(cfg, seed)
+        lev.run(6, 0.0, scenarios.dt_of(cfg))
+        lev.sync()
+        alone.append(_download(lev, U, B, boxes))
+        lev.close()
+        ctx.close()
+    made = [make(cfg, seed) for cfg, seed in cases]
+    batch = capi.Batch()
+    for (cfg, _), (ctx, lev, U, B, boxes) in zip(cases, made):
+        batch.add(lev, 0.0, scenarios.dt_of(cfg))
+    batch.run(6)
+    batch.sync()
+    got = []
+    for (cfg, _), (ctx, lev, U, B, boxes) in zip(cases, made):
+        batch.remove(lev)
+        lev.run(0, 0.0, scenarios.dt_of(cfg))
+        got.append(_download(lev, U, B, boxes))
+    batch.close()
+    for ctx, lev, *_ in made:
+        lev.close()
+        ctx.close()
+    assert np.max(alone[0][0][9]) > 0.0, "plastic flow expected somewhere"
+    for (gu, gb), (au, ab) in zip(got, alone):
+        if precision == capi.PREC_FP64:
+            assert np.array_equal(gu, au) and np.array_equal(gb, ab)
+        else:
+            assert np.all(_relerr(gu, au) <= 1e-5), _relerr(gu, au)
+            assert np.all(_relerr(gb[[6, 8, 10]], ab[[6, 8, 10]]) <= 1e-5)
+
+
 @pytest.mark.parametrize("precision,tol", [(capi.PREC_FP64, 0.0), (capi.PREC_FP32_BULK, 1e-4)])
 def test_box_layout_invariance(precision, tol):
     """The same scenario on one box and on 9 boxes.  FP64: bit-identical (every kernel rounds like

@@ -9,7 +9,8 @@ do not depend on the box layout).
 C1 reduced to 256x128 (rate-and-state, aging law, dx = 25 m, 400 steps): exp/log/asinh of the
 device math library and of glibc differ in the last bit on fault faces; the run stays within
 1e-12 relative of the oracle on state, slip and psi, and of each ruptured face's own rupture
-time (unruptured faces hold the 1e10 sentinel exactly).
+time (unruptured faces hold the 1e10 sentinel exactly).  The same run in FP32 bulk is
+bit-identical between strip heights and within 1e-3 of the FP64 oracle.
 """
 import os
 
@@ -26,8 +27,10 @@ G = scenarios.GHOST
 NTHREADS = min(16, os.cpu_count() or 1)
 
 
-def _run_level(cfg, boxes, nsteps, U, B):
-    ctx = capi.Context(cfg, capi.PREC_FP64)
+def _run_level(cfg, boxes, nsteps, U, B, precision=capi.PREC_FP64, march=None):
+    ctx = capi.Context(cfg, precision)
+    if march is not None:
+        ctx.set_march(march)
     lev = capi.Level(ctx, boxes)
     for k, b in enumerate(boxes):
         lev.upload(k, scenarios.box_view(U, b), scenarios.box_bview(B, b) if b[1] == 0 else None)
@@ -81,3 +84,27 @@ def test_c1_reduced_long_run_within_tolerance():
     assert np.all(e[[8, 9, 10]] <= 1e-12), e  # slip, rupture time (per face), psi
     assert np.max(Br[2]) > 1e-9  # the nucleation has started slipping
     assert new_ruptures(B0[:, G:-G], Br) >= 20  # and ruptured faces inside the window
+
+
+def test_c1_reduced_long_run_fp32_bulk():
+    """The same 400 steps in FP32 bulk (the row march without plasticity, its LDS ring of 4
+    slots): with strips of 8 and of 16 rows the results are bit-identical (a row read from a
+    slot before its loads landed would differ between the two), and within 1e-3 of the FP64
+    oracle on state, slip and psi."""
+    cfg = scenarios.config_c1(seed=0, nx=256, ny=128, dx=25.0)
+    nsteps = 400
+    U, B = scenarios.level_arrays(cfg)
+    Ur, Br = U.copy(), B.copy()
+    assert tmo.level_steps(tmo.OracleParams(cfg), Ur, Br, 0.0, scenarios.dt_of(cfg), nsteps,
+                           nthreads=NTHREADS, maxbox=32) == 0
+    Ur, Br = Ur[:, G:-G, G:-G], Br[:, G:-G]
+    boxes = scenarios.split_boxes(256, 128, 32)
+    runs = [_run_level(cfg, boxes, nsteps, U.copy(), B.copy(), capi.PREC_FP32_BULK, rows) for rows in (8, 16)]
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    Ug, Bg = runs[0]
+    for c in range(22):
+        scale = max(np.max(np.abs(Ur[c])), 1e-300)
+        assert np.max(np.abs(Ug[c] - Ur[c])) <= 1e-3 * scale, (c, np.max(np.abs(Ug[c] - Ur[c])) / scale)
+    for c in (8, 10):  # slip, psi
+        scale = max(np.max(np.abs(Br[c])), 1e-300)
+        assert np.max(np.abs(Bg[c] - Br[c])) <= 1e-3 * scale, (c, np.max(np.abs(Bg[c] - Br[c])) / scale)

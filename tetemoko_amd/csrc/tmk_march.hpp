@@ -258,8 +258,8 @@ k_godunov_march(const KArgs<T> a)
     constexpr int RING_FIT = (63 + ROW_STORES) / (SLOT_LOADS + ROW_STORES);
     constexpr int RING = RING_FIT < MARCH_RING ? RING_FIT : MARCH_RING;
     static_assert(RING >= 2 && RING * SLOT_LOADS + (RING - 1) * ROW_STORES <= 63, "vmcnt counts to 63");
-    // one LDS object: the ring, then the FP32 plastic return's constants (see PlasticConstL), then
-    // the lanes' flux-register face masks (below)
+    // one LDS object: the ring, then the plastic return's constants (PlasticConstL for FP32,
+    // PlasticConstL64 for FP64), then the lanes' flux-register face masks (below)
     constexpr int RING_FLOATS = (ring_fetch ? RING * SLOT_LOADS : 0) * 64;
     __shared__ float ring[RING_FLOATS + 16 + 4 * 64];
     float* const pconst = &ring[RING_FLOATS];

@@ -34,8 +34,8 @@ constexpr unsigned kDropOffset = 0xFFFFFF00u;  // buffer offset of a lane whose 
 // row q and the pointwise planes szz, gam of row q - 3), filled by buffer loads that write LDS
 // directly, so the requests in flight hold no VGPRs.  FP32: 4 slots x 7 planes x 64 lanes x 4 B =
 // 7 KiB per wavefront, 16 wavefronts per CU use 112 of its 160 KiB.  FP64: each plane row is two
-// loads (low and high words), 3 slots (10.5 KiB with plasticity), 8 wavefronts per CU.  Every
-// vector memory operation that can be in flight at a ring wait fits the 6-bit counter.
+// loads (low and high words), 3 slots (10.5 KiB with plasticity), 8 wavefronts per CU.  The
+// ring's operations that can be in flight at a wait fit the 6-bit counter.
 constexpr int MARCH_RING = 4;  // slots at most
 constexpr int MARCH_RING_LOADS = 7;   // loads per slot
 constexpr int MARCH_ROW_STORES = 8;   // stores per row: 5 stencil planes, szz, lam, gam
@@ -248,9 +248,9 @@ k_godunov_march(const KArgs<T> a)
     // the ring: an element is WORDS dwords, each plane row of a slot WORDS loads of one dword per
     // lane (FP64: low words, then high words); a slot holds NPL planes (gam only with plasticity)
     // and a row of the strip stores ROW_STORES planes (lam and gam only with plasticity).  As many
-    // slots, at most MARCH_RING, as keep every operation that can be in flight at a wait -- the
-    // slot's own loads, the later slots' and the stores of RING - 1 rows -- within what the 6-bit
-    // counter holds
+    // slots, at most MARCH_RING, as keep the ring's operations that can be in flight at a wait --
+    // the slot's own loads, the later slots' and the stores of RING - 1 rows -- within what the
+    // 6-bit counter holds (flux-register stores of hierarchy levels come on top)
     constexpr int WORDS = (int)(sizeof(T) / sizeof(float));
     constexpr int NPL = PL ? MARCH_RING_LOADS : MARCH_RING_LOADS - 1;
     constexpr int SLOT_LOADS = NPL * WORDS;

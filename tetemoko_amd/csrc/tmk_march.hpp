@@ -609,6 +609,11 @@ k_godunov_march(const KArgs<T> a)
         row(std::false_type{}, r + 1, B, A);
         if (r + 2 > rlast) break;
     }
+    // the last rows requested rows past the strip (loads the bound check drops, which still write
+    // their zeros to the ring): every ring load lands before the wavefront ends and its LDS can
+    // go to another workgroup (the last row's ROW_STORES stores are the only operations after
+    // the last request)
+    if constexpr (ring_fetch) (void)wait_vmem<ROW_STORES>(ring_lane);
 }
 
 }  // namespace tmk

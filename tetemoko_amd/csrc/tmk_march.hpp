@@ -384,10 +384,11 @@ k_godunov_march(const KArgs<T> a)
 
     // one row of the march: after loading row r, the predictors of row r - 2 and the update of
     // row r - 3, from what c carries in; n receives what the next row needs.  The first row
-    // computes row y0 - 1's face states (used) and row y0 - 2's (not used).  Loads run RING - 1
-    // rows ahead: row r + RING - 1's stencil planes (ghost rows of the array at the last rows)
-    // and row r + RING - 4's pointwise planes are requested before the row's arithmetic, so the
-    // wavefront computes while they arrive.
+    // computes row y0 - 1's face states (used) and row y0 - 2's (not used).  With the ring, loads
+    // run RING - 1 rows ahead: row r + RING - 1's stencil planes (ghost rows of the array at the
+    // last rows) and row r + RING - 4's pointwise planes are requested before the row's
+    // arithmetic, so the wavefront computes while they arrive; without it (FP64 without
+    // plasticity), row r + 1's stencil planes into VGPRs.
     // The first two rows of the march only feed the faces of the rows after them: they skip row
     // r - 3's final x faces, update and stores.  FP64 runs them as separate code (head = true),
     // so that the loop's rows have no branch around the update; FP32 runs every row in the loop
@@ -406,9 +407,9 @@ k_godunov_march(const KArgs<T> a)
         if constexpr (ring_fetch) {
             // request the row RING - 1 ahead, then wait for this row's slot.  Issued after that
             // slot's loads: the RING - 1 later slots (SLOT_LOADS loads each) and the stores of the
-            // rows since (ROW_STORES each).  The slots of the first rows were requested before the march,
-            // and its first two rows (rows y0 + 1, y0 + 2) store nothing, so the rows that stored
-            // since this slot's request number r - (y0 + 3), at most RING - 1.
+            // rows since (ROW_STORES each).  The slots of the first rows were requested before
+            // the march, and its first two rows (rows y0 + 1, y0 + 2) store nothing, so the rows
+            // that stored since this slot's request number r - (y0 + 3), at most RING - 1.
             request(r + RING - 1);
             const unsigned sl = (unsigned)(r - y0) % (unsigned)RING;
             const unsigned at = wait_slot<RING, SLOT_LOADS, ROW_STORES>(

@@ -771,16 +771,18 @@ def test_march_level_matches_oracle(rows, maxbox):
     assert np.array_equal(Ug[:, far:, :], Ur[:, far:, :])
 
 
+@pytest.mark.parametrize("rows", [1, 2, 3, 24])
 @pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_BULK])
-def test_march_level_plasticity(precision):
-    """The Drucker-Prager return fused into the marching update."""
+def test_march_level_plasticity(precision, rows):
+    """The Drucker-Prager return fused into the marching update, with strips of 1-3 rows (the LDS
+    ring's waits at a strip's start, where its first two rows store nothing) and of 24."""
     cfg = _cfg("rs", "PPM", nx=200, ny=120)
     cfg["plasticity"] = 1
     cfg["p_mu"], cfg["p_beta"], cfg["p_eta"], cfg["p_b"] = 0.6, 0.3, 0.03 * scenarios.G_MPA, 1.0
     cfg["rs"]["state_law"] = 1
     cfg["background"] = (-120.0, -120.0, -120.0, 70.0, 0.0, 0.0)
     (Ug, Ur), (Bg, Br) = _level_case(cfg, scenarios.split_boxes(200, 120, 32), 4, seed=16, precision=precision,
-                                     march=24)
+                                     march=rows)
     if precision == capi.PREC_FP64:
         assert np.all(_relerr(Ug, Ur) <= TOL), _relerr(Ug, Ur)
         assert np.array_equal(Ug[:, 15:, :], Ur[:, 15:, :])
